@@ -1,0 +1,205 @@
+"""gpuboscia_amd: MI355X (gfx950) Frank-Wolfe / DICG node relaxation.
+
+The product is the C-ABI library ``libgpuboscia_amd.so`` (declared in
+``include/gpuboscia_amd.h``) and the C++ drop-in header
+``include/gpuboscia/gpuboscia.hpp``.  This module is a thin ctypes binding
+used by the tests and the benchmark; it never falls back to any CPU path and
+raises if the HIP library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpuboscia_amd.so")
+
+GB_OK, GB_ERR_INVALID_ARGUMENT, GB_ERR_RUNTIME, GB_ERR_DEVICE = 0, 1, 2, 3
+ALG_FRANK_WOLFE, ALG_DICG = 0, 1
+LS_OPEN_LOOP, LS_SCALED, LS_SAFEGUARDED, LS_SECANT, LS_QUADRATIC_EXACT = 0, 1, 2, 3, 4
+OBJ_DENSE, OBJ_QAP = 0, 1
+LMO_BOX, LMO_SIMPLEX, LMO_BIRKHOFF = 0, 1, 2
+HEUR_ROUND, HEUR_FLOOR_OFFSET = 0, 1
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+BRANCH_POW2, BRANCH_MOST_FRACTIONAL, BRANCH_LEARNED, BRANCH_CALLBACK = 0, 1, 2, 3
+NODE_INFEASIBLE = 2
+NODE_INITIAL_POINT_SUPPLIED = 0x80
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class ObjectiveDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("Qt", ctypes.c_void_p),
+                ("linear", ctypes.c_void_p), ("grad_scale", ctypes.c_double),
+                ("linear_scale", ctypes.c_double), ("constant", ctypes.c_double),
+                ("order", ctypes.c_int), ("A", ctypes.c_void_p), ("B", ctypes.c_void_p)]
+
+
+class LmoDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("right_side", ctypes.c_double),
+                ("equality", ctypes.c_int), ("is_int", ctypes.c_void_p), ("order", ctypes.c_int)]
+
+
+class AlgorithmDesc(ctypes.Structure):
+    _fields_ = [("algorithm", ctypes.c_int), ("line_search", ctypes.c_int),
+                ("secant_iters", ctypes.c_int), ("secant_tol", ctypes.c_double),
+                ("secant_denom_eps", ctypes.c_double), ("heuristic", ctypes.c_int),
+                ("heuristic_offset", ctypes.c_double)]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("n_vars", ctypes.c_int), ("lb", _dp), ("ub", _dp), ("int_vars", _ip),
+                ("n_int_vars", ctypes.c_int)]
+
+
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_int)
+VALIDATOR_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int,
+                                ctypes.c_double)
+BRANCH_SCORE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int,
+                                   ctypes.c_int, _dp)
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [("gpu_batch_size", ctypes.c_int), ("num_streams", ctypes.c_int),
+                ("gpu_block_threads", ctypes.c_int), ("gpu_max_iters", ctypes.c_int),
+                ("gpu_eps", ctypes.c_double), ("gpu_eps_decay", ctypes.c_double),
+                ("gpu_eps_min", ctypes.c_double), ("integer_tol", ctypes.c_double),
+                ("abs_gap_tol", ctypes.c_double), ("rel_gap_tol", ctypes.c_double),
+                ("algorithm", AlgorithmDesc), ("branch_rule", ctypes.c_int),
+                ("branch_k", ctypes.c_int), ("time_limit_seconds", ctypes.c_double),
+                ("natural_lower_bound", ctypes.c_double), ("root_initial_point", _dp),
+                ("root_initial_point_size", ctypes.c_int),
+                ("root_validator", VALIDATOR_FN), ("root_validator_user", ctypes.c_void_p),
+                ("progress", PROGRESS_FN), ("progress_user", ctypes.c_void_p),
+                ("branch_score", BRANCH_SCORE_FN), ("branch_score_user", ctypes.c_void_p),
+                ("precision", ctypes.c_int), ("learned_model", ctypes.c_int),
+                ("max_nodes", ctypes.c_longlong)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("objective", ctypes.c_double), ("x", _dp), ("dual_bound", ctypes.c_double),
+                ("nodes_processed", ctypes.c_int), ("elapsed_seconds", ctypes.c_double),
+                ("reached_tolerance", ctypes.c_int), ("reached_time_limit", ctypes.c_int),
+                ("fw_iterations", ctypes.c_longlong), ("relax_seconds", ctypes.c_double)]
+
+
+_lib = None
+
+#: Every symbol that include/gpuboscia_amd.h declares.
+EXPORTED = ["gb_relax_batch", "gb_default_settings", "gb_solve", "gb_lmo_relaxation_feasible_cpu",
+            "gb_last_error", "gb_version", "gb_device_count"]
+
+
+class GpuBosciaError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class InvalidArgument(GpuBosciaError, ValueError):
+    """Mirrors std::invalid_argument in the reference."""
+
+
+def load():
+    """Load the in-tree HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.gb_relax_batch.restype = ctypes.c_int
+    lib.gb_relax_batch.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
+                                   ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
+        + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    lib.gb_default_settings.restype = None
+    lib.gb_default_settings.argtypes = [ctypes.POINTER(Settings)]
+    lib.gb_solve.restype = ctypes.c_int
+    lib.gb_solve.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(ObjectiveDesc),
+                             ctypes.POINTER(LmoDesc), ctypes.POINTER(Settings),
+                             ctypes.POINTER(Result)]
+    lib.gb_lmo_relaxation_feasible_cpu.restype = ctypes.c_int
+    lib.gb_lmo_relaxation_feasible_cpu.argtypes = [ctypes.POINTER(LmoDesc), ctypes.c_void_p, _dp,
+                                                   _dp, _dp, ctypes.c_int, ctypes.c_double]
+    lib.gb_last_error.restype = ctypes.c_char_p
+    lib.gb_version.restype = ctypes.c_char_p
+    lib.gb_device_count.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code == GB_OK:
+        return
+    msg = load().gb_last_error().decode()
+    if code == GB_ERR_INVALID_ARGUMENT:
+        raise InvalidArgument(code, msg)
+    raise GpuBosciaError(code, msg)
+
+
+def default_settings() -> Settings:
+    s = Settings()
+    load().gb_default_settings(ctypes.byref(s))
+    return s
+
+
+def algorithm(dicg=True, line_search=LS_SECANT, secant_iters=4, secant_tol=1e-10,
+              secant_denom_eps=1e-14, heuristic=HEUR_ROUND, heuristic_offset=0.0) -> AlgorithmDesc:
+    return AlgorithmDesc(ALG_DICG if dicg else ALG_FRANK_WOLFE, line_search, secant_iters,
+                         secant_tol, secant_denom_eps, heuristic, heuristic_offset)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def relax_batch(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb, ub,
+                eps, iterations, status, gap, value, best_vertex, best_vertex_val, flags=None,
+                candidate_tol=1e-6, block_threads=0, stream=None):
+    """gb_relax_batch on torch device tensors (float64/int32/uint8)."""
+    lib = load()
+    code = lib.gb_relax_batch(ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
+                              int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
+                              _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),
+                              _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
+                              float(candidate_tol), int(block_threads),
+                              ctypes.c_void_p(stream or 0))
+    check(code)
+
+
+def solve(n, lb, ub, int_vars, obj: ObjectiveDesc, lmo: LmoDesc, settings: Settings | None = None,
+          root_initial_point=None):
+    """gb_solve with host lists/arrays for the problem; returns a dict."""
+    lib = load()
+    s = settings if settings is not None else default_settings()
+    lb_a = (ctypes.c_double * n)(*[float(v) for v in lb])
+    ub_a = (ctypes.c_double * n)(*[float(v) for v in ub])
+    iv = list(int_vars)
+    iv_a = (ctypes.c_int * max(1, len(iv)))(*iv)
+    prob = Problem(n, lb_a, ub_a, iv_a, len(iv))
+    keep = []
+    if root_initial_point is not None:
+        rp = (ctypes.c_double * len(root_initial_point))(*[float(v) for v in root_initial_point])
+        keep.append(rp)
+        s.root_initial_point = rp
+        s.root_initial_point_size = len(root_initial_point)
+    xo = (ctypes.c_double * n)()
+    res = Result()
+    res.x = xo
+    code = lib.gb_solve(ctypes.byref(prob), ctypes.byref(obj), ctypes.byref(lmo), ctypes.byref(s),
+                        ctypes.byref(res))
+    check(code)
+    return {"objective": res.objective, "x": [xo[j] for j in range(n)],
+            "dual_bound": res.dual_bound, "nodes_processed": res.nodes_processed,
+            "elapsed_seconds": res.elapsed_seconds,
+            "reached_tolerance": bool(res.reached_tolerance),
+            "reached_time_limit": bool(res.reached_time_limit),
+            "fw_iterations": res.fw_iterations, "relax_seconds": res.relax_seconds}
+
+
+def inf():
+    return math.inf

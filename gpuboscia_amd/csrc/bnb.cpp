@@ -1,0 +1,467 @@
+// Host branch-and-bound driver behind gb_solve.
+//
+// Follows gpuboscia::solve / DeviceResidentHybridSolver
+// (gpuboscia.cuh:56-170, device_resident_solver.cuh:1184-2085) with one
+// deliberate change of schedule: one batch is in flight at a time and every
+// node of a completed batch is post-processed in batch order before the next
+// pop.  The reference overlaps up to num_streams batches on host threads,
+// which makes node counts and tie-breaking schedule dependent; a single
+// in-flight batch makes the search deterministic, so the GPU run and the CPU
+// oracle expand the same tree.  The frontier is a best-first heap ordered by
+// (dual_bound, id).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <queue>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gb_host.h"
+
+namespace gb {
+namespace {
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+struct HostNode {
+    int id = 0, parent = -1, depth = 0;
+    std::vector<double> lb, ub, rx;  // rx: warm start (relaxation_x)
+    int riters = 0;                  // relaxation_iterations
+    double bound = -1e300;           // dual_bound
+};
+
+struct HeapOrder {
+    bool operator()(const HostNode& a, const HostNode& b) const {
+        return a.bound > b.bound || (a.bound == b.bound && a.id > b.id);
+    }
+};
+
+// gap_utils.h:14-62
+double certified_fw_lower_bound(double inherited, double primal, double gap) {
+    if (!std::isfinite(primal) || !std::isfinite(gap) || gap < 0.0) return inherited;
+    return std::max(inherited, primal - gap);
+}
+double gap_absolute(double bound, double inc) {
+    if (inc >= kInf || bound <= -kInf) return kInf;
+    return std::fabs(inc - bound);
+}
+double gap_relative(double bound, double inc) {
+    if (!std::isfinite(bound) || !std::isfinite(inc)) return kInf;
+    const double scale = std::min(std::fabs(bound), std::fabs(inc));
+    if (scale == 0.0) return kInf;
+    return (inc - bound) / scale;
+}
+bool gap_within_tolerance(double bound, double inc, double at, double rt) {
+    if (!std::isfinite(bound) || !std::isfinite(inc)) return false;
+    return gap_absolute(bound, inc) <= at || gap_relative(bound, inc) <= rt;
+}
+double depth_tolerance(double root, double decay, double minimum, int depth) {
+    if (depth <= 0) return std::max(root, minimum);
+    return std::max(minimum, root * std::pow(decay, depth));
+}
+
+// top_k_infeasible (branching.h:43-65).  Same candidate vector and
+// comparator as the reference so std::sort orders ties identically.
+std::vector<int> top_k_infeasible(const double* x, const std::vector<int>& iv, int k, double tol) {
+    struct Cand { int idx; double dist; };
+    std::vector<Cand> c;
+    c.reserve(iv.size());
+    for (int i : iv) {
+        const double frac = x[i] - std::floor(x[i]);
+        const double dist = std::min(frac, 1.0 - frac);
+        if (dist > tol) c.push_back({i, dist});
+    }
+    std::sort(c.begin(), c.end(), [](const Cand& a, const Cand& b) { return a.dist > b.dist; });
+    const int m = std::min(k, (int)c.size());
+    std::vector<int> r(m);
+    for (int i = 0; i < m; ++i) r[i] = c[i].idx;
+    return r;
+}
+
+// Single most fractional variable (branching.h:24-36), ignoring variables
+// within the integer tolerance.
+int most_fractional(const double* x, const std::vector<int>& iv, double tol) {
+    int best = -1;
+    double bd = -1.0;
+    for (int i : iv) {
+        const double frac = x[i] - std::floor(x[i]);
+        const double dist = std::min(frac, 1.0 - frac);
+        if (dist > tol && dist > bd) { bd = dist; best = i; }
+    }
+    return best;
+}
+
+struct DeviceBatch {
+    int cap = 0, n = 0;
+    double *x = nullptr, *lb = nullptr, *ub = nullptr, *bv = nullptr;
+    double *eps = nullptr, *gap = nullptr, *val = nullptr, *bvv = nullptr;
+    int* it = nullptr;
+    uint8_t *st = nullptr, *flags = nullptr;
+    double *hx = nullptr, *hlb = nullptr, *hub = nullptr, *hbv = nullptr;
+    double *heps = nullptr, *hgap = nullptr, *hval = nullptr, *hbvv = nullptr;
+    int* hit = nullptr;
+    uint8_t* hst = nullptr;
+    hipStream_t stream = nullptr;
+
+    gb_status alloc(int capacity, int nv) {
+        cap = capacity;
+        n = nv;
+        const size_t xn = (size_t)cap * n;
+        GB_HIP(hipMalloc(&x, xn * 8));
+        GB_HIP(hipMalloc(&lb, xn * 8));
+        GB_HIP(hipMalloc(&ub, xn * 8));
+        GB_HIP(hipMalloc(&bv, xn * 8));
+        GB_HIP(hipMalloc(&eps, cap * 8));
+        GB_HIP(hipMalloc(&gap, cap * 8));
+        GB_HIP(hipMalloc(&val, cap * 8));
+        GB_HIP(hipMalloc(&bvv, cap * 8));
+        GB_HIP(hipMalloc(&it, cap * 4));
+        GB_HIP(hipMalloc(&st, cap));
+        GB_HIP(hipMalloc(&flags, n));
+        GB_HIP(hipHostMalloc(&hx, xn * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hlb, xn * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hub, xn * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hbv, xn * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&heps, cap * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hgap, cap * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hval, cap * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hbvv, cap * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hit, cap * 4, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hst, cap, hipHostMallocDefault));
+        GB_HIP(hipStreamCreate(&stream));
+        return GB_OK;
+    }
+    ~DeviceBatch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : {(void*)x, (void*)lb, (void*)ub, (void*)bv, (void*)eps, (void*)gap,
+                        (void*)val, (void*)bvv, (void*)it, (void*)st, (void*)flags})
+            if (p) (void)hipFree(p);
+        for (void* p : {(void*)hx, (void*)hlb, (void*)hub, (void*)hbv, (void*)heps, (void*)hgap,
+                        (void*)hval, (void*)hbvv, (void*)hit, (void*)hst})
+            if (p) (void)hipHostFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, const gb_lmo_desc* l,
+                            const gb_settings* s) {
+    if (!pr || !o || !l || !s) return fail(GB_ERR_INVALID_ARGUMENT, "null argument");
+    if (pr->n_vars <= 0 || !pr->lb || !pr->ub)
+        return fail(GB_ERR_INVALID_ARGUMENT, "problem requires n_vars > 0 and bound vectors");
+    // device_resident_solver.cuh:1198-1218
+    if (s->gpu_max_iters <= 0 || s->gpu_batch_size <= 0 || s->num_streams <= 0)
+        return fail(GB_ERR_INVALID_ARGUMENT, "GPU iterations, batch size and streams must be positive");
+    for (int t = 0; t < pr->n_int_vars; ++t)
+        if (pr->int_vars[t] < 0 || pr->int_vars[t] >= pr->n_vars)
+            return fail(GB_ERR_INVALID_ARGUMENT, "integer variable index out of range");
+    if (!std::isfinite(s->gpu_eps) || s->gpu_eps <= 0.0 || !std::isfinite(s->gpu_eps_min)
+            || s->gpu_eps_min <= 0.0 || !std::isfinite(s->gpu_eps_decay) || s->gpu_eps_decay <= 0.0
+            || s->gpu_eps_decay > 1.0 || !std::isfinite(s->abs_gap_tol) || s->abs_gap_tol < 0.0
+            || !std::isfinite(s->rel_gap_tol) || s->rel_gap_tol < 0.0)
+        return fail(GB_ERR_INVALID_ARGUMENT, "invalid tolerance configuration");
+    if (std::isnan(s->time_limit_seconds) || s->time_limit_seconds <= 0.0)
+        return fail(GB_ERR_INVALID_ARGUMENT, "time limit must be positive");
+    if (s->gpu_block_threads != 0
+            && (s->gpu_block_threads < 64 || s->gpu_block_threads % 64 != 0 || s->gpu_block_threads > 1024))
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "gpu_block_threads must be zero or a positive multiple of 64 up to 1024");
+    if (o->n != pr->n_vars)
+        return fail(GB_ERR_RUNTIME, "DeviceObjective::n_vars does not match Problem::n_vars");
+    // gpuboscia.cuh:86-90
+    if (std::isnan(s->natural_lower_bound) || s->natural_lower_bound == kInf)
+        return fail(GB_ERR_INVALID_ARGUMENT, "natural lower bound must be finite or negative infinity");
+    if (s->branch_rule < GB_BRANCH_POW2 || s->branch_rule > GB_BRANCH_CALLBACK)
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown branch rule");
+    if (s->branch_rule == GB_BRANCH_POW2 && (s->branch_k < 1 || s->branch_k > 16))
+        return fail(GB_ERR_INVALID_ARGUMENT, "branch_k must be in [1, 16]");
+    if (s->branch_rule == GB_BRANCH_CALLBACK && !s->branch_score)
+        return fail(GB_ERR_INVALID_ARGUMENT, "callback branching requires branch_score");
+    if (s->precision != GB_PREC_FP64)
+        return fail(GB_ERR_INVALID_ARGUMENT, "gb_solve runs the FP64 plain form; use gb_solve_batched for mixed precision");
+    return GB_OK;
+}
+
+// validate_root_initial_point (tree.h:115-147)
+gb_status validate_root(const gb_problem* pr, const gb_lmo_desc* l, const gb_settings* s) {
+    if (!s->root_initial_point || s->root_initial_point_size == 0) return GB_OK;
+    const double tol = 1e-9;
+    const int n = pr->n_vars;
+    if (s->root_initial_point_size != n)
+        return fail(GB_ERR_INVALID_ARGUMENT, "invalid root initial point dimensions or tolerance");
+    const double* x = s->root_initial_point;
+    for (int j = 0; j < n; ++j)
+        if (!std::isfinite(x[j]) || std::isnan(pr->lb[j]) || std::isnan(pr->ub[j])
+                || pr->lb[j] > pr->ub[j] || x[j] < pr->lb[j] - tol || x[j] > pr->ub[j] + tol)
+            return fail(GB_ERR_INVALID_ARGUMENT, "root initial point is nonfinite or violates bounds");
+    bool feasible;
+    if (s->root_validator) {
+        feasible = s->root_validator(s->root_validator_user, x, pr->lb, pr->ub, n, tol) != 0;
+    } else {
+        feasible = gb_lmo_relaxation_feasible_cpu(l, nullptr, x, pr->lb, pr->ub, n, tol) != 0;
+    }
+    if (!feasible)
+        return fail(GB_ERR_INVALID_ARGUMENT, "root initial point violates the LMO's continuous feasible set");
+    return GB_OK;
+}
+
+gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_lmo_desc* l,
+                     const gb_settings* s, gb_result* res) {
+    gb_status st = validate_settings(pr, o, l, s);
+    if (st != GB_OK) return st;
+    st = validate_descriptors(o, &s->algorithm, l);
+    if (st != GB_OK) return st;
+    st = validate_root(pr, l, s);
+    if (st != GB_OK) return st;
+
+    using Clock = std::chrono::steady_clock;
+    const auto t0 = Clock::now();
+    const int n = pr->n_vars;
+    const std::vector<int> iv(pr->int_vars, pr->int_vars + pr->n_int_vars);
+    const int B = s->gpu_batch_size;
+
+    DeviceBatch db;
+    st = db.alloc(B, n);
+    if (st != GB_OK) return st;
+    std::vector<uint8_t> hflags(n, 0);
+    for (int i : iv) hflags[i] = 1;
+    GB_HIP(hipMemcpy(db.flags, hflags.data(), n, hipMemcpyHostToDevice));
+
+    std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
+    {
+        HostNode root;
+        root.id = 0;
+        root.lb.assign(pr->lb, pr->lb + n);
+        root.ub.assign(pr->ub, pr->ub + n);
+        if (s->root_initial_point && s->root_initial_point_size == n)
+            root.rx.assign(s->root_initial_point, s->root_initial_point + n);
+        root.bound = s->natural_lower_bound;
+        q.push(std::move(root));
+    }
+    double inc = kInf;
+    std::vector<double> inc_x;
+    int next_id = 1, done_nodes = 0;
+    long long fw_iters = 0;
+    double final_bound = -kInf, relax_s = 0.0;
+    bool tol_hit = false, time_hit = false, finished = false;
+    std::multiset<double> active;
+
+    auto live_bound = [&]() {
+        double b = inc;
+        if (!q.empty()) b = std::min(b, q.top().bound);
+        if (!active.empty()) b = std::min(b, *active.begin());
+        return b;
+    };
+    auto finish_check = [&](bool batch_remaining) {
+        if (q.empty() && !batch_remaining) { final_bound = inc; finished = true; return; }
+        const double b = live_bound();
+        if (gap_within_tolerance(b, inc, s->abs_gap_tol, s->rel_gap_tol)) {
+            final_bound = b;
+            tol_hit = true;
+            finished = true;
+        }
+    };
+    auto progress = [&](bool new_inc, int open_in_batch) {
+        if (!s->progress) return;
+        s->progress(s->progress_user, done_nodes, (int)q.size() + open_in_batch, live_bound(), inc,
+                    new_inc ? 1 : 0);
+    };
+
+    while (!finished) {
+        const double el = std::chrono::duration<double>(Clock::now() - t0).count();
+        if (el >= s->time_limit_seconds || (s->max_nodes > 0 && done_nodes >= s->max_nodes)) {
+            final_bound = live_bound();
+            time_hit = true;
+            break;
+        }
+        std::vector<HostNode> batch;
+        while (!q.empty() && (int)batch.size() < B) {
+            HostNode nd = q.top();
+            q.pop();
+            if (nd.bound >= inc) continue;
+            batch.push_back(std::move(nd));
+        }
+        if (batch.empty()) { final_bound = inc; finished = true; break; }
+        const int N = (int)batch.size();
+        for (int i = 0; i < N; ++i) {
+            const HostNode& nd = batch[i];
+            for (int j = 0; j < n; ++j) {
+                db.hx[(size_t)i * n + j] = nd.rx.empty() ? nd.lb[j] : nd.rx[j];
+                db.hlb[(size_t)i * n + j] = nd.lb[j];
+                db.hub[(size_t)i * n + j] = nd.ub[j];
+            }
+            db.heps[i] = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth);
+            db.hit[i] = nd.riters;
+            db.hst[i] = nd.rx.empty() ? 0 : GB_NODE_INITIAL_POINT_SUPPLIED;
+            active.insert(nd.bound);
+        }
+        const auto r0 = Clock::now();
+        const size_t xb = (size_t)N * n * 8;
+        GB_HIP(hipMemcpyAsync(db.x, db.hx, xb, hipMemcpyHostToDevice, db.stream));
+        GB_HIP(hipMemcpyAsync(db.lb, db.hlb, xb, hipMemcpyHostToDevice, db.stream));
+        GB_HIP(hipMemcpyAsync(db.ub, db.hub, xb, hipMemcpyHostToDevice, db.stream));
+        GB_HIP(hipMemcpyAsync(db.eps, db.heps, N * 8, hipMemcpyHostToDevice, db.stream));
+        GB_HIP(hipMemcpyAsync(db.it, db.hit, N * 4, hipMemcpyHostToDevice, db.stream));
+        GB_HIP(hipMemcpyAsync(db.st, db.hst, N, hipMemcpyHostToDevice, db.stream));
+        st = relax_batch_fp64(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub, db.eps,
+                              db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags, s->integer_tol,
+                              s->gpu_block_threads, db.stream);
+        if (st != GB_OK) return st;
+        GB_HIP(hipMemcpyAsync(db.hx, db.x, xb, hipMemcpyDeviceToHost, db.stream));
+        GB_HIP(hipMemcpyAsync(db.hbv, db.bv, xb, hipMemcpyDeviceToHost, db.stream));
+        GB_HIP(hipMemcpyAsync(db.hgap, db.gap, N * 8, hipMemcpyDeviceToHost, db.stream));
+        GB_HIP(hipMemcpyAsync(db.hval, db.val, N * 8, hipMemcpyDeviceToHost, db.stream));
+        GB_HIP(hipMemcpyAsync(db.hbvv, db.bvv, N * 8, hipMemcpyDeviceToHost, db.stream));
+        GB_HIP(hipMemcpyAsync(db.hit, db.it, N * 4, hipMemcpyDeviceToHost, db.stream));
+        GB_HIP(hipMemcpyAsync(db.hst, db.st, N, hipMemcpyDeviceToHost, db.stream));
+        GB_HIP(hipStreamSynchronize(db.stream));
+        relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
+
+        for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
+        for (int i = 0; i < N && !finished; ++i) {
+            HostNode nd = std::move(batch[i]);
+            active.erase(active.find(nd.bound));
+            const int remaining = N - i - 1;
+            if (db.hst[i] == GB_NODE_INFEASIBLE) {
+                ++done_nodes;
+                progress(false, remaining);
+                finish_check(remaining > 0);
+                continue;
+            }
+            const double node_eps = db.heps[i];
+            const double g = db.hgap[i];
+            nd.bound = certified_fw_lower_bound(nd.bound, db.hval[i], g);
+            bool new_inc = false;
+            const double bvv = db.hbvv[i];
+            if (bvv < 1e299 && std::isfinite(bvv) && bvv < inc) {
+                inc = bvv;
+                inc_x.assign(db.hbv + (size_t)i * n, db.hbv + (size_t)(i + 1) * n);
+                new_inc = true;
+            }
+            const bool cert = std::isfinite(g) && g >= 0.0 && g < node_eps;
+            if (nd.bound >= inc) {
+                ++done_nodes;
+                progress(new_inc, remaining);
+                finish_check(remaining > 0);
+                continue;
+            }
+            const double* xi = db.hx + (size_t)i * n;
+            if (db.hst[i] >= GB_NODE_INTEGRAL_UNCONVERGED && !cert) {
+                // device_resident_solver.cuh:1970-1980: keep the node live.
+                nd.rx.assign(xi, xi + n);
+                nd.riters = db.hit[i];
+                q.push(std::move(nd));
+                progress(new_inc, remaining);
+                continue;
+            }
+            if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
+                std::vector<int> vars;
+                if (s->branch_rule == GB_BRANCH_POW2) {
+                    vars = top_k_infeasible(xi, iv, s->branch_k, s->integer_tol);
+                } else if (s->branch_rule == GB_BRANCH_CALLBACK) {
+                    std::vector<int> cand;
+                    for (int v : iv) {
+                        const double frac = xi[v] - std::floor(xi[v]);
+                        if (std::min(frac, 1.0 - frac) > s->integer_tol) cand.push_back(v);
+                    }
+                    if (!cand.empty()) {
+                        std::vector<double> score(cand.size(), 0.0);
+                        s->branch_score(s->branch_score_user, xi, n, cand.data(), (int)cand.size(),
+                                        nd.depth, score.data());
+                        size_t bi = 0;
+                        for (size_t c = 1; c < cand.size(); ++c)
+                            if (score[c] > score[bi]) bi = c;
+                        vars.push_back(cand[bi]);
+                    }
+                } else {
+                    const int v = most_fractional(xi, iv, s->integer_tol);
+                    if (v >= 0) vars.push_back(v);
+                }
+                const int base = next_id;
+                next_id += 8;  // device_resident_solver.cuh:1989
+                int nid = base;
+                const int k = (int)vars.size();
+                for (int mask = 0; k > 0 && mask < (1 << k); ++mask) {
+                    HostNode c;
+                    c.id = nid++;
+                    c.parent = nd.id;
+                    c.depth = nd.depth + 1;
+                    c.lb = nd.lb;
+                    c.ub = nd.ub;
+                    c.bound = nd.bound;
+                    for (int b = 0; b < k; ++b) {
+                        const int vi = vars[b];
+                        if (mask & (1 << b)) c.lb[vi] = std::ceil(xi[vi]);
+                        else c.ub[vi] = std::floor(xi[vi]);
+                    }
+                    if (c.bound < inc) q.push(std::move(c));
+                }
+            }
+            ++done_nodes;
+            progress(new_inc, remaining);
+            finish_check(remaining > 0);
+        }
+        if (!finished && q.empty()) { final_bound = inc; finished = true; }
+    }
+    if (final_bound == -kInf) final_bound = q.empty() ? inc : std::min(inc, q.top().bound);
+    res->objective = inc;
+    res->dual_bound = final_bound;
+    res->nodes_processed = done_nodes;
+    res->elapsed_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+    res->reached_tolerance = tol_hit ? 1 : 0;
+    res->reached_time_limit = time_hit ? 1 : 0;
+    res->fw_iterations = fw_iters;
+    res->relax_seconds = relax_s;
+    if (res->x)
+        for (int j = 0; j < n; ++j) res->x[j] = inc_x.empty() ? std::nan("") : inc_x[j];
+    return GB_OK;
+}
+
+}  // namespace
+}  // namespace gb
+
+extern "C" void gb_default_settings(gb_settings* s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    s->gpu_batch_size = 1024;
+    s->num_streams = 8;
+    s->gpu_block_threads = 0;
+    s->gpu_max_iters = 20000;
+    s->gpu_eps = 1e-3;
+    s->gpu_eps_decay = 0.8;
+    s->gpu_eps_min = 1e-6;
+    s->integer_tol = 1e-6;
+    s->abs_gap_tol = 1e-6;
+    s->rel_gap_tol = 1e-2;
+    s->algorithm.algorithm = GB_ALG_DICG;
+    s->algorithm.line_search = GB_LS_SECANT;
+    s->algorithm.secant_iters = 4;
+    s->algorithm.secant_tol = 1e-10;
+    s->algorithm.secant_denom_eps = 1e-14;
+    s->algorithm.heuristic = GB_HEUR_ROUND;
+    s->algorithm.heuristic_offset = 0.0;
+    s->branch_rule = GB_BRANCH_POW2;
+    s->branch_k = 3;
+    s->time_limit_seconds = std::numeric_limits<double>::infinity();
+    s->natural_lower_bound = -std::numeric_limits<double>::infinity();
+    s->precision = GB_PREC_FP64;
+    s->max_nodes = 0;
+}
+
+extern "C" gb_status gb_solve(const gb_problem* problem, const gb_objective_desc* objective,
+                              const gb_lmo_desc* lmo, const gb_settings* settings,
+                              gb_result* result) {
+    if (!result) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null result");
+    try {
+        return gb::solve_impl(problem, objective, lmo, settings, result);
+    } catch (const std::bad_alloc&) {
+        return gb::fail(GB_ERR_RUNTIME, "host memory exhausted");
+    } catch (const std::exception& e) {
+        return gb::fail(GB_ERR_RUNTIME, e.what());
+    }
+}

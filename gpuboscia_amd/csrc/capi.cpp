@@ -1,0 +1,244 @@
+// C-ABI entry points: descriptor validation, workspace management and the
+// FP64 batch relaxation launch (gb_relax_batch).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gb_internal.h"
+#include "gb_host.h"
+
+namespace gb {
+
+thread_local std::string g_last_error;
+
+gb_status fail(gb_status code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+gb_status hip_fail(hipError_t e, const char* file, int line) {
+    return fail(GB_ERR_DEVICE, std::string("HIP ") + file + ":" + std::to_string(line) + " - "
+                                   + hipGetErrorString(e));
+}
+
+// Per-device grow-only workspace for the per-node work vectors.
+struct Workspace {
+    std::mutex mtx;
+    double* ptr = nullptr;
+    size_t bytes = 0;
+};
+static Workspace g_ws[64];
+
+gb_status workspace_get(size_t bytes, double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
+    if (dev < 0 || dev >= 64) return fail(GB_ERR_RUNTIME, "device index out of range");
+    Workspace& w = g_ws[dev];
+    std::lock_guard<std::mutex> lk(w.mtx);
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            e = hipDeviceSynchronize();
+            if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
+            e = hipFree(w.ptr);
+            if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        void* p = nullptr;
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
+        w.ptr = static_cast<double*>(p);
+        w.bytes = bytes;
+    }
+    *out = w.ptr;
+    return GB_OK;
+}
+
+gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                               const gb_lmo_desc* l) {
+    if (!o || !a || !l) return fail(GB_ERR_INVALID_ARGUMENT, "null descriptor");
+    if (o->n <= 0) return fail(GB_ERR_INVALID_ARGUMENT, "objective n must be positive");
+    if (o->kind == GB_OBJ_DENSE_QUADRATIC) {
+        if (!o->Qt) return fail(GB_ERR_INVALID_ARGUMENT, "dense objective requires Qt");
+        if (o->n > kMaxDenseN)
+            return fail(GB_ERR_INVALID_ARGUMENT, "dense objective n exceeds the LDS-staged limit");
+    } else if (o->kind == GB_OBJ_QAP) {
+        if (o->order <= 0 || o->n != o->order * o->order || !o->A || !o->B)
+            return fail(GB_ERR_INVALID_ARGUMENT, "QAP objective requires order, A, B and n == order^2");
+    } else {
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown objective kind");
+    }
+    if (l->kind == GB_LMO_SIMPLEX) {
+        if (!l->is_int) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO requires is_int");
+        if (o->n > kMaxSimplexN) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO n exceeds the LDS limit");
+    } else if (l->kind == GB_LMO_BIRKHOFF) {
+        if (l->order <= 0 || l->order > kMaxBirkhoffOrder)
+            return fail(GB_ERR_INVALID_ARGUMENT, "Birkhoff order must be in [1, 256]");
+    } else if (l->kind != GB_LMO_BOX) {
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown LMO kind");
+    }
+    if (a->algorithm != GB_ALG_FRANK_WOLFE && a->algorithm != GB_ALG_DICG)
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown FW algorithm");
+    if (a->line_search < GB_LS_OPEN_LOOP || a->line_search > GB_LS_QUADRATIC_EXACT)
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown line search");
+    // gpuboscia.cuh:146-149
+    if (a->algorithm == GB_ALG_FRANK_WOLFE
+            && (a->line_search == GB_LS_SCALED_OPEN_LOOP
+                || a->line_search == GB_LS_SAFEGUARDED_SCALED_OPEN_LOOP))
+        return fail(GB_ERR_INVALID_ARGUMENT, "scaled open-loop is available only for DICG");
+    if (a->line_search == GB_LS_SECANT && a->secant_iters < 0)
+        return fail(GB_ERR_INVALID_ARGUMENT, "secant iterations must be non-negative");
+    if (a->heuristic != GB_HEUR_ROUND && a->heuristic != GB_HEUR_FLOOR_OFFSET)
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown heuristic");
+    return GB_OK;
+}
+
+int auto_block_threads(int n) {
+    const int wanted = ((n + 63) / 64) * 64;
+    return wanted < 64 ? 64 : (wanted > 256 ? 256 : wanted);
+}
+
+gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                           const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
+                           const double* ub, const double* eps, int* iterations, uint8_t* status,
+                           double* gap, double* value, double* best_vertex, double* best_vertex_val,
+                           const uint8_t* flags, double ctol, int block_threads, void* stream) {
+    gb_status st = validate_descriptors(o, a, l);
+    if (st != GB_OK) return st;
+    if (N < 0 || max_iters < 0) return fail(GB_ERR_INVALID_ARGUMENT, "N and max_iters must be non-negative");
+    if (N == 0) return GB_OK;
+    if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
+            || !best_vertex_val)
+        return fail(GB_ERR_INVALID_ARGUMENT, "null node buffer");
+    if (block_threads == 0) block_threads = auto_block_threads(o->n);
+    if (block_threads < 64 || block_threads > 1024 || block_threads % 64 != 0)
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "gpu_block_threads must be zero or a positive multiple of 64 up to 1024");
+    const int n = o->n;
+    const LdsLayout L = lds_layout(o->kind, l->kind, n, l->order);
+    if (L.total > kMaxLdsBytes) return fail(GB_ERR_INVALID_ARGUMENT, "problem exceeds the LDS budget");
+    double* work = nullptr;
+    st = workspace_get((size_t)N * kWorkVectors * n * sizeof(double), &work);
+    if (st != GB_OK) return st;
+    RelaxParams P{};
+    P.obj_kind = o->kind;
+    P.n = n;
+    P.Qt = o->Qt;
+    P.lin = o->linear;
+    P.gs = o->grad_scale;
+    P.ls = o->linear_scale;
+    P.kappa = o->constant;
+    P.order = o->order;
+    P.A = o->A;
+    P.B = o->B;
+    P.lmo_kind = l->kind;
+    P.rhs = l->right_side;
+    P.equality = l->equality;
+    P.is_int = l->is_int;
+    P.lorder = l->order;
+    P.dicg = a->algorithm == GB_ALG_DICG;
+    P.ls_kind = a->line_search;
+    P.secant_iters = a->secant_iters;
+    P.secant_tol = a->secant_tol;
+    P.secant_denom = a->secant_denom_eps;
+    P.heuristic = a->heuristic;
+    P.heuristic_offset = a->heuristic_offset;
+    P.N = N;
+    P.max_iters = max_iters;
+    P.x = x;
+    P.lb = lb;
+    P.ub = ub;
+    P.eps = eps;
+    P.iters = iterations;
+    P.status = status;
+    P.gap = gap;
+    P.value = value;
+    P.best_v = best_vertex;
+    P.best_val = best_vertex_val;
+    P.flags = flags;
+    P.ctol = ctol;
+    P.work = work;
+    const int e = launch_relax_fp64(P, block_threads, L.total, L, stream);
+    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    return GB_OK;
+}
+
+}  // namespace gb
+
+using namespace gb;
+
+extern "C" gb_status gb_relax_batch(const gb_objective_desc* objective,
+                                    const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,
+                                    int N, int max_iters, double* x, const double* lb,
+                                    const double* ub, const double* eps, int* iterations,
+                                    uint8_t* status, double* gap, double* value,
+                                    double* best_vertex, double* best_vertex_val,
+                                    const uint8_t* integer_flags, double candidate_tol,
+                                    int block_threads, void* stream) {
+    try {
+        return relax_batch_fp64(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps,
+                                iterations, status, gap, value, best_vertex, best_vertex_val,
+                                integer_flags, candidate_tol, block_threads, stream);
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
+
+extern "C" const char* gb_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" const char* gb_version(void) { return "gpuboscia_amd 0.1.0 (gfx950)"; }
+
+extern "C" int gb_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+extern "C" int gb_lmo_relaxation_feasible_cpu(const gb_lmo_desc* lmo, const uint8_t* is_int_host,
+                                              const double* x, const double* lb, const double* ub,
+                                              int n, double tol) {
+    (void)is_int_host;
+    (void)lb;
+    (void)ub;
+    if (!lmo || !x || n <= 0) return 0;
+    switch (lmo->kind) {
+    case GB_LMO_BOX:  // lmo.cuh:214-217: bounds are validated separately
+        return 1;
+    case GB_LMO_SIMPLEX: {  // lmo.cuh:476-485, lmo_dicg.h:156-178
+        double mass = 0.0;
+        if (lmo->equality) {
+            for (int j = 0; j < n; ++j) {
+                if (x[j] < -tol) return 0;
+                mass += x[j];
+            }
+            return std::isfinite(mass) && std::fabs(mass - lmo->right_side) <= tol;
+        }
+        for (int j = 0; j < n; ++j) {
+            if (x[j] < 0.0) return 0;
+            mass += x[j];
+        }
+        return mass <= lmo->right_side + tol;
+    }
+    case GB_LMO_BIRKHOFF: {  // lmo.cuh:922-934
+        const int m = lmo->order;
+        if (n != m * m) return 0;
+        for (int j = 0; j < n; ++j)
+            if (x[j] < -tol || x[j] > 1 + tol) return 0;
+        for (int i = 0; i < m; ++i) {
+            double row = 0, col = 0;
+            for (int j = 0; j < m; ++j) {
+                row += x[i * m + j];
+                col += x[j * m + i];
+            }
+            if (std::fabs(row - 1) > tol || std::fabs(col - 1) > tol) return 0;
+        }
+        return 1;
+    }
+    }
+    return 0;
+}

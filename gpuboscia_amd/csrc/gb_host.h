@@ -1,0 +1,37 @@
+// Host-side internal helpers of the library (error state, limits, shared
+// launch helpers).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "gb_internal.h"
+
+namespace gb {
+
+constexpr int kMaxDenseN = 8192;         // vector staged in LDS (64 KiB FP64)
+constexpr int kMaxSimplexN = 2048;       // sort/scan arrays in LDS
+constexpr int kMaxBirkhoffOrder = 256;   // Hungarian state in LDS
+constexpr size_t kMaxLdsBytes = 160 * 1024;
+
+extern thread_local std::string g_last_error;
+
+gb_status fail(gb_status code, const std::string& msg);
+gb_status hip_fail(hipError_t e, const char* file, int line);
+gb_status workspace_get(size_t bytes, double** out);
+gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                               const gb_lmo_desc* l);
+int auto_block_threads(int n);
+gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                           const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
+                           const double* ub, const double* eps, int* iterations, uint8_t* status,
+                           double* gap, double* value, double* best_vertex, double* best_vertex_val,
+                           const uint8_t* flags, double ctol, int block_threads, void* stream);
+
+#define GB_HIP(call)                                                   \
+    do {                                                               \
+        hipError_t _e = (call);                                        \
+        if (_e != hipSuccess) return ::gb::hip_fail(_e, __FILE__, __LINE__); \
+    } while (0)
+
+}  // namespace gb

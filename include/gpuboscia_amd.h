@@ -1,0 +1,264 @@
+/*
+ * gpuboscia_amd — C-ABI of the MI355X (gfx950) Frank-Wolfe / DICG
+ * node-relaxation library.
+ *
+ * The reference (LJS42/GPUBoscia) is a header-only C++/CUDA template
+ * library; its plugin contract is compile-time (objective, algorithm and LMO
+ * functors copied into kernels).  This C-ABI replaces that contract with
+ * plain descriptors: every entry point takes plain pointers and sizes.  The
+ * C++ drop-in header <gpuboscia/gpuboscia.hpp> maps the reference's types
+ * (Settings, Result, BoxLMO, UnitSimplexLMO<N>, BirkhoffLMO<N>,
+ * GlobalCachedDenseQuadraticObjective<N,T>, ...) onto these descriptors.
+ *
+ * Errors: every function returns a gb_status; on failure gb_last_error()
+ * returns a thread-local message.  Device errors read "HIP <file>:<line> -
+ * <message>" (the reference's DRS_CUDA_CHECK reads "CUDA ...").
+ *
+ * Device pointers are caller-owned, as in the reference (lmo.cuh:6-9,
+ * heuristics.h:22).  Calls on one stream are ordered; calls that share a
+ * library-owned workspace serialise on an internal mutex.
+ */
+#ifndef GPUBOSCIA_AMD_H
+#define GPUBOSCIA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GB_OK = 0,
+    GB_ERR_INVALID_ARGUMENT = 1, /* reference: std::invalid_argument   */
+    GB_ERR_RUNTIME = 2,          /* reference: std::runtime_error      */
+    GB_ERR_DEVICE = 3            /* reference: DRS_CUDA_CHECK failure  */
+} gb_status;
+
+/* FWAlgorithmKind (line_search_options.h:13-16). */
+typedef enum { GB_ALG_FRANK_WOLFE = 0, GB_ALG_DICG = 1 } gb_algorithm_kind;
+
+/* LineSearchKind (line_search_options.h:5-11). */
+typedef enum {
+    GB_LS_OPEN_LOOP = 0,
+    GB_LS_SCALED_OPEN_LOOP = 1,
+    GB_LS_SAFEGUARDED_SCALED_OPEN_LOOP = 2,
+    GB_LS_SECANT = 3,
+    GB_LS_QUADRATIC_EXACT = 4
+} gb_line_search_kind;
+
+/* Objective plugins. */
+typedef enum {
+    /* GlobalCachedDenseQuadraticObjective (dense_quadratic_objectives.cuh:18-112):
+       f = 1/2 s_g x'Qx + s_g s_l c'x + kappa, Qt[k*n+j] == Q[j,k]. */
+    GB_OBJ_DENSE_QUADRATIC = 0,
+    /* Quadratic assignment over a Birkhoff polytope of order m (n = m*m):
+       f(P) = ||A P - P B||_F^2, P row-major. No reference counterpart
+       (analogue: examples/main.cu:131-181). */
+    GB_OBJ_QAP = 1
+} gb_objective_kind;
+
+/* LMO plugins. */
+typedef enum {
+    GB_LMO_BOX = 0,       /* BoxLMO (lmo.cuh:212-321) */
+    GB_LMO_SIMPLEX = 1,   /* UnitSimplexLMO / EqualitySimplexLMO (lmo.cuh:335-912) */
+    GB_LMO_BIRKHOFF = 2   /* BirkhoffLMO (lmo.cuh:918-1419) */
+} gb_lmo_kind;
+
+/* Candidate heuristics (heuristics.h:23-35). */
+typedef enum {
+    GB_HEUR_ROUND = 0,        /* RoundingHeuristic: round integer coordinates */
+    GB_HEUR_FLOOR_OFFSET = 1  /* floor(x) + offset on integer coordinates    */
+} gb_heuristic_kind;
+
+/* Arithmetic of the relaxation. */
+typedef enum {
+    GB_PREC_FP64 = 0,      /* plain form: everything FP64 */
+    GB_PREC_FP32 = 1,      /* iterates FP32, certificate FP64 */
+    GB_PREC_BF16 = 2       /* BF16 MFMA gradient (FP32 accumulate), FP64 certificate */
+} gb_precision;
+
+typedef struct {
+    int kind;                 /* gb_objective_kind */
+    int n;                    /* number of variables */
+    const double* Qt;         /* dense: device, n*n, column-major Q */
+    const double* linear;     /* dense: device, n, may be NULL */
+    double grad_scale;        /* s_g */
+    double linear_scale;      /* s_l */
+    double constant;          /* kappa */
+    int order;                /* qap: m */
+    const double* A;          /* qap: device, m*m row-major */
+    const double* B;          /* qap: device, m*m row-major */
+} gb_objective_desc;
+
+typedef struct {
+    int kind;                 /* gb_lmo_kind */
+    double right_side;        /* simplex: rhs (default 1) */
+    int equality;             /* simplex: 1 = EqualitySimplexLMO */
+    const uint8_t* is_int;    /* simplex: device, n bytes (UnitSimplexLMO::is_int) */
+    int order;                /* birkhoff: Order (n = Order*Order) */
+} gb_lmo_desc;
+
+typedef struct {
+    int algorithm;            /* gb_algorithm_kind */
+    int line_search;          /* gb_line_search_kind */
+    int secant_iters;         /* LineSearchOptions (line_search_options.h:18-23) */
+    double secant_tol;
+    double secant_denom_eps;
+    int heuristic;            /* gb_heuristic_kind */
+    double heuristic_offset;  /* GB_HEUR_FLOOR_OFFSET */
+} gb_algorithm_desc;
+
+/* Per-node status (device_resident_solver.cuh:192-202). */
+enum {
+    GB_NODE_FRACTIONAL_UNCONVERGED = 0,
+    GB_NODE_FRACTIONAL_CONVERGED = 1,
+    GB_NODE_INFEASIBLE = 2,
+    GB_NODE_INTEGRAL_UNCONVERGED = 3,
+    GB_NODE_INTEGRAL_CONVERGED = 4,
+    GB_NODE_INITIAL_POINT_SUPPLIED = 0x80  /* input-only flag */
+};
+
+/*
+ * Relax a batch of N nodes on one GPU, one workgroup per node, FP64.
+ * Replaces drs_fw_relax_batch_kernel<Obj,Alg,LMO,Heur> and its launch
+ * (device_resident_solver.cuh:631-660, :1834-1851); argument meaning as at
+ * :634-654.  All arrays are device pointers, node-major [N][n]:
+ *   x             in: warm start / ignored for fresh nodes; out: final iterate
+ *   lb, ub        node bounds
+ *   eps           [N] per-node FW gap tolerance
+ *   iterations    [N] in: cumulative count (continuation); out: new count
+ *   status        [N] in: 0 or GB_NODE_INITIAL_POINT_SUPPLIED; out: 0..4
+ *   gap, value    [N] final FW gap and objective at the returned x
+ *   best_vertex   [N][n], best_vertex_val [N]: best integer candidate (1e300 = none)
+ *   integer_flags [n] u8, shared by all nodes (may be NULL)
+ * block_threads: 0 = automatic (min(256, round_up(n, 64))), else a multiple
+ * of 64 up to 1024.  stream: a hipStream_t (NULL = default stream).  The call
+ * is asynchronous with respect to the host.
+ */
+gb_status gb_relax_batch(const gb_objective_desc* objective,
+                         const gb_algorithm_desc* algorithm,
+                         const gb_lmo_desc* lmo,
+                         int N, int max_iters,
+                         double* x, const double* lb, const double* ub,
+                         const double* eps, int* iterations, uint8_t* status,
+                         double* gap, double* value,
+                         double* best_vertex, double* best_vertex_val,
+                         const uint8_t* integer_flags, double candidate_tol,
+                         int block_threads, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Branch-and-bound (gpuboscia::solve, gpuboscia.cuh:56-170)               */
+/* ---------------------------------------------------------------------- */
+
+/* Problem (problem.h:4-10); host arrays. */
+typedef struct {
+    int n_vars;
+    const double* lb;
+    const double* ub;
+    const int* int_vars;
+    int n_int_vars;
+} gb_problem;
+
+/* progress(iter, open_nodes, bound, incumbent, new_incumbent)
+   (gpuboscia.cuh:26, device_resident_solver.cuh:2078-2085). */
+typedef void (*gb_progress_fn)(void* user, int iter, int open_nodes,
+                               double bound, double incumbent,
+                               int new_incumbent);
+/* RootInitialPointValidator (tree.h:102-104): return nonzero if feasible. */
+typedef int (*gb_root_validator_fn)(void* user, const double* x,
+                                    const double* lb, const double* ub, int n,
+                                    double tolerance);
+/* Branching variable scorer: fills score[i] for each integer variable in
+   candidates[0..count); larger is branched first.  Used by
+   GB_BRANCH_CALLBACK. */
+typedef void (*gb_branch_score_fn)(void* user, const double* x, int n,
+                                   const int* candidates, int count,
+                                   int depth, double* score);
+
+typedef enum {
+    GB_BRANCH_POW2 = 0,            /* branch_pow2(k) on top_k_infeasible (default k=3) */
+    GB_BRANCH_MOST_FRACTIONAL = 1, /* single variable, most_infeasible */
+    GB_BRANCH_LEARNED = 2,         /* on-GPU MLP scorer, falls back to most-fractional */
+    GB_BRANCH_CALLBACK = 3         /* host scorer callback, 2-way split */
+} gb_branch_rule;
+
+/* Settings + DeviceResidentConfig (gpuboscia.cuh:21-32,
+   device_resident_solver.cuh:1004-1025). */
+typedef struct {
+    /* DeviceResidentConfig */
+    int gpu_batch_size;       /* 1024 */
+    int num_streams;          /* 8 (accepted; one batch in flight per stream) */
+    int gpu_block_threads;    /* 0 = automatic; else a multiple of 64 */
+    int gpu_max_iters;        /* 20000 */
+    double gpu_eps;           /* 1e-3 */
+    double gpu_eps_decay;     /* 0.8 */
+    double gpu_eps_min;       /* 1e-6 */
+    double integer_tol;       /* 1e-6 */
+    double abs_gap_tol;       /* 1e-6 */
+    double rel_gap_tol;       /* 1e-2 */
+    /* Settings */
+    gb_algorithm_desc algorithm;   /* DICG + Secant(4, 1e-10, 1e-14) */
+    int branch_rule;          /* gb_branch_rule */
+    int branch_k;             /* 3 */
+    double time_limit_seconds;     /* +inf */
+    double natural_lower_bound;    /* -inf */
+    const double* root_initial_point;  /* host, n_vars, or NULL */
+    int root_initial_point_size;
+    gb_root_validator_fn root_validator;
+    void* root_validator_user;
+    gb_progress_fn progress;
+    void* progress_user;
+    gb_branch_score_fn branch_score;
+    void* branch_score_user;
+    int precision;            /* gb_precision (GB_PREC_FP64 = plain form) */
+    int learned_model;        /* handle from gb_brancher_create, or 0 */
+    long long max_nodes;      /* safety cap; <= 0 = none (no reference counterpart) */
+} gb_settings;
+
+/* Result (gpuboscia.cuh:34-42). x is caller-provided, n_vars doubles. */
+typedef struct {
+    double objective;
+    double* x;
+    double dual_bound;
+    int nodes_processed;
+    double elapsed_seconds;
+    int reached_tolerance;
+    int reached_time_limit;
+    long long fw_iterations;  /* total FW iterations (no reference counterpart) */
+    double relax_seconds;     /* time inside node relaxations */
+} gb_result;
+
+/* Fill *s with the reference defaults (create_default_settings,
+   gpuboscia.cuh:44-47). */
+void gb_default_settings(gb_settings* s);
+
+/* Run branch-and-bound to the proven optimum (or a limit).
+   Validation and errors follow gpuboscia.cuh:56-170 and tree.h:115-147. */
+gb_status gb_solve(const gb_problem* problem,
+                   const gb_objective_desc* objective,
+                   const gb_lmo_desc* lmo,
+                   const gb_settings* settings,
+                   gb_result* result);
+
+/* Host-side continuous feasibility check of the built-in LMOs
+   (LMO::relaxation_feasible_cpu, lmo.cuh:30-35). is_int is a host array for
+   the simplex kind (may be NULL). Returns 1 feasible, 0 not. */
+int gb_lmo_relaxation_feasible_cpu(const gb_lmo_desc* lmo, const uint8_t* is_int_host,
+                                   const double* x, const double* lb,
+                                   const double* ub, int n, double tolerance);
+
+/* Thread-local message of the last failure. */
+const char* gb_last_error(void);
+
+/* Library version string. */
+const char* gb_version(void);
+
+/* Number of visible HIP devices (0 when none); never throws. */
+int gb_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUBOSCIA_AMD_H */

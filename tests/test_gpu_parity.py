@@ -1,0 +1,251 @@
+"""Parity of the HIP FP64 node-relaxation kernel (through the C-ABI) with the
+CPU oracle: every output is compared bit for bit on the same seeded inputs.
+Also the reference's GPU known-answer tests and end-to-end B&B optima."""
+import sys
+
+import numpy as np
+import pytest
+
+import gpuboscia_amd as gb
+from gpuboscia_amd import instances
+from tests.gpu_util import Problem, assert_bit_equal, auto_T
+
+pytestmark = pytest.mark.gpu
+
+LS = gb
+FW_KINDS = [gb.LS_OPEN_LOOP, gb.LS_SECANT, gb.LS_QUADRATIC_EXACT]
+DICG_KINDS = [gb.LS_OPEN_LOOP, gb.LS_SCALED, gb.LS_SAFEGUARDED, gb.LS_SECANT, gb.LS_QUADRATIC_EXACT]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def branched_nodes(n, N, seed, lo=0.0, hi=1.0, fix_frac=0.3):
+    """N nodes of a binary problem with random variable fixings (as B&B makes)."""
+    rng = np.random.default_rng(seed)
+    LB = np.full((N, n), lo)
+    UB = np.full((N, n), hi)
+    for i in range(1, N):
+        fix = rng.random(n) < fix_frac * rng.random()
+        val = rng.random(n) < 0.5
+        LB[i, fix & val] = hi
+        UB[i, fix & ~val] = lo
+    return LB, UB
+
+
+def compare(prob, alg, LB, UB, EPS, flags, max_iters, T=None, X0=None, ITERS=None, STATUS=None, label=""):
+    N, n = LB.shape
+    T = T or auto_T(n)
+    X0 = LB.copy() if X0 is None else X0
+    ITERS = np.zeros(N, dtype=np.int32) if ITERS is None else ITERS
+    STATUS = np.zeros(N, dtype=np.uint8) if STATUS is None else STATUS
+    say(f"[parity] {label} n={n} N={N} T={T} alg={alg} iters={max_iters}")
+    g = prob.run_gpu(alg, X0, LB, UB, EPS, ITERS, STATUS, flags, max_iters, T)
+    o = prob.run_orc(alg, X0, LB, UB, EPS, ITERS, STATUS, flags, max_iters, T)
+    assert_bit_equal(g, o, label)
+    return g
+
+
+@pytest.mark.parametrize("dicg,ls", [(False, k) for k in FW_KINDS] + [(True, k) for k in DICG_KINDS])
+def test_dense_box_parity_n20(dicg, ls):
+    Q, c, _ = instances.binary_qp(20, seed=1, rows=30)
+    prob = Problem("dense", Q, c)
+    LB, UB = branched_nodes(20, 24, seed=5)
+    compare(prob, {"dicg": dicg, "ls": ls}, LB, UB, np.full(24, 1e-6), np.ones(20, np.uint8), 400,
+            label="C0")
+
+
+@pytest.mark.parametrize("n,T", [(100, 0), (200, 0), (200, 64), (130, 192)])
+def test_dense_box_parity_wide(n, T):
+    Q, c, _ = instances.binary_qp(n, seed=7)
+    prob = Problem("dense", Q, c)
+    LB, UB = branched_nodes(n, 8, seed=n)
+    for alg in ({"dicg": True, "ls": gb.LS_SECANT}, {"dicg": True, "ls": gb.LS_QUADRATIC_EXACT},
+                {"dicg": False, "ls": gb.LS_OPEN_LOOP}):
+        compare(prob, alg, LB, UB, np.full(8, 1e-4), np.ones(n, np.uint8), 60, T=T or None,
+                label=f"wide{n}")
+
+
+@pytest.mark.parametrize("equality", [False, True])
+def test_simplex_parity(equality):
+    rng = np.random.default_rng(3 + equality)
+    n = 40
+    M = rng.standard_normal((n, n))
+    Q = M.T @ M / n + 0.05 * np.eye(n)
+    c = rng.uniform(-1, 1, n)
+    is_int = (rng.random(n) < 0.5).astype(np.uint8)
+    N = 12
+    LB = np.zeros((N, n))
+    UB = np.where(is_int, 3.0, 5.0) * np.ones((N, n))
+    for i in range(1, N):
+        j = rng.integers(0, n, 3)
+        UB[i, j] = np.where(is_int[j], rng.integers(0, 2, 3), UB[i, j])
+    prob = Problem("dense", Q, c, lmo="simplex", rhs=4.0, equality=equality, is_int=is_int)
+    for alg in ({"dicg": True, "ls": gb.LS_SECANT}, {"dicg": True, "ls": gb.LS_OPEN_LOOP},
+                {"dicg": False, "ls": gb.LS_QUADRATIC_EXACT}):
+        compare(prob, alg, LB, UB, np.full(N, 1e-6), is_int, 200, label="simplex")
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_birkhoff_dense_parity(order):
+    rng = np.random.default_rng(order)
+    n = order * order
+    target = np.zeros(n)
+    perm = rng.permutation(order)
+    target[np.arange(order) * order + perm] = 1.0
+    prob = Problem("dense", np.eye(n), target, 2.0, -1.0, float(target @ target), lmo="birkhoff",
+                   order=order)
+    N = 6
+    LB, UB = np.zeros((N, n)), np.ones((N, n))
+    LB[1, 0] = 1.0
+    UB[2, 1] = 0.0
+    UB[3, :order] = 0.0          # row 0 fully forbidden: infeasible
+    for alg in ({"dicg": True, "ls": gb.LS_OPEN_LOOP}, {"dicg": True, "ls": gb.LS_SECANT},
+                {"dicg": False, "ls": gb.LS_OPEN_LOOP}):
+        g = compare(prob, alg, LB, UB, np.full(N, 1e-8), np.ones(n, np.uint8), 100, label="birkhoff")
+        assert g["status"][3] == gb.NODE_INFEASIBLE
+
+
+def test_qap_birkhoff_parity():
+    A, B, perm, _ = instances.qap_instance(8, seed=4)
+    prob = Problem("qap", A=A, B=B, lmo="birkhoff", order=8)
+    N, n = 4, 64
+    LB, UB = np.zeros((N, n)), np.ones((N, n))
+    LB[1, 3] = 1.0
+    for alg in ({"dicg": True, "ls": gb.LS_QUADRATIC_EXACT}, {"dicg": False, "ls": gb.LS_OPEN_LOOP}):
+        compare(prob, alg, LB, UB, np.full(N, 1e-6), np.ones(n, np.uint8), 40, label="qap")
+
+
+def test_infeasible_and_mixed_batch():
+    Q, c, _ = instances.binary_qp(20, seed=2, rows=30)
+    prob = Problem("dense", Q, c)
+    LB, UB = branched_nodes(20, 10, seed=1)
+    LB[2, 4], UB[2, 4] = 0.2, 0.8       # integer interval without an integer
+    LB[5, 0], UB[5, 0] = 1.0, 0.0       # lb > ub
+    LB[7, 1] = np.nan
+    g = compare(prob, {"dicg": True, "ls": gb.LS_SECANT}, LB, UB, np.full(10, 1e-6),
+                np.ones(20, np.uint8), 100, label="mixed")
+    assert list(np.where(g["status"] == gb.NODE_INFEASIBLE)[0]) == [2, 5, 7]
+
+
+# --- reference GPU KATs (tests/test_pipeline.cu) ------------------------------
+
+def ils(target, n=2):
+    t = np.asarray(target, dtype=np.float64)
+    return Problem("dense", np.eye(n), t, 2.0, -1.0, float(t @ t))
+
+
+@pytest.mark.parametrize("dicg", [False, True])
+def test_gpu_continuation_two_thirds(dicg):
+    # tests/test_pipeline.cu:1398-1426
+    prob = ils([0.3, 0.4])
+    alg = {"dicg": dicg, "ls": gb.LS_OPEN_LOOP}
+    fl = np.ones(2, np.uint8)
+    z = np.zeros((1, 2))
+    o = np.ones((1, 2))
+    s1 = prob.run_gpu(alg, z, z, o, np.array([1e-12]), np.zeros(1), np.zeros(1), fl, 1, 64)
+    assert s1["status"][0] == 3
+    s2 = prob.run_gpu(alg, s1["x"], z, o, np.array([1e-12]), s1["iterations"], np.zeros(1), fl, 1, 64)
+    w = prob.run_gpu(alg, z, z, o, np.array([1e-12]), np.zeros(1), np.zeros(1), fl, 2, 64)
+    assert s2["iterations"][0] == 2
+    assert abs(s2["x"][0, 0] - 2 / 3) < 1e-12
+    assert np.allclose(s2["x"], w["x"], atol=1e-12) and abs(s2["gap"][0] - w["gap"][0]) < 1e-12
+
+
+def test_gpu_candidate_scoring():
+    # tests/test_pipeline.cu:1358-1396, :1428-1441
+    prob = ils([0.3, 0.4])
+    fl = np.array([1, 0], np.uint8)
+    z, o = np.zeros((1, 2)), np.ones((1, 2))
+    x0 = np.array([[0.6, 0.4]])
+    for dicg in (False, True):
+        base = {"dicg": dicg, "ls": gb.LS_OPEN_LOOP}
+        r = prob.run_gpu(base, x0, z, o, np.array([1e-12]), np.ones(1), np.zeros(1), fl, 0, 64)
+        assert abs(r["best_vertex_val"][0] - 0.45) < 1e-12
+        r = prob.run_gpu(dict(base, heur=gb.HEUR_FLOOR_OFFSET), x0, z, o, np.array([1e-12]), np.ones(1),
+                         np.zeros(1), fl, 0, 64)
+        assert abs(r["best_vertex_val"][0] - 0.09) < 1e-12
+        r = prob.run_gpu(dict(base, heur=gb.HEUR_FLOOR_OFFSET, off=2.0), x0, z, o, np.array([1e-12]),
+                         np.ones(1), np.zeros(1), fl, 0, 64)
+        assert abs(r["best_vertex_val"][0] - 0.45) < 1e-12
+    r = prob.run_gpu({"dicg": False, "ls": gb.LS_OPEN_LOOP}, np.array([[0.3, 0.4]]), z, o,
+                     np.array([1e-12]), np.full(1, 5), np.zeros(1), fl, 0, 64)
+    assert list(r["best_vertex"][0]) == [0.0, 0.4] and abs(r["best_vertex_val"][0] - 0.09) < 1e-12
+
+
+def test_gpu_root_point_observed():
+    # tests/test_root_initial_point.cu:29-71: with no budget the returned x is
+    # the first point the gradient would see: LMO start (1,1) or (.25,.75).
+    prob = ils([0.25, 0.75])
+    alg = {"dicg": True, "ls": gb.LS_OPEN_LOOP}
+    z, o = np.zeros((1, 2)), np.ones((1, 2))
+    r = prob.run_gpu(alg, z, z, o, np.array([1e-3]), np.zeros(1), np.zeros(1), np.ones(2, np.uint8), 0, 128)
+    assert list(r["x"][0]) == [1.0, 1.0]
+    r = prob.run_gpu(alg, np.array([[0.25, 0.75]]), z, o, np.array([1e-3]), np.zeros(1),
+                     np.full(1, gb.NODE_INITIAL_POINT_SUPPLIED), np.ones(2, np.uint8), 0, 128)
+    assert list(r["x"][0]) == [0.25, 0.75]
+
+
+def test_gpu_exact_interior_step():
+    # tests/test_pipeline.cu:1240-1270
+    prob = Problem("dense", np.eye(2), [0.25, 0.75], 2.0, -1.0, 0.625, lmo="simplex", rhs=1.0,
+                   equality=True, is_int=np.zeros(2))
+    for dicg in (False, True):
+        r = prob.run_gpu({"dicg": dicg, "ls": gb.LS_QUADRATIC_EXACT}, np.zeros((1, 2)),
+                         np.zeros((1, 2)), np.ones((1, 2)), np.array([1e-10]), np.zeros(1),
+                         np.zeros(1), None, 1, 64)
+        assert r["iterations"][0] == 1 and abs(r["gap"][0]) < 1e-10
+        assert np.allclose(r["x"][0], [0.25, 0.75], atol=1e-15)
+
+
+# --- end-to-end branch-and-bound ----------------------------------------------
+
+def dense_desc(Q, c, gs=2.0, ls=0.5, kappa=0.0):
+    p = Problem("dense", Q, c, gs, ls, kappa)
+    return p
+
+
+def test_minimal_example_gpu():
+    # examples/minimal.cu:49-50
+    p = ils([0.3, 0.7])
+    s = gb.default_settings()
+    s.algorithm.line_search = gb.LS_QUADRATIC_EXACT
+    s.natural_lower_bound = 0.0
+    s.time_limit_seconds = 10.0
+    r = gb.solve(2, [0, 0], [1, 1], [0, 1], p.gobj, p.glmo, s)
+    assert r["x"] == [0.0, 1.0] and abs(r["objective"] - 0.18) < 1e-8
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_c0_bnb_matches_oracle_and_enumeration(seed):
+    from oracle import pyoracle as orc
+    Q, c, sha = instances.binary_qp(20, seed=seed, rows=30)
+    say(f"[bnb] C0 seed={seed} sha={sha[:12]}")
+    p = dense_desc(Q, c)
+    s = gb.default_settings()
+    s.abs_gap_tol = 1e-6
+    s.rel_gap_tol = 1e-6
+    r = gb.solve(20, [0] * 20, [1] * 20, range(20), p.gobj, p.glmo, s)
+    o = orc.bnb_solve(p.oobj, orc.alg(), p.olmo, [0] * 20, [1] * 20, list(range(20)),
+                      abs_gap_tol=1e-6, rel_gap_tol=1e-6, threads=8)
+    f_enum, x_enum = orc.enumerate_binary_qp(Q, c, threads=8)
+    assert r["x"] == list(x_enum), "integer optimum differs from exhaustive enumeration"
+    assert abs(r["objective"] - f_enum) <= 1e-9 * max(1.0, abs(f_enum))
+    # same deterministic search as the oracle: identical tree and bounds
+    assert r["nodes_processed"] == o["nodes_processed"]
+    assert r["objective"] == o["objective"] and r["dual_bound"] == o["dual_bound"]
+    assert r["fw_iterations"] == o["fw_iterations"]
+
+
+def test_separable_bnb_gpu():
+    t = np.array([0.7, 1.6, -0.3, 2.8, 0.1, -1.4, 3.6, 0.9])
+    p = ils(t, 8)
+    s = gb.default_settings()
+    s.gpu_batch_size = 32
+    s.gpu_max_iters = 50000
+    s.gpu_eps = 1e-6
+    r = gb.solve(8, [-2.0] * 8, [4.0] * 8, range(8), p.gobj, p.glmo, s)
+    xs = np.clip(np.round(t), -2, 4)
+    assert r["x"] == list(xs) and abs(r["objective"] - float(((xs - t) ** 2).sum())) < 1e-9
