@@ -111,6 +111,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    # torch bundles its own libamdhip64.so.7 (same soname as /opt/rocm's).
+    # Whichever is loaded first serves the whole process; load torch's first
+    # so that torch tensors and this library share one HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     lib.gb_relax_batch.restype = ctypes.c_int
     lib.gb_relax_batch.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
