@@ -50,6 +50,11 @@ class AlgorithmDesc(ctypes.Structure):
                 ("heuristic_offset", ctypes.c_double)]
 
 
+class FastOptions(ctypes.Structure):
+    _fields_ = [("refresh_interval", ctypes.c_int), ("waves_per_block", ctypes.c_int),
+                ("blocks", ctypes.c_int)]
+
+
 class Problem(ctypes.Structure):
     _fields_ = [("n_vars", ctypes.c_int), ("lb", _dp), ("ub", _dp), ("int_vars", _ip),
                 ("n_int_vars", ctypes.c_int)]
@@ -76,7 +81,7 @@ class Settings(ctypes.Structure):
                 ("root_validator", VALIDATOR_FN), ("root_validator_user", ctypes.c_void_p),
                 ("progress", PROGRESS_FN), ("progress_user", ctypes.c_void_p),
                 ("branch_score", BRANCH_SCORE_FN), ("branch_score_user", ctypes.c_void_p),
-                ("precision", ctypes.c_int), ("learned_model", ctypes.c_int),
+                ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_int),
                 ("max_nodes", ctypes.c_longlong)]
 
 
@@ -90,7 +95,7 @@ class Result(ctypes.Structure):
 _lib = None
 
 #: Every symbol that include/gpuboscia_amd.h declares.
-EXPORTED = ["gb_relax_batch", "gb_default_settings", "gb_solve", "gb_lmo_relaxation_feasible_cpu",
+EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_solve", "gb_lmo_relaxation_feasible_cpu",
             "gb_last_error", "gb_version", "gb_device_count"]
 
 
@@ -123,6 +128,11 @@ def load():
     lib.gb_relax_batch.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
                                    ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
         + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    lib.gb_relax_batch_fast.restype = ctypes.c_int
+    lib.gb_relax_batch_fast.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
+                                        ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
+        + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_void_p, ctypes.POINTER(FastOptions),
+                                     ctypes.c_void_p]
     lib.gb_default_settings.restype = None
     lib.gb_default_settings.argtypes = [ctypes.POINTER(Settings)]
     lib.gb_solve.restype = ctypes.c_int
@@ -175,6 +185,22 @@ def relax_batch(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_ite
                               _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
                               float(candidate_tol), int(block_threads),
                               ctypes.c_void_p(stream or 0))
+    check(code)
+
+
+def relax_batch_fast(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb,
+                     ub, eps, iterations, status, gap, value, best_vertex, best_vertex_val,
+                     flags=None, candidate_tol=1e-6, cutoff=None, refresh=64, waves_per_block=4,
+                     blocks=0, stream=None):
+    """gb_relax_batch_fast on torch device tensors."""
+    lib = load()
+    opt = FastOptions(refresh, waves_per_block, blocks)
+    code = lib.gb_relax_batch_fast(ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
+                                   int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
+                                   _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),
+                                   _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
+                                   float(candidate_tol), _ptr(cutoff), ctypes.byref(opt),
+                                   ctypes.c_void_p(stream or 0))
     check(code)
 
 

@@ -100,11 +100,11 @@ int most_fractional(const double* x, const std::vector<int>& iv, double tol) {
 struct DeviceBatch {
     int cap = 0, n = 0;
     double *x = nullptr, *lb = nullptr, *ub = nullptr, *bv = nullptr;
-    double *eps = nullptr, *gap = nullptr, *val = nullptr, *bvv = nullptr;
+    double *eps = nullptr, *gap = nullptr, *val = nullptr, *bvv = nullptr, *cut = nullptr;
     int* it = nullptr;
     uint8_t *st = nullptr, *flags = nullptr;
     double *hx = nullptr, *hlb = nullptr, *hub = nullptr, *hbv = nullptr;
-    double *heps = nullptr, *hgap = nullptr, *hval = nullptr, *hbvv = nullptr;
+    double *heps = nullptr, *hgap = nullptr, *hval = nullptr, *hbvv = nullptr, *hcut = nullptr;
     int* hit = nullptr;
     uint8_t* hst = nullptr;
     hipStream_t stream = nullptr;
@@ -121,6 +121,7 @@ struct DeviceBatch {
         GB_HIP(hipMalloc(&gap, cap * 8));
         GB_HIP(hipMalloc(&val, cap * 8));
         GB_HIP(hipMalloc(&bvv, cap * 8));
+        GB_HIP(hipMalloc(&cut, cap * 8));
         GB_HIP(hipMalloc(&it, cap * 4));
         GB_HIP(hipMalloc(&st, cap));
         GB_HIP(hipMalloc(&flags, n));
@@ -132,6 +133,7 @@ struct DeviceBatch {
         GB_HIP(hipHostMalloc(&hgap, cap * 8, hipHostMallocDefault));
         GB_HIP(hipHostMalloc(&hval, cap * 8, hipHostMallocDefault));
         GB_HIP(hipHostMalloc(&hbvv, cap * 8, hipHostMallocDefault));
+        GB_HIP(hipHostMalloc(&hcut, cap * 8, hipHostMallocDefault));
         GB_HIP(hipHostMalloc(&hit, cap * 4, hipHostMallocDefault));
         GB_HIP(hipHostMalloc(&hst, cap, hipHostMallocDefault));
         GB_HIP(hipStreamCreate(&stream));
@@ -140,10 +142,10 @@ struct DeviceBatch {
     ~DeviceBatch() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : {(void*)x, (void*)lb, (void*)ub, (void*)bv, (void*)eps, (void*)gap,
-                        (void*)val, (void*)bvv, (void*)it, (void*)st, (void*)flags})
+                        (void*)val, (void*)bvv, (void*)cut, (void*)it, (void*)st, (void*)flags})
             if (p) (void)hipFree(p);
         for (void* p : {(void*)hx, (void*)hlb, (void*)hub, (void*)hbv, (void*)heps, (void*)hgap,
-                        (void*)hval, (void*)hbvv, (void*)hit, (void*)hst})
+                        (void*)hval, (void*)hbvv, (void*)hcut, (void*)hit, (void*)hst})
             if (p) (void)hipHostFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -183,7 +185,11 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     if (s->branch_rule == GB_BRANCH_CALLBACK && !s->branch_score)
         return fail(GB_ERR_INVALID_ARGUMENT, "callback branching requires branch_score");
     if (s->precision != GB_PREC_FP64)
-        return fail(GB_ERR_INVALID_ARGUMENT, "gb_solve runs the FP64 plain form; use gb_solve_batched for mixed precision");
+        return fail(GB_ERR_INVALID_ARGUMENT, "gb_solve relaxes nodes in FP64 (certified bounds)");
+    if (s->form != 0 && s->form != 1) return fail(GB_ERR_INVALID_ARGUMENT, "unknown solver form");
+    if (s->form == 1 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > kMaxFastN))
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "the batched form supports dense objectives over a box with n <= 512");
     return GB_OK;
 }
 
@@ -308,9 +314,18 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         GB_HIP(hipMemcpyAsync(db.eps, db.heps, N * 8, hipMemcpyHostToDevice, db.stream));
         GB_HIP(hipMemcpyAsync(db.it, db.hit, N * 4, hipMemcpyHostToDevice, db.stream));
         GB_HIP(hipMemcpyAsync(db.st, db.hst, N, hipMemcpyHostToDevice, db.stream));
-        st = relax_batch_fp64(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub, db.eps,
-                              db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags, s->integer_tol,
-                              s->gpu_block_threads, db.stream);
+        if (s->form == 1) {
+            for (int i = 0; i < N; ++i) db.hcut[i] = inc;
+            GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
+            const gb_fast_options fo{kFastRefresh, 4, 0};
+            st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
+                                  db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
+                                  s->integer_tol, db.cut, &fo, db.stream);
+        } else {
+            st = relax_batch_fp64(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
+                                  db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
+                                  s->integer_tol, s->gpu_block_threads, db.stream);
+        }
         if (st != GB_OK) return st;
         GB_HIP(hipMemcpyAsync(db.hx, db.x, xb, hipMemcpyDeviceToHost, db.stream));
         GB_HIP(hipMemcpyAsync(db.hbv, db.bv, xb, hipMemcpyDeviceToHost, db.stream));
@@ -398,6 +413,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         const int vi = vars[b];
                         if (mask & (1 << b)) c.lb[vi] = std::ceil(xi[vi]);
                         else c.ub[vi] = std::floor(xi[vi]);
+                    }
+                    if (s->form == 1) {
+                        // Batched form: the child starts from the parent's
+                        // iterate clipped to its box (any box point is feasible).
+                        c.rx.resize(n);
+                        for (int j = 0; j < n; ++j) c.rx[j] = std::min(std::max(xi[j], c.lb[j]), c.ub[j]);
                     }
                     if (c.bound < inc) q.push(std::move(c));
                 }

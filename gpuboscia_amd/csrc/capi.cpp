@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "gb_fast.h"
 #include "gb_internal.h"
 #include "gb_host.h"
 
@@ -168,9 +169,104 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
     return GB_OK;
 }
 
+// Device-resident node counter of the persistent batched kernel, one per device.
+static int* g_counter[64];
+static std::mutex g_counter_mtx;
+
+gb_status counter_get(int** out) {
+    int dev = 0;
+    GB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(GB_ERR_RUNTIME, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_counter_mtx);
+    if (!g_counter[dev]) {
+        void* p = nullptr;
+        GB_HIP(hipMalloc(&p, 256));
+        g_counter[dev] = static_cast<int*>(p);
+    }
+    *out = g_counter[dev];
+    return GB_OK;
+}
+
+gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                           const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
+                           const double* ub, const double* eps, int* iterations, uint8_t* status,
+                           double* gap, double* value, double* best_vertex, double* best_vertex_val,
+                           const uint8_t* flags, double ctol, const double* cutoff,
+                           const gb_fast_options* opt, void* stream) {
+    gb_status st = validate_descriptors(o, a, l);
+    if (st != GB_OK) return st;
+    if (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX)
+        return fail(GB_ERR_INVALID_ARGUMENT, "the batched form supports dense objectives over a box");
+    if (o->n > kMaxFastN) return fail(GB_ERR_INVALID_ARGUMENT, "the batched form supports n <= 512");
+    if (N < 0 || max_iters < 0) return fail(GB_ERR_INVALID_ARGUMENT, "N and max_iters must be non-negative");
+    if (N == 0) return GB_OK;
+    if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
+            || !best_vertex_val)
+        return fail(GB_ERR_INVALID_ARGUMENT, "null node buffer");
+    gb_fast_options op{64, 4, 0};
+    if (opt) op = *opt;
+    if (op.waves_per_block < 1 || op.waves_per_block > 4 || op.refresh_interval < 0 || op.blocks < 0)
+        return fail(GB_ERR_INVALID_ARGUMENT, "invalid batched-form options");
+    int* counter = nullptr;
+    st = counter_get(&counter);
+    if (st != GB_OK) return st;
+    FastParams P{};
+    P.n = o->n;
+    P.n_pad = (o->n + 1) & ~1;
+    P.Qt = o->Qt;
+    P.lin = o->linear;
+    P.gs = o->grad_scale;
+    P.ls = o->linear_scale;
+    P.kappa = o->constant;
+    P.dicg = a->algorithm == GB_ALG_DICG;
+    P.refresh = op.refresh_interval;
+    P.N = N;
+    P.max_iters = max_iters;
+    P.x = x;
+    P.lb = lb;
+    P.ub = ub;
+    P.eps = eps;
+    P.iters = iterations;
+    P.status = status;
+    P.gap = gap;
+    P.value = value;
+    P.best_v = best_vertex;
+    P.best_val = best_vertex_val;
+    P.flags = flags;
+    P.ctol = ctol;
+    P.cutoff = cutoff;
+    P.counter = counter;
+    int blocks = op.blocks;
+    if (blocks == 0) {
+        const int needed = (N + op.waves_per_block - 1) / op.waves_per_block;
+        blocks = needed < kMaxFastBlocks ? needed : kMaxFastBlocks;
+    }
+    const int e = launch_relax_fast(P, op.waves_per_block, blocks, stream);
+    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    return GB_OK;
+}
+
 }  // namespace gb
 
 using namespace gb;
+
+extern "C" gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
+                                         const gb_algorithm_desc* algorithm,
+                                         const gb_lmo_desc* lmo, int N, int max_iters, double* x,
+                                         const double* lb, const double* ub, const double* eps,
+                                         int* iterations, uint8_t* status, double* gap,
+                                         double* value, double* best_vertex,
+                                         double* best_vertex_val, const uint8_t* integer_flags,
+                                         double candidate_tol, const double* cutoff,
+                                         const gb_fast_options* options, void* stream) {
+    try {
+        return relax_batch_fast(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps,
+                                iterations, status, gap, value, best_vertex, best_vertex_val,
+                                integer_flags, candidate_tol, cutoff, options, stream);
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
 
 extern "C" gb_status gb_relax_batch(const gb_objective_desc* objective,
                                     const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,

@@ -1,0 +1,36 @@
+// Parameters of the batched-form relaxation kernel (relax_fast.hip).
+#pragma once
+#include <stdint.h>
+
+#include "../../include/gpuboscia_amd.h"
+
+namespace gb {
+
+struct FastParams {
+    int n;
+    int n_pad;              // per-wave LDS staging stride (doubles)
+    const double* Qt;       // column-major Q (Qt[k*n + j] == Q[j,k])
+    const double* lin;      // may be NULL
+    double gs, ls, kappa;
+    int dicg;
+    int refresh;            // direct recompute of Q x every `refresh` iterations (0 = never)
+    int N, max_iters;
+    double* x;
+    const double* lb;
+    const double* ub;
+    const double* eps;
+    int* iters;
+    uint8_t* status;
+    double* gap;
+    double* value;
+    double* best_v;
+    double* best_val;
+    const uint8_t* flags;
+    double ctol;
+    const double* cutoff;   // per node, may be NULL (= +inf)
+    int* counter;           // work queue head (reset by the launcher)
+};
+
+int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream);
+
+}  // namespace gb

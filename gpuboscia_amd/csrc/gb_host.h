@@ -13,6 +13,9 @@ constexpr int kMaxDenseN = 8192;         // vector staged in LDS (64 KiB FP64)
 constexpr int kMaxSimplexN = 2048;       // sort/scan arrays in LDS
 constexpr int kMaxBirkhoffOrder = 256;   // Hungarian state in LDS
 constexpr size_t kMaxLdsBytes = 160 * 1024;
+constexpr int kMaxFastN = 512;           // batched form: 8 coordinates per lane
+constexpr int kMaxFastBlocks = 2048;     // persistent workgroups of the batched form
+constexpr int kFastRefresh = 64;         // direct Q x recompute interval of the batched form
 
 extern thread_local std::string g_last_error;
 
@@ -27,6 +30,13 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
                            const double* ub, const double* eps, int* iterations, uint8_t* status,
                            double* gap, double* value, double* best_vertex, double* best_vertex_val,
                            const uint8_t* flags, double ctol, int block_threads, void* stream);
+
+gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                           const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
+                           const double* ub, const double* eps, int* iterations, uint8_t* status,
+                           double* gap, double* value, double* best_vertex, double* best_vertex_val,
+                           const uint8_t* flags, double ctol, const double* cutoff,
+                           const gb_fast_options* opt, void* stream);
 
 #define GB_HIP(call)                                                   \
     do {                                                               \

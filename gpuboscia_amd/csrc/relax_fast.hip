@@ -1,0 +1,365 @@
+// Batched-form node relaxation for dense quadratics over boxes, gfx950.
+//
+// One wave64 owns one B&B node; waves are persistent and pull node indices
+// from a device counter, so converged nodes hand their wave to the next node
+// of the batch (no tail of idle blocks).  Lane l holds coordinates
+// j = l + 64 t (t < NT) of x, Qx, v, Qv, a, Qa, lb, ub in registers: every
+// reduction is a wave shuffle and no workgroup barrier is executed.
+//
+// Algorithm (FP64 throughout):
+//   * products Q x, Q v, Q a are kept incrementally: a box vertex changes in
+//     few coordinates per iteration, so Q v_new = Q v + sum_k Q[:,k] dv_k over
+//     the changed k only; Q x follows x += gamma d as Qx += gamma Q d;
+//     Q x is recomputed directly every `refresh` iterations;
+//   * exact quadratic step gamma = clip(gap_pair / d'Hd, 0, gamma_max)
+//     (line_search_options.h:48-63), d'Hd from Q d = Qv - Qa;
+//   * the iteration stops at the per-node FW tolerance, or as soon as the
+//     FW lower bound f(x) - gap reaches the node's cutoff (the incumbent:
+//     the node will be pruned);
+//   * the returned certificate is always recomputed directly in FP64:
+//     Q x by a GEMV, then v, gap and f(x), exactly like the plain form's
+//     final step (device_resident_solver.cuh:588-621).
+// Vertex and rounded-candidate scoring follow drs_integer_value and the
+// rounding heuristic (device_resident_solver.cuh:297-335).
+#include "gb_device.h"
+#include "gb_fast.h"
+
+namespace gb {
+namespace {
+
+__device__ __forceinline__ double wave_sum_all(double v) {
+    return __shfl(wave_tree_sum(v), 0, kWave);
+}
+__device__ __forceinline__ double wave_min_all(double v) {
+    return __shfl(wave_tree_min(v), 0, kWave);
+}
+__device__ __forceinline__ bool wave_all(bool p) { return __ballot(!p) == 0ull; }
+
+__device__ __forceinline__ double quad_exact(double gap, double curvature, double maximum) {
+    if (maximum <= 0.0) return 0.0;
+    if (curvature > 0.0) {
+        if (gap <= 0.0) return 0.0;
+        const double step = gap / curvature;
+        return step < maximum ? step : maximum;
+    }
+    return -gap + 0.5 * curvature * maximum < 0.0 ? maximum : 0.0;
+}
+
+template <int NT>
+struct WaveNode {
+    const FastParams& P;
+    int lane;
+    double* zs;  // this wave's LDS staging vector (n doubles)
+
+    // y = Q z for the lane's rows; z given in registers.
+    __device__ void gemv(const double (&z)[NT], double (&y)[NT]) {
+        const int n = P.n;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            if (j < n) zs[j] = z[t];
+        }
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            double acc = 0.0;
+            if (j < n) {
+                const double* q = P.Qt + j;
+                int k = 0;
+                for (; k + 4 <= n; k += 4) {
+                    const double q0 = q[(size_t)(k + 0) * n], q1 = q[(size_t)(k + 1) * n];
+                    const double q2 = q[(size_t)(k + 2) * n], q3 = q[(size_t)(k + 3) * n];
+                    acc = fma(q0, zs[k + 0], acc);
+                    acc = fma(q1, zs[k + 1], acc);
+                    acc = fma(q2, zs[k + 2], acc);
+                    acc = fma(q3, zs[k + 3], acc);
+                }
+                for (; k < n; ++k) acc = fma(q[(size_t)k * n], zs[k], acc);
+            }
+            y[t] = acc;
+        }
+        wave_sync();
+    }
+
+    // y += sum over k with znew_k != zold_k of Q[:,k] (znew_k - zold_k), k ascending.
+    __device__ int update(const double (&znew)[NT], const double (&zold)[NT], double (&y)[NT]) {
+        const int n = P.n;
+        int changes = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            const bool ch = j < n && znew[t] != zold[t];
+            const double dz = znew[t] - zold[t];
+            unsigned long long mask = __ballot(ch);
+            while (mask) {
+                const int b = __builtin_ctzll(mask);
+                mask &= mask - 1ull;
+                const int k = b + kWave * t;
+                const double delta = __shfl(dz, b, kWave);
+                const double* col = P.Qt + (size_t)k * n;
+#pragma unroll
+                for (int tt = 0; tt < NT; ++tt) {
+                    const int jj = lane + kWave * tt;
+                    if (jj < n) y[tt] = fma(col[jj], delta, y[tt]);
+                }
+                ++changes;
+            }
+        }
+        return changes;
+    }
+};
+
+template <int NT>
+__global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
+    extern __shared__ __align__(16) double lds_fast[];
+    const int lane = lane_id();
+    const int n = P.n;
+    double* zs = lds_fast + (size_t)wave_id() * P.n_pad;
+    WaveNode<NT> W{P, lane, zs};
+    const double hs = 0.5 * P.gs;
+    const double lsc_scale = P.gs * P.ls;
+    const double atol = 1e-6, rtol = 1e-4;
+
+    while (true) {
+        int node = 0;
+        if (lane == 0) node = atomicAdd(P.counter, 1);
+        node = __shfl(node, 0, kWave);
+        if (node >= P.N) break;
+
+        const double* lbn = P.lb + (size_t)node * n;
+        const double* ubn = P.ub + (size_t)node * n;
+        double* xn = P.x + (size_t)node * n;
+        double* bvn = P.best_v + (size_t)node * n;
+        const double eps = P.eps[node];
+        const double cutoff = P.cutoff ? P.cutoff[node] : __builtin_inf();
+        const int initial = P.iters[node];
+        const bool supplied = (P.status[node] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0;
+
+        double lb[NT], ub[NT], lsc[NT], x[NT], qx[NT], v[NT], qv[NT], a[NT], qa[NT], g[NT];
+        bool fl[NT];
+        bool valid = true;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            const bool in = j < n;
+            lb[t] = in ? lbn[j] : 0.0;
+            ub[t] = in ? ubn[j] : 0.0;
+            lsc[t] = (in && P.lin) ? lsc_scale * P.lin[j] : 0.0;
+            fl[t] = in && P.flags && P.flags[j];
+            if (in && (isnan(lb[t]) || isnan(ub[t]) || lb[t] > ub[t]
+                       || (fl[t] && ceil(lb[t]) > floor(ub[t]))))
+                valid = false;
+        }
+        if (!wave_all(valid)) {
+            if (lane == 0) {
+                P.status[node] = GB_NODE_INFEASIBLE;
+                P.value[node] = __builtin_inf();
+                P.gap[node] = 0.0;
+                P.best_val[node] = kNone;
+                P.iters[node] = initial;
+            }
+            continue;
+        }
+
+        double best = kNone;
+        // Score an integral vertex z whose product Qz is known.
+        auto score = [&](const double (&z)[NT], const double (&qz)[NT]) {
+            bool ok = true;
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                if (j < n) {
+                    if (!isfinite(z[t]) || z[t] < lb[t] - P.ctol || z[t] > ub[t] + P.ctol
+                            || (fl[t] && z[t] != round(z[t])))
+                        ok = false;
+                    acc = fma(z[t], fma(hs, qz[t], lsc[t]), acc);
+                }
+            }
+            const bool all_ok = wave_all(ok);
+            const double s = wave_sum_all(acc);
+            if (!all_ok) return;
+            double f = s + P.kappa;
+            if (!isfinite(f)) return;
+            if (f < best) {
+                best = f;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int j = lane + kWave * t;
+                    if (j < n) bvn[j] = z[t];
+                }
+            }
+        };
+
+        // Starting point: LMO vertex of the artificial direction (ub for a
+        // box, since it is negative) or the supplied / warm-start iterate.
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            x[t] = (j < n) ? ((initial == 0 && !supplied) ? ub[t] : xn[j]) : 0.0;
+        }
+        W.gemv(x, qx);
+        auto grad = [&]() {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) g[t] = P.gs * qx[t] + lsc[t];
+        };
+        auto fwd = [&](double (&out)[NT]) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) out[t] = (g[t] > 0.0) ? lb[t] : ub[t];
+        };
+        grad();
+        fwd(v);
+        W.gemv(v, qv);
+        bool have_a = false;
+        int iterations = initial;
+        double gap = 0.0;
+        int it = 0;
+        for (; it < P.max_iters; ++it) {
+            iterations = initial + it + 1;
+            if (it > 0) {
+                grad();
+                double vn[NT];
+                fwd(vn);
+                W.update(vn, v, qv);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) v[t] = vn[t];
+            }
+            score(v, qv);
+            double ga = 0.0, fa = 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                if (j < n) {
+                    ga = fma(g[t], x[t] - v[t], ga);
+                    fa = fma(x[t], fma(hs, qx[t], lsc[t]), fa);
+                }
+            }
+            gap = wave_sum_all(ga);
+            const double fx = wave_sum_all(fa) + P.kappa;
+            if (gap >= 0.0 && gap < eps) break;
+            if (fx - gap >= cutoff) break;
+
+            double d[NT], qd[NT], gmax;
+            if (P.dicg) {
+                double an[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const double ut = atol + rtol * fabs(ub[t]);
+                    const double lt = atol + rtol * fabs(lb[t]);
+                    if (fabs(x[t] - ub[t]) <= ut) an[t] = ub[t];
+                    else if (fabs(x[t] - lb[t]) <= lt) an[t] = lb[t];
+                    else an[t] = (-g[t] > 0.0) ? lb[t] : ub[t];
+                }
+                if (have_a) W.update(an, a, qa);
+                else W.gemv(an, qa);
+                have_a = true;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) a[t] = an[t];
+                score(a, qa);
+                double r = 1.0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int j = lane + kWave * t;
+                    const double dv = a[t] - v[t];  // direction a - v as the reference
+                    if (j < n) {
+                        if (dv < 0.0) r = fmin(r, (ub[t] - x[t]) / -dv);
+                        else if (dv > 0.0) r = fmin(r, (x[t] - lb[t]) / dv);
+                    }
+                    d[t] = v[t] - a[t];
+                    qd[t] = qv[t] - qa[t];
+                }
+                gmax = fmax(0.0, fmin(wave_min_all(r), 1.0));
+            } else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    d[t] = v[t] - x[t];
+                    qd[t] = qv[t] - qx[t];
+                }
+                gmax = 1.0;
+            }
+            double pg = 0.0, cv = 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                if (j < n) {
+                    pg = fma(-g[t], d[t], pg);
+                    cv = fma(P.gs * d[t], qd[t], cv);
+                }
+            }
+            const double pgap = wave_sum_all(pg);
+            const double curv = wave_sum_all(cv);
+            const double gamma = quad_exact(pgap, curv, gmax);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                x[t] = fma(gamma, d[t], x[t]);
+                qx[t] = fma(gamma, qd[t], qx[t]);
+            }
+            if (P.refresh > 0 && (it + 1) % P.refresh == 0) W.gemv(x, qx);
+        }
+
+        // Direct FP64 certificate at the returned iterate.
+        W.gemv(x, qx);
+        grad();
+        fwd(v);
+        W.gemv(v, qv);
+        score(v, qv);
+        double ga = 0.0, fa = 0.0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            if (j < n) {
+                ga = fma(g[t], x[t] - v[t], ga);
+                fa = fma(x[t], fma(hs, qx[t], lsc[t]), fa);
+            }
+        }
+        const double fgap = wave_sum_all(ga);
+        const double fval = wave_sum_all(fa) + P.kappa;
+        uint8_t st = (fgap >= 0.0 && fgap < eps) ? 1 : 0;
+        bool integral = true;
+        if (P.flags) {
+            double cand[NT], qc[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cand[t] = fl[t] ? round(x[t]) : x[t];
+            W.gemv(cand, qc);
+            score(cand, qc);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            if (j < n) {
+                if (!isfinite(x[t]) || (fl[t] && fabs(x[t] - round(x[t])) > P.ctol)) integral = false;
+                xn[j] = x[t];
+            }
+        }
+        if (wave_all(integral)) st = st ? GB_NODE_INTEGRAL_CONVERGED : GB_NODE_INTEGRAL_UNCONVERGED;
+        if (lane == 0) {
+            P.value[node] = fval;
+            P.gap[node] = fgap;
+            P.status[node] = st;
+            P.iters[node] = iterations;
+            P.best_val[node] = best;
+        }
+    }
+}
+
+__global__ void reset_counter_kernel(int* counter) { *counter = 0; }
+
+}  // namespace
+
+int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream) {
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, p.counter);
+    const size_t lds = (size_t)waves_per_block * p.n_pad * sizeof(double);
+    const dim3 grid(blocks), block(waves_per_block * kWave);
+#define GB_FAST(NT) hipLaunchKernelGGL(relax_fast_kernel<NT>, grid, block, lds, st, p)
+    if (p.n <= 64) GB_FAST(1);
+    else if (p.n <= 128) GB_FAST(2);
+    else if (p.n <= 256) GB_FAST(4);
+    else if (p.n <= 512) GB_FAST(8);
+    else return (int)hipErrorInvalidValue;
+#undef GB_FAST
+    return (int)hipGetLastError();
+}
+
+}  // namespace gb

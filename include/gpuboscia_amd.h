@@ -147,6 +147,36 @@ gb_status gb_relax_batch(const gb_objective_desc* objective,
                          const uint8_t* integer_flags, double candidate_tol,
                          int block_threads, void* stream);
 
+/* Options of the batched-form relaxation (gb_relax_batch_fast). */
+typedef struct {
+    int refresh_interval;     /* recompute Q x directly every k iterations (0 = never) */
+    int waves_per_block;      /* nodes resident per workgroup (1..4) */
+    int blocks;               /* persistent workgroups (0 = automatic) */
+} gb_fast_options;
+
+/*
+ * Batched-form relaxation of N nodes of a dense quadratic over a box (no
+ * reference counterpart; it replaces the per-node block of
+ * drs_fw_relax_batch_kernel, device_resident_solver.cuh:631-660, for the
+ * box LMO).  One wave64 per node, persistent waves fed by a device node
+ * counter, incremental FP64 products Q x / Q v / Q a, exact quadratic step,
+ * early stop once f(x) - gap >= cutoff[i] (cutoff may be NULL), and a final
+ * direct FP64 certificate (gap, value) at the returned x.  Buffers and
+ * status codes as gb_relax_batch; the algorithm descriptor selects FW or
+ * DICG (its line search is ignored: the step is always exact).  n <= 512.
+ */
+gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
+                              const gb_algorithm_desc* algorithm,
+                              const gb_lmo_desc* lmo,
+                              int N, int max_iters,
+                              double* x, const double* lb, const double* ub,
+                              const double* eps, int* iterations, uint8_t* status,
+                              double* gap, double* value,
+                              double* best_vertex, double* best_vertex_val,
+                              const uint8_t* integer_flags, double candidate_tol,
+                              const double* cutoff, const gb_fast_options* options,
+                              void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Branch-and-bound (gpuboscia::solve, gpuboscia.cuh:56-170)               */
 /* ---------------------------------------------------------------------- */
@@ -212,6 +242,7 @@ typedef struct {
     gb_branch_score_fn branch_score;
     void* branch_score_user;
     int precision;            /* gb_precision (GB_PREC_FP64 = plain form) */
+    int form;                 /* 0 = plain (reference schedule), 1 = batched form */
     int learned_model;        /* handle from gb_brancher_create, or 0 */
     long long max_nodes;      /* safety cap; <= 0 = none (no reference counterpart) */
 } gb_settings;

@@ -1041,6 +1041,187 @@ extern "C" void orc_relax_batch(const orc_objective* o, const orc_alg* a, const 
 }
 
 // ---------------------------------------------------------------------------
+// Batched-form relaxation (box LMO, dense objective), mirroring the wave64
+// kernel gpuboscia_amd/csrc/relax_fast.hip operation for operation: one wave
+// (64 lanes) per node, incremental FP64 products, exact quadratic step,
+// cutoff stop, final direct certificate.  No reference counterpart: pinned
+// by the same KATs as the plain form (optima, enumeration) and by the kernel.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct FastNode {
+    const orc_objective& o;
+    int n;
+    std::vector<double> lsc;
+    double hs;
+
+    void gemv(const std::vector<double>& z, std::vector<double>& y) const {
+        for (int j = 0; j < n; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) acc = std::fma(o.Qt[(size_t)k * n + j], z[k], acc);
+            y[j] = acc;
+        }
+    }
+    void update(const std::vector<double>& zn, const std::vector<double>& zo,
+                std::vector<double>& y) const {
+        for (int k = 0; k < n; ++k) {  // k = lane + 64 t, t-major then lane: ascending k
+            if (zn[k] == zo[k]) continue;
+            const double delta = zn[k] - zo[k];
+            for (int j = 0; j < n; ++j) y[j] = std::fma(o.Qt[(size_t)k * n + j], delta, y[j]);
+        }
+    }
+    double dot_value(const std::vector<double>& z, const std::vector<double>& qz) const {
+        return block_sum(n, 64, [&](double acc, int j) {
+            return std::fma(z[j], std::fma(hs, qz[j], lsc[j]), acc);
+        });
+    }
+};
+
+}  // namespace
+
+extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, int max_iters,
+                                     double* X, const double* LB, const double* UB,
+                                     const double* EPS, int* ITERS, uint8_t* STATUS, double* GAP,
+                                     double* VALUE, double* BV, double* BVV, const uint8_t* flags,
+                                     double ctol, const double* cutoff, int refresh, int threads) {
+    const orc_objective& o = *op;
+    const int n = o.n;
+    auto work = [&](int node) {
+        FastNode F{o, n, std::vector<double>(n, 0.0), 0.5 * o.grad_scale};
+        const double lsc_scale = o.grad_scale * o.linear_scale;
+        for (int j = 0; j < n; ++j) F.lsc[j] = o.linear ? lsc_scale * o.linear[j] : 0.0;
+        const double* lb = LB + (size_t)node * n;
+        const double* ub = UB + (size_t)node * n;
+        double* xn = X + (size_t)node * n;
+        double* bvn = BV + (size_t)node * n;
+        const double eps = EPS[node];
+        const double cut = cutoff ? cutoff[node] : kInf;
+        const int initial = ITERS[node];
+        const bool supplied = (STATUS[node] & 0x80) != 0;
+        for (int j = 0; j < n; ++j)
+            if (std::isnan(lb[j]) || std::isnan(ub[j]) || lb[j] > ub[j]
+                    || (flags && flags[j] && std::ceil(lb[j]) > std::floor(ub[j]))) {
+                STATUS[node] = 2;
+                VALUE[node] = kInf;
+                GAP[node] = 0.0;
+                BVV[node] = kNone;
+                ITERS[node] = initial;
+                return;
+            }
+        double best = kNone;
+        auto score = [&](const std::vector<double>& z, const std::vector<double>& qz) {
+            bool ok = true;
+            for (int j = 0; j < n; ++j)
+                if (!std::isfinite(z[j]) || z[j] < lb[j] - ctol || z[j] > ub[j] + ctol
+                        || (flags && flags[j] && z[j] != std::round(z[j])))
+                    ok = false;
+            const double s = F.dot_value(z, qz);
+            if (!ok) return;
+            const double f = s + o.constant;
+            if (!std::isfinite(f)) return;
+            if (f < best) { best = f; std::copy(z.begin(), z.end(), bvn); }
+        };
+        std::vector<double> x(n), qx(n), v(n), qv(n), a(n), qa(n), g(n), vn(n), an(n), d(n), qd(n);
+        for (int j = 0; j < n; ++j) x[j] = (initial == 0 && !supplied) ? ub[j] : xn[j];
+        F.gemv(x, qx);
+        auto grad = [&]() { for (int j = 0; j < n; ++j) g[j] = o.grad_scale * qx[j] + F.lsc[j]; };
+        auto fwd = [&](std::vector<double>& out) { for (int j = 0; j < n; ++j) out[j] = g[j] > 0.0 ? lb[j] : ub[j]; };
+        grad();
+        fwd(v);
+        F.gemv(v, qv);
+        bool have_a = false;
+        int iterations = initial;
+        const double atol = 1e-6, rtol = 1e-4;
+        for (int it = 0; it < max_iters; ++it) {
+            iterations = initial + it + 1;
+            if (it > 0) {
+                grad();
+                fwd(vn);
+                F.update(vn, v, qv);
+                v = vn;
+            }
+            score(v, qv);
+            const double gap = block_sum(n, 64, [&](double acc, int j) { return std::fma(g[j], x[j] - v[j], acc); });
+            const double fx = F.dot_value(x, qx) + o.constant;
+            if (gap >= 0.0 && gap < eps) break;
+            if (fx - gap >= cut) break;
+            double gmax;
+            if (dicg) {
+                for (int j = 0; j < n; ++j) {
+                    const double ut = atol + rtol * std::fabs(ub[j]);
+                    const double lt = atol + rtol * std::fabs(lb[j]);
+                    if (std::fabs(x[j] - ub[j]) <= ut) an[j] = ub[j];
+                    else if (std::fabs(x[j] - lb[j]) <= lt) an[j] = lb[j];
+                    else an[j] = (-g[j] > 0.0) ? lb[j] : ub[j];
+                }
+                if (have_a) F.update(an, a, qa);
+                else F.gemv(an, qa);
+                have_a = true;
+                a = an;
+                score(a, qa);
+                std::vector<double> lane(64, 1.0);
+                for (int l = 0; l < 64; ++l) {
+                    double r = 1.0;
+                    for (int j = l; j < n; j += 64) {
+                        const double dv = a[j] - v[j];
+                        if (dv < 0.0) r = std::fmin(r, (ub[j] - x[j]) / -dv);
+                        else if (dv > 0.0) r = std::fmin(r, (x[j] - lb[j]) / dv);
+                    }
+                    lane[l] = r;
+                }
+                gmax = std::fmax(0.0, std::fmin(lanes_min(lane), 1.0));
+                for (int j = 0; j < n; ++j) { d[j] = v[j] - a[j]; qd[j] = qv[j] - qa[j]; }
+            } else {
+                for (int j = 0; j < n; ++j) { d[j] = v[j] - x[j]; qd[j] = qv[j] - qx[j]; }
+                gmax = 1.0;
+            }
+            const double pgap = block_sum(n, 64, [&](double acc, int j) { return std::fma(-g[j], d[j], acc); });
+            const double curv = block_sum(n, 64, [&](double acc, int j) { return std::fma(o.grad_scale * d[j], qd[j], acc); });
+            const double gamma = quad_exact_step(pgap, curv, gmax);
+            for (int j = 0; j < n; ++j) {
+                x[j] = std::fma(gamma, d[j], x[j]);
+                qx[j] = std::fma(gamma, qd[j], qx[j]);
+            }
+            if (refresh > 0 && (it + 1) % refresh == 0) F.gemv(x, qx);
+        }
+        F.gemv(x, qx);
+        grad();
+        fwd(v);
+        F.gemv(v, qv);
+        score(v, qv);
+        const double fgap = block_sum(n, 64, [&](double acc, int j) { return std::fma(g[j], x[j] - v[j], acc); });
+        const double fval = F.dot_value(x, qx) + o.constant;
+        uint8_t st = (fgap >= 0.0 && fgap < eps) ? 1 : 0;
+        if (flags) {
+            std::vector<double> cand(n), qc(n);
+            for (int j = 0; j < n; ++j) cand[j] = flags[j] ? std::round(x[j]) : x[j];
+            F.gemv(cand, qc);
+            score(cand, qc);
+        }
+        bool integral = true;
+        for (int j = 0; j < n; ++j) {
+            if (!std::isfinite(x[j]) || (flags && flags[j] && std::fabs(x[j] - std::round(x[j])) > ctol)) integral = false;
+            xn[j] = x[j];
+        }
+        if (integral) st = st ? 4 : 3;
+        VALUE[node] = fval;
+        GAP[node] = fgap;
+        STATUS[node] = st;
+        ITERS[node] = iterations;
+        BVV[node] = best;
+    };
+    if (threads <= 1 || N <= 1) {
+        for (int i = 0; i < N; ++i) work(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::min(threads, N); ++t)
+        pool.emplace_back([&] { for (int i = next.fetch_add(1); i < N; i = next.fetch_add(1)) work(i); });
+    for (auto& th : pool) th.join();
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic best-first branch-and-bound (device_resident_solver.cuh:
 // 1570-2085, single batch in flight; see the product's host driver for the
 // same schedule).  Queue order: (dual_bound, id) ascending.
@@ -1120,9 +1301,16 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
             ST[i] = nd.rx.empty() ? 0 : 0x80;
             active.insert(nd.bound);
         }
-        orc_relax_batch(o, a, l, N, s->max_iters, X.data(), L.data(), U.data(), E.data(), IT.data(),
-                        ST.data(), G.data(), V.data(), BV.data(), BVV.data(), flags.data(),
-                        s->integer_tol, s->lanes, s->threads);
+        if (s->form == 1) {
+            std::vector<double> CUT(N, inc);
+            orc_relax_fast_batch(o, a->dicg, N, s->max_iters, X.data(), L.data(), U.data(), E.data(),
+                                 IT.data(), ST.data(), G.data(), V.data(), BV.data(), BVV.data(),
+                                 flags.data(), s->integer_tol, CUT.data(), s->refresh, s->threads);
+        } else {
+            orc_relax_batch(o, a, l, N, s->max_iters, X.data(), L.data(), U.data(), E.data(), IT.data(),
+                            ST.data(), G.data(), V.data(), BV.data(), BVV.data(), flags.data(),
+                            s->integer_tol, s->lanes, s->threads);
+        }
         for (int i = 0; i < N; ++i) fw_iters += IT[i] - batch[i].riters;
         for (int i = 0; i < N && !finished; ++i) {
             BNode nd = std::move(batch[i]);
@@ -1156,6 +1344,10 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
                         const int vi = vars[b];
                         if (mask & (1 << b)) c.lb[vi] = std::ceil(xi[vi]);
                         else c.ub[vi] = std::floor(xi[vi]);
+                    }
+                    if (s->form == 1) {
+                        c.rx.resize(n);
+                        for (int j = 0; j < n; ++j) c.rx[j] = std::min(std::max(xi[j], c.lb[j]), c.ub[j]);
                     }
                     if (c.bound < inc) q.push(std::move(c));
                 }

@@ -125,6 +125,14 @@ void orc_relax_batch(const orc_objective* o, const orc_alg* a,
                      double* best_vertex_val, const uint8_t* flags,
                      double candidate_tol, int lanes, int threads);
 
+/* Batched-form relaxation (box LMO, dense objective), mirroring the wave64
+   kernel operation for operation; cutoff may be NULL. */
+void orc_relax_fast_batch(const orc_objective* o, int dicg, int N, int max_iters, double* x,
+                          const double* lb, const double* ub, const double* eps,
+                          int* iterations, uint8_t* status, double* gap, double* value,
+                          double* best_vertex, double* best_vertex_val, const uint8_t* flags,
+                          double candidate_tol, const double* cutoff, int refresh, int threads);
+
 /* Deterministic best-first B&B over orc_relax_batch. */
 typedef struct {
     int batch_size;
@@ -136,6 +144,8 @@ typedef struct {
     double natural_lower_bound;
     double time_limit_seconds;
     long long max_nodes;      /* safety cap; <=0 = none */
+    int form;                 /* 0 = plain schedule, 1 = batched form (fast relax, warm starts, cutoff) */
+    int refresh;              /* batched form: direct Q x recompute interval */
 } orc_bnb_settings;
 
 typedef struct {

@@ -46,7 +46,8 @@ class BnbSettings(ctypes.Structure):
                 ("abs_gap_tol", ctypes.c_double), ("rel_gap_tol", ctypes.c_double),
                 ("branch_k", ctypes.c_int), ("lanes", ctypes.c_int), ("threads", ctypes.c_int),
                 ("natural_lower_bound", ctypes.c_double),
-                ("time_limit_seconds", ctypes.c_double), ("max_nodes", ctypes.c_longlong)]
+                ("time_limit_seconds", ctypes.c_double), ("max_nodes", ctypes.c_longlong),
+                ("form", ctypes.c_int), ("refresh", ctypes.c_int)]
 
 
 class BnbResult(ctypes.Structure):
@@ -154,14 +155,14 @@ def relax_batch(obj, a, lmo, x, lb, ub, eps, iterations, status, flags=None, max
 def bnb_solve(obj, a, lmo, lb, ub, int_vars, batch_size=1024, max_iters=20000, eps=1e-3,
               eps_decay=0.8, eps_min=1e-6, integer_tol=1e-6, abs_gap_tol=1e-6, rel_gap_tol=1e-2,
               branch_k=3, lanes=64, threads=1, natural_lower_bound=float("-inf"),
-              time_limit_seconds=float("inf"), max_nodes=0):
+              time_limit_seconds=float("inf"), max_nodes=0, form=0, refresh=64):
     lib = load()
     n = len(lb)
     lb_a, ub_a = f64(lb), f64(ub)
     iv = np.ascontiguousarray(int_vars, dtype=np.int32)
     s = BnbSettings(batch_size, max_iters, eps, eps_decay, eps_min, integer_tol, abs_gap_tol,
                     rel_gap_tol, branch_k, lanes, threads, natural_lower_bound,
-                    time_limit_seconds, max_nodes)
+                    time_limit_seconds, max_nodes, form, refresh)
     r = BnbResult()
     xo = np.zeros(n)
     lib.orc_bnb_solve(ctypes.byref(obj), ctypes.byref(a), ctypes.byref(lmo), n, p(lb_a), p(ub_a),
@@ -179,3 +180,21 @@ def enumerate_binary_qp(Q, c, threads=8):
     xo = np.zeros(n)
     f = lib.orc_enumerate_binary_qp(n, p(Q), p(c), threads, p(xo))
     return f, xo
+
+
+def relax_fast_batch(obj, dicg, x, lb, ub, eps, iterations, status, flags=None, max_iters=100,
+                     candidate_tol=1e-6, cutoff=None, refresh=64, threads=1):
+    """orc_relax_fast_batch on numpy arrays (mirror of the batched-form kernel)."""
+    lib = load()
+    N, n = x.shape
+    gap = np.zeros(N)
+    value = np.zeros(N)
+    bv = np.zeros((N, n))
+    bvv = np.full(N, 1e300)
+    fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+    cut = None if cutoff is None else f64(cutoff)
+    lib.orc_relax_fast_batch(ctypes.byref(obj), int(dicg), N, max_iters, p(x), p(lb), p(ub), p(eps),
+                             p(iterations), p(status), p(gap), p(value), p(bv), p(bvv), p(fl),
+                             ctypes.c_double(candidate_tol), p(cut), refresh, threads)
+    return {"x": x, "gap": gap, "value": value, "iterations": iterations, "status": status,
+            "best_vertex": bv, "best_vertex_val": bvv}
