@@ -22,7 +22,7 @@ OBJ_DENSE, OBJ_QAP = 0, 1
 LMO_BOX, LMO_SIMPLEX, LMO_BIRKHOFF = 0, 1, 2
 HEUR_ROUND, HEUR_FLOOR_OFFSET = 0, 1
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
-BRANCH_POW2, BRANCH_MOST_FRACTIONAL, BRANCH_LEARNED, BRANCH_CALLBACK = 0, 1, 2, 3
+BRANCH_POW2, BRANCH_MOST_FRACTIONAL, BRANCH_LEARNED, BRANCH_CALLBACK, BRANCH_CUSTOM = 0, 1, 2, 3, 4
 NODE_INFEASIBLE = 2
 NODE_INITIAL_POINT_SUPPLIED = 0x80
 
@@ -52,7 +52,7 @@ class AlgorithmDesc(ctypes.Structure):
 
 class FastOptions(ctypes.Structure):
     _fields_ = [("refresh_interval", ctypes.c_int), ("waves_per_block", ctypes.c_int),
-                ("blocks", ctypes.c_int)]
+                ("blocks", ctypes.c_int), ("stats", ctypes.c_void_p)]
 
 
 class Problem(ctypes.Structure):
@@ -66,6 +66,8 @@ VALIDATOR_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp, _dp, ct
                                 ctypes.c_double)
 BRANCH_SCORE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int,
                                    ctypes.c_int, _dp)
+BRANCH_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, _dp, _dp, ctypes.c_int)
 
 
 class Settings(ctypes.Structure):
@@ -81,6 +83,7 @@ class Settings(ctypes.Structure):
                 ("root_validator", VALIDATOR_FN), ("root_validator_user", ctypes.c_void_p),
                 ("progress", PROGRESS_FN), ("progress_user", ctypes.c_void_p),
                 ("branch_score", BRANCH_SCORE_FN), ("branch_score_user", ctypes.c_void_p),
+                ("branch", BRANCH_FN), ("branch_user", ctypes.c_void_p),
                 ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_int),
                 ("max_nodes", ctypes.c_longlong)]
 
@@ -191,10 +194,10 @@ def relax_batch(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_ite
 def relax_batch_fast(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb,
                      ub, eps, iterations, status, gap, value, best_vertex, best_vertex_val,
                      flags=None, candidate_tol=1e-6, cutoff=None, refresh=64, waves_per_block=4,
-                     blocks=0, stream=None):
-    """gb_relax_batch_fast on torch device tensors."""
+                     blocks=0, stream=None, stats=None):
+    """gb_relax_batch_fast on torch device tensors (stats: optional int64 [N,2])."""
     lib = load()
-    opt = FastOptions(refresh, waves_per_block, blocks)
+    opt = FastOptions(refresh, waves_per_block, blocks, 0 if stats is None else stats.data_ptr())
     code = lib.gb_relax_batch_fast(ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
                                    int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
                                    _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <queue>
 #include <set>
 #include <stdexcept>
@@ -29,9 +30,30 @@ namespace {
 
 constexpr double kInf = std::numeric_limits<double>::infinity();
 
+// Node bounds and warm starts live in a chunked slab (3 n doubles per slot:
+// lb, ub, relaxation_x); heap entries are small records.  This keeps host
+// post-processing at a few hundred ns per node (no per-child allocation).
+struct NodePool {
+    int n = 0;
+    static constexpr int kChunk = 4096;
+    std::vector<std::unique_ptr<double[]>> chunks;
+    std::vector<int> free_slots;
+    int next = 0;
+    int alloc() {
+        if (!free_slots.empty()) { const int s = free_slots.back(); free_slots.pop_back(); return s; }
+        if (next % kChunk == 0) chunks.emplace_back(new double[(size_t)kChunk * 3 * n]);
+        return next++;
+    }
+    void release(int slot) { free_slots.push_back(slot); }
+    double* lb(int slot) { return chunks[slot / kChunk].get() + (size_t)(slot % kChunk) * 3 * n; }
+    double* ub(int slot) { return lb(slot) + n; }
+    double* rx(int slot) { return lb(slot) + 2 * n; }
+};
+
 struct HostNode {
     int id = 0, parent = -1, depth = 0;
-    std::vector<double> lb, ub, rx;  // rx: warm start (relaxation_x)
+    int slot = -1;                   // NodePool slot holding lb, ub, relaxation_x
+    bool has_rx = false;             // warm start present (relaxation_x non-empty)
     int riters = 0;                  // relaxation_iterations
     double bound = -1e300;           // dual_bound
 };
@@ -178,8 +200,10 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     // gpuboscia.cuh:86-90
     if (std::isnan(s->natural_lower_bound) || s->natural_lower_bound == kInf)
         return fail(GB_ERR_INVALID_ARGUMENT, "natural lower bound must be finite or negative infinity");
-    if (s->branch_rule < GB_BRANCH_POW2 || s->branch_rule > GB_BRANCH_CALLBACK)
+    if (s->branch_rule < GB_BRANCH_POW2 || s->branch_rule > GB_BRANCH_CUSTOM)
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown branch rule");
+    if (s->branch_rule == GB_BRANCH_CUSTOM && !s->branch)
+        return fail(GB_ERR_INVALID_ARGUMENT, "custom branching requires a branch callback");
     if (s->branch_rule == GB_BRANCH_POW2 && (s->branch_k < 1 || s->branch_k > 16))
         return fail(GB_ERR_INVALID_ARGUMENT, "branch_k must be in [1, 16]");
     if (s->branch_rule == GB_BRANCH_CALLBACK && !s->branch_score)
@@ -239,15 +263,20 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     GB_HIP(hipMemcpy(db.flags, hflags.data(), n, hipMemcpyHostToDevice));
 
     std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
+    NodePool pool;
+    pool.n = n;
     {
         HostNode root;
         root.id = 0;
-        root.lb.assign(pr->lb, pr->lb + n);
-        root.ub.assign(pr->ub, pr->ub + n);
-        if (s->root_initial_point && s->root_initial_point_size == n)
-            root.rx.assign(s->root_initial_point, s->root_initial_point + n);
+        root.slot = pool.alloc();
+        std::copy(pr->lb, pr->lb + n, pool.lb(root.slot));
+        std::copy(pr->ub, pr->ub + n, pool.ub(root.slot));
+        if (s->root_initial_point && s->root_initial_point_size == n) {
+            std::copy(s->root_initial_point, s->root_initial_point + n, pool.rx(root.slot));
+            root.has_rx = true;
+        }
         root.bound = s->natural_lower_bound;
-        q.push(std::move(root));
+        q.push(root);
     }
     double inc = kInf;
     std::vector<double> inc_x;
@@ -289,21 +318,21 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         while (!q.empty() && (int)batch.size() < B) {
             HostNode nd = q.top();
             q.pop();
-            if (nd.bound >= inc) continue;
-            batch.push_back(std::move(nd));
+            if (nd.bound >= inc) { pool.release(nd.slot); continue; }
+            batch.push_back(nd);
         }
         if (batch.empty()) { final_bound = inc; finished = true; break; }
         const int N = (int)batch.size();
         for (int i = 0; i < N; ++i) {
             const HostNode& nd = batch[i];
-            for (int j = 0; j < n; ++j) {
-                db.hx[(size_t)i * n + j] = nd.rx.empty() ? nd.lb[j] : nd.rx[j];
-                db.hlb[(size_t)i * n + j] = nd.lb[j];
-                db.hub[(size_t)i * n + j] = nd.ub[j];
-            }
+            const double* nlb = pool.lb(nd.slot);
+            std::copy(nd.has_rx ? pool.rx(nd.slot) : nlb, (nd.has_rx ? pool.rx(nd.slot) : nlb) + n,
+                      db.hx + (size_t)i * n);
+            std::copy(nlb, nlb + n, db.hlb + (size_t)i * n);
+            std::copy(pool.ub(nd.slot), pool.ub(nd.slot) + n, db.hub + (size_t)i * n);
             db.heps[i] = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth);
             db.hit[i] = nd.riters;
-            db.hst[i] = nd.rx.empty() ? 0 : GB_NODE_INITIAL_POINT_SUPPLIED;
+            db.hst[i] = nd.has_rx ? GB_NODE_INITIAL_POINT_SUPPLIED : 0;
             active.insert(nd.bound);
         }
         const auto r0 = Clock::now();
@@ -317,7 +346,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         if (s->form == 1) {
             for (int i = 0; i < N; ++i) db.hcut[i] = inc;
             GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
-            const gb_fast_options fo{kFastRefresh, 4, 0};
+            const gb_fast_options fo{kFastRefresh, 4, 0, nullptr};
             st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
                                   s->integer_tol, db.cut, &fo, db.stream);
@@ -339,10 +368,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
         for (int i = 0; i < N && !finished; ++i) {
-            HostNode nd = std::move(batch[i]);
+            HostNode nd = batch[i];
             active.erase(active.find(nd.bound));
             const int remaining = N - i - 1;
             if (db.hst[i] == GB_NODE_INFEASIBLE) {
+                pool.release(nd.slot);
                 ++done_nodes;
                 progress(false, remaining);
                 finish_check(remaining > 0);
@@ -360,6 +390,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             }
             const bool cert = std::isfinite(g) && g >= 0.0 && g < node_eps;
             if (nd.bound >= inc) {
+                pool.release(nd.slot);
                 ++done_nodes;
                 progress(new_inc, remaining);
                 finish_check(remaining > 0);
@@ -368,13 +399,36 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             const double* xi = db.hx + (size_t)i * n;
             if (db.hst[i] >= GB_NODE_INTEGRAL_UNCONVERGED && !cert) {
                 // device_resident_solver.cuh:1970-1980: keep the node live.
-                nd.rx.assign(xi, xi + n);
+                std::copy(xi, xi + n, pool.rx(nd.slot));
+                nd.has_rx = true;
                 nd.riters = db.hit[i];
-                q.push(std::move(nd));
+                q.push(nd);
                 progress(new_inc, remaining);
                 continue;
             }
-            if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
+            if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED && s->branch_rule == GB_BRANCH_CUSTOM) {
+                // Host BranchStrategy: children as returned, ids from the
+                // same fetch_add(8) sequence (device_resident_solver.cuh:1989).
+                std::vector<double> clbs((size_t)GB_MAX_CUSTOM_CHILDREN * n);
+                std::vector<double> cubs((size_t)GB_MAX_CUSTOM_CHILDREN * n);
+                const int cnt = s->branch(s->branch_user, pool.lb(nd.slot), pool.ub(nd.slot), xi, n,
+                                          nd.depth, nd.id, clbs.data(), cubs.data(),
+                                          GB_MAX_CUSTOM_CHILDREN);
+                int nid = next_id;
+                next_id += cnt > 8 ? cnt : 8;
+                for (int cc = 0; cc < cnt && cc < GB_MAX_CUSTOM_CHILDREN; ++cc) {
+                    HostNode c;
+                    c.id = nid++;
+                    c.parent = nd.id;
+                    c.depth = nd.depth + 1;
+                    c.bound = nd.bound;
+                    if (!(c.bound < inc)) continue;
+                    c.slot = pool.alloc();
+                    std::copy(clbs.data() + (size_t)cc * n, clbs.data() + (size_t)(cc + 1) * n, pool.lb(c.slot));
+                    std::copy(cubs.data() + (size_t)cc * n, cubs.data() + (size_t)(cc + 1) * n, pool.ub(c.slot));
+                    q.push(c);
+                }
+            } else if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
                 std::vector<int> vars;
                 if (s->branch_rule == GB_BRANCH_POW2) {
                     vars = top_k_infeasible(xi, iv, s->branch_k, s->integer_tol);
@@ -406,23 +460,29 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     c.id = nid++;
                     c.parent = nd.id;
                     c.depth = nd.depth + 1;
-                    c.lb = nd.lb;
-                    c.ub = nd.ub;
                     c.bound = nd.bound;
+                    if (!(c.bound < inc)) continue;
+                    c.slot = pool.alloc();
+                    double* clb = pool.lb(c.slot);
+                    double* cub = pool.ub(c.slot);
+                    std::copy(pool.lb(nd.slot), pool.lb(nd.slot) + n, clb);
+                    std::copy(pool.ub(nd.slot), pool.ub(nd.slot) + n, cub);
                     for (int b = 0; b < k; ++b) {
                         const int vi = vars[b];
-                        if (mask & (1 << b)) c.lb[vi] = std::ceil(xi[vi]);
-                        else c.ub[vi] = std::floor(xi[vi]);
+                        if (mask & (1 << b)) clb[vi] = std::ceil(xi[vi]);
+                        else cub[vi] = std::floor(xi[vi]);
                     }
                     if (s->form == 1) {
                         // Batched form: the child starts from the parent's
                         // iterate clipped to its box (any box point is feasible).
-                        c.rx.resize(n);
-                        for (int j = 0; j < n; ++j) c.rx[j] = std::min(std::max(xi[j], c.lb[j]), c.ub[j]);
+                        double* crx = pool.rx(c.slot);
+                        for (int j = 0; j < n; ++j) crx[j] = std::min(std::max(xi[j], clb[j]), cub[j]);
+                        c.has_rx = true;
                     }
-                    if (c.bound < inc) q.push(std::move(c));
+                    q.push(c);
                 }
             }
+            pool.release(nd.slot);
             ++done_nodes;
             progress(new_inc, remaining);
             finish_check(remaining > 0);

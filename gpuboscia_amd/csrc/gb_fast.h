@@ -29,6 +29,7 @@ struct FastParams {
     double ctol;
     const double* cutoff;   // per node, may be NULL (= +inf)
     int* counter;           // work queue head (reset by the launcher)
+    long long* stats;       // optional [N][2]: column updates, direct GEMVs (may be NULL)
 };
 
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream);

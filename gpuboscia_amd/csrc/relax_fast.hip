@@ -45,11 +45,14 @@ __device__ __forceinline__ double quad_exact(double gap, double curvature, doubl
     return -gap + 0.5 * curvature * maximum < 0.0 ? maximum : 0.0;
 }
 
-template <int NT>
+template <int NT, bool QL>
 struct WaveNode {
     const FastParams& P;
+    const double* Q;    // Qt in LDS (QL) or global memory
     int lane;
     double* zs;  // this wave's LDS staging vector (n doubles)
+    long long columns;  // column updates performed (work accounting)
+    long long gemvs;    // direct products performed
 
     // y = Q z for the lane's rows; z given in registers.
     __device__ void gemv(const double (&z)[NT], double (&y)[NT]) {
@@ -65,58 +68,97 @@ struct WaveNode {
             const int j = lane + kWave * t;
             double acc = 0.0;
             if (j < n) {
-                const double* q = P.Qt + j;
+                const double* q = Q + j;
                 int k = 0;
-                for (; k + 4 <= n; k += 4) {
-                    const double q0 = q[(size_t)(k + 0) * n], q1 = q[(size_t)(k + 1) * n];
-                    const double q2 = q[(size_t)(k + 2) * n], q3 = q[(size_t)(k + 3) * n];
-                    acc = fma(q0, zs[k + 0], acc);
-                    acc = fma(q1, zs[k + 1], acc);
-                    acc = fma(q2, zs[k + 2], acc);
-                    acc = fma(q3, zs[k + 3], acc);
+                for (; k + 8 <= n; k += 8) {
+                    double qq[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) qq[u] = q[(size_t)(k + u) * n];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc = fma(qq[u], zs[k + u], acc);
                 }
                 for (; k < n; ++k) acc = fma(q[(size_t)k * n], zs[k], acc);
             }
             y[t] = acc;
         }
         wave_sync();
+        ++gemvs;
     }
 
-    // y += sum over k with znew_k != zold_k of Q[:,k] (znew_k - zold_k), k ascending.
+    // Changed columns are compacted into this wave's LDS list (k ascending)
+    // and their loads issued kGroup at a time before the FMAs: the vertex of
+    // an interior iterate flips many coordinates per iteration, and one L2
+    // round trip per column would leave the wave latency-bound.
+    static constexpr int kGroup = NT >= 4 ? 4 : 8;
+    int* klist;      // this wave's LDS list of changed columns (n ints)
+    double* dlist;   // their coefficients (n doubles)
+
+    // y += sum over k with znew_k != zold_k of Q[:,k] (znew_k - zold_k), k
+    // ascending (the FMA order per output coordinate is k order).
     __device__ int update(const double (&znew)[NT], const double (&zold)[NT], double (&y)[NT]) {
         const int n = P.n;
-        int changes = 0;
+        int count = 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int j = lane + kWave * t;
             const bool ch = j < n && znew[t] != zold[t];
-            const double dz = znew[t] - zold[t];
-            unsigned long long mask = __ballot(ch);
-            while (mask) {
-                const int b = __builtin_ctzll(mask);
-                mask &= mask - 1ull;
-                const int k = b + kWave * t;
-                const double delta = __shfl(dz, b, kWave);
-                const double* col = P.Qt + (size_t)k * n;
+            const unsigned long long mask = __ballot(ch);
+            if (ch) {
+                const int pos = count + __popcll(mask & ((1ull << lane) - 1ull));
+                klist[pos] = j;
+                dlist[pos] = znew[t] - zold[t];
+            }
+            count += __popcll(mask);
+        }
+        wave_sync();
+        for (int g0 = 0; g0 < count; g0 += kGroup) {
+            double c[kGroup][NT];
+            double dd[kGroup];
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                const bool live = g0 + g < count;
+                const int k = live ? klist[g0 + g] : 0;
+                dd[g] = live ? dlist[g0 + g] : 0.0;
 #pragma unroll
                 for (int tt = 0; tt < NT; ++tt) {
                     const int jj = lane + kWave * tt;
-                    if (jj < n) y[tt] = fma(col[jj], delta, y[tt]);
+                    c[g][tt] = (live && jj < n) ? Q[(size_t)k * n + jj] : 0.0;
                 }
-                ++changes;
+            }
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) {
+                if (g0 + g < count) {
+#pragma unroll
+                    for (int tt = 0; tt < NT; ++tt) {
+                        const int jj = lane + kWave * tt;
+                        if (jj < n) y[tt] = fma(c[g][tt], dd[g], y[tt]);
+                    }
+                }
             }
         }
-        return changes;
+        wave_sync();
+        columns += count;
+        return count;
     }
 };
 
-template <int NT>
+// QL: Q is copied once into LDS and shared by the block's resident waves
+// (the block is persistent, so the copy is amortised over all its nodes).
+template <int NT, bool QL>
 __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
     extern __shared__ __align__(16) double lds_fast[];
     const int lane = lane_id();
     const int n = P.n;
-    double* zs = lds_fast + (size_t)wave_id() * P.n_pad;
-    WaveNode<NT> W{P, lane, zs};
+    const size_t qsize = QL ? (size_t)n * n : 0;
+    if (QL) {
+        for (size_t i = threadIdx.x; i < qsize; i += blockDim.x) lds_fast[i] = P.Qt[i];
+        __syncthreads();
+    }
+    // Per-wave LDS: staging vector (n_pad doubles), changed-column
+    // coefficients (n_pad doubles), changed-column indices (n_pad ints).
+    double* wbase = lds_fast + qsize + (size_t)wave_id() * (size_t)P.n_pad * 5 / 2;
+    WaveNode<NT, QL> W{P, QL ? lds_fast : P.Qt, lane, wbase, 0, 0,
+                       reinterpret_cast<int*>(wbase + 2 * P.n_pad), wbase + P.n_pad};
     const double hs = 0.5 * P.gs;
     const double lsc_scale = P.gs * P.ls;
     const double atol = 1e-6, rtol = 1e-4;
@@ -126,6 +168,8 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
         if (lane == 0) node = atomicAdd(P.counter, 1);
         node = __shfl(node, 0, kWave);
         if (node >= P.N) break;
+        W.columns = 0;
+        W.gemvs = 0;
 
         const double* lbn = P.lb + (size_t)node * n;
         const double* ubn = P.ub + (size_t)node * n;
@@ -339,24 +383,43 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
             P.status[node] = st;
             P.iters[node] = iterations;
             P.best_val[node] = best;
+            if (P.stats) {
+                P.stats[2 * (size_t)node] = W.columns;
+                P.stats[2 * (size_t)node + 1] = W.gemvs;
+            }
         }
     }
 }
 
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
 
+constexpr size_t kFastLdsBudget = 160 * 1024;
+
 }  // namespace
 
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, p.counter);
-    const size_t lds = (size_t)waves_per_block * p.n_pad * sizeof(double);
+    // per wave: 2 n_pad doubles + n_pad ints (n_pad even keeps 8-byte alignment)
+    const size_t wave_lds = (size_t)waves_per_block * p.n_pad * 5 / 2 * sizeof(double);
+    const size_t q_lds = (size_t)p.n * p.n * sizeof(double);
+    const bool ql = q_lds + wave_lds <= kFastLdsBudget;
+    const size_t lds = wave_lds + (ql ? q_lds : 0);
     const dim3 grid(blocks), block(waves_per_block * kWave);
-#define GB_FAST(NT) hipLaunchKernelGGL(relax_fast_kernel<NT>, grid, block, lds, st, p)
-    if (p.n <= 64) GB_FAST(1);
-    else if (p.n <= 128) GB_FAST(2);
-    else if (p.n <= 256) GB_FAST(4);
-    else if (p.n <= 512) GB_FAST(8);
+#define GB_FAST(NT, QL)                                                                   \
+    do {                                                                                  \
+        auto k = relax_fast_kernel<NT, QL>;                                               \
+        if (lds > 64 * 1024) {                                                            \
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),          \
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                    \
+            if (e != hipSuccess) return (int)e;                                           \
+        }                                                                                 \
+        hipLaunchKernelGGL(k, grid, block, lds, st, p);                                   \
+    } while (0)
+    if (p.n <= 64) { if (ql) GB_FAST(1, true); else GB_FAST(1, false); }
+    else if (p.n <= 128) { if (ql) GB_FAST(2, true); else GB_FAST(2, false); }
+    else if (p.n <= 256) GB_FAST(4, false);
+    else if (p.n <= 512) GB_FAST(8, false);
     else return (int)hipErrorInvalidValue;
 #undef GB_FAST
     return (int)hipGetLastError();

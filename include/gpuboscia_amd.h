@@ -152,6 +152,8 @@ typedef struct {
     int refresh_interval;     /* recompute Q x directly every k iterations (0 = never) */
     int waves_per_block;      /* nodes resident per workgroup (1..4) */
     int blocks;               /* persistent workgroups (0 = automatic) */
+    long long* stats;         /* optional device [N][2]: Q-column updates and direct
+                                 n x n products per node (work accounting), or NULL */
 } gb_fast_options;
 
 /*
@@ -206,12 +208,23 @@ typedef void (*gb_branch_score_fn)(void* user, const double* x, int n,
                                    const int* candidates, int count,
                                    int depth, double* score);
 
+/* Custom BranchStrategy (branching.h:13-18): called on the host with the
+   node's bounds and final iterate; writes up to max_children child bound
+   vectors (n each) into child_lb / child_ub and returns their count (0 = the
+   node is not branched). */
+typedef int (*gb_branch_fn)(void* user, const double* lb, const double* ub,
+                            const double* x, int n, int depth, int node_id,
+                            double* child_lb, double* child_ub, int max_children);
+
 typedef enum {
     GB_BRANCH_POW2 = 0,            /* branch_pow2(k) on top_k_infeasible (default k=3) */
     GB_BRANCH_MOST_FRACTIONAL = 1, /* single variable, most_infeasible */
     GB_BRANCH_LEARNED = 2,         /* on-GPU MLP scorer, falls back to most-fractional */
-    GB_BRANCH_CALLBACK = 3         /* host scorer callback, 2-way split */
+    GB_BRANCH_CALLBACK = 3,        /* host scorer callback, 2-way split */
+    GB_BRANCH_CUSTOM = 4           /* host BranchStrategy callback (gb_branch_fn) */
 } gb_branch_rule;
+
+#define GB_MAX_CUSTOM_CHILDREN 64
 
 /* Settings + DeviceResidentConfig (gpuboscia.cuh:21-32,
    device_resident_solver.cuh:1004-1025). */
@@ -241,6 +254,8 @@ typedef struct {
     void* progress_user;
     gb_branch_score_fn branch_score;
     void* branch_score_user;
+    gb_branch_fn branch;      /* GB_BRANCH_CUSTOM */
+    void* branch_user;
     int precision;            /* gb_precision (GB_PREC_FP64 = plain form) */
     int form;                 /* 0 = plain (reference schedule), 1 = batched form */
     int learned_model;        /* handle from gb_brancher_create, or 0 */
