@@ -52,7 +52,7 @@ class AlgorithmDesc(ctypes.Structure):
 
 class FastOptions(ctypes.Structure):
     _fields_ = [("refresh_interval", ctypes.c_int), ("waves_per_block", ctypes.c_int),
-                ("blocks", ctypes.c_int), ("stats", ctypes.c_void_p)]
+                ("blocks", ctypes.c_int), ("stats", ctypes.c_void_p), ("gradient", ctypes.c_void_p)]
 
 
 class Problem(ctypes.Structure):
@@ -84,7 +84,7 @@ class Settings(ctypes.Structure):
                 ("progress", PROGRESS_FN), ("progress_user", ctypes.c_void_p),
                 ("branch_score", BRANCH_SCORE_FN), ("branch_score_user", ctypes.c_void_p),
                 ("branch", BRANCH_FN), ("branch_user", ctypes.c_void_p),
-                ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_int),
+                ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_void_p),
                 ("max_nodes", ctypes.c_longlong)]
 
 
@@ -98,8 +98,13 @@ class Result(ctypes.Structure):
 _lib = None
 
 #: Every symbol that include/gpuboscia_amd.h declares.
-EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_solve", "gb_lmo_relaxation_feasible_cpu",
-            "gb_last_error", "gb_version", "gb_device_count"]
+EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_solve",
+            "gb_lmo_relaxation_feasible_cpu", "gb_last_error", "gb_version", "gb_device_count",
+            "gb_brancher_create", "gb_brancher_destroy", "gb_brancher_set_params",
+            "gb_brancher_get_params", "gb_brancher_set_normalization", "gb_brancher_predict",
+            "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset"]
+MLP_FEATURES = 8
+MLP_PARAMS = 8961
 
 
 class GpuBosciaError(RuntimeError):
@@ -145,6 +150,23 @@ def load():
     lib.gb_lmo_relaxation_feasible_cpu.restype = ctypes.c_int
     lib.gb_lmo_relaxation_feasible_cpu.argtypes = [ctypes.POINTER(LmoDesc), ctypes.c_void_p, _dp,
                                                    _dp, _dp, ctypes.c_int, ctypes.c_double]
+    vp = ctypes.c_void_p
+    lib.gb_brancher_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    lib.gb_brancher_destroy.argtypes = [vp]
+    lib.gb_brancher_set_params.argtypes = [vp, vp, ctypes.c_int]
+    lib.gb_brancher_get_params.argtypes = [vp, vp, ctypes.c_int]
+    lib.gb_brancher_set_normalization.argtypes = [vp, vp, vp]
+    lib.gb_brancher_predict.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+    lib.gb_brancher_loss_grad.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), vp]
+    lib.gb_brancher_train.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+    lib.gb_branching_dataset.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(ObjectiveDesc),
+                                         ctypes.POINTER(LmoDesc), ctypes.POINTER(Settings), ctypes.c_int,
+                                         vp, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    for name in ("gb_brancher_create", "gb_brancher_destroy", "gb_brancher_set_params",
+                 "gb_brancher_get_params", "gb_brancher_set_normalization", "gb_brancher_predict",
+                 "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset"):
+        getattr(lib, name).restype = ctypes.c_int
     lib.gb_last_error.restype = ctypes.c_char_p
     lib.gb_version.restype = ctypes.c_char_p
     lib.gb_device_count.restype = ctypes.c_int
@@ -197,7 +219,7 @@ def relax_batch_fast(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, ma
                      blocks=0, stream=None, stats=None):
     """gb_relax_batch_fast on torch device tensors (stats: optional int64 [N,2])."""
     lib = load()
-    opt = FastOptions(refresh, waves_per_block, blocks, 0 if stats is None else stats.data_ptr())
+    opt = FastOptions(refresh, waves_per_block, blocks, 0 if stats is None else stats.data_ptr(), 0)
     code = lib.gb_relax_batch_fast(ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
                                    int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
                                    _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),
@@ -205,6 +227,98 @@ def relax_batch_fast(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, ma
                                    float(candidate_tol), _ptr(cutoff), ctypes.byref(opt),
                                    ctypes.c_void_p(stream or 0))
     check(code)
+
+
+class Brancher:
+    """The on-GPU MLP branching scorer (gb_brancher_*)."""
+
+    def __init__(self, seed=0):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.gb_brancher_create(ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().gb_brancher_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        import numpy as np
+        out = np.zeros(MLP_PARAMS, dtype=np.float32)
+        check(load().gb_brancher_get_params(self.handle, out.ctypes.data, MLP_PARAMS))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(load().gb_brancher_set_params(self.handle, p.ctypes.data, int(p.size)))
+
+    def set_normalization(self, mu, sigma):
+        import numpy as np
+        mu = np.ascontiguousarray(mu, dtype=np.float32)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float32)
+        check(load().gb_brancher_set_normalization(self.handle, mu.ctypes.data, sigma.ctypes.data))
+        self._norm = (mu.copy(), sigma.copy())
+
+    def save(self, path):
+        """Writes weights and feature normalization to an .npz file."""
+        import numpy as np
+        mu, sigma = getattr(self, "_norm", (np.zeros(MLP_FEATURES, np.float32), np.ones(MLP_FEATURES, np.float32)))
+        np.savez(path, params=self.params(), mu=mu, sigma=sigma)
+
+    @classmethod
+    def from_file(cls, path):
+        """A scorer restored from Brancher.save output (plain arrays, no pickles)."""
+        import numpy as np
+        with np.load(path, allow_pickle=False) as z:
+            b = cls(0)
+            b.set_params(z["params"])
+            b.set_normalization(z["mu"], z["sigma"])
+        return b
+
+    def predict(self, X, Y, stream=None):
+        """X, Y: torch device float32 tensors [M,8], [M]."""
+        check(load().gb_brancher_predict(self.handle, X.data_ptr(), int(X.shape[0]), Y.data_ptr(),
+                                         stream or 0))
+
+    def loss_grad(self, X, T):
+        import numpy as np
+        loss = ctypes.c_double()
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        check(load().gb_brancher_loss_grad(self.handle, X.data_ptr(), T.data_ptr(), int(X.shape[0]),
+                                           ctypes.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def train(self, X, T, epochs, batch, lr):
+        loss = ctypes.c_double()
+        check(load().gb_brancher_train(self.handle, X.data_ptr(), T.data_ptr(), int(X.shape[0]),
+                                       int(epochs), int(batch), ctypes.c_float(lr), ctypes.byref(loss)))
+        return loss.value
+
+
+def branching_dataset(n, lb, ub, int_vars, obj, lmo, settings, max_nodes, capacity):
+    """gb_branching_dataset: strong-branching (features, labels) rows."""
+    import numpy as np
+    lib = load()
+    lb_a = (ctypes.c_double * n)(*[float(v) for v in lb])
+    ub_a = (ctypes.c_double * n)(*[float(v) for v in ub])
+    iv = list(int_vars)
+    iv_a = (ctypes.c_int * max(1, len(iv)))(*iv)
+    prob = Problem(n, lb_a, ub_a, iv_a, len(iv))
+    F = np.zeros((capacity, MLP_FEATURES), dtype=np.float32)
+    L = np.zeros(capacity, dtype=np.float32)
+    cnt = ctypes.c_int()
+    check(lib.gb_branching_dataset(ctypes.byref(prob), ctypes.byref(obj), ctypes.byref(lmo),
+                                   ctypes.byref(settings), int(max_nodes), F.ctypes.data, L.ctypes.data,
+                                   int(capacity), ctypes.byref(cnt)))
+    return F[:cnt.value], L[:cnt.value]
 
 
 def solve(n, lb, ub, int_vars, obj: ObjectiveDesc, lmo: LmoDesc, settings: Settings | None = None,

@@ -23,7 +23,9 @@
 #include <string>
 #include <vector>
 
+#include "gb_fast.h"
 #include "gb_host.h"
+#include "gb_mlp.h"
 
 namespace gb {
 namespace {
@@ -56,6 +58,101 @@ struct HostNode {
     bool has_rx = false;             // warm start present (relaxation_x non-empty)
     int riters = 0;                  // relaxation_iterations
     double bound = -1e300;           // dual_bound
+    // single-variable branching record for pseudocosts
+    int bvar = -1;                   // branched variable (-1: none / multi-variable)
+    int bdir = 0;                    // 0 = down (ub = floor), 1 = up (lb = ceil)
+    double bfrac = 0.0;              // distance moved by the branching bound
+    double pbound = 0.0;             // parent's certified bound
+};
+
+// Pseudocosts: mean certified-bound gain per unit of fractionality removed,
+// per variable and direction, from the children relaxed so far.
+struct Pseudocosts {
+    std::vector<double> up_sum, dn_sum;
+    std::vector<int> up_cnt, dn_cnt;
+    double up_all = 0.0, dn_all = 0.0;
+    int up_n = 0, dn_n = 0;
+    void init(int n) {
+        up_sum.assign(n, 0.0); dn_sum.assign(n, 0.0);
+        up_cnt.assign(n, 0); dn_cnt.assign(n, 0);
+    }
+    void record(int var, int dir, double gain_per_unit) {
+        if (dir) { up_sum[var] += gain_per_unit; ++up_cnt[var]; up_all += gain_per_unit; ++up_n; }
+        else { dn_sum[var] += gain_per_unit; ++dn_cnt[var]; dn_all += gain_per_unit; ++dn_n; }
+    }
+    // relative pseudocost: variable mean / mean over all records (1 without history)
+    double up(int v) const {
+        if (up_n == 0) return 1.0;
+        const double mean = up_all / up_n;
+        const double pv = up_cnt[v] ? up_sum[v] / up_cnt[v] : mean;
+        return pv / (mean + 1e-12);
+    }
+    double dn(int v) const {
+        if (dn_n == 0) return 1.0;
+        const double mean = dn_all / dn_n;
+        const double pv = dn_cnt[v] ? dn_sum[v] / dn_cnt[v] : mean;
+        return pv / (mean + 1e-12);
+    }
+};
+
+// The 8 scorer features of candidate variables (order documented in
+// gpuboscia_amd.h next to GB_MLP_FEATURES).  Features 3 and 4 are the
+// estimated bound gains of the up and down branches: pseudocost (mean gain
+// per unit from relaxed children) times the distance moved, or, without
+// history for the variable, the quadratic's own estimate 1/2 s_g Q_ii d^2;
+// each is divided by its mean over the node's candidates.
+void candidate_features(const double* x, const double* g, const double* lb, const double* ub, int n,
+                        int depth, double gap, double fval, const Pseudocosts& pc,
+                        const std::vector<double>& qdiag, double gs, const std::vector<int>& cands,
+                        float* out) {
+    double gmax = 0.0;
+    for (int j = 0; j < n; ++j) gmax = std::max(gmax, std::fabs(g[j]));
+    const double rgap = gap / (1.0 + std::fabs(fval));
+    std::vector<double> gu(cands.size()), gd(cands.size());
+    double su = 0.0, sd = 0.0;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        const int v = cands[c];
+        const double du = std::ceil(x[v]) - x[v], dd = x[v] - std::floor(x[v]);
+        const double curv = 0.5 * gs * (qdiag.empty() ? 0.0 : qdiag[v]);
+        gu[c] = (pc.up_cnt[v] ? pc.up_sum[v] / pc.up_cnt[v] : curv * du) * du;
+        gd[c] = (pc.dn_cnt[v] ? pc.dn_sum[v] / pc.dn_cnt[v] : curv * dd) * dd;
+        su += gu[c];
+        sd += gd[c];
+    }
+    const double mu = su / (cands.empty() ? 1.0 : (double)cands.size());
+    const double md = sd / (cands.empty() ? 1.0 : (double)cands.size());
+    for (size_t c = 0; c < cands.size(); ++c) {
+        const int v = cands[c];
+        const double frac = x[v] - std::floor(x[v]);
+        float* f = out + c * kMlpFeatures;
+        f[0] = (float)x[v];
+        f[1] = (float)std::min(frac, 1.0 - frac);
+        f[2] = (float)(g[v] / (gmax + 1e-12));
+        f[3] = (float)(gu[c] / (mu + 1e-12));
+        f[4] = (float)(gd[c] / (md + 1e-12));
+        f[5] = (float)depth / (float)n;
+        f[6] = (float)rgap;
+        f[7] = (float)(ub[v] - lb[v]);
+    }
+}
+
+std::vector<int> fractional_candidates(const double* x, const std::vector<int>& iv, double tol) {
+    std::vector<int> c;
+    for (int v : iv) {
+        const double frac = x[v] - std::floor(x[v]);
+        if (std::min(frac, 1.0 - frac) > tol) c.push_back(v);
+    }
+    return c;
+}
+
+// Strong-branching label collection (gb_branching_dataset).
+struct Labeler {
+    int max_nodes = 0;
+    float* features = nullptr;
+    float* labels = nullptr;
+    int capacity = 0;
+    int count = 0;
+    int labeled = 0;
 };
 
 struct HeapOrder {
@@ -130,6 +227,20 @@ struct DeviceBatch {
     int* hit = nullptr;
     uint8_t* hst = nullptr;
     hipStream_t stream = nullptr;
+    // learned branching / labelling: gradients, feature rows, scores
+    double *g = nullptr, *hg = nullptr;
+    float *feat = nullptr, *hfeat = nullptr, *score = nullptr, *hscore = nullptr;
+
+    gb_status alloc_features() {
+        const size_t xn = (size_t)cap * n;
+        GB_HIP(hipMalloc(&g, xn * 8));
+        GB_HIP(hipHostMalloc(&hg, xn * 8, hipHostMallocDefault));
+        GB_HIP(hipMalloc(&feat, xn * kMlpFeatures * 4));
+        GB_HIP(hipHostMalloc(&hfeat, xn * kMlpFeatures * 4, hipHostMallocDefault));
+        GB_HIP(hipMalloc(&score, xn * 4));
+        GB_HIP(hipHostMalloc(&hscore, xn * 4, hipHostMallocDefault));
+        return GB_OK;
+    }
 
     gb_status alloc(int capacity, int nv) {
         cap = capacity;
@@ -164,10 +275,12 @@ struct DeviceBatch {
     ~DeviceBatch() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : {(void*)x, (void*)lb, (void*)ub, (void*)bv, (void*)eps, (void*)gap,
-                        (void*)val, (void*)bvv, (void*)cut, (void*)it, (void*)st, (void*)flags})
+                        (void*)val, (void*)bvv, (void*)cut, (void*)it, (void*)st, (void*)flags,
+                        (void*)g, (void*)feat, (void*)score})
             if (p) (void)hipFree(p);
         for (void* p : {(void*)hx, (void*)hlb, (void*)hub, (void*)hbv, (void*)heps, (void*)hgap,
-                        (void*)hval, (void*)hbvv, (void*)hcut, (void*)hit, (void*)hst})
+                        (void*)hval, (void*)hbvv, (void*)hcut, (void*)hit, (void*)hst, (void*)hg,
+                        (void*)hfeat, (void*)hscore})
             if (p) (void)hipHostFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -204,6 +317,8 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown branch rule");
     if (s->branch_rule == GB_BRANCH_CUSTOM && !s->branch)
         return fail(GB_ERR_INVALID_ARGUMENT, "custom branching requires a branch callback");
+    if (s->branch_rule == GB_BRANCH_LEARNED && s->learned_model && o->kind != GB_OBJ_DENSE_QUADRATIC)
+        return fail(GB_ERR_INVALID_ARGUMENT, "the learned scorer needs a dense quadratic objective");
     if (s->branch_rule == GB_BRANCH_POW2 && (s->branch_k < 1 || s->branch_k > 16))
         return fail(GB_ERR_INVALID_ARGUMENT, "branch_k must be in [1, 16]");
     if (s->branch_rule == GB_BRANCH_CALLBACK && !s->branch_score)
@@ -240,8 +355,88 @@ gb_status validate_root(const gb_problem* pr, const gb_lmo_desc* l, const gb_set
     return GB_OK;
 }
 
+// Strong branching at one node: both children of every fractional candidate
+// relaxed in one batched-form launch; appends (features, normalized product
+// score) rows.
+gb_status strong_branch_labels(const gb_objective_desc* o, const gb_lmo_desc* l, const gb_settings* s,
+                               const HostNode& nd, const double* xi, double fval, double gap,
+                               NodePool& pool, const Pseudocosts& pc, const std::vector<double>& qdiag,
+                               const std::vector<int>& iv, DeviceBatch& sb, Labeler& lab) {
+    const int n = pool.n;
+    const std::vector<int> cands = fractional_candidates(xi, iv, s->integer_tol);
+    if (cands.empty()) return GB_OK;
+    const int C = (int)cands.size();
+    const int N = 2 * C;
+    const double* plb = pool.lb(nd.slot);
+    const double* pub = pool.ub(nd.slot);
+    const double eps = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth + 1);
+    for (int c = 0; c < C; ++c)
+        for (int d = 0; d < 2; ++d) {
+            const int r = 2 * c + d;
+            double* lb = sb.hlb + (size_t)r * n;
+            double* ub = sb.hub + (size_t)r * n;
+            std::copy(plb, plb + n, lb);
+            std::copy(pub, pub + n, ub);
+            const int v = cands[c];
+            if (d) lb[v] = std::ceil(xi[v]);
+            else ub[v] = std::floor(xi[v]);
+            for (int j = 0; j < n; ++j) sb.hx[(size_t)r * n + j] = std::min(std::max(xi[j], lb[j]), ub[j]);
+            sb.heps[r] = eps;
+            sb.hit[r] = 0;
+            sb.hst[r] = GB_NODE_INITIAL_POINT_SUPPLIED;
+        }
+    const size_t xb = (size_t)N * n * 8;
+    GB_HIP(hipMemcpyAsync(sb.x, sb.hx, xb, hipMemcpyHostToDevice, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.lb, sb.hlb, xb, hipMemcpyHostToDevice, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.ub, sb.hub, xb, hipMemcpyHostToDevice, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.eps, sb.heps, N * 8, hipMemcpyHostToDevice, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.it, sb.hit, N * 4, hipMemcpyHostToDevice, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.st, sb.hst, N, hipMemcpyHostToDevice, sb.stream));
+    const gb_fast_options fo{kFastRefresh, 4, 0, nullptr, nullptr};
+    gb_status st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, sb.x, sb.lb, sb.ub, sb.eps,
+                                    sb.it, sb.st, sb.gap, sb.val, sb.bv, sb.bvv, sb.flags,
+                                    s->integer_tol, nullptr, &fo, sb.stream);
+    if (st != GB_OK) return st;
+    // gradient at the parent's iterate for the feature rows
+    GB_HIP(hipMemcpyAsync(sb.g, xi, (size_t)n * 8, hipMemcpyHostToDevice, sb.stream));
+    const int ge = launch_batched_gradient(o->Qt, o->linear, o->grad_scale, o->linear_scale, n, 1, sb.g,
+                                           sb.g + n, sb.stream);
+    if (ge != 0) return hip_fail(static_cast<hipError_t>(ge), __FILE__, __LINE__);
+    GB_HIP(hipMemcpyAsync(sb.hgap, sb.gap, N * 8, hipMemcpyDeviceToHost, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.hval, sb.val, N * 8, hipMemcpyDeviceToHost, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.hst, sb.st, N, hipMemcpyDeviceToHost, sb.stream));
+    GB_HIP(hipMemcpyAsync(sb.hg, sb.g + n, (size_t)n * 8, hipMemcpyDeviceToHost, sb.stream));
+    GB_HIP(hipStreamSynchronize(sb.stream));
+    const double big = 1e3 * (1.0 + std::fabs(fval));
+    std::vector<double> score(C);
+    double smax = 0.0;
+    for (int c = 0; c < C; ++c) {
+        double gain[2];
+        for (int d = 0; d < 2; ++d) {
+            const int r = 2 * c + d;
+            if (sb.hst[r] == GB_NODE_INFEASIBLE) { gain[d] = big; continue; }
+            const double b = certified_fw_lower_bound(nd.bound, sb.hval[r], sb.hgap[r]);
+            gain[d] = std::isfinite(b) && std::isfinite(nd.bound) ? std::min(big, std::max(0.0, b - nd.bound))
+                                                                   : 0.0;
+        }
+        score[c] = std::max(gain[0], 1e-6) * std::max(gain[1], 1e-6);
+        smax = std::max(smax, score[c]);
+    }
+    std::vector<float> rows((size_t)C * kMlpFeatures);
+    candidate_features(xi, sb.hg, plb, pub, n, nd.depth, gap, fval, pc, qdiag, o->grad_scale, cands,
+                       rows.data());
+    for (int c = 0; c < C && lab.count < lab.capacity; ++c) {
+        std::copy(rows.begin() + (size_t)c * kMlpFeatures, rows.begin() + (size_t)(c + 1) * kMlpFeatures,
+                  lab.features + (size_t)lab.count * kMlpFeatures);
+        lab.labels[lab.count] = (float)(smax > 0.0 ? score[c] / smax : 0.0);
+        ++lab.count;
+    }
+    ++lab.labeled;
+    return GB_OK;
+}
+
 gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_lmo_desc* l,
-                     const gb_settings* s, gb_result* res) {
+                     const gb_settings* s, gb_result* res, Labeler* lab = nullptr) {
     gb_status st = validate_settings(pr, o, l, s);
     if (st != GB_OK) return st;
     st = validate_descriptors(o, &s->algorithm, l);
@@ -258,9 +453,32 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     DeviceBatch db;
     st = db.alloc(B, n);
     if (st != GB_OK) return st;
+    const bool learned = s->branch_rule == GB_BRANCH_LEARNED && s->learned_model != nullptr;
+    if (learned || lab) {
+        st = db.alloc_features();
+        if (st != GB_OK) return st;
+    }
+    // Child batch of the strong-branching labeller (2 children per candidate).
+    DeviceBatch sb;
+    if (lab) {
+        st = sb.alloc(2 * n, n);
+        if (st != GB_OK) return st;
+        st = sb.alloc_features();
+        if (st != GB_OK) return st;
+    }
+    Pseudocosts pc;
+    pc.init(n);
+    // Q diagonal on the host (feature estimate of branch gains)
+    std::vector<double> qdiag;
+    if ((learned || lab) && o->kind == GB_OBJ_DENSE_QUADRATIC) {
+        qdiag.resize(n);
+        GB_HIP(hipMemcpy2D(qdiag.data(), sizeof(double), o->Qt, (size_t)(n + 1) * sizeof(double),
+                           sizeof(double), n, hipMemcpyDeviceToHost));
+    }
     std::vector<uint8_t> hflags(n, 0);
     for (int i : iv) hflags[i] = 1;
     GB_HIP(hipMemcpy(db.flags, hflags.data(), n, hipMemcpyHostToDevice));
+    if (lab) GB_HIP(hipMemcpy(sb.flags, hflags.data(), n, hipMemcpyHostToDevice));
 
     std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
     NodePool pool;
@@ -346,7 +564,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         if (s->form == 1) {
             for (int i = 0; i < N; ++i) db.hcut[i] = inc;
             GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
-            const gb_fast_options fo{kFastRefresh, 4, 0, nullptr};
+            const gb_fast_options fo{kFastRefresh, 4, 0, nullptr, learned ? db.g : nullptr};
             st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
                                   s->integer_tol, db.cut, &fo, db.stream);
@@ -363,8 +581,46 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         GB_HIP(hipMemcpyAsync(db.hbvv, db.bvv, N * 8, hipMemcpyDeviceToHost, db.stream));
         GB_HIP(hipMemcpyAsync(db.hit, db.it, N * 4, hipMemcpyDeviceToHost, db.stream));
         GB_HIP(hipMemcpyAsync(db.hst, db.st, N, hipMemcpyDeviceToHost, db.stream));
+        if (learned) {
+            if (s->form != 1) {
+                const int ge = launch_batched_gradient(o->Qt, o->linear, o->grad_scale, o->linear_scale,
+                                                       n, N, db.x, db.g, db.stream);
+                if (ge != 0) return hip_fail(static_cast<hipError_t>(ge), __FILE__, __LINE__);
+            }
+            GB_HIP(hipMemcpyAsync(db.hg, db.g, (size_t)N * n * 8, hipMemcpyDeviceToHost, db.stream));
+        }
         GB_HIP(hipStreamSynchronize(db.stream));
         relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
+
+        // Learned scores for every fractional candidate of the batch: one
+        // gradient launch, host feature rows, one MLP launch.
+        std::vector<int> score_off(N + 1, 0);
+        std::vector<std::vector<int>> score_cand(learned ? N : 0);
+        if (learned) {
+            int rows = 0;
+            for (int i = 0; i < N; ++i) {
+                score_off[i] = rows;
+                if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
+                    const double* xi = db.hx + (size_t)i * n;
+                    score_cand[i] = fractional_candidates(xi, iv, s->integer_tol);
+                    candidate_features(xi, db.hg + (size_t)i * n, pool.lb(batch[i].slot),
+                                       pool.ub(batch[i].slot), n, batch[i].depth, db.hgap[i], db.hval[i],
+                                       pc, qdiag, o->grad_scale, score_cand[i],
+                                       db.hfeat + (size_t)rows * kMlpFeatures);
+                    rows += (int)score_cand[i].size();
+                }
+            }
+            score_off[N] = rows;
+            if (rows > 0) {
+                GB_HIP(hipMemcpyAsync(db.feat, db.hfeat, (size_t)rows * kMlpFeatures * 4,
+                                      hipMemcpyHostToDevice, db.stream));
+                st = gb_brancher_predict(s->learned_model, db.feat, rows, db.score, db.stream);
+                if (st != GB_OK) return st;
+                GB_HIP(hipMemcpyAsync(db.hscore, db.score, (size_t)rows * 4, hipMemcpyDeviceToHost,
+                                      db.stream));
+                GB_HIP(hipStreamSynchronize(db.stream));
+            }
+        }
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
         for (int i = 0; i < N && !finished; ++i) {
@@ -381,6 +637,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             const double node_eps = db.heps[i];
             const double g = db.hgap[i];
             nd.bound = certified_fw_lower_bound(nd.bound, db.hval[i], g);
+            if (nd.bvar >= 0 && nd.riters == 0 && nd.bfrac > 0.0 && std::isfinite(nd.bound)
+                    && std::isfinite(nd.pbound))
+                pc.record(nd.bvar, nd.bdir, std::max(0.0, nd.bound - nd.pbound) / nd.bfrac);
             bool new_inc = false;
             const double bvv = db.hbvv[i];
             if (bvv < 1e299 && std::isfinite(bvv) && bvv < inc) {
@@ -447,9 +706,23 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                             if (score[c] > score[bi]) bi = c;
                         vars.push_back(cand[bi]);
                     }
+                } else if (learned && score_off[i + 1] > score_off[i]) {
+                    // highest predicted strong-branching score, ties to the lowest index
+                    int bi = 0;
+                    const float* sc = db.hscore + score_off[i];
+                    const int cnt = score_off[i + 1] - score_off[i];
+                    for (int c = 1; c < cnt; ++c)
+                        if (sc[c] > sc[bi]) bi = c;
+                    vars.push_back(score_cand[i][bi]);
                 } else {
+                    // GB_BRANCH_MOST_FRACTIONAL, and the learned rule without a model
                     const int v = most_fractional(xi, iv, s->integer_tol);
                     if (v >= 0) vars.push_back(v);
+                }
+                if (lab && lab->labeled < lab->max_nodes) {
+                    st = strong_branch_labels(o, l, s, nd, xi, db.hval[i], db.hgap[i], pool, pc, qdiag,
+                                              iv, sb, *lab);
+                    if (st != GB_OK) return st;
                 }
                 const int base = next_id;
                 next_id += 8;  // device_resident_solver.cuh:1989
@@ -478,6 +751,13 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         double* crx = pool.rx(c.slot);
                         for (int j = 0; j < n; ++j) crx[j] = std::min(std::max(xi[j], clb[j]), cub[j]);
                         c.has_rx = true;
+                    }
+                    if (k == 1) {
+                        const int vi = vars[0];
+                        c.bvar = vi;
+                        c.bdir = mask & 1;
+                        c.bfrac = c.bdir ? std::ceil(xi[vi]) - xi[vi] : xi[vi] - std::floor(xi[vi]);
+                        c.pbound = nd.bound;
                     }
                     q.push(c);
                 }
@@ -532,6 +812,33 @@ extern "C" void gb_default_settings(gb_settings* s) {
     s->natural_lower_bound = -std::numeric_limits<double>::infinity();
     s->precision = GB_PREC_FP64;
     s->max_nodes = 0;
+}
+
+extern "C" gb_status gb_branching_dataset(const gb_problem* problem, const gb_objective_desc* objective,
+                                          const gb_lmo_desc* lmo, const gb_settings* settings,
+                                          int max_nodes, float* features, float* labels, int capacity,
+                                          int* count) {
+    if (!settings || !features || !labels || !count || capacity < 0 || max_nodes < 0)
+        return gb::fail(GB_ERR_INVALID_ARGUMENT, "invalid dataset arguments");
+    if (!objective || objective->kind != GB_OBJ_DENSE_QUADRATIC || !lmo || lmo->kind != GB_LMO_BOX)
+        return gb::fail(GB_ERR_INVALID_ARGUMENT, "strong-branching labels need a dense objective over a box");
+    gb_settings s = *settings;
+    s.form = 1;
+    s.branch_rule = GB_BRANCH_MOST_FRACTIONAL;
+    gb::Labeler lab;
+    lab.max_nodes = max_nodes;
+    lab.features = features;
+    lab.labels = labels;
+    lab.capacity = capacity;
+    gb_result r{};
+    gb_status st;
+    try {
+        st = gb::solve_impl(problem, objective, lmo, &s, &r, &lab);
+    } catch (const std::exception& e) {
+        return gb::fail(GB_ERR_RUNTIME, e.what());
+    }
+    *count = lab.count;
+    return st;
 }
 
 extern "C" gb_status gb_solve(const gb_problem* problem, const gb_objective_desc* objective,

@@ -203,7 +203,7 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
             || !best_vertex_val)
         return fail(GB_ERR_INVALID_ARGUMENT, "null node buffer");
-    gb_fast_options op{64, 4, 0, nullptr};
+    gb_fast_options op{64, 4, 0, nullptr, nullptr};
     if (opt) op = *opt;
     if (op.waves_per_block < 1 || op.waves_per_block > 4 || op.refresh_interval < 0 || op.blocks < 0)
         return fail(GB_ERR_INVALID_ARGUMENT, "invalid batched-form options");
@@ -237,6 +237,7 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     P.cutoff = cutoff;
     P.counter = counter;
     P.stats = op.stats;
+    P.gout = op.gradient;
     int blocks = op.blocks;
     if (blocks == 0) {
         const int needed = (N + op.waves_per_block - 1) / op.waves_per_block;

@@ -30,8 +30,11 @@ struct FastParams {
     const double* cutoff;   // per node, may be NULL (= +inf)
     int* counter;           // work queue head (reset by the launcher)
     long long* stats;       // optional [N][2]: column updates, direct GEMVs (may be NULL)
+    double* gout;           // optional [N][n]: gradient at the returned iterate (may be NULL)
 };
 
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream);
+int launch_batched_gradient(const double* Qt, const double* lin, double gs, double ls, int n, int N,
+                            const double* X, double* G, void* stream);
 
 }  // namespace gb

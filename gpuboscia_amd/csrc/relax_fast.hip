@@ -345,6 +345,14 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
         // Direct FP64 certificate at the returned iterate.
         W.gemv(x, qx);
         grad();
+        if (P.gout) {
+            double* go = P.gout + (size_t)node * n;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                if (j < n) go[j] = g[t];
+            }
+        }
         fwd(v);
         W.gemv(v, qv);
         score(v, qv);
@@ -393,9 +401,32 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
 
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
 
+// G[i] = s_g (Q x_i) + s_g s_l c for a batch of iterates: one wave per node,
+// lane rows j, k-ordered FMA chain (features of the learned scorer).
+__global__ void __launch_bounds__(64) batched_gradient_kernel(const double* Qt, const double* lin,
+                                                              double gs, double ls, int n, int N,
+                                                              const double* X, double* G) {
+    const int node = blockIdx.x;
+    if (node >= N) return;
+    const double* x = X + (size_t)node * n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double acc = 0.0;
+        for (int k = 0; k < n; ++k) acc = fma(Qt[(size_t)k * n + j], x[k], acc);
+        G[(size_t)node * n + j] = gs * acc + (lin ? (gs * ls) * lin[j] : 0.0);
+    }
+}
+
 constexpr size_t kFastLdsBudget = 160 * 1024;
 
 }  // namespace
+
+int launch_batched_gradient(const double* Qt, const double* lin, double gs, double ls, int n, int N,
+                            const double* X, double* G, void* stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(batched_gradient_kernel, dim3(N), dim3(64), 0,
+                       reinterpret_cast<hipStream_t>(stream), Qt, lin, gs, ls, n, N, X, G);
+    return (int)hipGetLastError();
+}
 
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);

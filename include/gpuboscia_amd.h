@@ -154,6 +154,7 @@ typedef struct {
     int blocks;               /* persistent workgroups (0 = automatic) */
     long long* stats;         /* optional device [N][2]: Q-column updates and direct
                                  n x n products per node (work accounting), or NULL */
+    double* gradient;         /* optional device [N][n]: gradient at the returned x, or NULL */
 } gb_fast_options;
 
 /*
@@ -178,6 +179,42 @@ gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
                               const uint8_t* integer_flags, double candidate_tol,
                               const double* cutoff, const gb_fast_options* options,
                               void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Learned branching scorer (no reference counterpart; the reference       */
+/* branches on the most fractional variables, branching.h:24-118).          */
+/* ---------------------------------------------------------------------- */
+
+/* 8 -> 64 -> 64 -> 64 -> 1 ReLU MLP; parameters as floats
+   W1[8][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4. */
+#define GB_MLP_FEATURES 8
+#define GB_MLP_PARAMS 8961
+
+/* Per-candidate features, in order:
+   0 x_i, 1 fractionality min(x_i - floor, ceil - x_i), 2 g_i / max_j |g_j|,
+   3 pseudocost-up gain, 4 pseudocost-down gain: mean bound gain per unit
+   from the variable's relaxed children (or, without history, the
+   quadratic estimate 1/2 s_g Q_ii d) times the distance d moved, divided
+   by its mean over the node's candidates; 5 depth / n,
+   6 FW gap / (1 + |f(x)|), 7 ub_i - lb_i. */
+
+typedef struct gb_brancher gb_brancher;
+
+gb_status gb_brancher_create(unsigned long long seed, gb_brancher** out);
+gb_status gb_brancher_destroy(gb_brancher* b);
+gb_status gb_brancher_set_params(gb_brancher* b, const float* params, int count);
+gb_status gb_brancher_get_params(gb_brancher* b, float* params, int count);
+/* Feature standardization (x - mu) / sigma applied inside the network. */
+gb_status gb_brancher_set_normalization(gb_brancher* b, const float* mu, const float* sigma);
+/* Scores for M feature rows (device float [M][8] -> device float [M]). */
+gb_status gb_brancher_predict(gb_brancher* b, const float* X, int M, float* Y, void* stream);
+/* Mean squared error and its gradient (host float [GB_MLP_PARAMS], may be
+   NULL) on device data; used for gradient checks. */
+gb_status gb_brancher_loss_grad(gb_brancher* b, const float* X, const float* T, int M,
+                                double* loss, float* grad_host);
+/* Adam training on device data (X [M][8], T [M]), mini-batches in order. */
+gb_status gb_brancher_train(gb_brancher* b, const float* X, const float* T, int M, int epochs,
+                            int batch, float lr, double* final_loss);
 
 /* ---------------------------------------------------------------------- */
 /* Branch-and-bound (gpuboscia::solve, gpuboscia.cuh:56-170)               */
@@ -258,7 +295,7 @@ typedef struct {
     void* branch_user;
     int precision;            /* gb_precision (GB_PREC_FP64 = plain form) */
     int form;                 /* 0 = plain (reference schedule), 1 = batched form */
-    int learned_model;        /* handle from gb_brancher_create, or 0 */
+    gb_brancher* learned_model;   /* GB_BRANCH_LEARNED scorer, or NULL (most fractional) */
     long long max_nodes;      /* safety cap; <= 0 = none (no reference counterpart) */
 } gb_settings;
 
@@ -286,6 +323,18 @@ gb_status gb_solve(const gb_problem* problem,
                    const gb_lmo_desc* lmo,
                    const gb_settings* settings,
                    gb_result* result);
+
+/* Strong-branching labels generated by the solver itself: runs the batched
+   B&B with most-fractional branching and, for the first max_nodes branched
+   nodes, relaxes both children of every fractional candidate on the GPU.
+   Each candidate yields a feature row (host float [8]) and the label
+   score_i / max_j score_j with score = max(dn, 1e-6) * max(up, 1e-6) over
+   the certified child-bound improvements.  Rows are appended up to
+   capacity; *count receives the number written. */
+gb_status gb_branching_dataset(const gb_problem* problem, const gb_objective_desc* objective,
+                               const gb_lmo_desc* lmo, const gb_settings* settings,
+                               int max_nodes, float* features, float* labels, int capacity,
+                               int* count);
 
 /* Host-side continuous feasibility check of the built-in LMOs
    (LMO::relaxation_feasible_cpu, lmo.cuh:30-35). is_int is a host array for

@@ -1,0 +1,109 @@
+"""Learned branching on the GPU: the MLP kernels against the CPU reference
+(forward values and gradients), training by the library's own kernels, the
+solver's strong-branching labels, and the learned rule reaching the same
+proven optimum as most-fractional branching."""
+import sys
+
+import numpy as np
+import pytest
+
+import gpuboscia_amd as gb
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def tensors(X, T=None):
+    import torch
+    dX = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
+    dT = None if T is None else torch.from_numpy(np.ascontiguousarray(T, dtype=np.float32)).cuda()
+    return dX, dT
+
+
+def test_forward_and_gradient_match_cpu_reference():
+    import torch
+    from oracle import mlp_oracle as mo
+    rng = np.random.default_rng(1)
+    M = 1000  # not a multiple of 16: exercises the ragged tail tile
+    X = rng.standard_normal((M, 8)).astype(np.float32) * 2 + 0.5
+    T = rng.random(M).astype(np.float32)
+    mu = X.mean(0).astype(np.float32)
+    sigma = X.std(0).astype(np.float32)
+    b = gb.Brancher(seed=7)
+    b.set_normalization(mu, sigma)
+    p = b.params()
+    dX, dT = tensors(X, T)
+    Y = torch.zeros(M, dtype=torch.float32, device="cuda")
+    b.predict(dX, Y)
+    torch.cuda.synchronize()
+    y_ref, _ = mo.forward(p, X, mu, sigma)
+    y = Y.cpu().numpy()
+    # same BF16 operands, FP32 accumulation in a different order
+    assert np.allclose(y, y_ref, rtol=1e-4, atol=1e-4), np.abs(y - y_ref).max()
+    loss, g = b.loss_grad(dX, dT)
+    loss_ref, g_ref = mo.loss_grad(p, X, T, mu, sigma)
+    assert abs(loss - loss_ref) <= 1e-5 * max(1.0, loss_ref)
+    scale = np.abs(g_ref).max()
+    assert np.allclose(g, g_ref, rtol=2e-3, atol=2e-4 * scale), np.abs(g - g_ref).max() / scale
+    # BF16 weights: the rounded network is close to the exact one
+    y64, _ = mo.forward(p, X, mu, sigma, rounding=False)
+    assert np.abs(y - y64).max() <= 0.05 * (1 + np.abs(y64).max())
+
+
+def test_training_reduces_loss_deterministically():
+    rng = np.random.default_rng(2)
+    M = 4096
+    X = rng.standard_normal((M, 8)).astype(np.float32)
+    T = (np.tanh(X[:, 0] - 0.5 * X[:, 1]) * 0.5 + 0.5).astype(np.float32)
+    dX, dT = tensors(X, T)
+    losses = []
+    for rep in range(2):
+        b = gb.Brancher(seed=3)
+        b.set_normalization(np.zeros(8), np.ones(8))
+        l0, _ = b.loss_grad(dX, dT)
+        l1 = b.train(dX, dT, epochs=20, batch=512, lr=3e-3)
+        losses.append((l0, l1, b.params().copy()))
+        say(f"[mlp] train rep {rep}: loss {l0:.4f} -> {l1:.5f}")
+    assert losses[0][1] < 0.2 * losses[0][0]
+    assert losses[0][1] == losses[1][1] and np.array_equal(losses[0][2], losses[1][2])
+
+
+def test_strong_branching_labels_and_learned_search():
+    from gpuboscia_amd import learned
+    n = 30
+    X, T = learned.generate(n, seeds=range(100, 124), labeled_nodes=8)
+    say(f"[learned] rows={len(T)}")
+    assert len(T) > 100
+    assert np.all((T >= 0) & (T <= 1)) and np.isclose(T.max(), 1.0)
+    assert np.all(np.isfinite(X))
+    # each labelled node has its best candidate at 1.0
+    b, info = learned.fit(X, T, epochs=40, batch=512, lr=3e-3)
+    say(f"[learned] fit {info['initial_loss']:.4f} -> {info['final_loss']:.4f}")
+    assert info["final_loss"] < info["initial_loss"]
+    rows = learned.evaluate(b, n, seeds=range(900, 906))
+    for r in rows:
+        say(f"[learned] seed {r['seed']}: mf {r['mf_nodes']} nodes, learned {r['learned_nodes']} nodes")
+        assert r["proven"] and r["same_optimum"]
+
+
+def test_scorer_save_and_restore(tmp_path):
+    import torch
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((300, 8)).astype(np.float32)
+    b = gb.Brancher(seed=11)
+    b.set_normalization(X.mean(0), X.std(0))
+    path = tmp_path / "scorer.npz"
+    b.save(path)
+    r = gb.Brancher.from_file(path)
+    assert np.array_equal(r.params(), b.params())
+    dX, _ = tensors(X)
+    y1 = torch.zeros(300, dtype=torch.float32, device="cuda")
+    y2 = torch.zeros_like(y1)
+    b.predict(dX, y1)
+    r.predict(dX, y2)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
