@@ -187,7 +187,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(B&B nodes would shard across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(B&B open nodes would shard across GPUs with a min-allreduce of incumbent "
+              "and bound per round)", file=sys.stderr)
         sys.exit(2)
     import gpuboscia_amd as gb
     from gpuboscia_amd import instances
