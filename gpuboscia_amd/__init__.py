@@ -55,6 +55,13 @@ class FastOptions(ctypes.Structure):
                 ("blocks", ctypes.c_int), ("stats", ctypes.c_void_p), ("gradient", ctypes.c_void_p)]
 
 
+class MixedOptions(ctypes.Structure):
+    _fields_ = [("row_tiles_per_wave", ctypes.c_int), ("columns", ctypes.c_int), ("blocks", ctypes.c_int),
+                ("rel_floor", ctypes.c_float),
+                ("fp64_max_iters", ctypes.c_int), ("mixed_iterations", ctypes.c_void_p),
+                ("fp64_stats", ctypes.c_void_p)]
+
+
 class Problem(ctypes.Structure):
     _fields_ = [("n_vars", ctypes.c_int), ("lb", _dp), ("ub", _dp), ("int_vars", _ip),
                 ("n_int_vars", ctypes.c_int)]
@@ -102,7 +109,10 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_lmo_relaxation_feasible_cpu", "gb_last_error", "gb_version", "gb_device_count",
             "gb_brancher_create", "gb_brancher_destroy", "gb_brancher_set_params",
             "gb_brancher_get_params", "gb_brancher_set_normalization", "gb_brancher_predict",
-            "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset"]
+            "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset",
+            "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
+            "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
+            "gb_relax_batch_mixed"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -167,6 +177,23 @@ def load():
                  "gb_brancher_get_params", "gb_brancher_set_normalization", "gb_brancher_predict",
                  "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset"):
         getattr(lib, name).restype = ctypes.c_int
+    lib.gb_relax_graph_create.argtypes = lib.gb_relax_batch_fast.argtypes[:-1] + [ctypes.POINTER(vp)]
+    lib.gb_relax_graph_launch.argtypes = [vp, vp]
+    lib.gb_relax_graph_destroy.argtypes = [vp]
+    for name in ("gb_relax_graph_create", "gb_relax_graph_launch", "gb_relax_graph_destroy"):
+        getattr(lib, name).restype = ctypes.c_int
+    lib.gb_mixed_prepare.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.c_int, vp, ctypes.POINTER(vp)]
+    lib.gb_mixed_destroy.argtypes = [vp]
+    lib.gb_mixed_products.argtypes = [vp, vp, ctypes.c_int, vp, vp]
+    lib.gb_relax_batch_mixed.argtypes = [vp, ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
+                                         ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
+        + [vp] * 11 + [ctypes.c_double, vp, ctypes.POINTER(MixedOptions), vp]
+    for name in ("gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products", "gb_relax_batch_mixed"):
+        getattr(lib, name).restype = ctypes.c_int
+    ll = ctypes.POINTER(ctypes.c_longlong)
+    lib.gb_grid_barrier_selftest.argtypes = [ctypes.c_int] * 5 + [ctypes.c_longlong, ll, ll,
+                                                                  ctypes.POINTER(ctypes.c_double)]
+    lib.gb_grid_barrier_selftest.restype = ctypes.c_int
     lib.gb_last_error.restype = ctypes.c_char_p
     lib.gb_version.restype = ctypes.c_char_p
     lib.gb_device_count.restype = ctypes.c_int
@@ -227,6 +254,85 @@ def relax_batch_fast(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, ma
                                    float(candidate_tol), _ptr(cutoff), ctypes.byref(opt),
                                    ctypes.c_void_p(stream or 0))
     check(code)
+
+
+class MixedQ:
+    """gb_mixed_*: Q of a dense objective split into bf16 planes for the
+    mixed-precision form (planes 1 or 3)."""
+
+    def __init__(self, obj: ObjectiveDesc, planes=3, stream=None):
+        lib = load()
+        self._obj = obj
+        h = ctypes.c_void_p()
+        check(lib.gb_mixed_prepare(ctypes.byref(obj), int(planes), ctypes.c_void_p(stream or 0),
+                                   ctypes.byref(h)))
+        self.handle = h
+        self.planes = planes
+
+    def products(self, V, Y, stream=None):
+        """Y = Q V through the MFMA fragments (torch float64 [N, n])."""
+        check(load().gb_mixed_products(self.handle, _ptr(V), int(V.shape[0]), _ptr(Y),
+                                       ctypes.c_void_p(stream or 0)))
+
+    def close(self):
+        if self.handle:
+            load().gb_mixed_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def relax_batch_mixed(mq: MixedQ, obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x,
+                      lb, ub, eps, iterations, status, gap, value, best_vertex, best_vertex_val, flags=None,
+                      candidate_tol=1e-6, cutoff=None, rel_floor=1e-6, fp64_max_iters=0, row_tiles_per_wave=0,
+                      columns=0, blocks=0, mixed_iterations=None, stream=None):
+    """gb_relax_batch_mixed on torch device tensors."""
+    opt = MixedOptions(row_tiles_per_wave, columns, blocks, rel_floor, fp64_max_iters,
+                       0 if mixed_iterations is None else mixed_iterations.data_ptr(), 0)
+    check(load().gb_relax_batch_mixed(mq.handle, ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo),
+                                      int(N), int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
+                                      _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),
+                                      _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
+                                      float(candidate_tol), _ptr(cutoff), ctypes.byref(opt),
+                                      ctypes.c_void_p(stream or 0)))
+
+
+class RelaxGraph:
+    """gb_relax_graph_*: a captured batched-form launch over fixed tensors."""
+
+    def __init__(self, obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb, ub,
+                 eps, iterations, status, gap, value, best_vertex, best_vertex_val, flags=None,
+                 candidate_tol=1e-6, cutoff=None, refresh=64, waves_per_block=4, blocks=0):
+        lib = load()
+        self._keep = (obj, alg, lmo, x, lb, ub, eps, iterations, status, gap, value, best_vertex,
+                      best_vertex_val, flags, cutoff)
+        opt = FastOptions(refresh, waves_per_block, blocks, 0, 0)
+        h = ctypes.c_void_p()
+        check(lib.gb_relax_graph_create(ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
+                                        int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
+                                        _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),
+                                        _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
+                                        float(candidate_tol), _ptr(cutoff), ctypes.byref(opt),
+                                        ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self, stream=None):
+        check(load().gb_relax_graph_launch(self.handle, ctypes.c_void_p(stream or 0)))
+
+    def close(self):
+        if self.handle:
+            load().gb_relax_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Brancher:
@@ -301,6 +407,15 @@ class Brancher:
         check(load().gb_brancher_train(self.handle, X.data_ptr(), T.data_ptr(), int(X.shape[0]),
                                        int(epochs), int(batch), ctypes.c_float(lr), ctypes.byref(loss)))
         return loss.value
+
+
+def grid_barrier_selftest(groups, group_blocks, threads=256, phases_base=2000, phase_step=101,
+                          spin_limit=1 << 22):
+    """gb_grid_barrier_selftest: (errors, timeouts, seconds per sync)."""
+    e, t, s = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+    check(load().gb_grid_barrier_selftest(groups, group_blocks, threads, phases_base, phase_step,
+                                          spin_limit, ctypes.byref(e), ctypes.byref(t), ctypes.byref(s)))
+    return e.value, t.value, s.value
 
 
 def branching_dataset(n, lb, ub, int_vars, obj, lmo, settings, max_nodes, capacity):

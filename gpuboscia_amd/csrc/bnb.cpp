@@ -323,12 +323,20 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
         return fail(GB_ERR_INVALID_ARGUMENT, "branch_k must be in [1, 16]");
     if (s->branch_rule == GB_BRANCH_CALLBACK && !s->branch_score)
         return fail(GB_ERR_INVALID_ARGUMENT, "callback branching requires branch_score");
-    if (s->precision != GB_PREC_FP64)
-        return fail(GB_ERR_INVALID_ARGUMENT, "gb_solve relaxes nodes in FP64 (certified bounds)");
-    if (s->form != 0 && s->form != 1) return fail(GB_ERR_INVALID_ARGUMENT, "unknown solver form");
+    if (s->form != 0 && s->form != 1 && s->form != 2)
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown solver form");
+    if (s->form != 2 && s->precision != GB_PREC_FP64)
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "the plain and batched forms relax in FP64; reduced precision needs form 2");
+    if (s->form == 2 && s->precision != GB_PREC_FP64 && s->precision != GB_PREC_FP32
+            && s->precision != GB_PREC_BF16)
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown precision");
     if (s->form == 1 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > kMaxFastN))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "the batched form supports dense objectives over a box with n <= 512");
+    if (s->form == 2 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 256))
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "the mixed-precision form supports dense objectives over a box with n <= 256");
     return GB_OK;
 }
 
@@ -478,6 +486,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     std::vector<uint8_t> hflags(n, 0);
     for (int i : iv) hflags[i] = 1;
     GB_HIP(hipMemcpy(db.flags, hflags.data(), n, hipMemcpyHostToDevice));
+    // Mixed-precision form: Q split into bf16 planes once per solve (one
+    // plane for GB_PREC_BF16, three otherwise).
+    std::unique_ptr<gb_mixed_q, gb_status (*)(gb_mixed_q*)> mq(nullptr, gb_mixed_destroy);
+    if (s->form == 2) {
+        gb_mixed_q* h = nullptr;
+        st = gb_mixed_prepare(o, s->precision == GB_PREC_BF16 ? 1 : 3, db.stream, &h);
+        if (st != GB_OK) return st;
+        mq.reset(h);
+    }
     if (lab) GB_HIP(hipMemcpy(sb.flags, hflags.data(), n, hipMemcpyHostToDevice));
 
     std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
@@ -568,6 +585,17 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
                                   s->integer_tol, db.cut, &fo, db.stream);
+        } else if (s->form == 2) {
+            for (int i = 0; i < N; ++i) {
+                db.hcut[i] = inc;
+                db.hbvv[i] = 1e300;
+            }
+            GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
+            GB_HIP(hipMemcpyAsync(db.bvv, db.hbvv, N * 8, hipMemcpyHostToDevice, db.stream));
+            const gb_mixed_options mo{0, 0, 0, 1e-6f, 0, nullptr, nullptr};
+            st = gb_relax_batch_mixed(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb,
+                                      db.ub, db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv,
+                                      db.flags, s->integer_tol, db.cut, &mo, db.stream);
         } else {
             st = relax_batch_fp64(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
@@ -745,8 +773,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         if (mask & (1 << b)) clb[vi] = std::ceil(xi[vi]);
                         else cub[vi] = std::floor(xi[vi]);
                     }
-                    if (s->form == 1) {
-                        // Batched form: the child starts from the parent's
+                    if (s->form != 0) {
+                        // Batched forms: the child starts from the parent's
                         // iterate clipped to its box (any box point is feasible).
                         double* crx = pool.rx(c.slot);
                         for (int j = 0; j < n; ++j) crx[j] = std::min(std::max(xi[j], clb[j]), cub[j]);

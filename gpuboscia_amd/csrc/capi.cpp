@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <atomic>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -26,24 +28,32 @@ gb_status hip_fail(hipError_t e, const char* file, int line) {
                                    + hipGetErrorString(e));
 }
 
-// Per-device grow-only workspace for the per-node work vectors.
+// Grow-only workspace for the per-node work vectors, one per (device,
+// stream): launches on different streams never share work vectors, and
+// launches on one stream are ordered.
 struct Workspace {
     std::mutex mtx;
     double* ptr = nullptr;
     size_t bytes = 0;
 };
-static Workspace g_ws[64];
+static std::mutex g_ws_mtx;
+static std::map<std::pair<int, void*>, Workspace> g_ws;
 
-gb_status workspace_get(size_t bytes, double** out) {
+gb_status workspace_get(size_t bytes, void* stream, double** out) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
-    if (dev < 0 || dev >= 64) return fail(GB_ERR_RUNTIME, "device index out of range");
-    Workspace& w = g_ws[dev];
+    Workspace* wp = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mtx);
+        wp = &g_ws[{dev, stream}];
+    }
+    Workspace& w = *wp;
     std::lock_guard<std::mutex> lk(w.mtx);
     if (w.bytes < bytes) {
         if (w.ptr) {
-            e = hipDeviceSynchronize();
+            // Work already queued on this stream may still use the old buffer.
+            e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
             if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
             e = hipFree(w.ptr);
             if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
@@ -124,7 +134,7 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
     const LdsLayout L = lds_layout(o->kind, l->kind, n, l->order);
     if (L.total > kMaxLdsBytes) return fail(GB_ERR_INVALID_ARGUMENT, "problem exceeds the LDS budget");
     double* work = nullptr;
-    st = workspace_get((size_t)N * kWorkVectors * n * sizeof(double), &work);
+    st = workspace_get((size_t)N * kWorkVectors * n * sizeof(double), stream, &work);
     if (st != GB_OK) return st;
     RelaxParams P{};
     P.obj_kind = o->kind;
@@ -169,21 +179,31 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
     return GB_OK;
 }
 
-// Device-resident node counter of the persistent batched kernel, one per device.
+// Device-resident node counters of the persistent batched kernel: a ring of
+// kCounterSlots counters per device, 64 bytes apart; each call takes the next
+// slot, so up to kCounterSlots launches may be in flight on one device (on
+// any streams) without sharing a counter.  Each launch resets its counter
+// with a kernel before use.
+constexpr int kCounterSlots = 1024;
+constexpr int kCounterStride = 16;  // ints
 static int* g_counter[64];
+static std::atomic<unsigned> g_counter_next[64];
 static std::mutex g_counter_mtx;
 
 gb_status counter_get(int** out) {
     int dev = 0;
     GB_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(GB_ERR_RUNTIME, "device index out of range");
-    std::lock_guard<std::mutex> lk(g_counter_mtx);
-    if (!g_counter[dev]) {
-        void* p = nullptr;
-        GB_HIP(hipMalloc(&p, 256));
-        g_counter[dev] = static_cast<int*>(p);
+    {
+        std::lock_guard<std::mutex> lk(g_counter_mtx);
+        if (!g_counter[dev]) {
+            void* p = nullptr;
+            GB_HIP(hipMalloc(&p, sizeof(int) * kCounterSlots * kCounterStride));
+            g_counter[dev] = static_cast<int*>(p);
+        }
     }
-    *out = g_counter[dev];
+    const unsigned slot = g_counter_next[dev].fetch_add(1u) % kCounterSlots;
+    *out = g_counter[dev] + static_cast<size_t>(slot) * kCounterStride;
     return GB_OK;
 }
 
@@ -192,7 +212,7 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
                            const double* ub, const double* eps, int* iterations, uint8_t* status,
                            double* gap, double* value, double* best_vertex, double* best_vertex_val,
                            const uint8_t* flags, double ctol, const double* cutoff,
-                           const gb_fast_options* opt, void* stream) {
+                           const gb_fast_options* opt, void* stream, int* counter, bool rescore) {
     gb_status st = validate_descriptors(o, a, l);
     if (st != GB_OK) return st;
     if (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX)
@@ -207,9 +227,10 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     if (opt) op = *opt;
     if (op.waves_per_block < 1 || op.waves_per_block > 4 || op.refresh_interval < 0 || op.blocks < 0)
         return fail(GB_ERR_INVALID_ARGUMENT, "invalid batched-form options");
-    int* counter = nullptr;
-    st = counter_get(&counter);
-    if (st != GB_OK) return st;
+    if (!counter) {
+        st = counter_get(&counter);
+        if (st != GB_OK) return st;
+    }
     FastParams P{};
     P.n = o->n;
     P.n_pad = (o->n + 1) & ~1;
@@ -238,6 +259,7 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     P.counter = counter;
     P.stats = op.stats;
     P.gout = op.gradient;
+    P.rescore = rescore ? 1 : 0;
     int blocks = op.blocks;
     if (blocks == 0) {
         const int needed = (N + op.waves_per_block - 1) / op.waves_per_block;
@@ -268,6 +290,80 @@ extern "C" gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
     } catch (const std::exception& ex) {
         return fail(GB_ERR_RUNTIME, ex.what());
     }
+}
+
+// A captured batched-form launch: counter reset kernel + relaxation kernel,
+// bound to fixed buffers and its own node counter.
+struct gb_relax_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int* counter = nullptr;
+    int device = 0;
+};
+
+extern "C" gb_status gb_relax_graph_destroy(gb_relax_graph* g) {
+    if (!g) return GB_OK;
+    int prev = 0;
+    const bool have_prev = hipGetDevice(&prev) == hipSuccess;
+    (void)hipSetDevice(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->counter) (void)hipFree(g->counter);
+    if (have_prev) (void)hipSetDevice(prev);
+    delete g;
+    return GB_OK;
+}
+
+extern "C" gb_status gb_relax_graph_create(const gb_objective_desc* objective,
+                                           const gb_algorithm_desc* algorithm,
+                                           const gb_lmo_desc* lmo, int N, int max_iters, double* x,
+                                           const double* lb, const double* ub, const double* eps,
+                                           int* iterations, uint8_t* status, double* gap,
+                                           double* value, double* best_vertex,
+                                           double* best_vertex_val, const uint8_t* integer_flags,
+                                           double candidate_tol, const double* cutoff,
+                                           const gb_fast_options* options, gb_relax_graph** out) {
+    if (!out) return fail(GB_ERR_INVALID_ARGUMENT, "null graph handle");
+    *out = nullptr;
+    if (N <= 0) return fail(GB_ERR_INVALID_ARGUMENT, "a captured launch needs N > 0");
+    try {
+        auto* g = new gb_relax_graph();
+        hipStream_t s = nullptr;
+        hipError_t e = hipGetDevice(&g->device);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&g->counter), 64);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            gb_relax_graph_destroy(g);
+            return hip_fail(e, __FILE__, __LINE__);
+        }
+        gb_status st = GB_OK;
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            st = relax_batch_fast(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps, iterations,
+                                  status, gap, value, best_vertex, best_vertex_val, integer_flags,
+                                  candidate_tol, cutoff, options, s, g->counter);
+            const hipError_t e2 = hipStreamEndCapture(s, &g->graph);
+            if (st == GB_OK && e2 != hipSuccess) e = e2;
+        }
+        if (st == GB_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(s);
+        if (st != GB_OK || e != hipSuccess) {
+            const std::string msg = g_last_error;
+            gb_relax_graph_destroy(g);
+            if (st != GB_OK) return fail(st, msg);
+            return hip_fail(e, __FILE__, __LINE__);
+        }
+        *out = g;
+        return GB_OK;
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
+
+extern "C" gb_status gb_relax_graph_launch(gb_relax_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(GB_ERR_INVALID_ARGUMENT, "null graph handle");
+    GB_HIP(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)));
+    return GB_OK;
 }
 
 extern "C" gb_status gb_relax_batch(const gb_objective_desc* objective,

@@ -31,6 +31,7 @@ struct FastParams {
     int* counter;           // work queue head (reset by the launcher)
     long long* stats;       // optional [N][2]: column updates, direct GEMVs (may be NULL)
     double* gout;           // optional [N][n]: gradient at the returned iterate (may be NULL)
+    int rescore;            // re-score best_v[i] in FP64 first when best_val[i] < 1e300
 };
 
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream);

@@ -21,7 +21,7 @@ extern thread_local std::string g_last_error;
 
 gb_status fail(gb_status code, const std::string& msg);
 gb_status hip_fail(hipError_t e, const char* file, int line);
-gb_status workspace_get(size_t bytes, double** out);
+gb_status workspace_get(size_t bytes, void* stream, double** out);
 gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
                                const gb_lmo_desc* l);
 int auto_block_threads(int n);
@@ -36,7 +36,9 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
                            const double* ub, const double* eps, int* iterations, uint8_t* status,
                            double* gap, double* value, double* best_vertex, double* best_vertex_val,
                            const uint8_t* flags, double ctol, const double* cutoff,
-                           const gb_fast_options* opt, void* stream);
+                           const gb_fast_options* opt, void* stream, int* counter = nullptr,
+                           bool rescore = false);
+gb_status counter_get(int** out);
 
 #define GB_HIP(call)                                                   \
     do {                                                               \

@@ -1,0 +1,89 @@
+// Mixed-precision batched relaxation (relax_mixed.hip): parameters shared by
+// the host launcher and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gpuboscia_amd.h"
+
+namespace gb {
+
+constexpr int kMixedMaxN = 256;   // 16 row tiles of 16: x, Qx, Qv, Qa in registers
+constexpr int kMixedMaxPlanes = 3;
+
+// Q in MFMA A-fragment order, split into bf16 planes q = q0 + q1 + q2:
+// fragment ((p * RT + t) * S + s) * 64 + lane holds 8 bf16 values
+// Q[16 t + lane % 16][16 (2 s + (i >> 2)) + 4 (lane / 16) + (i & 3)], i < 8,
+// so lane l's B-operand slots of k-step s are exactly the rows it owns in
+// the output tiles 2s and 2s+1 (no cross-lane exchange of the vertex).
+struct MixedQ {
+    int n = 0;
+    int rt = 0;        // row tiles (even); n_pad = 16 rt
+    int planes = 0;
+    uint4* frag = nullptr;
+    float* lsc = nullptr;   // s_g s_l c, n_pad floats (0 beyond n)
+    int device = 0;
+};
+
+struct MixedParams {
+    int n, rt, planes;
+    int cols;               // node columns per 16-column tile (1..16)
+    const uint4* frag;
+    const float* lsc;
+    float gs;
+    double kappa;
+    int N, max_iters;
+    double* x;
+    const double* lb;
+    const double* ub;
+    const double* eps;
+    int* iters;
+    uint8_t* status;
+    double* best_v;
+    double* best_val;
+    const double* cutoff;
+    int* counter;
+    float rel_floor;        // stop once gap < max(eps, rel_floor (1 + |f|))
+    long long* stats;       // optional [N]: iterations of the mixed phase
+};
+
+int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double ls, int n, int rt,
+                         int planes, uint4* frag, float* lsc, void* stream);
+// LDS of one tile (W waves): s_g s_l c, vertex nibbles per (row tile, lane),
+// per-column partials per wave, fetched node ids, feasibility flags.
+struct MixedLds {
+    size_t lsc, bits, red, fetch, ok, total;
+};
+__host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves) {
+    MixedLds L{};
+    size_t o = 0;
+    L.lsc = o;
+    o += sizeof(float) * 16 * static_cast<size_t>(rt);
+    o = (o + 15) & ~static_cast<size_t>(15);
+    L.bits = o;
+    o += static_cast<size_t>(rt) * 64;
+    o = (o + 15) & ~static_cast<size_t>(15);
+    L.red = o;
+    o += sizeof(double) * static_cast<size_t>(waves) * 16 * 7;
+    L.fetch = o;
+    o += sizeof(int) * 16;
+    L.ok = o;
+    o += sizeof(int) * static_cast<size_t>(waves) * 16;
+    L.total = o;
+    return L;
+}
+
+constexpr int kMixedMaxRowTilesPerWave = 4;
+
+struct MixedShape {
+    int rw = 2;       // row tiles per wave
+    int cols = 16;    // node columns per tile
+    int blocks = 0;   // tiles launched
+};
+MixedShape mixed_default_shape(int rt, int N);
+
+// rw: row tiles per wave (1..4); the tile runs ceil(rt / rw) waves.
+int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void* stream);
+int launch_mixed_products(const MixedQ& q, const double* V, int N, double* Y, void* stream);
+
+}  // namespace gb

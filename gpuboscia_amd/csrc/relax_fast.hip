@@ -236,6 +236,19 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
             }
         };
 
+        // A candidate from an earlier phase (the mixed-precision pass) is
+        // re-scored exactly before this node's own candidates.
+        if (P.rescore && P.best_val[node] < kNone) {
+            double z[NT], qz[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                z[t] = (j < n) ? bvn[j] : 0.0;
+            }
+            W.gemv(z, qz);
+            score(z, qz);
+        }
+
         // Starting point: LMO vertex of the artificial direction (ub for a
         // box, since it is negative) or the supplied / warm-start iterate.
 #pragma unroll

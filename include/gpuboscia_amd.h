@@ -15,8 +15,10 @@
  * <message>" (the reference's DRS_CUDA_CHECK reads "CUDA ...").
  *
  * Device pointers are caller-owned, as in the reference (lmo.cuh:6-9,
- * heuristics.h:22).  Calls on one stream are ordered; calls that share a
- * library-owned workspace serialise on an internal mutex.
+ * heuristics.h:22).  Calls on one stream are ordered.  Library-owned
+ * scratch (work vectors of gb_relax_batch, node counters of the batched
+ * form) is kept per (device, stream) or per launch, so calls on different
+ * streams may run concurrently.
  */
 #ifndef GPUBOSCIA_AMD_H
 #define GPUBOSCIA_AMD_H
@@ -179,6 +181,109 @@ gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
                               const uint8_t* integer_flags, double candidate_tol,
                               const double* cutoff, const gb_fast_options* options,
                               void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Mixed-precision batched relaxation (no reference counterpart; the       */
+/* reference computes every product in FP64, dense_quadratic_objectives.cuh:92-111). */
+/* ---------------------------------------------------------------------- */
+
+/* Q of a dense objective prepared for the BF16 MFMA products: split into
+   `planes` bf16 planes (1: ~2^-8 relative per entry, 3: ~2^-24) in MFMA
+   fragment order, plus s_g s_l c in FP32.  n <= 256.  Asynchronous on
+   `stream`; the objective's Qt / linear must stay valid until it completes. */
+typedef struct gb_mixed_q gb_mixed_q;
+gb_status gb_mixed_prepare(const gb_objective_desc* objective, int planes, void* stream,
+                           gb_mixed_q** out);
+gb_status gb_mixed_destroy(gb_mixed_q* q);
+
+/* Y[i] = Q V[i] (device [N][n] FP64 in and out) computed by the same MFMA
+   fragments, V rounded to bf16 (exact for 0/1 vectors): the product error of
+   the mixed form against FP64. */
+gb_status gb_mixed_products(const gb_mixed_q* q, const double* V, int N, double* Y, void* stream);
+
+typedef struct {
+    int row_tiles_per_wave;   /* 16-row tiles per wave (1..4; 0 = automatic); a tile
+                                 (workgroup) runs ceil(n_pad / 16 / this) waves */
+    int columns;              /* nodes per 16-column MFMA tile (1..16; 0 = automatic):
+                                 fewer columns spread a small batch over more CUs */
+    int blocks;               /* persistent tiles (0 = automatic) */
+    float rel_floor;          /* the mixed pass stops at gap < max(eps, rel_floor (1 + |f|)) */
+    int fp64_max_iters;       /* budget of the FP64 pass (0 = max(1, max_iters / 16), < 0 = skip it) */
+    long long* mixed_iterations;  /* optional device [N]: iterations of the mixed pass */
+    long long* fp64_stats;        /* optional device [N][2]: as gb_fast_options.stats */
+} gb_mixed_options;
+
+/*
+ * Mixed-precision relaxation of N nodes of a dense quadratic over a 0/1 box.
+ * Pass 1 (BF16 MFMA): 16 nodes per wave64 in lockstep, products Q v / Q a
+ * from the prepared planes with FP32 accumulation, FP32 iterates, FP64
+ * step and reductions, DICG (or FW) with the exact step, per-node refill
+ * from a device counter; it writes its iterate (status
+ * GB_NODE_INITIAL_POINT_SUPPLIED) and its best vertex.  Pass 2: the FP64
+ * batched kernel (gb_relax_batch_fast) continues from that iterate,
+ * re-scores the vertex exactly, and returns FP64 gap, value and
+ * certificate, so pruning decisions and incumbents are FP64.  Pass 1
+ * starts a node at the box vertex nearest to its supplied point (the upper
+ * bounds for a fresh node); nodes whose bounds are not all 0/1 and
+ * continuations (iterations > 0 on entry) skip pass 1.  fp64_max_iters < 0
+ * runs pass 1 only (diagnostics).  Buffers as gb_relax_batch_fast;
+ * best_vertex_val must be 1e300 on entry (as for the reference kernel).
+ */
+gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
+                               const gb_objective_desc* objective,
+                               const gb_algorithm_desc* algorithm,
+                               const gb_lmo_desc* lmo,
+                               int N, int max_iters,
+                               double* x, const double* lb, const double* ub,
+                               const double* eps, int* iterations, uint8_t* status,
+                               double* gap, double* value,
+                               double* best_vertex, double* best_vertex_val,
+                               const uint8_t* integer_flags, double candidate_tol,
+                               const double* cutoff, const gb_mixed_options* options,
+                               void* stream);
+
+/*
+ * Captured batched-form launch (hipGraph): the counter reset and the
+ * persistent relaxation kernel of gb_relax_batch_fast recorded once for
+ * fixed buffers and N, replayed with one gb_relax_graph_launch per batch.
+ * Replays read the buffers' current contents, so a caller refills x / lb /
+ * ub / eps / iterations / status in place between replays.  Each graph owns
+ * its node counter; replays of one graph must not overlap.  No reference
+ * counterpart: it replaces the per-batch launch path of
+ * device_resident_solver.cuh:1775-1863 (run_slot).
+ */
+typedef struct gb_relax_graph gb_relax_graph;
+gb_status gb_relax_graph_create(const gb_objective_desc* objective,
+                                const gb_algorithm_desc* algorithm,
+                                const gb_lmo_desc* lmo,
+                                int N, int max_iters,
+                                double* x, const double* lb, const double* ub,
+                                const double* eps, int* iterations, uint8_t* status,
+                                double* gap, double* value,
+                                double* best_vertex, double* best_vertex_val,
+                                const uint8_t* integer_flags, double candidate_tol,
+                                const double* cutoff, const gb_fast_options* options,
+                                gb_relax_graph** out);
+gb_status gb_relax_graph_launch(gb_relax_graph* graph, void* stream);
+gb_status gb_relax_graph_destroy(gb_relax_graph* graph);
+
+/*
+ * Self-test of the persistent-kernel group barrier on device atomics that
+ * replaces the reference's cooperative grid syncs (drs_dicg_paired_batch_kernel,
+ * device_resident_solver.cuh:764-1000; MultiBlockContext::node_sync,
+ * multiblock.cuh:72-102).  Modelled on tests/test_multiblock.cu:8-31:
+ * groups x group_blocks co-resident workgroups (cooperative launch) of
+ * `threads` threads; group g runs phases_base + g*phase_step generations
+ * of publish / sync / check / sync, and groups g > 0 start only after group
+ * 0 has completed 100 generations.  Every wait is bounded by spin_limit
+ * polls.  Outputs: *errors = visibility errors, *timeouts = waits that ran
+ * out (both 0 on success), *seconds_per_sync = group 0's time per sync.
+ * Synchronous; allocates its own buffers.
+ */
+gb_status gb_grid_barrier_selftest(int groups, int group_blocks, int threads,
+                                   int phases_base, int phase_step, long long spin_limit,
+                                   long long* errors, long long* timeouts,
+                                   double* seconds_per_sync);
 
 /* ---------------------------------------------------------------------- */
 /* Learned branching scorer (no reference counterpart; the reference       */

@@ -1083,7 +1083,8 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
                                      double* X, const double* LB, const double* UB,
                                      const double* EPS, int* ITERS, uint8_t* STATUS, double* GAP,
                                      double* VALUE, double* BV, double* BVV, const uint8_t* flags,
-                                     double ctol, const double* cutoff, int refresh, int threads) {
+                                     double ctol, const double* cutoff, int refresh, int rescore,
+                                     int threads) {
     const orc_objective& o = *op;
     const int n = o.n;
     auto work = [&](int node) {
@@ -1122,6 +1123,12 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
             if (f < best) { best = f; std::copy(z.begin(), z.end(), bvn); }
         };
         std::vector<double> x(n), qx(n), v(n), qv(n), a(n), qa(n), g(n), vn(n), an(n), d(n), qd(n);
+        if (rescore && BVV[node] < kNone) {
+            // candidate of an earlier (mixed-precision) phase, re-scored exactly
+            std::vector<double> z(bvn, bvn + n), qz(n);
+            F.gemv(z, qz);
+            score(z, qz);
+        }
         for (int j = 0; j < n; ++j) x[j] = (initial == 0 && !supplied) ? ub[j] : xn[j];
         F.gemv(x, qx);
         auto grad = [&]() { for (int j = 0; j < n; ++j) g[j] = o.grad_scale * qx[j] + F.lsc[j]; };
@@ -1305,7 +1312,7 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
             std::vector<double> CUT(N, inc);
             orc_relax_fast_batch(o, a->dicg, N, s->max_iters, X.data(), L.data(), U.data(), E.data(),
                                  IT.data(), ST.data(), G.data(), V.data(), BV.data(), BVV.data(),
-                                 flags.data(), s->integer_tol, CUT.data(), s->refresh, s->threads);
+                                 flags.data(), s->integer_tol, CUT.data(), s->refresh, 0, s->threads);
         } else {
             orc_relax_batch(o, a, l, N, s->max_iters, X.data(), L.data(), U.data(), E.data(), IT.data(),
                             ST.data(), G.data(), V.data(), BV.data(), BVV.data(), flags.data(),

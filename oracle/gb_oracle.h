@@ -131,7 +131,8 @@ void orc_relax_fast_batch(const orc_objective* o, int dicg, int N, int max_iters
                           const double* lb, const double* ub, const double* eps,
                           int* iterations, uint8_t* status, double* gap, double* value,
                           double* best_vertex, double* best_vertex_val, const uint8_t* flags,
-                          double candidate_tol, const double* cutoff, int refresh, int threads);
+                          double candidate_tol, const double* cutoff, int refresh, int rescore,
+                          int threads);
 
 /* Deterministic best-first B&B over orc_relax_batch. */
 typedef struct {

@@ -183,18 +183,22 @@ def enumerate_binary_qp(Q, c, threads=8):
 
 
 def relax_fast_batch(obj, dicg, x, lb, ub, eps, iterations, status, flags=None, max_iters=100,
-                     candidate_tol=1e-6, cutoff=None, refresh=64, threads=1):
-    """orc_relax_fast_batch on numpy arrays (mirror of the batched-form kernel)."""
+                     candidate_tol=1e-6, cutoff=None, refresh=64, threads=1, best_vertex=None,
+                     best_vertex_val=None):
+    """orc_relax_fast_batch on numpy arrays (mirror of the batched-form kernel).
+    best_vertex / best_vertex_val given: candidates of an earlier phase,
+    re-scored first (the FP64 pass after the mixed-precision pass)."""
     lib = load()
     N, n = x.shape
     gap = np.zeros(N)
     value = np.zeros(N)
-    bv = np.zeros((N, n))
-    bvv = np.full(N, 1e300)
+    rescore = best_vertex_val is not None
+    bv = np.zeros((N, n)) if best_vertex is None else np.ascontiguousarray(best_vertex, dtype=np.float64)
+    bvv = np.full(N, 1e300) if best_vertex_val is None else np.ascontiguousarray(best_vertex_val, dtype=np.float64)
     fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
     cut = None if cutoff is None else f64(cutoff)
     lib.orc_relax_fast_batch(ctypes.byref(obj), int(dicg), N, max_iters, p(x), p(lb), p(ub), p(eps),
                              p(iterations), p(status), p(gap), p(value), p(bv), p(bvv), p(fl),
-                             ctypes.c_double(candidate_tol), p(cut), refresh, threads)
+                             ctypes.c_double(candidate_tol), p(cut), refresh, int(rescore), threads)
     return {"x": x, "gap": gap, "value": value, "iterations": iterations, "status": status,
             "best_vertex": bv, "best_vertex_val": bvv}
