@@ -1,12 +1,27 @@
-"""Benchmark: FW iterations/s of the node-relaxation hot path on one MI355X,
-plus time-to-proven-optimum on the tractable binary-QP sizes.
+"""Benchmark of the node-relaxation hot path on one MI355X.
 
-One step = one gb_relax_batch launch over a fixed batch of open B&B nodes of
-a seeded n=200 binary QP (A 300x200 ~ N(0,1), Q = A'A/300 + 0.1 I,
-c ~ U(-1,1)), reference default algorithm (DICG + secant line search, 4
-secant iterations), per-node FW tolerance 1e-3 * 0.8^depth, iteration budget
-per step.  value = total FW iterations executed / timed wall time (inputs
-resident in HBM).  Prints ONE JSON line.
+Workload (one step): one relaxation launch over a fixed batch of 1024 open
+B&B nodes of a seeded n=200 binary QP (A 300x200 ~ N(0,1), Q = A'A/300 +
+0.1 I, c ~ U(-1,1); nodes = the root plus random fixings), DICG, per-node FW
+tolerance 1e-3 * 0.8^depth, 200 iterations per node per step.  Inputs are
+resident in HBM; each step resets the per-node iteration counters and
+statuses (fill kernels) and relaxes the batch again.
+
+  value      FW iterations/s of the mixed-precision form (BF16 MFMA pass +
+             FP64 certification pass), whole batch, HIP events on the launch
+             stream bracket the timed steps
+  plain      the reference algorithm (FP64, DICG + secant), same batch
+  batched    the FP64 batched form, same batch
+  scale      the mixed form on 16384 nodes (16 batches in flight)
+  roofline   the BF16 MFMA kernel (relax_mixed_kernel): algorithmic flops
+             2 n^2 per FW iteration / its launch time, against 2.5 PFLOP/s
+  cpu_baseline  the CPU oracle (FP64 restatement of the plain form) on a
+             bounded sample of the same nodes, 16 host threads
+  time_to_optimum  gb_solve to the proven optimum (abs = rel gap 1e-6) on the
+             tractable binary-QP sizes: plain vs batched vs mixed vs
+             learned + mixed
+
+Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--nodes 1024] [--budget 200]
 """
@@ -14,7 +29,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -25,7 +39,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_FP64_TFLOPS = 78.6
-MI355X_HBM_GBS = 8000.0
+MI355X_BF16_TFLOPS = 2500.0
+SCORER = os.path.join(ROOT, "gpuboscia_amd", "scorer_binary_qp_n40.npz")
+
+
+def say(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def node_batch(n, N, seed):
@@ -44,80 +63,101 @@ def node_batch(n, N, seed):
     return LB, UB, eps
 
 
-def gpu_relax_throughput(args, Q, c, LB, UB, EPS, form):
-    """Times `steps` launches over the same node batch.  form 'plain': the
-    FP64 per-node-workgroup kernel with the reference defaults (DICG +
-    secant); form 'batched': the persistent wave64 kernel (DICG, exact step,
-    incremental products), no cutoff so every node relaxes to its tolerance
-    or the budget."""
-    import torch
+class Batch:
+    """Device buffers of one node batch and the launch of each form."""
 
-    import gpuboscia_amd as gb
-    n = Q.shape[0]
-    N = LB.shape[0]
-    cuda = torch.device("cuda:0")
-    dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).to(cuda)
-    dc = torch.from_numpy(c).to(cuda)
-    obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
-    lmo = gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0)
-    alg = gb.algorithm()  # DICG + secant(4, 1e-10, 1e-14): reference defaults
-    x = torch.zeros((N, n), dtype=torch.float64, device=cuda)
-    lb = torch.from_numpy(LB).to(cuda)
-    ub = torch.from_numpy(UB).to(cuda)
-    eps = torch.from_numpy(EPS).to(cuda)
-    it = torch.zeros(N, dtype=torch.int32, device=cuda)
-    st = torch.zeros(N, dtype=torch.uint8, device=cuda)
-    gap = torch.zeros(N, dtype=torch.float64, device=cuda)
-    val = torch.zeros(N, dtype=torch.float64, device=cuda)
-    bv = torch.zeros((N, n), dtype=torch.float64, device=cuda)
-    bvv = torch.zeros(N, dtype=torch.float64, device=cuda)
-    flags = torch.ones(n, dtype=torch.uint8, device=cuda)
-    stats = torch.zeros((N, 2), dtype=torch.int64, device=cuda)
-    stream = torch.cuda.Stream(device=cuda)
+    def __init__(self, Q, c, LB, UB, EPS):
+        import torch
 
-    def launch():
+        import gpuboscia_amd as gb
+        self.gb = gb
+        self.torch = torch
+        n = Q.shape[0]
+        N = LB.shape[0]
+        self.n, self.N = n, N
+        cuda = torch.device("cuda:0")
+        f64 = dict(dtype=torch.float64, device=cuda)
+        self.dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).to(cuda)
+        self.dc = torch.from_numpy(c).to(cuda)
+        self.obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, self.dQt.data_ptr(), self.dc.data_ptr(), 2.0, 0.5, 0.0, 0,
+                                    0, 0)
+        self.lmo = gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0)
+        self.alg = gb.algorithm()  # DICG + secant(4, 1e-10, 1e-14): reference defaults
+        self.x = torch.zeros((N, n), **f64)
+        self.lb = torch.from_numpy(LB).to(cuda)
+        self.ub = torch.from_numpy(UB).to(cuda)
+        self.eps = torch.from_numpy(EPS).to(cuda)
+        self.it = torch.zeros(N, dtype=torch.int32, device=cuda)
+        self.st = torch.zeros(N, dtype=torch.uint8, device=cuda)
+        self.gap = torch.zeros(N, **f64)
+        self.val = torch.zeros(N, **f64)
+        self.bv = torch.zeros((N, n), **f64)
+        self.bvv = torch.full((N,), 1e300, **f64)
+        self.flags = torch.ones(n, dtype=torch.uint8, device=cuda)
+        self.mit = torch.zeros(N, dtype=torch.int64, device=cuda)
+        self.stats = torch.zeros((N, 2), dtype=torch.int64, device=cuda)
+        self.stream = torch.cuda.Stream(device=cuda)
+        self.mq = None
+
+    def reset(self):
+        self.it.zero_()
+        self.st.zero_()
+        self.bvv.fill_(1e300)
+
+    def launch(self, form, budget):
+        gb, s = self.gb, self.stream.cuda_stream
         if form == "plain":
-            gb.relax_batch(obj, alg, lmo, N, args.budget, x, lb, ub, eps, it, st, gap, val, bv, bvv,
-                           flags, 1e-6, 0, stream.cuda_stream)
+            gb.relax_batch(self.obj, self.alg, self.lmo, self.N, budget, self.x, self.lb, self.ub, self.eps,
+                           self.it, self.st, self.gap, self.val, self.bv, self.bvv, self.flags, 1e-6, 0, s)
+        elif form == "batched":
+            gb.relax_batch_fast(self.obj, self.alg, self.lmo, self.N, budget, self.x, self.lb, self.ub,
+                                self.eps, self.it, self.st, self.gap, self.val, self.bv, self.bvv, self.flags,
+                                1e-6, None, 64, 4, 0, s, self.stats)
         else:
-            gb.relax_batch_fast(obj, alg, lmo, N, args.budget, x, lb, ub, eps, it, st, gap, val, bv,
-                                bvv, flags, 1e-6, None, 64, 4, 0, stream.cuda_stream, stats)
+            if self.mq is None:
+                self.mq = gb.MixedQ(self.obj, 1, s)
+            gb.relax_batch_mixed(self.mq, self.obj, self.alg, self.lmo, self.N, budget, self.x, self.lb,
+                                 self.ub, self.eps, self.it, self.st, self.gap, self.val, self.bv, self.bvv,
+                                 self.flags, 1e-6, None, fp64_max_iters=-1 if form == "mixed_pass1" else 0,
+                                 mixed_iterations=self.mit, stream=s)
 
-    with torch.cuda.stream(stream):
-        for _ in range(args.warmup):
-            it.zero_()
-            st.zero_()
-            launch()
-        stream.synchronize()
-        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            it.zero_()
-            st.zero_()
-            starts[k].record(stream)
-            launch()
-            ends[k].record(stream)
-        stream.synchronize()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-    iters_per_step = int(it.sum().item())
-    kernel_ms = [s_.elapsed_time(e_) for s_, e_ in zip(starts, ends)]
-    out = {"wall_s": wall, "total_iters": iters_per_step * args.steps,
-           "iters_per_step": iters_per_step, "kernel_ms_avg": float(np.mean(kernel_ms)),
-           "status_hist": np.bincount(st.cpu().numpy(), minlength=5).tolist(),
-           "value": iters_per_step * args.steps / wall}
-    if form == "batched":
-        sv = stats.cpu().numpy()
-        out["column_updates"] = int(sv[:, 0].sum())
-        out["direct_products"] = int(sv[:, 1].sum())
-    return out
+    def measure(self, form, budget, steps, warmup):
+        """Times `steps` launches; HIP events on the launch stream give the
+        average launch duration, wall time the throughput."""
+        torch = self.torch
+        with torch.cuda.stream(self.stream):
+            for _ in range(warmup):
+                self.reset()
+                self.launch(form, budget)
+            self.stream.synchronize()
+            starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+            ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(steps):
+                self.reset()
+                starts[k].record(self.stream)
+                self.launch(form, budget)
+                ends[k].record(self.stream)
+            self.stream.synchronize()
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        iters = int(self.it.sum().item())
+        out = {"value": iters * steps / wall, "wall_s": wall, "fw_iters_per_step": iters,
+               "launch_ms": float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)])),
+               "node_status_hist": np.bincount(self.st.cpu().numpy() & 0x7F, minlength=5)[:5].tolist()}
+        if form.startswith("mixed"):
+            out["mixed_iters_per_step"] = int(self.mit.sum().item())
+        if form == "batched":
+            sv = self.stats.cpu().numpy()
+            out["column_updates"] = int(sv[:, 0].sum())
+            out["direct_products"] = int(sv[:, 1].sum())
+        return out
 
 
 def cpu_baseline(args, Q, c, LB, UB, EPS):
-    """The CPU oracle (FP64 restatement of the reference algorithm) on a
-    bounded sample of the same node batch, node-parallel on host threads."""
+    """The CPU oracle (FP64 restatement of the plain form) on a bounded sample
+    of the same node batch, node-parallel on host threads."""
     from oracle import pyoracle as orc
     threads = max(1, min(16, os.cpu_count() or 1))
     sample = min(len(LB), max(threads, args.cpu_nodes))
@@ -133,43 +173,55 @@ def cpu_baseline(args, Q, c, LB, UB, EPS):
     iters = int(r["iterations"].sum())
     return {"value": iters / dt, "unit": "FW iterations/s", "cores": threads, "kind": "port",
             "sample": f"{sample} of the {len(LB)} nodes, budget {args.budget}, {iters} iterations "
-                      f"in {dt:.2f} s"}
+                      f"in {dt:.2f} s (plain-form algorithm, FP64)"}
+
+
+def solve_row(gb, Q, c, n, form, learned=None):
+    import torch
+    dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
+    dc = torch.from_numpy(c).cuda()
+    obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
+    s = gb.default_settings()
+    s.abs_gap_tol = 1e-6
+    s.rel_gap_tol = 1e-6
+    s.time_limit_seconds = 120.0
+    s.form = form
+    if form == 2:
+        s.precision = gb.PREC_BF16
+    if learned is not None:
+        s.branch_rule = gb.BRANCH_LEARNED
+        s.learned_model = learned.handle
+    r = gb.solve(n, [0] * n, [1] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), s)
+    return {"seconds": round(r["elapsed_seconds"], 5), "relax_seconds": round(r["relax_seconds"], 5),
+            "nodes": r["nodes_processed"], "fw_iterations": r["fw_iterations"], "objective": r["objective"],
+            "dual_bound": r["dual_bound"], "proven": not r["reached_time_limit"]}
 
 
 def time_to_optimum(args):
-    """gb_solve to the proven optimum (abs = rel gap 1e-6), plain form
-    (reference defaults) and batched form, on the tractable binary-QP sizes."""
-    import torch
-
+    """gb_solve to the proven optimum (abs = rel gap 1e-6): plain form
+    (reference defaults), batched FP64 form, mixed form (BF16 MFMA + FP64
+    certification) and learned + mixed, on the tractable binary-QP sizes."""
     import gpuboscia_amd as gb
     from gpuboscia_amd import instances
+    scorer = gb.Brancher.from_file(SCORER) if os.path.exists(SCORER) else None
+    forms = [("plain", 0, None), ("batched", 1, None), ("mixed", 2, None)]
+    if scorer is not None:
+        forms.append(("learned_mixed", 2, scorer))
     out = []
     for n, rows, seeds in ((20, 30, (1, 2, 3)), (40, 60, (1, 2, 3)), (60, 90, (1, 2, 3))):
         for seed in seeds:
             Q, c, sha = instances.binary_qp(n, seed=seed, rows=rows)
-            dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
-            dc = torch.from_numpy(c).cuda()
-            obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
             row = {"n": n, "seed": seed, "sha256": sha[:16]}
-            for form in (0, 1):
-                s = gb.default_settings()
-                s.abs_gap_tol = 1e-6
-                s.rel_gap_tol = 1e-6
-                s.time_limit_seconds = 120.0
-                s.form = form
-                r = gb.solve(n, [0] * n, [1] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), s)
-                key = "plain" if form == 0 else "batched"
-                row[key] = {"seconds": round(r["elapsed_seconds"], 5),
-                            "relax_seconds": round(r["relax_seconds"], 5), "nodes": r["nodes_processed"],
-                            "fw_iterations": r["fw_iterations"], "objective": r["objective"],
-                            "dual_bound": r["dual_bound"], "proven": not r["reached_time_limit"]}
-            fp, fb = row["plain"]["objective"], row["batched"]["objective"]
-            row["same_optimum"] = abs(fp - fb) <= 1e-9 * max(1.0, abs(fp))
-            row["speedup"] = round(row["plain"]["seconds"] / max(row["batched"]["seconds"], 1e-9), 2)
+            for key, form, learned in forms:
+                row[key] = solve_row(gb, Q, c, n, form, learned)
+            fp = row["plain"]["objective"]
+            row["same_optimum"] = all(abs(row[k]["objective"] - fp) <= 1e-9 * max(1.0, abs(fp))
+                                      for k, _, _ in forms)
+            for key, _, _ in forms[1:]:
+                row[f"speedup_{key}"] = round(row["plain"]["seconds"] / max(row[key]["seconds"], 1e-9), 2)
             out.append(row)
-            print(f"[bench] time-to-optimum n={n} seed={seed}: plain {row['plain']['seconds']:.4f}s "
-                  f"batched {row['batched']['seconds']:.4f}s nodes {row['plain']['nodes']}/"
-                  f"{row['batched']['nodes']}", file=sys.stderr, flush=True)
+            say(f"time-to-optimum n={n} seed={seed}: " +
+                " ".join(f"{k} {row[k]['seconds']:.4f}s/{row[k]['nodes']}n" for k, _, _ in forms))
     return out
 
 
@@ -182,8 +234,12 @@ def main():
     ap.add_argument("--n", type=int, default=200)
     ap.add_argument("--budget", type=int, default=200)
     ap.add_argument("--cpu-nodes", type=int, default=32)
+    ap.add_argument("--scale-nodes", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tto", action="store_true")
+    ap.add_argument("--no-scale", action="store_true")
+    ap.add_argument("--forms", default="plain,batched,mixed,mixed_pass1",
+                    help="forms to time (the profiled runs time one)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -196,50 +252,67 @@ def main():
     n = args.n
     Q, c, sha = instances.binary_qp(n, seed=1)
     LB, UB, EPS = node_batch(n, args.nodes, seed=2)
-    print(f"[bench] n={n} nodes={args.nodes} budget={args.budget} steps={args.steps}",
-          file=sys.stderr, flush=True)
-    plain = gpu_relax_throughput(args, Q, c, LB, UB, EPS, "plain")
-    print(f"[bench] plain: {plain['value']:.4g} FW it/s, kernel {plain['kernel_ms_avg']:.3f} ms",
-          file=sys.stderr, flush=True)
-    fast = gpu_relax_throughput(args, Q, c, LB, UB, EPS, "batched")
-    print(f"[bench] batched: {fast['value']:.4g} FW it/s, kernel {fast['kernel_ms_avg']:.3f} ms",
-          file=sys.stderr, flush=True)
-    kernel_s = fast["kernel_ms_avg"] / 1e3
-    # FP64 flops the batched kernel executed on matrix work: 2n per Q-column
-    # update, 2n^2 per direct product (counted by the kernel itself).
-    flops = (fast["column_updates"] * 2.0 * n + fast["direct_products"] * 2.0 * n * n) / args.steps
-    achieved = flops / kernel_s / 1e12
-    plain_flops = plain["iters_per_step"] * 2.0 * n * n
+    say(f"n={n} nodes={args.nodes} budget={args.budget} steps={args.steps}")
+    batch = Batch(Q, c, LB, UB, EPS)
+    forms = args.forms.split(",")
+    res = {}
+    for form in forms:
+        res[form] = batch.measure(form, args.budget, args.steps, args.warmup)
+        say(f"{form}: {res[form]['value']:.4g} FW it/s, launch {res[form]['launch_ms']:.3f} ms")
+    main_form = "mixed" if "mixed" in res else forms[0]
+    m = res[main_form]
     result = {
-        "metric": "FW iterations/s of the node relaxation (batched form) on n=200 binary-QP nodes; "
-                  "time-to-proven-optimum @gap 1e-6 plain vs batched in time_to_optimum",
-        "value": fast["value"], "unit": "FW iterations/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": fast["wall_s"] / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "metric": "FW iterations/s of the node relaxation on n=200 binary-QP nodes (mixed-precision form: BF16 "
+                  "MFMA pass + FP64 certification); time-to-proven-optimum @gap 1e-6 plain vs batched / mixed / "
+                  "learned+mixed in time_to_optimum",
+        "value": m["value"], "unit": "FW iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": m["wall_s"] / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic",
-        "config": {"workload": f"binary QP n={n} (A {int(1.5*n)}x{n}), {args.nodes} open B&B nodes, "
-                               f"DICG, budget {args.budget} iterations/node/step",
-                   "instance_sha256": sha[:16], "form": "batched (persistent wave64, FP64)"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_TFLOPS, "traffic": None,
-                     "kernel": "relax_fast_kernel<4>", "kernel_ms": fast["kernel_ms_avg"],
-                     "basis": "FP64 matrix flops executed (2n per column update + 2n^2 per "
-                              "direct product, kernel-counted) / kernel time; FP64 peak"},
-        "fw_iters_per_step": fast["iters_per_step"], "node_status_hist": fast["status_hist"],
-        "plain": {"value": plain["value"], "kernel_ms": plain["kernel_ms_avg"],
-                  "fw_iters_per_step": plain["iters_per_step"],
-                  "roofline_achieved_tflops": plain_flops / (plain["kernel_ms_avg"] / 1e3) / 1e12,
-                  "basis": "2 n^2 per FW iteration (one gradient), DICG + secant reference defaults",
-                  "node_status_hist": plain["status_hist"]},
-        "speedup_fw_iters_per_s": fast["value"] / plain["value"],
+        "config": {"workload": f"binary QP n={n} (A {int(1.5 * n)}x{n}), {args.nodes} open B&B nodes, DICG, "
+                               f"budget {args.budget} iterations/node/step",
+                   "instance_sha256": sha[:16], "form": main_form,
+                   "precision": "BF16 MFMA products (FP32 accumulate), FP32 iterates, FP64 steps; FP64 "
+                                "certificate of gap, value and incumbent"},
+        "forms": res,
     }
+    if "plain" in res:
+        result["speedup_vs_plain"] = m["value"] / res["plain"]["value"]
+    if "mixed_pass1" in res:
+        p1 = res["mixed_pass1"]
+        kernel_s = p1["launch_ms"] / 1e3
+        alg_flops = 2.0 * n * n * p1["mixed_iters_per_step"]
+        n_pad = 32 * ((n + 31) // 32)
+        mfma_flops = 2 * 2.0 * n_pad * n_pad * p1["mixed_iters_per_step"]  # Q v and Q a per iteration
+        achieved = alg_flops / kernel_s / 1e12
+        result["roofline"] = {
+            "bound": "mfma", "achieved": achieved, "peak": MI355X_BF16_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / MI355X_BF16_TFLOPS, "traffic": None, "kernel": "relax_mixed_kernel",
+            "kernel_ms": p1["launch_ms"],
+            "basis": "algorithmic 2 n^2 flops per FW iteration x iterations of one launch / launch time "
+                     "(HIP events on the launch stream); bf16 dense MFMA peak",
+            "useful_mfma_tflops": mfma_flops / kernel_s / 1e12,
+            "useful_mfma_basis": "2 products (Q v, Q a) x 2 n_pad^2 per column-iteration, n_pad = 32 ceil(n/32)"}
+    if not args.no_scale and "mixed" in forms:
+        LBs, UBs, EPSs = node_batch(n, args.scale_nodes, seed=3)
+        sb = Batch(Q, c, LBs, UBs, EPSs)
+        steps = max(2, args.steps // 2)
+        sc = {f: sb.measure(f, args.budget, steps, 1) for f in ("mixed", "batched")}
+        result["scale"] = {"nodes": args.scale_nodes, "steps": steps,
+                           "mixed": {"value": sc["mixed"]["value"], "launch_ms": sc["mixed"]["launch_ms"]},
+                           "batched": {"value": sc["batched"]["value"], "launch_ms": sc["batched"]["launch_ms"]}}
+        say(f"scale {args.scale_nodes}: mixed {sc['mixed']['value']:.4g} batched {sc['batched']['value']:.4g}")
+        del sb
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, Q, c, LB, UB, EPS)
     if not args.no_tto:
         tto = time_to_optimum(args)
         result["time_to_optimum"] = tto
-        sp = [r["speedup"] for r in tto if r["n"] >= 40]
-        result["time_to_optimum_speedup_geomean_n40_60"] = float(np.exp(np.mean(np.log(sp))))
+        for key in ("batched", "mixed", "learned_mixed"):
+            sp = [r[f"speedup_{key}"] for r in tto if f"speedup_{key}" in r and r["n"] >= 40]
+            if sp:
+                result[f"time_to_optimum_speedup_geomean_n40_60_{key}"] = float(np.exp(np.mean(np.log(sp))))
+        result["time_to_optimum_all_same_optimum"] = all(r["same_optimum"] for r in tto)
     print(json.dumps(result), flush=True)
 
 

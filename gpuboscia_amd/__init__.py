@@ -59,7 +59,7 @@ class MixedOptions(ctypes.Structure):
     _fields_ = [("row_tiles_per_wave", ctypes.c_int), ("columns", ctypes.c_int), ("blocks", ctypes.c_int),
                 ("rel_floor", ctypes.c_float),
                 ("fp64_max_iters", ctypes.c_int), ("mixed_iterations", ctypes.c_void_p),
-                ("fp64_stats", ctypes.c_void_p)]
+                ("fp64_stats", ctypes.c_void_p), ("gradient", ctypes.c_void_p)]
 
 
 class Problem(ctypes.Structure):
@@ -292,7 +292,7 @@ def relax_batch_mixed(mq: MixedQ, obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: L
                       columns=0, blocks=0, mixed_iterations=None, stream=None):
     """gb_relax_batch_mixed on torch device tensors."""
     opt = MixedOptions(row_tiles_per_wave, columns, blocks, rel_floor, fp64_max_iters,
-                       0 if mixed_iterations is None else mixed_iterations.data_ptr(), 0)
+                       0 if mixed_iterations is None else mixed_iterations.data_ptr(), 0, 0)
     check(load().gb_relax_batch_mixed(mq.handle, ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo),
                                       int(N), int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
                                       _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),

@@ -227,9 +227,13 @@ struct DeviceBatch {
     int* hit = nullptr;
     uint8_t* hst = nullptr;
     hipStream_t stream = nullptr;
-    // learned branching / labelling: gradients, feature rows, scores
+    // learned branching / labelling: gradients, feature rows, scores, node
+    // depths, per-variable pseudocost means (up, down; < 0 = no history), Q
+    // diagonal
     double *g = nullptr, *hg = nullptr;
     float *feat = nullptr, *hfeat = nullptr, *score = nullptr, *hscore = nullptr;
+    int *depth = nullptr, *hdepth = nullptr;
+    double *pc = nullptr, *hpc = nullptr, *qdiag = nullptr;
 
     gb_status alloc_features() {
         const size_t xn = (size_t)cap * n;
@@ -239,6 +243,11 @@ struct DeviceBatch {
         GB_HIP(hipHostMalloc(&hfeat, xn * kMlpFeatures * 4, hipHostMallocDefault));
         GB_HIP(hipMalloc(&score, xn * 4));
         GB_HIP(hipHostMalloc(&hscore, xn * 4, hipHostMallocDefault));
+        GB_HIP(hipMalloc(&depth, (size_t)cap * 4));
+        GB_HIP(hipHostMalloc(&hdepth, (size_t)cap * 4, hipHostMallocDefault));
+        GB_HIP(hipMalloc(&pc, (size_t)n * 16));
+        GB_HIP(hipHostMalloc(&hpc, (size_t)n * 16, hipHostMallocDefault));
+        GB_HIP(hipMalloc(&qdiag, (size_t)n * 8));
         return GB_OK;
     }
 
@@ -276,11 +285,11 @@ struct DeviceBatch {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : {(void*)x, (void*)lb, (void*)ub, (void*)bv, (void*)eps, (void*)gap,
                         (void*)val, (void*)bvv, (void*)cut, (void*)it, (void*)st, (void*)flags,
-                        (void*)g, (void*)feat, (void*)score})
+                        (void*)g, (void*)feat, (void*)score, (void*)depth, (void*)pc, (void*)qdiag})
             if (p) (void)hipFree(p);
         for (void* p : {(void*)hx, (void*)hlb, (void*)hub, (void*)hbv, (void*)heps, (void*)hgap,
                         (void*)hval, (void*)hbvv, (void*)hcut, (void*)hit, (void*)hst, (void*)hg,
-                        (void*)hfeat, (void*)hscore})
+                        (void*)hfeat, (void*)hscore, (void*)hdepth, (void*)hpc})
             if (p) (void)hipHostFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -319,7 +328,8 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
         return fail(GB_ERR_INVALID_ARGUMENT, "custom branching requires a branch callback");
     if (s->branch_rule == GB_BRANCH_LEARNED && s->learned_model && o->kind != GB_OBJ_DENSE_QUADRATIC)
         return fail(GB_ERR_INVALID_ARGUMENT, "the learned scorer needs a dense quadratic objective");
-    if (s->branch_rule == GB_BRANCH_POW2 && (s->branch_k < 1 || s->branch_k > 16))
+    if ((s->branch_rule == GB_BRANCH_POW2 || s->branch_rule == GB_BRANCH_LEARNED)
+            && (s->branch_k < 1 || s->branch_k > 16))
         return fail(GB_ERR_INVALID_ARGUMENT, "branch_k must be in [1, 16]");
     if (s->branch_rule == GB_BRANCH_CALLBACK && !s->branch_score)
         return fail(GB_ERR_INVALID_ARGUMENT, "callback branching requires branch_score");
@@ -482,6 +492,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         qdiag.resize(n);
         GB_HIP(hipMemcpy2D(qdiag.data(), sizeof(double), o->Qt, (size_t)(n + 1) * sizeof(double),
                            sizeof(double), n, hipMemcpyDeviceToHost));
+        if (learned) GB_HIP(hipMemcpy(db.qdiag, qdiag.data(), (size_t)n * 8, hipMemcpyHostToDevice));
     }
     std::vector<uint8_t> hflags(n, 0);
     for (int i : iv) hflags[i] = 1;
@@ -568,6 +579,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             db.heps[i] = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth);
             db.hit[i] = nd.riters;
             db.hst[i] = nd.has_rx ? GB_NODE_INITIAL_POINT_SUPPLIED : 0;
+            if (learned) db.hdepth[i] = nd.depth;
             active.insert(nd.bound);
         }
         const auto r0 = Clock::now();
@@ -592,7 +604,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             }
             GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
             GB_HIP(hipMemcpyAsync(db.bvv, db.hbvv, N * 8, hipMemcpyHostToDevice, db.stream));
-            const gb_mixed_options mo{0, 0, 0, 1e-6f, 0, nullptr, nullptr};
+            const gb_mixed_options mo{0, 0, 0, 1e-6f, 0, nullptr, nullptr, learned ? db.g : nullptr};
             st = gb_relax_batch_mixed(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb,
                                       db.ub, db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv,
                                       db.flags, s->integer_tol, db.cut, &mo, db.stream);
@@ -610,44 +622,44 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         GB_HIP(hipMemcpyAsync(db.hit, db.it, N * 4, hipMemcpyDeviceToHost, db.stream));
         GB_HIP(hipMemcpyAsync(db.hst, db.st, N, hipMemcpyDeviceToHost, db.stream));
         if (learned) {
-            if (s->form != 1) {
+            // Scores of every (node, variable) on the device, returned with
+            // the batch: gradient (from the relaxation kernels, or one
+            // product launch for the plain form), feature rows, MLP.
+            if (s->form == 0) {
                 const int ge = launch_batched_gradient(o->Qt, o->linear, o->grad_scale, o->linear_scale,
                                                        n, N, db.x, db.g, db.stream);
                 if (ge != 0) return hip_fail(static_cast<hipError_t>(ge), __FILE__, __LINE__);
             }
-            GB_HIP(hipMemcpyAsync(db.hg, db.g, (size_t)N * n * 8, hipMemcpyDeviceToHost, db.stream));
+            for (int j = 0; j < n; ++j) {
+                db.hpc[j] = pc.up_cnt[j] ? pc.up_sum[j] / pc.up_cnt[j] : -1.0;
+                db.hpc[n + j] = pc.dn_cnt[j] ? pc.dn_sum[j] / pc.dn_cnt[j] : -1.0;
+            }
+            GB_HIP(hipMemcpyAsync(db.pc, db.hpc, (size_t)n * 16, hipMemcpyHostToDevice, db.stream));
+            GB_HIP(hipMemcpyAsync(db.depth, db.hdepth, (size_t)N * 4, hipMemcpyHostToDevice, db.stream));
+            const int fe = launch_candidate_features(n, N, db.x, db.g, db.lb, db.ub, db.depth, db.gap, db.val,
+                                                     db.pc, db.pc + n, db.qdiag, o->grad_scale, db.flags,
+                                                     s->integer_tol, db.feat, db.stream);
+            if (fe != 0) return hip_fail(static_cast<hipError_t>(fe), __FILE__, __LINE__);
+            st = gb_brancher_predict(s->learned_model, db.feat, N * n, db.score, db.stream);
+            if (st != GB_OK) return st;
+            GB_HIP(hipMemcpyAsync(db.hscore, db.score, (size_t)N * n * 4, hipMemcpyDeviceToHost, db.stream));
         }
         GB_HIP(hipStreamSynchronize(db.stream));
         relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
 
-        // Learned scores for every fractional candidate of the batch: one
-        // gradient launch, host feature rows, one MLP launch.
+        // Learned scores of each node's fractional candidates.
         std::vector<int> score_off(N + 1, 0);
         std::vector<std::vector<int>> score_cand(learned ? N : 0);
+        std::vector<float> cand_score;
         if (learned) {
-            int rows = 0;
             for (int i = 0; i < N; ++i) {
-                score_off[i] = rows;
+                score_off[i] = (int)cand_score.size();
                 if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
-                    const double* xi = db.hx + (size_t)i * n;
-                    score_cand[i] = fractional_candidates(xi, iv, s->integer_tol);
-                    candidate_features(xi, db.hg + (size_t)i * n, pool.lb(batch[i].slot),
-                                       pool.ub(batch[i].slot), n, batch[i].depth, db.hgap[i], db.hval[i],
-                                       pc, qdiag, o->grad_scale, score_cand[i],
-                                       db.hfeat + (size_t)rows * kMlpFeatures);
-                    rows += (int)score_cand[i].size();
+                    score_cand[i] = fractional_candidates(db.hx + (size_t)i * n, iv, s->integer_tol);
+                    for (int v : score_cand[i]) cand_score.push_back(db.hscore[(size_t)i * n + v]);
                 }
             }
-            score_off[N] = rows;
-            if (rows > 0) {
-                GB_HIP(hipMemcpyAsync(db.feat, db.hfeat, (size_t)rows * kMlpFeatures * 4,
-                                      hipMemcpyHostToDevice, db.stream));
-                st = gb_brancher_predict(s->learned_model, db.feat, rows, db.score, db.stream);
-                if (st != GB_OK) return st;
-                GB_HIP(hipMemcpyAsync(db.hscore, db.score, (size_t)rows * 4, hipMemcpyDeviceToHost,
-                                      db.stream));
-                GB_HIP(hipStreamSynchronize(db.stream));
-            }
+            score_off[N] = (int)cand_score.size();
         }
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
@@ -735,13 +747,18 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         vars.push_back(cand[bi]);
                     }
                 } else if (learned && score_off[i + 1] > score_off[i]) {
-                    // highest predicted strong-branching score, ties to the lowest index
-                    int bi = 0;
-                    const float* sc = db.hscore + score_off[i];
+                    // the branch_k candidates with the highest predicted
+                    // strong-branching score (ties to the lower index), split
+                    // 2^k ways like branch_pow2
+                    const float* sc = cand_score.data() + score_off[i];
                     const int cnt = score_off[i + 1] - score_off[i];
-                    for (int c = 1; c < cnt; ++c)
-                        if (sc[c] > sc[bi]) bi = c;
-                    vars.push_back(score_cand[i][bi]);
+                    std::vector<int> order(cnt);
+                    for (int c = 0; c < cnt; ++c) order[c] = c;
+                    const int k = std::min(s->branch_k, cnt);
+                    std::partial_sort(order.begin(), order.begin() + k, order.end(), [&](int a, int b) {
+                        return sc[a] > sc[b] || (sc[a] == sc[b] && score_cand[i][a] < score_cand[i][b]);
+                    });
+                    for (int c = 0; c < k; ++c) vars.push_back(score_cand[i][order[c]]);
                 } else {
                     // GB_BRANCH_MOST_FRACTIONAL, and the learned rule without a model
                     const int v = most_fractional(xi, iv, s->integer_tol);
@@ -852,7 +869,9 @@ extern "C" gb_status gb_branching_dataset(const gb_problem* problem, const gb_ob
         return gb::fail(GB_ERR_INVALID_ARGUMENT, "strong-branching labels need a dense objective over a box");
     gb_settings s = *settings;
     s.form = 1;
-    s.branch_rule = GB_BRANCH_MOST_FRACTIONAL;
+    // the tree the scorer will be used in: branch_k-variable splits, chosen
+    // here by fractionality (top_k_infeasible)
+    s.branch_rule = GB_BRANCH_POW2;
     gb::Labeler lab;
     lab.max_nodes = max_nodes;
     lab.features = features;

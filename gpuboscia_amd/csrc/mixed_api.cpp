@@ -110,7 +110,7 @@ extern "C" gb_status gb_relax_batch_mixed(const gb_mixed_q* q, const gb_objectiv
         if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
                 || !best_vertex_val)
             return fail(GB_ERR_INVALID_ARGUMENT, "null node buffer");
-        gb_mixed_options op{0, 0, 0, 1e-6f, 0, nullptr, nullptr};
+        gb_mixed_options op{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
         if (options) op = *options;
         if (op.row_tiles_per_wave < 0 || op.row_tiles_per_wave > kMixedMaxRowTilesPerWave || op.columns < 0
                 || op.columns > 16 || op.blocks < 0 || !(op.rel_floor >= 0.0f))
@@ -158,7 +158,7 @@ extern "C" gb_status gb_relax_batch_mixed(const gb_mixed_q* q, const gb_objectiv
         // FP64-certified bound, and resumes as a continuation).
         const int fp64_iters = op.fp64_max_iters > 0 ? op.fp64_max_iters
                                                      : (max_iters / 16 > 1 ? max_iters / 16 : 1);
-        const gb_fast_options fo{64, 4, 0, op.fp64_stats, nullptr};
+        const gb_fast_options fo{64, 4, 0, op.fp64_stats, op.gradient};
         return relax_batch_fast(objective, algorithm, lmo, N, fp64_iters, x, lb, ub, eps, iterations,
                                 status, gap, value, best_vertex, best_vertex_val, integer_flags,
                                 candidate_tol, cutoff, &fo, stream, nullptr, true);

@@ -429,6 +429,78 @@ __global__ void __launch_bounds__(64) batched_gradient_kernel(const double* Qt, 
     }
 }
 
+// Feature rows of the learned scorer for every (node, variable) of a batch,
+// row node * n + j; rows of variables that are not fractional candidates
+// are zero.  Same definitions as the host rows of the label generator:
+// 0 x_j, 1 min(frac, 1 - frac), 2 g_j / max|g|, 3 / 4 up / down gain
+// (pseudocost mean gain per unit, or 1/2 s_g Q_jj d without history for
+// the variable, times the distance d moved) over its mean across the node's
+// candidates, 5 depth / n, 6 gap / (1 + |f|), 7 ub_j - lb_j.  pcu / pcd < 0
+// means no history.  One wave per node.
+__global__ void __launch_bounds__(64) candidate_features_kernel(
+    int n, int N, const double* X, const double* G, const double* LB, const double* UB, const int* depth,
+    const double* gap, const double* val, const double* pcu, const double* pcd, const double* qdiag,
+    double gs, const uint8_t* flags, double tol, float* F) {
+    const int node = blockIdx.x;
+    if (node >= N) return;
+    const int lane = threadIdx.x;
+    const size_t base = static_cast<size_t>(node) * n;
+    const double* x = X + base;
+    const double* g = G + base;
+    double gmax = 0.0, su = 0.0, sd = 0.0;
+    int cnt = 0;
+    auto is_cand = [&](int j, double& frac) {
+        frac = x[j] - floor(x[j]);
+        return flags[j] != 0 && fmin(frac, 1.0 - frac) > tol;
+    };
+    auto gains = [&](int j, double& gu, double& gd) {
+        const double du = ceil(x[j]) - x[j], dd = x[j] - floor(x[j]);
+        const double curv = 0.5 * gs * qdiag[j];
+        gu = (pcu[j] >= 0.0 ? pcu[j] : curv * du) * du;
+        gd = (pcd[j] >= 0.0 ? pcd[j] : curv * dd) * dd;
+    };
+    for (int j = lane; j < n; j += kWave) {
+        gmax = fmax(gmax, fabs(g[j]));
+        double frac;
+        if (is_cand(j, frac)) {
+            double gu, gd;
+            gains(j, gu, gd);
+            su += gu;
+            sd += gd;
+            ++cnt;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        gmax = fmax(gmax, __shfl_xor(gmax, off, kWave));
+        su += __shfl_xor(su, off, kWave);
+        sd += __shfl_xor(sd, off, kWave);
+        cnt += __shfl_xor(cnt, off, kWave);
+    }
+    const double mu = su / (cnt ? static_cast<double>(cnt) : 1.0);
+    const double md = sd / (cnt ? static_cast<double>(cnt) : 1.0);
+    const double rgap = gap[node] / (1.0 + fabs(val[node]));
+    for (int j = lane; j < n; j += kWave) {
+        float* f = F + (base + j) * 8;
+        double frac;
+        if (is_cand(j, frac)) {
+            double gu, gd;
+            gains(j, gu, gd);
+            f[0] = static_cast<float>(x[j]);
+            f[1] = static_cast<float>(fmin(frac, 1.0 - frac));
+            f[2] = static_cast<float>(g[j] / (gmax + 1e-12));
+            f[3] = static_cast<float>(gu / (mu + 1e-12));
+            f[4] = static_cast<float>(gd / (md + 1e-12));
+            f[5] = static_cast<float>(depth[node]) / static_cast<float>(n);
+            f[6] = static_cast<float>(rgap);
+            f[7] = static_cast<float>(UB[base + j] - LB[base + j]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) f[k] = 0.0f;
+        }
+    }
+}
+
 constexpr size_t kFastLdsBudget = 160 * 1024;
 
 }  // namespace
@@ -439,6 +511,16 @@ int launch_batched_gradient(const double* Qt, const double* lin, double gs, doub
     hipLaunchKernelGGL(batched_gradient_kernel, dim3(N), dim3(64), 0,
                        reinterpret_cast<hipStream_t>(stream), Qt, lin, gs, ls, n, N, X, G);
     return (int)hipGetLastError();
+}
+
+int launch_candidate_features(int n, int N, const double* X, const double* G, const double* LB,
+                              const double* UB, const int* depth, const double* gap, const double* val,
+                              const double* pcu, const double* pcd, const double* qdiag, double gs,
+                              const uint8_t* flags, double tol, float* F, void* stream) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(candidate_features_kernel, dim3(N), dim3(kWave), 0, reinterpret_cast<hipStream_t>(stream),
+                       n, N, X, G, LB, UB, depth, gap, val, pcu, pcd, qdiag, gs, flags, tol, F);
+    return static_cast<int>(hipGetLastError());
 }
 
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream) {

@@ -25,20 +25,24 @@ def _objective(Q, c):
     return obj, (dQt, dc)
 
 
-def _settings(time_limit=60.0, max_nodes=0):
+def _settings(time_limit=60.0, max_nodes=0, form=1):
     s = gb.default_settings()
     s.abs_gap_tol = 1e-6
     s.rel_gap_tol = 1e-6
-    s.form = 1
+    s.form = form
+    if form == 2:
+        s.precision = gb.PREC_BF16
     s.time_limit_seconds = time_limit
     s.max_nodes = max_nodes
     return s
 
 
-def generate(n, seeds, labeled_nodes=16, capacity_per_instance=4096, bnb_node_cap=5000):
+def generate(n, seeds, labeled_nodes=16, capacity_per_instance=4096, bnb_node_cap=5000, progress=False):
     """Strong-branching rows for binary QPs of size n (one instance per seed)."""
     Xs, Ts = [], []
-    for seed in seeds:
+    for k, seed in enumerate(seeds):
+        if progress and k % 256 == 0:
+            print(f"[learned] labelling instance {k}", flush=True)
         Q, c, _ = instances.binary_qp(n, seed=seed)
         obj, keep = _objective(Q, c)
         F, L = gb.branching_dataset(n, [0] * n, [1] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0),
@@ -48,8 +52,10 @@ def generate(n, seeds, labeled_nodes=16, capacity_per_instance=4096, bnb_node_ca
     return np.concatenate(Xs), np.concatenate(Ts)
 
 
-def fit(X, T, epochs=30, batch=1024, lr=2e-3, seed=0):
-    """Trains a scorer on (X, T) with the library's GPU kernels."""
+def fit(X, T, epochs=30, batch=1024, lr=2e-3, seed=0, progress=False):
+    """Trains a scorer on (X, T) with the library's GPU kernels (Adam state
+    persists in the scorer, so training in chunks of epochs is the same as
+    one call)."""
     import torch
     mu = X.mean(0)
     sigma = X.std(0)
@@ -61,15 +67,22 @@ def fit(X, T, epochs=30, batch=1024, lr=2e-3, seed=0):
     dX = torch.from_numpy(np.ascontiguousarray(X[perm], dtype=np.float32)).cuda()
     dT = torch.from_numpy(np.ascontiguousarray(T[perm], dtype=np.float32)).cuda()
     loss0, _ = b.loss_grad(dX, dT)
-    loss = b.train(dX, dT, epochs, batch, lr)
+    loss = loss0
+    done = 0
+    while done < epochs:
+        step = min(10, epochs - done)
+        loss = b.train(dX, dT, step, batch, lr)
+        done += step
+        if progress:
+            print(f"[learned] epoch {done}: loss {loss:.5f}", flush=True)
     return b, {"initial_loss": loss0, "final_loss": loss, "rows": int(len(T)), "mu": mu.tolist(),
                "sigma": sigma.tolist()}
 
 
-def solve_with(Q, c, rule, brancher=None, time_limit=120.0):
+def solve_with(Q, c, rule, brancher=None, time_limit=120.0, form=2):
     n = Q.shape[0]
     obj, keep = _objective(Q, c)
-    s = _settings(time_limit)
+    s = _settings(time_limit, form=form)
     s.branch_rule = rule
     if brancher is not None:
         s.learned_model = brancher.handle
@@ -79,14 +92,17 @@ def solve_with(Q, c, rule, brancher=None, time_limit=120.0):
     return r
 
 
-def evaluate(brancher, n, seeds, time_limit=120.0):
-    """Node count and time of learned vs most-fractional branching on the
-    same instances, both to the proven optimum (gap 1e-6)."""
+def evaluate(brancher, n, seeds, time_limit=120.0, form=2):
+    """Node count and time of the learned rule vs the rule without it
+    (branch_pow2 on the most fractional variables, the default) on the same
+    instances and the same tree shape (k = 3 variables per split), both to the
+    proven optimum (gap 1e-6).  Keys keep the name "mf" for the
+    fractionality-based rule."""
     rows = []
     for seed in seeds:
         Q, c, sha = instances.binary_qp(n, seed=seed)
-        mf = solve_with(Q, c, gb.BRANCH_MOST_FRACTIONAL, None, time_limit)
-        ln = solve_with(Q, c, gb.BRANCH_LEARNED, brancher, time_limit)
+        mf = solve_with(Q, c, gb.BRANCH_POW2, None, time_limit, form)
+        ln = solve_with(Q, c, gb.BRANCH_LEARNED, brancher, time_limit, form)
         rows.append({"seed": seed, "sha256": sha[:16], "mf_nodes": mf["nodes_processed"],
                      "learned_nodes": ln["nodes_processed"], "mf_seconds": mf["wall"],
                      "learned_seconds": ln["wall"], "mf_objective": mf["objective"],
@@ -94,3 +110,48 @@ def evaluate(brancher, n, seeds, time_limit=120.0):
                      "same_optimum": abs(mf["objective"] - ln["objective"]) <= 1e-9 * max(1.0, abs(mf["objective"])),
                      "proven": not (mf["reached_time_limit"] or ln["reached_time_limit"])})
     return rows
+
+
+def train_main(argv=None):
+    """Trains a scorer on solver-generated strong-branching labels and writes it
+    (weights + feature normalization, plain arrays) to --out:
+
+      python -m gpuboscia_amd.learned --n 40 --train 3072 --out scorer.npz
+    """
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=40)
+    ap.add_argument("--train", type=int, default=3072, help="training instances (seeds 1000..)")
+    ap.add_argument("--test", type=int, default=64, help="test instances (seeds 50000..)")
+    ap.add_argument("--labeled-nodes", type=int, default=16)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    X, T = generate(a.n, seeds=range(1000, 1000 + a.train), labeled_nodes=a.labeled_nodes, progress=True)
+    t1 = time.perf_counter()
+    b, info = fit(X, T, epochs=a.epochs, lr=a.lr, progress=True)
+    t2 = time.perf_counter()
+    b.save(a.out)
+    rows = evaluate(b, a.n, seeds=range(50000, 50000 + a.test)) if a.test > 0 else []
+    mf = np.array([r["mf_nodes"] for r in rows], dtype=float)
+    ln = np.array([r["learned_nodes"] for r in rows], dtype=float)
+    summary = {"n": a.n, "train_instances": a.train, "rows": info["rows"], "label_seconds": round(t1 - t0, 2),
+               "fit_seconds": round(t2 - t1, 2), "initial_loss": info["initial_loss"],
+               "final_loss": info["final_loss"], "test_instances": len(rows)}
+    if rows:
+        summary.update({"mf_nodes": int(mf.sum()), "learned_nodes": int(ln.sum()),
+                        "node_ratio": float(ln.sum() / mf.sum()),
+                        "node_ratio_geomean": float(np.exp(np.mean(np.log(ln / mf)))),
+                        "mf_seconds": float(sum(r["mf_seconds"] for r in rows)),
+                        "learned_seconds": float(sum(r["learned_seconds"] for r in rows)),
+                        "same_optimum": all(r["same_optimum"] for r in rows),
+                        "proven": all(r["proven"] for r in rows)})
+    print(json.dumps(summary), flush=True)
+    return summary
+
+
+if __name__ == "__main__":
+    train_main()

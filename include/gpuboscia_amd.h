@@ -211,6 +211,7 @@ typedef struct {
     int fp64_max_iters;       /* budget of the FP64 pass (0 = max(1, max_iters / 16), < 0 = skip it) */
     long long* mixed_iterations;  /* optional device [N]: iterations of the mixed pass */
     long long* fp64_stats;        /* optional device [N][2]: as gb_fast_options.stats */
+    double* gradient;             /* optional device [N][n]: gradient at the returned x */
 } gb_mixed_options;
 
 /*
@@ -361,7 +362,8 @@ typedef int (*gb_branch_fn)(void* user, const double* lb, const double* ub,
 typedef enum {
     GB_BRANCH_POW2 = 0,            /* branch_pow2(k) on top_k_infeasible (default k=3) */
     GB_BRANCH_MOST_FRACTIONAL = 1, /* single variable, most_infeasible */
-    GB_BRANCH_LEARNED = 2,         /* on-GPU MLP scorer, falls back to most-fractional */
+    GB_BRANCH_LEARNED = 2,         /* on-GPU MLP scorer: the branch_k best-scored candidates,
+                                      split 2^k ways; most-fractional without a model */
     GB_BRANCH_CALLBACK = 3,        /* host scorer callback, 2-way split */
     GB_BRANCH_CUSTOM = 4           /* host BranchStrategy callback (gb_branch_fn) */
 } gb_branch_rule;
@@ -430,7 +432,8 @@ gb_status gb_solve(const gb_problem* problem,
                    gb_result* result);
 
 /* Strong-branching labels generated by the solver itself: runs the batched
-   B&B with most-fractional branching and, for the first max_nodes branched
+   B&B with branch_pow2 on the settings' branch_k most fractional variables
+   (the tree the scorer is used in) and, for the first max_nodes branched
    nodes, relaxes both children of every fractional candidate on the GPU.
    Each candidate yields a feature row (host float [8]) and the label
    score_i / max_j score_j with score = max(dn, 1e-6) * max(up, 1e-6) over
