@@ -62,6 +62,13 @@ class MixedOptions(ctypes.Structure):
                 ("fp64_stats", ctypes.c_void_p), ("gradient", ctypes.c_void_p)]
 
 
+class InstanceBatch(ctypes.Structure):
+    _fields_ = [("objective", ctypes.c_void_p), ("lmo", ctypes.c_void_p), ("mixed", ctypes.c_void_p),
+                ("N", ctypes.c_int)] + [(k, ctypes.c_void_p) for k in (
+                    "x", "lb", "ub", "eps", "iterations", "status", "gap", "value", "best_vertex",
+                    "best_vertex_val", "integer_flags", "cutoff")]
+
+
 class Problem(ctypes.Structure):
     _fields_ = [("n_vars", ctypes.c_int), ("lb", _dp), ("ub", _dp), ("int_vars", _ip),
                 ("n_int_vars", ctypes.c_int)]
@@ -92,7 +99,7 @@ class Settings(ctypes.Structure):
                 ("branch_score", BRANCH_SCORE_FN), ("branch_score_user", ctypes.c_void_p),
                 ("branch", BRANCH_FN), ("branch_user", ctypes.c_void_p),
                 ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_void_p),
-                ("max_nodes", ctypes.c_longlong)]
+                ("max_nodes", ctypes.c_longlong), ("autotune", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -112,7 +119,7 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset",
             "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
             "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
-            "gb_relax_batch_mixed"]
+            "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -188,7 +195,13 @@ def load():
     lib.gb_relax_batch_mixed.argtypes = [vp, ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
                                          ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
         + [vp] * 11 + [ctypes.c_double, vp, ctypes.POINTER(MixedOptions), vp]
-    for name in ("gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products", "gb_relax_batch_mixed"):
+    lib.gb_mixed_autotune.argtypes = [vp, ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
+                                      ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] + [vp] * 7 \
+        + [ctypes.c_double, vp, ctypes.POINTER(MixedOptions), ctypes.POINTER(ctypes.c_double), vp]
+    lib.gb_relax_instances.argtypes = [ctypes.c_int, vp, ctypes.POINTER(AlgorithmDesc), ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, vp]
+    for name in ("gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products", "gb_relax_batch_mixed",
+                 "gb_mixed_autotune", "gb_relax_instances"):
         getattr(lib, name).restype = ctypes.c_int
     ll = ctypes.POINTER(ctypes.c_longlong)
     lib.gb_grid_barrier_selftest.argtypes = [ctypes.c_int] * 5 + [ctypes.c_longlong, ll, ll,
@@ -299,6 +312,43 @@ def relax_batch_mixed(mq: MixedQ, obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: L
                                       _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
                                       float(candidate_tol), _ptr(cutoff), ctypes.byref(opt),
                                       ctypes.c_void_p(stream or 0)))
+
+
+def mixed_autotune(mq: MixedQ, obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb, ub,
+                   eps, iterations, status, flags=None, candidate_tol=1e-6, cutoff=None, stream=None):
+    """gb_mixed_autotune: (MixedOptions chosen, seconds of the winning full relaxation)."""
+    opt = MixedOptions(0, 0, 0, 1e-6, 0, 0, 0, 0)
+    sec = ctypes.c_double()
+    check(load().gb_mixed_autotune(mq.handle, ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
+                                   int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps), _ptr(iterations),
+                                   _ptr(status), _ptr(flags), float(candidate_tol), _ptr(cutoff),
+                                   ctypes.byref(opt), ctypes.byref(sec), ctypes.c_void_p(stream or 0)))
+    return opt, sec.value
+
+
+def relax_instances(entries, alg: AlgorithmDesc, max_iters, candidate_tol=1e-6, concurrency=4, stream=None):
+    """gb_relax_instances.  entries: dicts with obj, lmo, mixed (MixedQ or None) and
+    the torch buffers x, lb, ub, eps, iterations, status, gap, value,
+    best_vertex, best_vertex_val, flags, cutoff (None allowed for flags/cutoff)."""
+    arr = (InstanceBatch * len(entries))()
+    keep = []
+    for k, e in enumerate(entries):
+        keep.append((e["obj"], e["lmo"]))
+        b = arr[k]
+        b.objective = ctypes.addressof(e["obj"])
+        b.lmo = ctypes.addressof(e["lmo"])
+        b.mixed = e["mixed"].handle.value if e.get("mixed") is not None else None
+        b.N = int(e["x"].shape[0])
+        for name, key in (("x", "x"), ("lb", "lb"), ("ub", "ub"), ("eps", "eps"), ("iterations", "iterations"),
+                          ("status", "status"), ("gap", "gap"), ("value", "value"),
+                          ("best_vertex", "best_vertex"), ("best_vertex_val", "best_vertex_val"),
+                          ("integer_flags", "flags"), ("cutoff", "cutoff")):
+            t = e.get(key)
+            setattr(b, name, None if t is None else t.data_ptr())
+    check(load().gb_relax_instances(len(entries), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(alg),
+                                    int(max_iters), float(candidate_tol), int(concurrency),
+                                    ctypes.c_void_p(stream or 0)))
+    del keep
 
 
 class RelaxGraph:

@@ -506,6 +506,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         if (st != GB_OK) return st;
         mq.reset(h);
     }
+    gb_mixed_options tune{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
+    bool tuned = false;
     if (lab) GB_HIP(hipMemcpy(sb.flags, hflags.data(), n, hipMemcpyHostToDevice));
 
     std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
@@ -604,7 +606,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             }
             GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
             GB_HIP(hipMemcpyAsync(db.bvv, db.hbvv, N * 8, hipMemcpyHostToDevice, db.stream));
-            const gb_mixed_options mo{0, 0, 0, 1e-6f, 0, nullptr, nullptr, learned ? db.g : nullptr};
+            if (s->autotune && !tuned && N >= 64) {
+                st = gb_mixed_autotune(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
+                                       db.eps, db.it, db.st, db.flags, s->integer_tol, db.cut, &tune, nullptr,
+                                       db.stream);
+                if (st != GB_OK) return st;
+                tuned = true;
+            }
+            gb_mixed_options mo = tune;
+            mo.gradient = learned ? db.g : nullptr;
             st = gb_relax_batch_mixed(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb,
                                       db.ub, db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv,
                                       db.flags, s->integer_tol, db.cut, &mo, db.stream);
@@ -857,6 +867,7 @@ extern "C" void gb_default_settings(gb_settings* s) {
     s->natural_lower_bound = -std::numeric_limits<double>::infinity();
     s->precision = GB_PREC_FP64;
     s->max_nodes = 0;
+    s->autotune = 0;
 }
 
 extern "C" gb_status gb_branching_dataset(const gb_problem* problem, const gb_objective_desc* objective,

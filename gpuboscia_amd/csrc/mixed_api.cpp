@@ -3,7 +3,9 @@
 // polishes, certifies and re-scores), and the product diagnostic.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "gb_host.h"
 #include "gb_mixed.h"
@@ -91,16 +93,15 @@ extern "C" gb_status gb_mixed_products(const gb_mixed_q* q, const double* V, int
     return GB_OK;
 }
 
-extern "C" gb_status gb_relax_batch_mixed(const gb_mixed_q* q, const gb_objective_desc* objective,
-                                          const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,
-                                          int N, int max_iters, double* x, const double* lb,
-                                          const double* ub, const double* eps, int* iterations,
-                                          uint8_t* status, double* gap, double* value,
-                                          double* best_vertex, double* best_vertex_val,
-                                          const uint8_t* integer_flags, double candidate_tol,
-                                          const double* cutoff, const gb_mixed_options* options,
-                                          void* stream) {
-    try {
+namespace gb {
+
+gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objective,
+                           const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo, int N, int max_iters,
+                           double* x, const double* lb, const double* ub, const double* eps, int* iterations,
+                           uint8_t* status, double* gap, double* value, double* best_vertex,
+                           double* best_vertex_val, const uint8_t* integer_flags, double candidate_tol,
+                           const double* cutoff, const gb_mixed_options* options, void* stream) {
+    {
         gb_status st = validate_descriptors(objective, algorithm, lmo);
         if (st != GB_OK) return st;
         st = check_mixed_scope(q, objective, lmo);
@@ -162,6 +163,226 @@ extern "C" gb_status gb_relax_batch_mixed(const gb_mixed_q* q, const gb_objectiv
         return relax_batch_fast(objective, algorithm, lmo, N, fp64_iters, x, lb, ub, eps, iterations,
                                 status, gap, value, best_vertex, best_vertex_val, integer_flags,
                                 candidate_tol, cutoff, &fo, stream, nullptr, true);
+    }
+}
+
+}  // namespace gb
+
+extern "C" gb_status gb_relax_batch_mixed(const gb_mixed_q* q, const gb_objective_desc* objective,
+                                          const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,
+                                          int N, int max_iters, double* x, const double* lb,
+                                          const double* ub, const double* eps, int* iterations,
+                                          uint8_t* status, double* gap, double* value,
+                                          double* best_vertex, double* best_vertex_val,
+                                          const uint8_t* integer_flags, double candidate_tol,
+                                          const double* cutoff, const gb_mixed_options* options,
+                                          void* stream) {
+    try {
+        return relax_mixed_impl(q, objective, algorithm, lmo, N, max_iters, x, lb, ub, eps, iterations, status,
+                                gap, value, best_vertex, best_vertex_val, integer_flags, candidate_tol, cutoff,
+                                options, stream);
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
+
+namespace {
+
+// Scratch copy of a batch for the autotuner.
+struct TuneScratch {
+    double *x = nullptr, *gap = nullptr, *val = nullptr, *bv = nullptr, *bvv = nullptr;
+    int* it = nullptr;
+    uint8_t* st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~TuneScratch() {
+        for (void* p : {(void*)x, (void*)gap, (void*)val, (void*)bv, (void*)bvv, (void*)it, (void*)st})
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+}  // namespace
+
+extern "C" gb_status gb_mixed_autotune(const gb_mixed_q* q, const gb_objective_desc* objective,
+                                       const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo, int N,
+                                       int max_iters, const double* x, const double* lb, const double* ub,
+                                       const double* eps, const int* iterations, const uint8_t* status,
+                                       const uint8_t* integer_flags, double candidate_tol, const double* cutoff,
+                                       gb_mixed_options* out, double* seconds, void* stream) {
+    try {
+        gb_status st = validate_descriptors(objective, algorithm, lmo);
+        if (st != GB_OK) return st;
+        st = check_mixed_scope(q, objective, lmo);
+        if (st != GB_OK) return st;
+        if (!out || N <= 0 || max_iters <= 0 || !x || !lb || !ub || !eps || !iterations || !status)
+            return fail(GB_ERR_INVALID_ARGUMENT, "invalid autotune arguments");
+        const int n = q->n;
+        hipStream_t hs = reinterpret_cast<hipStream_t>(stream);
+        TuneScratch S;
+        const size_t xb = (size_t)N * n * sizeof(double);
+        GB_HIP(hipMalloc(reinterpret_cast<void**>(&S.x), xb));
+        GB_HIP(hipMalloc(reinterpret_cast<void**>(&S.bv), xb));
+        GB_HIP(hipMalloc(reinterpret_cast<void**>(&S.gap), (size_t)N * 8));
+        GB_HIP(hipMalloc(reinterpret_cast<void**>(&S.val), (size_t)N * 8));
+        GB_HIP(hipMalloc(reinterpret_cast<void**>(&S.bvv), (size_t)N * 8));
+        GB_HIP(hipMalloc(reinterpret_cast<void**>(&S.it), (size_t)N * 4));
+        GB_HIP(hipMalloc(reinterpret_cast<void**>(&S.st), (size_t)N));
+        GB_HIP(hipEventCreate(&S.e0));
+        GB_HIP(hipEventCreate(&S.e1));
+        const std::vector<double> none(N, 1e300);
+        std::vector<uint8_t> hst(N);
+        auto restore = [&]() -> gb_status {
+            GB_HIP(hipMemcpyAsync(S.x, x, xb, hipMemcpyDeviceToDevice, hs));
+            GB_HIP(hipMemcpyAsync(S.it, iterations, (size_t)N * 4, hipMemcpyDeviceToDevice, hs));
+            GB_HIP(hipMemcpyAsync(S.st, status, (size_t)N, hipMemcpyDeviceToDevice, hs));
+            GB_HIP(hipMemcpyAsync(S.bvv, none.data(), (size_t)N * 8, hipMemcpyHostToDevice, hs));
+            GB_HIP(hipStreamSynchronize(hs));
+            return GB_OK;
+        };
+        // Runs one relaxation of the scratch batch; returns its time.
+        auto run = [&](const gb_mixed_options& op, double* sec) -> gb_status {
+            gb_status r = restore();
+            if (r != GB_OK) return r;
+            GB_HIP(hipEventRecord(S.e0, hs));
+            r = relax_mixed_impl(q, objective, algorithm, lmo, N, max_iters, S.x, lb, ub, eps, S.it, S.st, S.gap,
+                                 S.val, S.bv, S.bvv, integer_flags, candidate_tol, cutoff, &op, stream);
+            if (r != GB_OK) return r;
+            GB_HIP(hipEventRecord(S.e1, hs));
+            GB_HIP(hipEventSynchronize(S.e1));
+            float ms = 0.f;
+            GB_HIP(hipEventElapsedTime(&ms, S.e0, S.e1));
+            *sec = ms * 1e-3;
+            return GB_OK;
+        };
+        gb_mixed_options op = *out;
+        op.blocks = 0;
+        op.mixed_iterations = nullptr;
+        op.fp64_stats = nullptr;
+        op.gradient = nullptr;
+        // Stage 1: tile shape of the mixed pass.
+        int best_rw = 0, best_cols = 0;
+        double best_t = 1e300;
+        for (int rw = 1; rw <= kMixedMaxRowTilesPerWave; ++rw) {
+            if (rw == 1 && q->rt > 8) continue;
+            for (int cols : {2, 4, 8, 16}) {
+                op.row_tiles_per_wave = rw;
+                op.columns = cols;
+                op.fp64_max_iters = -1;
+                double t0 = 0.0, t1 = 0.0;
+                st = run(op, &t0);  // first launch of a variant also loads its code
+                if (st != GB_OK) return st;
+                st = run(op, &t1);
+                if (st != GB_OK) return st;
+                if (t1 < best_t) {
+                    best_t = t1;
+                    best_rw = rw;
+                    best_cols = cols;
+                }
+            }
+        }
+        // Stage 2: FP64-pass budget, by time per certified-converged node.
+        op.row_tiles_per_wave = best_rw;
+        op.columns = best_cols;
+        int best_budget = 0;
+        double best_cost = 1e300, best_sec = 0.0;
+        for (int div : {32, 16, 8}) {
+            op.fp64_max_iters = max_iters / div > 1 ? max_iters / div : 1;
+            double t = 0.0;
+            st = run(op, &t);
+            if (st != GB_OK) return st;
+            GB_HIP(hipMemcpy(hst.data(), S.st, (size_t)N, hipMemcpyDeviceToHost));
+            int conv = 0;
+            for (uint8_t v : hst)
+                if (v == GB_NODE_FRACTIONAL_CONVERGED || v == GB_NODE_INTEGRAL_CONVERGED
+                        || v == GB_NODE_INFEASIBLE)
+                    ++conv;
+            const double cost = t / (conv > 0 ? conv : 1);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_budget = op.fp64_max_iters;
+                best_sec = t;
+            }
+        }
+        out->row_tiles_per_wave = best_rw;
+        out->columns = best_cols;
+        out->fp64_max_iters = best_budget;
+        if (seconds) *seconds = best_sec;
+        return GB_OK;
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
+
+namespace gb {
+namespace {
+
+// Internal fork streams of gb_relax_instances, per device, default priority.
+struct ForkStreams {
+    std::mutex mtx;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> done;
+    hipEvent_t start = nullptr;
+};
+ForkStreams g_fork[64];
+
+}  // namespace
+}  // namespace gb
+
+extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batches,
+                                        const gb_algorithm_desc* algorithm, int max_iters, double candidate_tol,
+                                        int concurrency, void* stream) {
+    try {
+        if (count < 0 || (count > 0 && !batches) || !algorithm || concurrency < 1 || concurrency > 8 || max_iters < 0)
+            return fail(GB_ERR_INVALID_ARGUMENT, "invalid instance batch arguments");
+        for (int i = 0; i < count; ++i) {
+            const gb_instance_batch& b = batches[i];
+            if (!b.objective || !b.lmo) return fail(GB_ERR_INVALID_ARGUMENT, "instance batch without descriptors");
+            gb_status st = validate_descriptors(b.objective, algorithm, b.lmo);
+            if (st != GB_OK) return st;
+            if (b.mixed) {
+                st = check_mixed_scope(b.mixed, b.objective, b.lmo);
+                if (st != GB_OK) return st;
+            }
+        }
+        if (count == 0) return GB_OK;
+        int dev = 0;
+        GB_HIP(hipGetDevice(&dev));
+        if (dev < 0 || dev >= 64) return fail(GB_ERR_RUNTIME, "device index out of range");
+        ForkStreams& F = g_fork[dev];
+        std::lock_guard<std::mutex> lk(F.mtx);
+        while ((int)F.streams.size() < concurrency) {
+            hipStream_t s = nullptr;
+            hipEvent_t e = nullptr;
+            GB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            GB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            F.streams.push_back(s);
+            F.done.push_back(e);
+        }
+        if (!F.start) GB_HIP(hipEventCreateWithFlags(&F.start, hipEventDisableTiming));
+        hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
+        GB_HIP(hipEventRecord(F.start, caller));
+        const int k = concurrency < count ? concurrency : count;
+        for (int j = 0; j < k; ++j) GB_HIP(hipStreamWaitEvent(F.streams[j], F.start, 0));
+        for (int i = 0; i < count; ++i) {
+            const gb_instance_batch& b = batches[i];
+            void* s = F.streams[i % k];
+            gb_status st;
+            if (b.mixed)
+                st = relax_mixed_impl(b.mixed, b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
+                                      b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
+                                      b.integer_flags, candidate_tol, b.cutoff, nullptr, s);
+            else
+                st = relax_batch_fast(b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
+                                      b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
+                                      b.integer_flags, candidate_tol, b.cutoff, nullptr, s);
+            if (st != GB_OK) return st;
+        }
+        for (int j = 0; j < k; ++j) {
+            GB_HIP(hipEventRecord(F.done[j], F.streams[j]));
+            GB_HIP(hipStreamWaitEvent(caller, F.done[j], 0));
+        }
+        return GB_OK;
     } catch (const std::exception& ex) {
         return fail(GB_ERR_RUNTIME, ex.what());
     }

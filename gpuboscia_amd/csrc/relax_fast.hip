@@ -355,8 +355,12 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
             if (P.refresh > 0 && (it + 1) % P.refresh == 0) W.gemv(x, qx);
         }
 
-        // Direct FP64 certificate at the returned iterate.
-        W.gemv(x, qx);
+        // Direct FP64 certificate at the returned iterate.  A node that
+        // stopped before its first step still holds the direct products
+        // Q x and Q v of this iterate and vertex (recomputing them gives the
+        // same bits), so they are reused.
+        const bool fresh = it == 0;
+        if (!fresh) W.gemv(x, qx);
         grad();
         if (P.gout) {
             double* go = P.gout + (size_t)node * n;
@@ -366,8 +370,10 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
                 if (j < n) go[j] = g[t];
             }
         }
-        fwd(v);
-        W.gemv(v, qv);
+        if (!fresh) {
+            fwd(v);
+            W.gemv(v, qv);
+        }
         score(v, qv);
         double ga = 0.0, fa = 0.0;
 #pragma unroll

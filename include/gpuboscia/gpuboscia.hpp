@@ -227,10 +227,12 @@ struct Settings {
     double natural_lower_bound = -std::numeric_limits<double>::infinity();
     std::vector<double> root_initial_point;
     RootInitialPointValidator root_initial_point_feasible;
-    // Additions (no reference counterpart): the batched form and the
-    // branching rule of the search.
-    int form = 0;                       // 0 = plain, 1 = batched form
+    // Additions (no reference counterpart): the relaxation form, its
+    // arithmetic, the branching rule of the search and the autotuner.
+    int form = 0;                       // 0 = plain, 1 = batched FP64, 2 = mixed precision
+    int precision = GB_PREC_FP64;       // form 2: GB_PREC_BF16 (1 bf16 plane) or FP32 (3 planes)
     int branch_rule = GB_BRANCH_POW2;   // used when `branch` is empty
+    bool autotune = false;              // form 2: tune tile shape and FP64-pass budget
 };
 
 struct Result {
@@ -323,7 +325,9 @@ Result solve(const Problem& problem, DeviceObjective objective, LMO lmo,
     s.time_limit_seconds = settings.time_limit_seconds;
     s.natural_lower_bound = settings.natural_lower_bound;
     s.form = settings.form;
+    s.precision = settings.precision;
     s.branch_rule = settings.branch_rule;
+    s.autotune = settings.autotune ? 1 : 0;
     detail::Callbacks cb{&settings, &problem};
     if (!settings.root_initial_point.empty()) {
         s.root_initial_point = settings.root_initial_point.data();

@@ -244,6 +244,58 @@ gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
                                void* stream);
 
 /*
+ * Heterogeneous batches (no reference counterpart; the reference fixes n at
+ * compile time, device_resident_solver.cuh:362, :1217-1218): independent
+ * node batches of different problems and sizes relaxed in one call.  Each
+ * entry is relaxed like gb_relax_batch_mixed (when `mixed` is set, n <= 256)
+ * or gb_relax_batch_fast (n <= 512), with the entry's own buffers, on one of
+ * `concurrency` internal streams (1..8) forked from and joined back into
+ * `stream`, so small batches of different instances share the GPU.  Every
+ * entry's results equal those of its solo call.
+ */
+typedef struct {
+    const gb_objective_desc* objective;
+    const gb_lmo_desc* lmo;
+    const gb_mixed_q* mixed;      /* prepared Q for the mixed form, or NULL */
+    int N;
+    double* x;
+    const double* lb;
+    const double* ub;
+    const double* eps;
+    int* iterations;
+    uint8_t* status;
+    double* gap;
+    double* value;
+    double* best_vertex;
+    double* best_vertex_val;
+    const uint8_t* integer_flags;
+    const double* cutoff;
+} gb_instance_batch;
+
+gb_status gb_relax_instances(int count, const gb_instance_batch* batches,
+                             const gb_algorithm_desc* algorithm, int max_iters,
+                             double candidate_tol, int concurrency, void* stream);
+
+/*
+ * Autotuner of the mixed-precision form (no reference counterpart): on a
+ * scratch copy of a batch (the caller's buffers are read, never written) it
+ * times the mixed pass for every valid tile shape (row tiles per wave 1..4,
+ * node columns per tile 2..16) and keeps the fastest, then times the full
+ * relaxation for FP64-pass budgets max_iters/32, /16, /8 and keeps the
+ * lowest time per node that ends certified-converged.  Results are written
+ * to *out (other fields of *out are kept); *seconds receives the winning
+ * full-relaxation time.  Tuning changes speed only: every shape computes the
+ * same algorithm, and the FP64 pass certifies whatever it returns.
+ * Synchronous on `stream`; allocates its scratch buffers.
+ */
+gb_status gb_mixed_autotune(const gb_mixed_q* q, const gb_objective_desc* objective,
+                            const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,
+                            int N, int max_iters, const double* x, const double* lb, const double* ub,
+                            const double* eps, const int* iterations, const uint8_t* status,
+                            const uint8_t* integer_flags, double candidate_tol, const double* cutoff,
+                            gb_mixed_options* out, double* seconds, void* stream);
+
+/*
  * Captured batched-form launch (hipGraph): the counter reset and the
  * persistent relaxation kernel of gb_relax_batch_fast recorded once for
  * fixed buffers and N, replayed with one gb_relax_graph_launch per batch.
@@ -401,9 +453,12 @@ typedef struct {
     gb_branch_fn branch;      /* GB_BRANCH_CUSTOM */
     void* branch_user;
     int precision;            /* gb_precision (GB_PREC_FP64 = plain form) */
-    int form;                 /* 0 = plain (reference schedule), 1 = batched form */
+    int form;                 /* 0 = plain (reference schedule), 1 = batched FP64 form,
+                                 2 = mixed-precision form (BF16/FP32 MFMA + FP64 certificate) */
     gb_brancher* learned_model;   /* GB_BRANCH_LEARNED scorer, or NULL (most fractional) */
     long long max_nodes;      /* safety cap; <= 0 = none (no reference counterpart) */
+    int autotune;             /* form 2: tune tile shape and FP64-pass budget on the first
+                                 batch of at least 64 nodes (gb_mixed_autotune); 0 = defaults */
 } gb_settings;
 
 /* Result (gpuboscia.cuh:34-42). x is caller-provided, n_vars doubles. */

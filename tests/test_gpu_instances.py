@@ -1,0 +1,81 @@
+"""Heterogeneous instance batches (gb_relax_instances): batches of different
+problems and sizes relaxed in one call give, entry by entry, bit for bit
+what each entry's solo call gives."""
+import sys
+
+import numpy as np
+import pytest
+
+import gpuboscia_amd as gb
+from gpuboscia_amd import instances
+from tests.gpu_util import assert_bit_equal, dev
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_entry(n, N, seed, mixed_planes):
+    import torch
+    Q, c, _ = instances.binary_qp(n, seed=seed)
+    dQt, dc = dev(np.ascontiguousarray(Q.T)), dev(c)
+    obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
+    rng = np.random.default_rng(seed)
+    LB, UB = np.zeros((N, n)), np.ones((N, n))
+    for i in range(1, N):
+        fix = rng.random(n) < 0.3 * rng.random()
+        val = rng.random(n) < 0.5
+        LB[i, fix & val] = 1.0
+        UB[i, fix & ~val] = 0.0
+    eps = np.maximum(1e-6, 1e-3 * 0.8 ** rng.integers(0, 10, N))
+    f64 = dict(dtype=torch.float64, device="cuda")
+    e = {"obj": obj, "lmo": gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), "keep": (dQt, dc),
+         "mixed": gb.MixedQ(obj, mixed_planes) if mixed_planes else None,
+         "x": dev(LB.copy()), "lb": dev(LB), "ub": dev(UB), "eps": dev(eps),
+         "iterations": torch.zeros(N, dtype=torch.int32, device="cuda"),
+         "status": torch.zeros(N, dtype=torch.uint8, device="cuda"),
+         "gap": torch.zeros(N, **f64), "value": torch.zeros(N, **f64), "best_vertex": torch.zeros((N, n), **f64),
+         "best_vertex_val": torch.full((N,), 1e300, **f64), "flags": dev(np.ones(n, np.uint8)),
+         "cutoff": dev(np.full(N, -1.0)) if seed % 2 else None}
+    return e
+
+
+def results(e):
+    return {k: e[k].cpu().numpy() for k in ("x", "gap", "value", "iterations", "status", "best_vertex",
+                                              "best_vertex_val")}
+
+
+@pytest.mark.parametrize("concurrency", [1, 4])
+def test_instances_equal_solo_runs(concurrency):
+    import torch
+    sizes = [(32, 40, 1), (45, 17, 3), (64, 100, 3), (100, 64, 1), (150, 33, 3), (200, 80, 1), (256, 20, 3),
+             (300, 50, 0), (420, 25, 0), (512, 16, 0)]
+    alg = gb.algorithm(dicg=True)
+    budget = 250
+    together = [make_entry(n, N, 100 + k, planes) for k, (n, N, planes) in enumerate(sizes)]
+    solo = [make_entry(n, N, 100 + k, planes) for k, (n, N, planes) in enumerate(sizes)]
+    say(f"[instances] {len(sizes)} instances, n from 32 to 512, concurrency {concurrency}")
+    gb.relax_instances(together, alg, budget, concurrency=concurrency)
+    for e in solo:
+        if e["mixed"] is not None:
+            gb.relax_batch_mixed(e["mixed"], e["obj"], alg, e["lmo"], e["x"].shape[0], budget, e["x"], e["lb"],
+                                 e["ub"], e["eps"], e["iterations"], e["status"], e["gap"], e["value"],
+                                 e["best_vertex"], e["best_vertex_val"], e["flags"], 1e-6, e["cutoff"])
+        else:
+            gb.relax_batch_fast(e["obj"], alg, e["lmo"], e["x"].shape[0], budget, e["x"], e["lb"], e["ub"],
+                                e["eps"], e["iterations"], e["status"], e["gap"], e["value"], e["best_vertex"],
+                                e["best_vertex_val"], e["flags"], 1e-6, e["cutoff"])
+    torch.cuda.synchronize()
+    for k, (a, b) in enumerate(zip(together, solo)):
+        assert_bit_equal(results(a), results(b), f"instance {k} n={sizes[k][0]}")
+        assert int(a["iterations"].sum().item()) > 0
+
+
+def test_instances_reject_bad_entries():
+    e = make_entry(40, 8, 1, 0)
+    e["obj"] = gb.ObjectiveDesc(gb.OBJ_DENSE, 40, 0, 0, 2.0, 0.5, 0.0, 0, 0, 0)  # no Qt
+    with pytest.raises(gb.InvalidArgument):
+        gb.relax_instances([e], gb.algorithm(), 10)

@@ -182,3 +182,78 @@ def test_mixed_bnb_matches_plain_optimum(n, seed):
             f"{res[form]['nodes_processed']} {res[form]['elapsed_seconds']:.4f}s")
     assert not res[2]["reached_time_limit"]
     assert abs(res[1]["objective"] - res[2]["objective"]) <= 1e-9 * max(1.0, abs(res[1]["objective"]))
+
+
+def test_autotuner_picks_a_valid_shape_and_leaves_inputs_untouched():
+    from oracle import pyoracle as orc
+    import torch
+    n, N, budget = 120, 256, 300
+    Q, c, _ = instances.binary_qp(n, seed=31)
+    obj, keep = dense(Q, c)
+    mq = gb.MixedQ(obj, 1)
+    LB, UB, eps = node_set(n, N, 8)
+    x, lb, ub, e = dev(LB.copy()), dev(LB), dev(UB), dev(eps)
+    it = dev(np.zeros(N, np.int32))
+    st = dev(np.zeros(N, np.uint8))
+    flags = dev(np.ones(n, np.uint8))
+    alg = gb.algorithm(dicg=True)
+    box = gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0)
+    opt, sec = gb.mixed_autotune(mq, obj, alg, box, N, budget, x, lb, ub, e, it, st, flags)
+    say(f"[mixed] autotune n={n} N={N}: rw={opt.row_tiles_per_wave} cols={opt.columns} "
+        f"fp64={opt.fp64_max_iters} {sec * 1e3:.3f} ms")
+    assert 1 <= opt.row_tiles_per_wave <= 4 and opt.columns in (2, 4, 8, 16)
+    assert opt.fp64_max_iters in (max(1, budget // d) for d in (32, 16, 8))
+    torch.cuda.synchronize()
+    assert np.array_equal(x.cpu().numpy(), LB) and not it.cpu().numpy().any() and not st.cpu().numpy().any()
+    # the tuned shape computes the same algorithm: its FP64 pass matches the
+    # oracle started from its own mixed-pass output
+    zeros = np.zeros(N)
+    p1 = run_mixed_opts(mq, obj, LB, UB, eps, budget, -1, opt)
+    full = run_mixed_opts(mq, obj, LB, UB, eps, budget, opt.fp64_max_iters, opt)
+    o = orc.relax_fast_batch(orc.dense_objective(Q, c, 2.0, 0.5, 0.0), True, p1["x"].copy(), LB.copy(),
+                             UB.copy(), eps.copy(), p1["iterations"].astype(np.int32).copy(), p1["status"].copy(),
+                             flags=np.ones(n, np.uint8), max_iters=opt.fp64_max_iters, refresh=64, threads=8,
+                             best_vertex=p1["best_vertex"].copy(), best_vertex_val=p1["best_vertex_val"].copy())
+    assert_bit_equal(full, o, "tuned shape")
+    del zeros
+
+
+def run_mixed_opts(mq, obj, LB, UB, eps, budget, fp64_iters, opt):
+    import torch
+    N, n = LB.shape
+    x = dev(LB.copy())
+    it = dev(np.zeros(N, np.int32))
+    st = dev(np.zeros(N, np.uint8))
+    gap = torch.zeros(N, dtype=torch.float64, device="cuda")
+    val = torch.zeros(N, dtype=torch.float64, device="cuda")
+    bv = torch.zeros((N, n), dtype=torch.float64, device="cuda")
+    bvv = torch.full((N,), 1e300, dtype=torch.float64, device="cuda")
+    gb.relax_batch_mixed(mq, obj, gb.algorithm(dicg=True), gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), N, budget, x,
+                         dev(LB), dev(UB), dev(eps), it, st, gap, val, bv, bvv, dev(np.ones(n, np.uint8)), 1e-6,
+                         None, fp64_max_iters=fp64_iters, row_tiles_per_wave=opt.row_tiles_per_wave,
+                         columns=opt.columns)
+    torch.cuda.synchronize()
+    return {"x": x.cpu().numpy(), "gap": gap.cpu().numpy(), "value": val.cpu().numpy(),
+            "iterations": it.cpu().numpy(), "status": st.cpu().numpy(), "best_vertex": bv.cpu().numpy(),
+            "best_vertex_val": bvv.cpu().numpy()}
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_autotuned_bnb_reaches_the_same_optimum(seed):
+    n = 60
+    Q, c, _ = instances.binary_qp(n, seed=seed)
+    obj, keep = dense(Q, c)
+    res = {}
+    for tune in (0, 1):
+        s = gb.default_settings()
+        s.abs_gap_tol = 1e-6
+        s.rel_gap_tol = 1e-6
+        s.form = 2
+        s.precision = gb.PREC_BF16
+        s.autotune = tune
+        s.time_limit_seconds = 60.0
+        res[tune] = gb.solve(n, [0] * n, [1] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), s)
+        say(f"[mixed] autotune={tune} seed={seed}: {res[tune]['objective']:.12f} nodes "
+            f"{res[tune]['nodes_processed']} {res[tune]['elapsed_seconds']:.4f}s")
+    assert not res[1]["reached_time_limit"]
+    assert abs(res[0]["objective"] - res[1]["objective"]) <= 1e-9 * max(1.0, abs(res[0]["objective"]))
