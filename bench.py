@@ -155,6 +155,17 @@ class Batch:
         return out
 
 
+def traffic_bytes():
+    """HBM bytes per launch of relax_mixed_kernel on this workload, from the
+    committed rocprofv3 counter passes (FETCH_SIZE + WRITE_SIZE, scaled by
+    the same counters on a copy of known size); None if not recorded."""
+    path = os.path.join(ROOT, "profiles", "relax_mixed_kernel_n200_1024nodes_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
 def cpu_baseline(args, Q, c, LB, UB, EPS):
     """The CPU oracle (FP64 restatement of the plain form) on a bounded sample
     of the same node batch, node-parallel on host threads."""
@@ -240,6 +251,8 @@ def main():
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--forms", default="plain,batched,mixed,mixed_pass1",
                     help="forms to time (the profiled runs time one)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -259,6 +272,15 @@ def main():
     for form in forms:
         res[form] = batch.measure(form, args.budget, args.steps, args.warmup)
         say(f"{form}: {res[form]['value']:.4g} FW it/s, launch {res[form]['launch_ms']:.3f} ms")
+    if args.copy_probe:
+        import torch
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        say("copy probe: 268435456 bytes copied device to device")
+        del src, dst
     main_form = "mixed" if "mixed" in res else forms[0]
     m = res[main_form]
     result = {
@@ -287,7 +309,7 @@ def main():
         achieved = alg_flops / kernel_s / 1e12
         result["roofline"] = {
             "bound": "mfma", "achieved": achieved, "peak": MI355X_BF16_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / MI355X_BF16_TFLOPS, "traffic": None, "kernel": "relax_mixed_kernel",
+            "frac": achieved / MI355X_BF16_TFLOPS, "traffic": traffic_bytes(), "kernel": "relax_mixed_kernel",
             "kernel_ms": p1["launch_ms"],
             "basis": "algorithmic 2 n^2 flops per FW iteration x iterations of one launch / launch time "
                      "(HIP events on the launch stream); bf16 dense MFMA peak",

@@ -19,7 +19,7 @@ GB_OK, GB_ERR_INVALID_ARGUMENT, GB_ERR_RUNTIME, GB_ERR_DEVICE = 0, 1, 2, 3
 ALG_FRANK_WOLFE, ALG_DICG = 0, 1
 LS_OPEN_LOOP, LS_SCALED, LS_SAFEGUARDED, LS_SECANT, LS_QUADRATIC_EXACT = 0, 1, 2, 3, 4
 OBJ_DENSE, OBJ_QAP = 0, 1
-LMO_BOX, LMO_SIMPLEX, LMO_BIRKHOFF = 0, 1, 2
+LMO_BOX, LMO_SIMPLEX, LMO_BIRKHOFF, LMO_PORTFOLIO = 0, 1, 2, 3
 HEUR_ROUND, HEUR_FLOOR_OFFSET = 0, 1
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 BRANCH_POW2, BRANCH_MOST_FRACTIONAL, BRANCH_LEARNED, BRANCH_CALLBACK, BRANCH_CUSTOM = 0, 1, 2, 3, 4
@@ -40,7 +40,8 @@ class ObjectiveDesc(ctypes.Structure):
 
 class LmoDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("right_side", ctypes.c_double),
-                ("equality", ctypes.c_int), ("is_int", ctypes.c_void_p), ("order", ctypes.c_int)]
+                ("equality", ctypes.c_int), ("is_int", ctypes.c_void_p), ("order", ctypes.c_int),
+                ("cap", ctypes.c_double), ("cardinality", ctypes.c_double)]
 
 
 class AlgorithmDesc(ctypes.Structure):

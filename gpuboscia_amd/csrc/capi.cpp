@@ -90,6 +90,15 @@ gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_de
     } else if (l->kind == GB_LMO_BIRKHOFF) {
         if (l->order <= 0 || l->order > kMaxBirkhoffOrder)
             return fail(GB_ERR_INVALID_ARGUMENT, "Birkhoff order must be in [1, 256]");
+    } else if (l->kind == GB_LMO_PORTFOLIO) {
+        if (o->kind != GB_OBJ_DENSE_QUADRATIC || o->n % 2 != 0 || o->n > kMaxSimplexN)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "portfolio LMO needs a dense objective over (x, z) with n = 2 * assets <= 2048");
+        if (!(l->cap > 0.0) || !(l->cardinality >= 0.0) || !(l->right_side > 0.0))
+            return fail(GB_ERR_INVALID_ARGUMENT, "portfolio LMO needs cap > 0, cardinality >= 0, right_side > 0");
+        if (a && a->algorithm == GB_ALG_DICG)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "the portfolio LMO has no in-face oracle: use the Frank-Wolfe algorithm");
     } else if (l->kind != GB_LMO_BOX) {
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown LMO kind");
     }
@@ -152,6 +161,8 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
     P.equality = l->equality;
     P.is_int = l->is_int;
     P.lorder = l->order;
+    P.pcap = l->cap;
+    P.pcard = l->cardinality;
     P.dicg = a->algorithm == GB_ALG_DICG;
     P.ls_kind = a->line_search;
     P.secant_iters = a->secant_iters;
@@ -397,8 +408,6 @@ extern "C" int gb_lmo_relaxation_feasible_cpu(const gb_lmo_desc* lmo, const uint
                                               const double* x, const double* lb, const double* ub,
                                               int n, double tol) {
     (void)is_int_host;
-    (void)lb;
-    (void)ub;
     if (!lmo || !x || n <= 0) return 0;
     switch (lmo->kind) {
     case GB_LMO_BOX:  // lmo.cuh:214-217: bounds are validated separately
@@ -417,6 +426,18 @@ extern "C" int gb_lmo_relaxation_feasible_cpu(const gb_lmo_desc* lmo, const uint
             mass += x[j];
         }
         return mass <= lmo->right_side + tol;
+    }
+    case GB_LMO_PORTFOLIO: {  // sum x = rhs, 0 <= x_i <= cap z_i, lb <= z <= ub, sum z <= card
+        if (n % 2 != 0) return 0;
+        const int m = n / 2;
+        double xs = 0.0, zs = 0.0;
+        for (int i = 0; i < m; ++i) {
+            if (x[i] < -tol || x[i] > lmo->cap * x[m + i] + tol) return 0;
+            if (lb && ub && (x[m + i] < lb[m + i] - tol || x[m + i] > ub[m + i] + tol)) return 0;
+            xs += x[i];
+            zs += x[m + i];
+        }
+        return std::fabs(xs - lmo->right_side) <= tol && zs <= lmo->cardinality + tol;
     }
     case GB_LMO_BIRKHOFF: {  // lmo.cuh:922-934
         const int m = lmo->order;

@@ -27,6 +27,7 @@ struct RelaxParams {
     int equality;
     const uint8_t* is_int;
     int lorder;
+    double pcap, pcard;     // portfolio LMO
     // algorithm
     int dicg;
     int ls_kind;
@@ -73,7 +74,7 @@ inline LdsLayout lds_layout(int obj_kind, int lmo_kind, int n, int order) {
     L.red = take(16 * sizeof(double));
     L.ired = take(16 * sizeof(int));
     L.vec = take(obj_kind == GB_OBJ_DENSE_QUADRATIC ? (size_t)n * sizeof(double) : 0);
-    if (lmo_kind == GB_LMO_SIMPLEX) {
+    if (lmo_kind == GB_LMO_SIMPLEX || lmo_kind == GB_LMO_PORTFOLIO) {
         const int S = next_pow2_host(n);
         L.sg = take((size_t)n * sizeof(double));
         L.sidx = take((size_t)S * sizeof(int));

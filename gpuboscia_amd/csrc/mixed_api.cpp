@@ -28,8 +28,9 @@ gb_status check_mixed_scope(const gb_mixed_q* q, const gb_objective_desc* o, con
 
 MixedShape mixed_default_shape(int rt, int N) {
     MixedShape sh;
-    // Row split: about 4 waves per tile.
-    sh.rw = rt <= 4 ? 1 : (rt <= 8 ? 2 : 4);
+    // Row split (measured on MI355X at n = 60 and 200): one row tile per
+    // wave up to 8 waves, two beyond.
+    sh.rw = rt <= 8 ? 1 : 2;
     // Columns: enough tiles to cover the 256 CUs of the chip.
     const int want = (N + 255) / 256;
     sh.cols = want < 1 ? 1 : (want > 16 ? 16 : want);

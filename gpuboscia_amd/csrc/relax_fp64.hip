@@ -352,6 +352,114 @@ struct Lmo<GB_LMO_SIMPLEX> {
     }
 };
 
+// Cardinality-portfolio LMO over v = (x, z), n = 2m (no reference
+// counterpart): sum x = rhs, 0 <= x_i <= cap z_i, lb_z <= z <= ub_z,
+// sum z <= card.  With z at its least value, asset i offers x in segment
+// q = 2i (up to cap lz_i at cost gx_i) and q = 2i + 1 (up to cap uz_i at
+// cost gx_i + max(gz_i, 0) / cap, drawing 1/cap per unit on the pool
+// cap (card - sum lz)); segments, pool and total form a laminar family, on
+// which the cheapest-first fill is an optimal vertex.  Segments are ranked
+// by (cost, q) with an all-pairs count; thread 0 fills in rank order.
+// Frank-Wolfe only: the solver rejects DICG for this LMO.
+template <>
+struct Lmo<GB_LMO_PORTFOLIO> {
+    __device__ static bool feasible(const RelaxParams& P, const Lds& s, const double* lb, const double* ub,
+                                    double*) {
+        const int m = P.n / 2;
+        if (threadIdx.x == 0) {
+            double zsum = 0.0, a = 0.0, b = 0.0;
+            for (int i = 0; i < m; ++i) {
+                zsum = zsum + lb[m + i];
+                a = a + P.pcap * lb[m + i];
+                b = b + P.pcap * (ub[m + i] - lb[m + i]);
+            }
+            const double capacity = a + fmin(b, P.pcap * (P.pcard - zsum));
+            s.ired[0] = (zsum <= P.pcard + 1e-9 && capacity >= P.rhs - 1e-9) ? 1 : 0;
+        }
+        __syncthreads();
+        const bool ok = s.ired[0] != 0;
+        __syncthreads();
+        return ok;
+    }
+    __device__ static void compute_v(const RelaxParams& P, const Lds& s, const double* g, const double* lb,
+                                     const double* ub, double* v) {
+        const int m = P.n / 2, S2 = 2 * m;
+        const double cap = P.pcap;
+        for (int q = threadIdx.x; q < S2; q += blockDim.x) {
+            const int i = q >> 1;
+            s.sg[q] = (q & 1) ? g[i] + fmax(g[m + i], 0.0) / cap : g[i];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < S2; q += blockDim.x) {
+            const double kq = s.sg[q];
+            int r = 0;
+            for (int t = 0; t < S2; ++t) {
+                const double kt = s.sg[t];
+                r += (kt < kq || (kt == kq && t < q)) ? 1 : 0;
+            }
+            s.sidx[r] = q;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double zsum = 0.0;
+            for (int i = 0; i < m; ++i) {
+                v[i] = 0.0;
+                v[m + i] = lb[m + i];
+                zsum = zsum + lb[m + i];
+            }
+            double pool = cap * (P.pcard - zsum);
+            double rem = P.rhs;
+            for (int r = 0; r < S2; ++r) {
+                if (!(rem > 0.0)) break;
+                const int q = s.sidx[r];
+                const int i = q >> 1;
+                double take;
+                if ((q & 1) == 0) {
+                    take = fmin(rem, cap * lb[m + i]);
+                } else {
+                    take = fmin(fmin(rem, cap * (ub[m + i] - lb[m + i])), pool);
+                    if (take > 0.0) {
+                        pool = pool - take;
+                        v[m + i] = v[m + i] + take / cap;
+                    }
+                }
+                if (take > 0.0) {
+                    v[i] = v[i] + take;
+                    rem = rem - take;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // No in-face oracle (DICG is rejected for this LMO); present for the
+    // common node loop.
+    __device__ static void inface(const RelaxParams& P, const Lds& s, const double* g, const double*,
+                                  const double* lb, const double* ub, double* a) {
+        compute_v(P, s, g, lb, ub, a);
+    }
+    __device__ static double max_step(const RelaxParams&, const Lds&, const double*, const double*,
+                                      const double*, const double*) {
+        return 0.0;
+    }
+    __device__ static bool rounded_feasible(const RelaxParams& P, const Lds& s, const double* x) {
+        const int m = P.n / 2;
+        if (threadIdx.x == 0) {
+            bool ok = true;
+            double xs = 0.0, zs = 0.0;
+            for (int i = 0; i < m && ok; ++i) {
+                if (x[i] < -P.ctol || x[i] > P.pcap * x[m + i] + P.ctol) ok = false;
+                xs = xs + x[i];
+                zs = zs + x[m + i];
+            }
+            s.ired[0] = (ok && fabs(xs - P.rhs) <= P.ctol && zs <= P.pcard + P.ctol) ? 1 : 0;
+        }
+        __syncthreads();
+        const bool ok = s.ired[0] != 0;
+        __syncthreads();
+        return ok;
+    }
+};
+
 // BirkhoffLMO (lmo.cuh:918-1419).  Forced edges (lb > 1/2, or face entries
 // >= 1 - 1e-9) are fixed; the residual assignment is solved by the
 // shortest-augmenting-path Hungarian method on wave 0 alone: each lane owns
@@ -812,6 +920,7 @@ int launch_relax_fp64(const RelaxParams& p, int block_threads, size_t lds_bytes,
         if (p.lmo_kind == GB_LMO_BOX) GB_LAUNCH(GB_OBJ_DENSE_QUADRATIC, GB_LMO_BOX);
         if (p.lmo_kind == GB_LMO_SIMPLEX) GB_LAUNCH(GB_OBJ_DENSE_QUADRATIC, GB_LMO_SIMPLEX);
         if (p.lmo_kind == GB_LMO_BIRKHOFF) GB_LAUNCH(GB_OBJ_DENSE_QUADRATIC, GB_LMO_BIRKHOFF);
+        if (p.lmo_kind == GB_LMO_PORTFOLIO) GB_LAUNCH(GB_OBJ_DENSE_QUADRATIC, GB_LMO_PORTFOLIO);
     } else {
         if (p.lmo_kind == GB_LMO_BOX) GB_LAUNCH(GB_OBJ_QAP, GB_LMO_BOX);
         if (p.lmo_kind == GB_LMO_SIMPLEX) GB_LAUNCH(GB_OBJ_QAP, GB_LMO_SIMPLEX);

@@ -80,3 +80,39 @@ def qap_instance(m: int, seed: int, p: float = 0.3):
     Pi[np.arange(m), perm] = 1.0
     B = Pi @ A @ Pi.T
     return A, B, perm, sha256(A, B)
+
+
+def student_t(seed, stream, count, df=5):
+    """Student-t(df) draws: N(0,1) / sqrt(chi2_df / df), chi2 from df squared normals."""
+    z = normal(seed, stream, count)
+    g = normal(seed, stream + 1, count * df).reshape(count, df)
+    return z / np.sqrt((g * g).sum(axis=1) / df)
+
+
+def portfolio(assets: int = 100, seed: int = 1, periods: int = 600, factors: int = 3, lam: float = 1.0,
+              cap: float = 0.2, card: int = 10):
+    """Cardinality-constrained mean-variance portfolio (the cardinality
+    portfolio configuration).  Monthly returns r_t = B f_t + e_t: factor
+    returns f ~ N(0, 0.04^2), loadings B ~ N(1, 0.3), idiosyncratic e =
+    0.05 t_5 / sqrt(5/3) (Student-t with 5 degrees of freedom scaled to a 5%
+    standard deviation); Sigma and mu are the sample covariance and mean of T
+    periods.  Objective over v = (x, z), n = 2 assets:
+        f(v) = x' Sigma x - lam mu' x = 1/2 v'Qv + c'v,  Q = blockdiag(2 Sigma, 0),
+        c = (-lam mu, 0),
+    LMO: sum x = 1, 0 <= x_i <= cap z_i, z in {0,1}, sum z <= card.
+    Returns (Q, c, Sigma, mu, sha256)."""
+    m, T, k = assets, periods, factors
+    F = 0.04 * normal(seed, 11, T * k).reshape(T, k)
+    B = 1.0 + 0.3 * normal(seed, 12, m * k).reshape(m, k)
+    E = 0.05 * student_t(seed, 13, T * m).reshape(T, m) / np.sqrt(5.0 / 3.0)
+    R = F @ B.T + E
+    mu = R.mean(axis=0)
+    D = R - mu
+    Sigma = D.T @ D / (T - 1)
+    Sigma = 0.5 * (Sigma + Sigma.T)
+    n = 2 * m
+    Q = np.zeros((n, n))
+    Q[:m, :m] = 2.0 * Sigma
+    c = np.zeros(n)
+    c[:m] = -lam * mu
+    return Q, c, Sigma, mu, sha256(Q, c)

@@ -64,7 +64,13 @@ typedef enum {
 typedef enum {
     GB_LMO_BOX = 0,       /* BoxLMO (lmo.cuh:212-321) */
     GB_LMO_SIMPLEX = 1,   /* UnitSimplexLMO / EqualitySimplexLMO (lmo.cuh:335-912) */
-    GB_LMO_BIRKHOFF = 2   /* BirkhoffLMO (lmo.cuh:918-1419) */
+    GB_LMO_BIRKHOFF = 2,  /* BirkhoffLMO (lmo.cuh:918-1419) */
+    /* Cardinality portfolio over v = (x, z), n = 2m: sum x = right_side,
+       0 <= x_i <= cap z_i, lb_z <= z <= ub_z, sum z <= cardinality (no
+       reference counterpart; analogue LinkedCardinalityLMO,
+       examples/sparse_sharpe.cu:266-546).  Exact for gradients with gz >= 0;
+       Frank-Wolfe only (no in-face oracle). */
+    GB_LMO_PORTFOLIO = 3
 } gb_lmo_kind;
 
 /* Candidate heuristics (heuristics.h:23-35). */
@@ -99,6 +105,8 @@ typedef struct {
     int equality;             /* simplex: 1 = EqualitySimplexLMO */
     const uint8_t* is_int;    /* simplex: device, n bytes (UnitSimplexLMO::is_int) */
     int order;                /* birkhoff: Order (n = Order*Order) */
+    double cap;               /* portfolio: x_i <= cap z_i */
+    double cardinality;       /* portfolio: sum z <= cardinality */
 } gb_lmo_desc;
 
 typedef struct {

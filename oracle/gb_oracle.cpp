@@ -24,7 +24,8 @@
 //                                       (tests/test_pipeline.cu:1211-1441)
 //   B&B ............................... exhaustive 2^n enumeration, analytic
 //                                       separable optima (test_pipeline.cu:449-708)
-//   portfolio LMO ..................... vertex enumeration on tiny instances
+//   portfolio LMO ..................... LP optima of tiny instances (scipy, committed
+//                                       fixtures tests/golden/portfolio_lmo.json)
 //   QAP objective ..................... finite differences, optimum 0 at P=Pi^T
 #include "gb_oracle.h"
 
@@ -333,63 +334,73 @@ bool birkhoff_solve(int m, const double* cost, const double* lb, const double* u
 }  // namespace
 
 // ===========================================================================
-// Portfolio LMO.  Variables: x = v[0..m), z = v[m..2m).  Bounds lb/ub apply
-// to z (branching); x bounds come from the coupling.  For a multiplier
-// mu >= 0 on sum z <= card, an asset's best contribution per unit of x is
-// rho_i = gx_i + gz_i/cap (z grows with x) once z_i's own free value is
-// accounted for.  Exact solution by enumerating the candidate breakpoints of
-// mu: the optimal vertex has at most one fractional x and one fractional z.
-// Implemented as a direct vertex search over the structure described in
-// the function body.
+// Cardinality-portfolio LMO (no reference counterpart; the nearest analogue
+// is LinkedCardinalityLMO, examples/sparse_sharpe.cu:266-546, which has no
+// sum x = rhs).  Variables x = v[0..m), z = v[m..2m):
+//   sum x = rhs, 0 <= x_i <= cap z_i, lz_i <= z_i <= uz_i, sum z <= card.
+// For a gradient with gz >= 0 (the mean-variance objective has gz = 0), z_i
+// takes its least value max(lz_i, x_i / cap), and asset i offers x in two
+// segments: A, up to cap lz_i at unit cost gx_i; B, from cap lz_i to cap uz_i
+// at unit cost gx_i + gz_i / cap, each unit using 1 / cap of the cardinality
+// budget card - sum lz.  Segment, pool-of-B and total capacities form a
+// laminar family, on which filling the cheapest units first is an optimal LP
+// vertex.  Segments are ordered by (unit cost, asset, A before B); negative
+// gz_i count as 0 (the vertex is then exact for max(g, 0) on z).
 // ===========================================================================
 namespace {
 
-struct PortfolioSolution { double cost; };
-
-// Evaluate the optimal (x, z) for a fixed multiplier mu on sum z <= card:
-// minimise sum_i [gx_i x_i + (gz_i + mu) z_i] s.t. sum x = 1,
-// 0 <= x_i <= cap z_i, lz_i <= z_i <= uz_i.  For fixed mu the problem is a
-// fractional knapsack: each asset i offers x at unit cost
-// c_i = gx_i + (gz_i+mu)/cap up to cap*uz_i, beyond the x already "free"
-// from its lower-bound z (cap*lz_i at cost gx_i), and z_i is independently
-// raised to uz_i when gz_i + mu < 0.
-void portfolio_fixed_mu(int m, double cap, const double* g, const double* lz,
-                        const double* uz, double mu, double* v) {
-    // Base: z at its cheapest level ignoring x, x = 0.
+void portfolio_vertex(int m, double rhs, double cap, double card, const double* g, const double* lz,
+                      const double* uz, double* v) {
+    const int S = 2 * m;
+    std::vector<double> key(S);
+    for (int i = 0; i < m; ++i) {
+        key[2 * i] = g[i];
+        key[2 * i + 1] = g[i] + std::fmax(g[m + i], 0.0) / cap;
+    }
+    std::vector<int> order(S);
+    for (int q = 0; q < S; ++q) order[q] = q;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+    double zsum = 0.0;
     for (int i = 0; i < m; ++i) {
         v[i] = 0.0;
-        const double gz = g[m + i] + mu;
-        v[m + i] = gz < 0.0 ? uz[i] : lz[i];
+        v[m + i] = lz[i];
+        zsum = zsum + lz[i];
     }
-    // Units of x: segment A, x up to cap*z_i at cost gx_i (z already paid);
-    // segment B, x beyond cap*z_i up to cap*uz_i at cost gx_i + gz/cap.
-    struct Seg { double cost; int i; int kind; double amount; };
-    std::vector<Seg> segs;
-    for (int i = 0; i < m; ++i) {
-        const double gz = g[m + i] + mu;
-        const double freex = cap * v[m + i];
-        if (freex > 0.0) segs.push_back({g[i], i, 0, freex});
-        const double extra = cap * uz[i] - freex;
-        if (extra > 0.0) segs.push_back({g[i] + gz / cap, i, 1, extra});
-    }
-    std::sort(segs.begin(), segs.end(), [](const Seg& a, const Seg& b) {
-        return a.cost < b.cost || (a.cost == b.cost && (a.i < b.i || (a.i == b.i && a.kind < b.kind)));
-    });
-    double rem = 1.0;
-    for (const Seg& s : segs) {
-        if (rem <= 0.0) break;
-        const double take = std::fmin(rem, s.amount);
-        v[s.i] = v[s.i] + take;
-        if (s.kind == 1) v[m + s.i] = std::fmax(v[m + s.i], v[s.i] / cap);
-        rem = rem - take;
+    double pool = cap * (card - zsum);
+    double rem = rhs;
+    for (int r = 0; r < S; ++r) {
+        if (!(rem > 0.0)) break;
+        const int q = order[r];
+        const int i = q >> 1;
+        double take;
+        if ((q & 1) == 0) {
+            take = std::fmin(rem, cap * lz[i]);
+        } else {
+            take = std::fmin(std::fmin(rem, cap * (uz[i] - lz[i])), pool);
+            if (take > 0.0) {
+                pool = pool - take;
+                v[m + i] = v[m + i] + take / cap;
+            }
+        }
+        if (take > 0.0) {
+            v[i] = v[i] + take;
+            rem = rem - take;
+        }
     }
 }
 
-double portfolio_cost(int m, const double* g, const double* v) {
-    double c = 0.0;
-    for (int j = 0; j < 2 * m; ++j) c = std::fma(g[j], v[j], c);
-    return c;
+// Largest x mass the node's bounds allow (feasibility: >= rhs).
+double portfolio_capacity(int m, double cap, double card, const double* lz, const double* uz, double* zsum_out) {
+    double zsum = 0.0, a = 0.0, b = 0.0;
+    for (int i = 0; i < m; ++i) {
+        zsum = zsum + lz[i];
+        a = a + cap * lz[i];
+        b = b + cap * (uz[i] - lz[i]);
+    }
+    *zsum_out = zsum;
+    return a + std::fmin(b, cap * (card - zsum));
 }
+
 
 }  // namespace
 
@@ -436,31 +447,7 @@ void compute_v(const orc_lmo& l, int n, const double* g, const double* lb,
         break;
     case ORC_LMO_PORTFOLIO: {
         const int m = l.assets;
-        // Exact LP: the optimal multiplier of sum z <= card is 0 or a
-        // breakpoint where some asset's preference changes; search all.
-        std::vector<double> cand(2 * m), best(2 * m);
-        std::vector<double> mus{0.0};
-        for (int i = 0; i < m; ++i) {
-            const double a = -g[m + i];                 // gz + mu = 0
-            if (a > 0.0) mus.push_back(a);
-            for (int k = 0; k < m; ++k) {                // rho_i(mu) = gx_k crossings
-                const double b = (g[k] - g[i]) * l.cap - g[m + i];
-                if (b > 0.0) mus.push_back(b);
-            }
-        }
-        std::sort(mus.begin(), mus.end());
-        mus.erase(std::unique(mus.begin(), mus.end()), mus.end());
-        double best_cost = kInf;
-        bool found = false;
-        for (double mu : mus) {
-            portfolio_fixed_mu(m, l.cap, g, lb + m, ub + m, mu, cand.data());
-            double zs = 0.0, xs = 0.0;
-            for (int i = 0; i < m; ++i) { zs = zs + cand[m + i]; xs = xs + cand[i]; }
-            if (zs > l.card + 1e-9 || std::fabs(xs - 1.0) > 1e-9) continue;
-            const double c = portfolio_cost(m, g, cand.data());
-            if (!found || c < best_cost) { best_cost = c; best = cand; found = true; }
-        }
-        for (int j = 0; j < 2 * m; ++j) v[j] = found ? best[j] : 0.0;
+        portfolio_vertex(m, l.right_side, l.cap, l.card, g, lb + m, ub + m, v);
         for (int j = 2 * m; j < n; ++j) v[j] = 0.0;
         break;
     }
@@ -618,12 +605,9 @@ bool node_feasible(const orc_lmo& l, int n, const double* lb, const double* ub,
     case ORC_LMO_PORTFOLIO: {
         const int m = l.assets;
         if (n != 2 * m) return false;
-        double zmin = 0.0, xcap = 0.0;
-        for (int i = 0; i < m; ++i) {
-            zmin = zmin + std::fmax(lb[m + i], 0.0);
-            xcap = xcap + l.cap * std::fmin(ub[m + i], 1.0);
-        }
-        return zmin <= l.card + 1e-9 && xcap >= 1.0 - 1e-9;
+        double zsum = 0.0;
+        const double capacity = portfolio_capacity(m, l.cap, l.card, lb + m, ub + m, &zsum);
+        return zsum <= l.card + 1e-9 && capacity >= l.right_side - 1e-9;
     }
     }
     return false;
@@ -663,7 +647,7 @@ bool rounded_feasible(const orc_lmo& l, int n, const double* x, double tol, int 
             xs = xs + x[i];
             zs = zs + x[m + i];
         }
-        return std::fabs(xs - 1.0) <= tol && zs <= l.card + tol;
+        return std::fabs(xs - l.right_side) <= tol && zs <= l.card + tol;
     }
     }
     return false;

@@ -129,6 +129,11 @@ def birkhoff_lmo(order):
     return Lmo(LMO_BIRKHOFF, 1.0, 0, None, int(order), 0, 0.0, 0.0)
 
 
+def portfolio_lmo(assets, cap, card, rhs=1.0):
+    """Cardinality portfolio: v = (x, z), sum x = rhs, 0 <= x_i <= cap z_i, sum z <= card."""
+    return Lmo(LMO_PORTFOLIO, float(rhs), 0, None, 0, int(assets), float(cap), float(card))
+
+
 def alg(dicg=True, ls=LS_SECANT, secant_iters=4, secant_tol=1e-10, secant_denom_eps=1e-14,
         heuristic=0, heuristic_offset=0.0):
     return Alg(1 if dicg else 0, ls, secant_iters, secant_tol, secant_denom_eps, heuristic,

@@ -25,7 +25,7 @@ class Problem:
     """A dense-quadratic or QAP objective plus an LMO, held on both sides."""
 
     def __init__(self, kind="dense", Q=None, c=None, gs=2.0, ls=0.5, kappa=0.0, A=None, B=None,
-                 lmo="box", rhs=1.0, equality=False, is_int=None, order=0):
+                 lmo="box", rhs=1.0, equality=False, is_int=None, order=0, cap=0.2, card=10.0):
         from oracle import pyoracle as orc
         self.orc = orc
         self.kind, self.lmo_kind = kind, lmo
@@ -59,6 +59,10 @@ class Problem:
             self.glmo = gb.LmoDesc(gb.LMO_SIMPLEX, float(rhs), 1 if equality else 0,
                                    self.dii.data_ptr(), 0)
             self.olmo = orc.simplex_lmo(rhs, ii, equality)
+        elif lmo == "portfolio":
+            m = n // 2
+            self.glmo = gb.LmoDesc(gb.LMO_PORTFOLIO, float(rhs), 0, 0, 0, float(cap), float(card))
+            self.olmo = orc.portfolio_lmo(m, cap, card, rhs)
         else:
             self.glmo = gb.LmoDesc(gb.LMO_BIRKHOFF, 1.0, 0, 0, int(order))
             self.olmo = orc.birkhoff_lmo(order)
