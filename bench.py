@@ -7,9 +7,11 @@ tolerance 1e-3 * 0.8^depth, 200 iterations per node per step.  Inputs are
 resident in HBM; each step resets the per-node iteration counters and
 statuses (fill kernels) and relaxes the batch again.
 
-  value      FW iterations/s of the mixed-precision form (BF16 MFMA pass +
-             FP64 certification pass), whole batch, HIP events on the launch
-             stream bracket the timed steps
+  value      FW iterations/s of the mixed-precision form (BF16 MFMA pass,
+             one bf16 plane of Q, + FP64 certification pass), whole batch,
+             HIP events on the launch stream bracket the timed steps
+  mixed_fp32 the same with three bf16 planes of Q (FP32-class products), the
+             precision the searches in time_to_optimum use
   plain      the reference algorithm (FP64, DICG + secant), same batch
   batched    the FP64 batched form, same batch
   scale      the mixed form on 16384 nodes (16 batches in flight)
@@ -97,7 +99,7 @@ class Batch:
         self.mit = torch.zeros(N, dtype=torch.int64, device=cuda)
         self.stats = torch.zeros((N, 2), dtype=torch.int64, device=cuda)
         self.stream = torch.cuda.Stream(device=cuda)
-        self.mq = None
+        self.mq = {}
 
     def reset(self):
         self.it.zero_()
@@ -114,9 +116,10 @@ class Batch:
                                 self.eps, self.it, self.st, self.gap, self.val, self.bv, self.bvv, self.flags,
                                 1e-6, None, 64, 4, 0, s, self.stats)
         else:
-            if self.mq is None:
-                self.mq = gb.MixedQ(self.obj, 1, s)
-            gb.relax_batch_mixed(self.mq, self.obj, self.alg, self.lmo, self.N, budget, self.x, self.lb,
+            planes = 3 if form == "mixed_fp32" else 1
+            if self.mq.get(planes) is None:
+                self.mq[planes] = gb.MixedQ(self.obj, planes, s)
+            gb.relax_batch_mixed(self.mq[planes], self.obj, self.alg, self.lmo, self.N, budget, self.x, self.lb,
                                  self.ub, self.eps, self.it, self.st, self.gap, self.val, self.bv, self.bvv,
                                  self.flags, 1e-6, None, fp64_max_iters=-1 if form == "mixed_pass1" else 0,
                                  mixed_iterations=self.mit, stream=s)
@@ -198,7 +201,9 @@ def solve_row(gb, Q, c, n, form, learned=None):
     s.time_limit_seconds = 120.0
     s.form = form
     if form == 2:
-        s.precision = gb.PREC_BF16
+        # three bf16 planes (FP32-class products): fewer FP64 polish
+        # iterations in the search than one plane, at nearly the same pass cost
+        s.precision = gb.PREC_FP32
     if learned is not None:
         s.branch_rule = gb.BRANCH_LEARNED
         s.learned_model = learned.handle
@@ -249,7 +254,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tto", action="store_true")
     ap.add_argument("--no-scale", action="store_true")
-    ap.add_argument("--forms", default="plain,batched,mixed,mixed_pass1",
+    ap.add_argument("--forms", default="plain,batched,mixed,mixed_fp32,mixed_pass1",
                     help="forms to time (the profiled runs time one)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")

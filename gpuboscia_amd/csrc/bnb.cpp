@@ -508,6 +508,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     }
     gb_mixed_options tune{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
     bool tuned = false;
+    std::vector<int> score_off, cand_var, order_buf;
+    std::vector<float> cand_score;
     if (lab) GB_HIP(hipMemcpy(sb.flags, hflags.data(), n, hipMemcpyHostToDevice));
 
     std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
@@ -657,19 +659,26 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         GB_HIP(hipStreamSynchronize(db.stream));
         relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
 
-        // Learned scores of each node's fractional candidates.
-        std::vector<int> score_off(N + 1, 0);
-        std::vector<std::vector<int>> score_cand(learned ? N : 0);
-        std::vector<float> cand_score;
+        // Learned scores of each node's fractional candidates (flat arrays,
+        // reused across batches).
         if (learned) {
+            score_off.assign(N + 1, 0);
+            cand_var.clear();
+            cand_score.clear();
             for (int i = 0; i < N; ++i) {
-                score_off[i] = (int)cand_score.size();
+                score_off[i] = (int)cand_var.size();
                 if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
-                    score_cand[i] = fractional_candidates(db.hx + (size_t)i * n, iv, s->integer_tol);
-                    for (int v : score_cand[i]) cand_score.push_back(db.hscore[(size_t)i * n + v]);
+                    const double* xi = db.hx + (size_t)i * n;
+                    for (int v : iv) {
+                        const double frac = xi[v] - std::floor(xi[v]);
+                        if (std::min(frac, 1.0 - frac) > s->integer_tol) {
+                            cand_var.push_back(v);
+                            cand_score.push_back(db.hscore[(size_t)i * n + v]);
+                        }
+                    }
                 }
             }
-            score_off[N] = (int)cand_score.size();
+            score_off[N] = (int)cand_var.size();
         }
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
@@ -761,14 +770,14 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     // strong-branching score (ties to the lower index), split
                     // 2^k ways like branch_pow2
                     const float* sc = cand_score.data() + score_off[i];
+                    const int* cv = cand_var.data() + score_off[i];
                     const int cnt = score_off[i + 1] - score_off[i];
-                    std::vector<int> order(cnt);
-                    for (int c = 0; c < cnt; ++c) order[c] = c;
+                    order_buf.resize(cnt);
+                    for (int c = 0; c < cnt; ++c) order_buf[c] = c;
                     const int k = std::min(s->branch_k, cnt);
-                    std::partial_sort(order.begin(), order.begin() + k, order.end(), [&](int a, int b) {
-                        return sc[a] > sc[b] || (sc[a] == sc[b] && score_cand[i][a] < score_cand[i][b]);
-                    });
-                    for (int c = 0; c < k; ++c) vars.push_back(score_cand[i][order[c]]);
+                    std::partial_sort(order_buf.begin(), order_buf.begin() + k, order_buf.end(),
+                                      [&](int a, int b) { return sc[a] > sc[b] || (sc[a] == sc[b] && cv[a] < cv[b]); });
+                    for (int c = 0; c < k; ++c) vars.push_back(cv[order_buf[c]]);
                 } else {
                     // GB_BRANCH_MOST_FRACTIONAL, and the learned rule without a model
                     const int v = most_fractional(xi, iv, s->integer_tol);
