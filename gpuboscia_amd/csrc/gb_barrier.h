@@ -6,11 +6,13 @@
 // co-resident workgroups meets at a generation counter.  Each arrival is an
 // acq_rel fetch_add on `arrived`; the last arrival resets the counter and
 // release-stores generation + 1; the others spin on an acquire load of the
-// generation.  Every thread fences its own writes at device scope before the
-// workgroup barrier, so a group sync publishes all threads' writes, not only
-// those of the arriving thread.  A fast workgroup cannot complete the next
-// generation before every member has arrived, so a waiter never sees a
-// wrapped generation.
+// generation.  The workgroup barrier orders the other threads' writes before
+// the arriving thread's device-scope release (and its acquire before their
+// later reads), so a group sync publishes all threads' writes; the stress
+// self-test checks every thread's value each generation.  Waiters poll with
+// relaxed loads and acquire once, when the generation has moved.  A fast
+// workgroup cannot complete the next generation before every member has
+// arrived, so a waiter never sees a wrapped generation.
 //
 // Every wait is bounded: after `spin_limit` polls a waiter records a timeout,
 // raises `abort` and returns false, and every other waiter of the same
@@ -69,7 +71,6 @@ __device__ inline bool group_sync(GroupBarrier* b, unsigned int count, unsigned 
         __syncthreads();
         return true;
     }
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
         bool ok = true;
@@ -82,7 +83,7 @@ __device__ inline bool group_sync(GroupBarrier* b, unsigned int count, unsigned 
             __hip_atomic_store(&b->generation, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             unsigned long long polls = 0;
-            while (__hip_atomic_load(&b->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(&b->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (abort_raised(st)) {
                     ok = false;
                     break;
@@ -94,11 +95,11 @@ __device__ inline bool group_sync(GroupBarrier* b, unsigned int count, unsigned 
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         *flag = ok ? 1 : 0;
     }
     __syncthreads();
-    __threadfence();
     return *flag != 0;
 }
 
