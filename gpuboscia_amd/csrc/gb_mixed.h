@@ -49,22 +49,29 @@ struct MixedParams {
 
 int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double ls, int n, int rt,
                          int planes, uint4* frag, float* lsc, void* stream);
-// LDS of one tile (W waves): s_g s_l c, vertex nibbles per (row tile, lane),
-// per-column partials per wave, fetched node ids, feasibility flags.
+// LDS of one tile (W waves of rw row tiles each):
+//  vops, aops  B operands of the vertices v and a: 16-byte slot
+//              (s * 16 + column) * 4 + q holds rows 4q .. 4q+3 of tiles 2s
+//              and 2s+1 as 8 bf16 (1 KB per k-step);
+//  xch         per wave, Q v and Q a of its tiles as [array][j][column][row]
+//              floats (2 rw KB per wave);
+//  red         per-column partial sums of each wave, 8 floats;
+//  fetch, ok   fetched node ids, per-wave scope flags.
+// At most 8 + 8 + 32 + 8 KB for n <= 256.
 struct MixedLds {
-    size_t lsc, bits, red, fetch, ok, total;
+    size_t vops, aops, xch, red, fetch, ok, total;
 };
-__host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves) {
+__host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw) {
     MixedLds L{};
     size_t o = 0;
-    L.lsc = o;
-    o += sizeof(float) * 16 * static_cast<size_t>(rt);
-    o = (o + 15) & ~static_cast<size_t>(15);
-    L.bits = o;
-    o += static_cast<size_t>(rt) * 64;
-    o = (o + 15) & ~static_cast<size_t>(15);
+    L.vops = o;
+    o += 1024 * static_cast<size_t>(rt / 2);
+    L.aops = o;
+    o += 1024 * static_cast<size_t>(rt / 2);
+    L.xch = o;
+    o += sizeof(float) * 2 * 256 * static_cast<size_t>(rw) * waves;
     L.red = o;
-    o += sizeof(double) * static_cast<size_t>(waves) * 16 * 7;
+    o += sizeof(float) * 8 * 16 * static_cast<size_t>(waves);
     L.fetch = o;
     o += sizeof(int) * 16;
     L.ok = o;
@@ -72,6 +79,10 @@ __host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves) {
     L.total = o;
     return L;
 }
+
+// Rows per lane and tile in the element layout: 1, 2, 4 for up to 4, 8, 16
+// node columns per tile.
+inline int mixed_rows_per_lane(int cols) { return cols <= 4 ? 1 : (cols <= 8 ? 2 : 4); }
 
 constexpr int kMixedMaxRowTilesPerWave = 4;
 

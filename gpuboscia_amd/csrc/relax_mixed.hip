@@ -1,20 +1,29 @@
 // Mixed-precision batched relaxation of dense quadratics over 0/1 boxes,
 // gfx950 (the precision form of the batched B&B: BF16 MFMA products, FP32
-// iterates, FP64 steps and reductions; the bound that decides pruning and the
+// iterates, partial sums and step; the bound that decides pruning and the
 // incumbent value are certified afterwards in FP64 by the batched FP64
 // kernel, which continues from the iterate written here).
 //
-// Layout: one wave64 carries 16 node columns in lockstep.  Lane l serves
-// column c = l % 16 and owns rows 16 t + 4 (l / 16) + r (t < RT, r < 4) of
-// that column's x, Q x (registers, FP32).  Each pass computes Q v and Q a
-// for all 16 columns with v_mfma_f32_16x16x32_bf16: the A operand is Q
-// (pre-split into 1..3 bf16 planes q = q0 + q1 + q2, FP32 accumulation over
-// the planes), the B operand the vertices v, a.  Box vertices of a 0/1 box
+// Layout: a workgroup of W waves carries up to 16 node columns in lockstep;
+// wave w owns the row tiles t = w + W j (j < RW, t < RT) of every column.
+// Two lane layouts meet in each pass:
+//  - MFMA layout: lane l holds rows 16 t + 4 (l / 16) + r (r < 4) of column
+//    l % 16 in the accumulators of v_mfma_f32_16x16x32_bf16;
+//  - element layout: column k's x, Q x (FP32 registers) live in the G =
+//    16 / RPT adjacent lanes k G .. k G + G - 1, lane k G + m holding rows
+//    16 t + RPT m + r (r < RPT) of each of the wave's tiles.  RPT = 1, 2, 4
+//    serves up to 4, 8, 16 columns, so a tile with few columns still keeps
+//    every lane busy in the elementwise work, which bounds the pass.
+// One pass: the vertices v, a are chosen in the element layout and written
+// to LDS as bf16 0/1 B operands in MFMA k-slot order; Q v and Q a come from
+// the MFMA with the A operand Q (pre-split into 1..3 bf16 planes q = q0 +
+// q1 + q2, FP32 accumulation over the planes, fragments held in registers
+// where they fit); the accumulators return to the element layout through a
+// per-wave LDS exchange; partial sums are reduced over the column's lanes
+// (DPP) and over the waves (LDS, in wave order).  Box vertices of a 0/1 box
 // are 0/1 and exact in bf16, so the only rounding of the product is the
 // plane split of Q (about 2^-8 relative for one plane, 2^-24 for three) and
-// the FP32 accumulation.  The B-operand k-slots of lane l for k-step s are
-// the rows the lane owns in tiles 2s and 2s+1, so vertices go from the
-// lane's bit masks straight into the MFMA operand.
+// the FP32 accumulation.
 //
 // Per column the pass is one DICG (or FW) iteration of the batched form:
 // v = argmin over the box of g'v, the in-face away vertex a (face tolerance
@@ -26,11 +35,14 @@
 // and re-scored in FP64 by the certifying kernel.  A column stops at
 // gap < max(eps, rel_floor (1 + |f|)), at f - gap >= cutoff, or at the
 // iteration budget, writes its iterate and takes the next node from the
-// device counter; the wave exits when the counter is exhausted and all of
-// its columns are idle.  A fresh node starts at the box vertex nearest to
+// device counter; the workgroup exits when the counter is exhausted and all
+// of its columns are idle.  A fresh node starts at the box vertex nearest to
 // its supplied point (the upper bounds when none is supplied, as the plain
 // form's init vertex); nodes with bounds other than 0/1 and continuations
-// are left to the FP64 kernel.
+// are left to the FP64 kernel.  A node's trajectory depends only on the node
+// and the launch shape (RW, RPT), not on the column or tile that serves it.
+#include <type_traits>
+
 #include "gb_device.h"
 #include "gb_mixed.h"
 
@@ -61,45 +73,66 @@ __device__ unsigned short bf16_plane(double q, int p) {
     return b;
 }
 
-// 8 bits -> 8 bf16 values 0.0 / 1.0 (0x3F80), two per dword.
-__device__ __forceinline__ bf16x8 expand_bits(unsigned int bits) {
-    uint4 w;
-    w.x = ((bits & 1u) * 0x3F80u) | (((bits >> 1) & 1u) * 0x3F800000u);
-    w.y = (((bits >> 2) & 1u) * 0x3F80u) | (((bits >> 3) & 1u) * 0x3F800000u);
-    w.z = (((bits >> 4) & 1u) * 0x3F80u) | (((bits >> 5) & 1u) * 0x3F800000u);
-    w.w = (((bits >> 6) & 1u) * 0x3F80u) | (((bits >> 7) & 1u) * 0x3F800000u);
-    return __builtin_bit_cast(bf16x8, w);
-}
-
 __device__ __forceinline__ bf16x8 load_frag(const uint4* p) {
     return __builtin_bit_cast(bf16x8, *p);
 }
 
-__device__ __forceinline__ double quad_exact_step(double gap, double curvature, double maximum) {
-    if (maximum <= 0.0) return 0.0;
-    if (curvature > 0.0) {
-        if (gap <= 0.0) return 0.0;
-        const double step = gap / curvature;
-        return step < maximum ? step : maximum;
+constexpr unsigned int kBf16One = 0x3F80u;
+
+// RPT consecutive rows of a B-operand slot: bit r of `bits` -> bf16 1.0 or 0.0.
+template <int RPT>
+__device__ __forceinline__ void put_rows(unsigned short* p, uint32_t bits) {
+    if constexpr (RPT == 1) {
+        *p = static_cast<unsigned short>((bits & 1u) * kBf16One);
+    } else if constexpr (RPT == 2) {
+        *reinterpret_cast<uint32_t*>(p) = (bits & 1u) * kBf16One | ((bits >> 1) & 1u) * (kBf16One << 16);
+    } else {
+        uint2 v;
+        v.x = (bits & 1u) * kBf16One | ((bits >> 1) & 1u) * (kBf16One << 16);
+        v.y = ((bits >> 2) & 1u) * kBf16One | ((bits >> 3) & 1u) * (kBf16One << 16);
+        *reinterpret_cast<uint2*>(p) = v;
     }
-    return -gap + 0.5 * curvature * maximum < 0.0 ? maximum : 0.0;
 }
 
-// Sum / min over the 4 lanes of a column (lanes c, c+16, c+32, c+48); every
-// lane of the column receives the same value.
-__device__ __forceinline__ double col_sum(double v) {
-    v = v + __shfl_xor(v, 16, kWave);
-    return v + __shfl_xor(v, 32, kWave);
+// Exact quadratic step on the segment [0, maximum] (FP32).
+__device__ __forceinline__ float quad_exact_step(float gap, float curvature, float maximum) {
+    if (maximum <= 0.0f) return 0.0f;
+    if (curvature > 0.0f) {
+        if (gap <= 0.0f) return 0.0f;
+        const float step = gap / curvature;
+        return step < maximum ? step : maximum;
+    }
+    return -gap + 0.5f * curvature * maximum < 0.0f ? maximum : 0.0f;
 }
-__device__ __forceinline__ double col_min(double v) {
-    v = fmin(v, __shfl_xor(v, 16, kWave));
-    return fmin(v, __shfl_xor(v, 32, kWave));
+
+// In-row lane permutations (DPP); every lane of the wave must be active.
+constexpr int kDppSwap1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int kDppSwap2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;  // lane i <-> 7 - i within 8
+constexpr int kDppMirror = 0x140;      // lane i <-> 15 - i within 16
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
 }
-__device__ __forceinline__ bool col_and(bool p) {
-    int v = p ? 1 : 0;
-    v &= __shfl_xor(v, 16, kWave);
-    v &= __shfl_xor(v, 32, kWave);
-    return v != 0;
+
+// Sum / min over an aligned group of G = 4, 8, 16 lanes: a butterfly, so
+// every lane of the group ends with the same value (a + b == b + a).
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    v = v + dpp<kDppSwap1>(v);
+    v = v + dpp<kDppSwap2>(v);
+    if constexpr (G >= 8) v = v + dpp<kDppHalfMirror>(v);
+    if constexpr (G >= 16) v = v + dpp<kDppMirror>(v);
+    return v;
+}
+template <int G>
+__device__ __forceinline__ float group_min(float v) {
+    v = fminf(v, dpp<kDppSwap1>(v));
+    v = fminf(v, dpp<kDppSwap2>(v));
+    if constexpr (G >= 8) v = fminf(v, dpp<kDppHalfMirror>(v));
+    if constexpr (G >= 16) v = fminf(v, dpp<kDppMirror>(v));
+    return v;
 }
 
 __global__ void mixed_prepare_kernel(const double* Qt, const double* lin, double gs, double ls, int n,
@@ -135,38 +168,46 @@ __global__ void mixed_prepare_kernel(const double* Qt, const double* lin, double
 }
 
 constexpr int kPhaseDone = 0, kPhaseInit = 1, kPhaseActive = 2, kPhaseFetch = 3;
-constexpr int kRed = 7;  // per-column partials exchanged between the waves of a tile
 
-template <int RW, int PL, bool DICG>
+template <int RW, int PL, bool DICG, int RPT>
 __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) relax_mixed_kernel(MixedParams P) {
-    constexpr int EW = 4 * RW;  // elements per lane: rows 16 t + 4 q + r of the wave's tiles
+    constexpr int G = 16 / RPT;  // lanes of a column in the element layout
+    constexpr int E = RW * RPT;  // elements per lane: rows 16 t_j + RPT m + r, e = RPT j + r
+    constexpr unsigned long long kGroupMask = (1ull << G) - 1ull;
     extern __shared__ __align__(16) unsigned char mixed_smem[];
     const int RT = P.rt;
     const int S = RT / 2;
     const int W = blockDim.x / kWave;
-    const MixedLds L = mixed_lds_layout(RT, W);
-    float* lsc_lds = reinterpret_cast<float*>(mixed_smem + L.lsc);
-    uint8_t* bits = mixed_smem + L.bits;
-    double* red = reinterpret_cast<double*>(mixed_smem + L.red);
+    const MixedLds L = mixed_lds_layout(RT, W, RW);
+    uint4* vops = reinterpret_cast<uint4*>(mixed_smem + L.vops);
+    uint4* aops = reinterpret_cast<uint4*>(mixed_smem + L.aops);
+    float* red = reinterpret_cast<float*>(mixed_smem + L.red);
     int* fetch = reinterpret_cast<int*>(mixed_smem + L.fetch);
     int* okw = reinterpret_cast<int*>(mixed_smem + L.ok);
-    for (int i = threadIdx.x; i < 16 * RT; i += blockDim.x) lsc_lds[i] = P.lsc[i];
-    __syncthreads();
     const int lane = lane_id();
-    const int w = wave_id();
-    const int c = lane & 15;
+    const int w = __builtin_amdgcn_readfirstlane(wave_id());  // wave-uniform: tile tests branch on scalars
+    float* xch = reinterpret_cast<float*>(mixed_smem + L.xch) + static_cast<size_t>(w) * (2 * RW * 256);
+    // Columns that never hold a node multiply zeros.
+    for (int i = threadIdx.x; i < S * 64; i += blockDim.x) {
+        vops[i] = make_uint4(0u, 0u, 0u, 0u);
+        aops[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    const int c = lane & 15;   // MFMA layout: column c, rows 4 q + r
     const int q = lane >> 4;
+    const int col = lane / G;  // element layout: column col, rows RPT m + r
+    const int m = lane % G;
     const int n = P.n;
     const float gs = P.gs;
     const float hs = 0.5f * P.gs;
     const size_t tstride = static_cast<size_t>(S) * kWave;
     const uint4* fbase = P.frag + lane;
 
-    // Wave w owns row tiles t = w + W j (j < RW); element e = 4 j + r is row
-    // 16 t + 4 q + r of column c.
-    float x[EW], qx[EW];
+    float x[E], qx[E], ls[E];
 #pragma unroll
-    for (int e = 0; e < EW; ++e) {
+    for (int e = 0; e < E; ++e) {
+        const int t = w + W * (e / RPT);
+        const int row = 16 * t + RPT * m + e % RPT;
+        ls[e] = t < RT ? P.lsc[row] : 0.0f;
         x[e] = 0.0f;
         qx[e] = 0.0f;
     }
@@ -184,27 +225,28 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
 #pragma unroll
                 for (int s = 0; s < kSMax; ++s) {
                     const int t = w + W * j;
-                    if (t < RT && s < S)
-                        fc[p][j][s] = load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride +
-                                                static_cast<size_t>(s) * kWave);
+                    fc[p][j][s] = (t < RT && s < S) ? load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride +
+                                                                static_cast<size_t>(s) * kWave)
+                                                    : bf16x8{};
                 }
     }
     uint32_t lbm = 0, ubm = 0, x0m = 0, bestm = 0;
-    int phase = c < P.cols ? kPhaseFetch : kPhaseDone;
+    int phase = col < P.cols ? kPhaseFetch : kPhaseDone;
     int node = -1, it = 0, initial = 0;
     double eps = 0.0, cutoff = 0.0, best = kNone;
+    __syncthreads();
 
     while (true) {
         // Columns without a node take the next ones from the counter (wave 0
         // fetches, the tile's waves share the result); nodes outside the
         // 0/1-box scope and continuations are left to the FP64 kernel.  The
-        // column state is replicated in every wave, so all waves take the
-        // same branches and meet at the same barriers.
+        // column state is replicated in every wave and in the column's lanes,
+        // so all waves take the same branches and meet at the same barriers.
         while (__ballot(phase == kPhaseFetch) != 0ull) {
-            if (w == 0 && q == 0 && phase == kPhaseFetch) fetch[c] = atomicAdd(P.counter, 1);
+            if (w == 0 && m == 0 && phase == kPhaseFetch) fetch[col] = atomicAdd(P.counter, 1);
             __syncthreads();
             if (phase == kPhaseFetch) {
-                node = fetch[c];
+                node = fetch[col];
                 if (node >= P.N) phase = kPhaseDone;
             }
             const bool loading = phase == kPhaseFetch;
@@ -218,29 +260,25 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
                 init_it = P.iters[node];
                 const bool from_x = init_it != 0 || (P.status[node] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0;
 #pragma unroll 1
-                for (int j = 0; j < RW; ++j) {
-                    const int t = w + W * j;
-#pragma unroll 1
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = 16 * t + 4 * q + r;
-                        const int e = 4 * j + r;
-                        if (t < RT && row < n) {
-                            const double l = lbn[row], u = ubn[row];
-                            if (!(l <= u) || !(l == 0.0 || l == 1.0) || !(u == 0.0 || u == 1.0)) ok = false;
-                            const double xv = from_x ? xn[row] : u;
-                            const bool xb = (xv >= 0.5 * (l + u)) ? (u == 1.0) : (l == 1.0);
-                            lm |= static_cast<uint32_t>(l == 1.0) << e;
-                            um |= static_cast<uint32_t>(u == 1.0) << e;
-                            xm |= static_cast<uint32_t>(xb) << e;
-                        }
+                for (int e = 0; e < E; ++e) {
+                    const int t = w + W * (e / RPT);
+                    const int row = 16 * t + RPT * m + e % RPT;
+                    if (t < RT && row < n) {
+                        const double l = lbn[row], u = ubn[row];
+                        if (!(l <= u) || !(l == 0.0 || l == 1.0) || !(u == 0.0 || u == 1.0)) ok = false;
+                        const double xv = from_x ? xn[row] : u;
+                        const bool xb = (xv >= 0.5 * (l + u)) ? (u == 1.0) : (l == 1.0);
+                        lm |= static_cast<uint32_t>(l == 1.0) << e;
+                        um |= static_cast<uint32_t>(u == 1.0) << e;
+                        xm |= static_cast<uint32_t>(xb) << e;
                     }
                 }
             }
-            ok = col_and(ok && init_it == 0);
-            if (q == 0) okw[w * 16 + c] = ok ? 1 : 0;
+            const unsigned long long bad = __ballot(!(ok && init_it == 0));
+            if (m == 0) okw[w * 16 + col] = ((bad >> (col * G)) & kGroupMask) == 0ull ? 1 : 0;
             __syncthreads();
             bool all_ok = true;
-            for (int v = 0; v < W; ++v) all_ok = all_ok && okw[v * 16 + c] != 0;
+            for (int v = 0; v < W; ++v) all_ok = all_ok && okw[v * 16 + col] != 0;
             if (loading && all_ok) {
                 phase = kPhaseInit;
                 lbm = lm;
@@ -259,173 +297,204 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
         const bool act = phase == kPhaseActive;
         const bool ini = phase == kPhaseInit;
 
-        // Vertices of the wave's rows, partial sums at x, vertex nibbles to LDS.
+        // Vertices of the lane's rows, partial sums at x.
         uint32_t vm = 0, am = 0;
         float gp = 0.0f, fp = 0.0f;
+        float g[E];
 #pragma unroll
-        for (int e = 0; e < EW; ++e) {
-            const int t = w + W * (e >> 2);
-            const int row = 16 * t + 4 * q + (e & 3);
-            const float ls = t < RT ? lsc_lds[row] : 0.0f;
+        for (int e = 0; e < E; ++e) {
             const bool lb1 = (lbm >> e) & 1u;
             const bool ub1 = (ubm >> e) & 1u;
-            const float g = gs * qx[e] + ls;
-            const bool vb = g > 0.0f ? lb1 : ub1;
+            g[e] = gs * qx[e] + ls[e];
+            const bool vb = g[e] > 0.0f ? lb1 : ub1;
             bool ab = false;
             if (DICG) {
                 const float uf = ub1 ? 1.0f : 0.0f, lf = lb1 ? 1.0f : 0.0f;
-                if (fabsf(x[e] - uf) <= 1e-6f + 1e-4f * uf) ab = ub1;
-                else if (fabsf(x[e] - lf) <= 1e-6f + 1e-4f * lf) ab = lb1;
-                else ab = (-g > 0.0f) ? lb1 : ub1;
+                const bool at_u = fabsf(x[e] - uf) <= 1e-6f + 1e-4f * uf;
+                const bool at_l = fabsf(x[e] - lf) <= 1e-6f + 1e-4f * lf;
+                ab = at_u ? ub1 : (at_l ? lb1 : ((-g[e] > 0.0f) ? lb1 : ub1));
             }
             vm |= static_cast<uint32_t>(vb) << e;
             am |= static_cast<uint32_t>(ab) << e;
-            gp += g * (x[e] - (vb ? 1.0f : 0.0f));
-            fp += x[e] * (hs * qx[e] + ls);
+            gp += g[e] * (x[e] - (vb ? 1.0f : 0.0f));
+            fp += x[e] * (hs * qx[e] + ls[e]);
         }
-        if (ini) {
-            vm = x0m;
-            am = 0;
-        } else if (!act) {
-            vm = 0;
-            am = 0;
-        }
+        vm = ini ? x0m : (act ? vm : 0u);
+        am = act ? am : 0u;
+        // B operands: uint4 slot (s * 16 + k) * 4 + qq holds rows 4 qq .. 4 qq + 3
+        // of tiles 2s, 2s+1 of column k as 8 bf16 (k-slot i = 4 h + r).
+        const int qq = (RPT * m) >> 2;
+        const int r0 = (RPT * m) & 3;
 #pragma unroll
         for (int j = 0; j < RW; ++j) {
             const int t = w + W * j;
-            if (t < RT) bits[t * kWave + lane] = static_cast<uint8_t>(((vm >> (4 * j)) & 0xFu) | (((am >> (4 * j)) & 0xFu) << 4));
+            if (t < RT) {
+                const int slot = ((t >> 1) * 16 + col) * 4 + qq;
+                const int off = (t & 1) * 4 + r0;
+                constexpr uint32_t kRows = (1u << RPT) - 1u;
+                put_rows<RPT>(reinterpret_cast<unsigned short*>(vops + slot) + off, (vm >> (RPT * j)) & kRows);
+                if (DICG) put_rows<RPT>(reinterpret_cast<unsigned short*>(aops + slot) + off, (am >> (RPT * j)) & kRows);
+            }
         }
         __syncthreads();
 
-        // Q [v a] for the wave's row tiles over all k-steps; the next
-        // k-step's fragments are loaded while the current one multiplies.
+        // Q v, Q a for the wave's row tiles over all k-steps (tiles past RT
+        // multiply zero fragments; the k-step count is a compile-time
+        // constant of each case, so the MFMA chain has no branches).
         f32x4 accv[RW], acca[RW];
 #pragma unroll
         for (int j = 0; j < RW; ++j) {
             accv[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             acca[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
-        if (kCache) {
+        auto products = [&](auto steps) {
+            constexpr int SS = decltype(steps)::value;
+            if constexpr (kCache) {
 #pragma unroll
-            for (int s = 0; s < kSMax; ++s) {
-                if (s < S) {
-                    const unsigned int b0 = bits[(2 * s) * kWave + lane];
-                    const unsigned int b1 = bits[(2 * s + 1) * kWave + lane];
-                    const bf16x8 bv = expand_bits((b0 & 0xFu) | ((b1 & 0xFu) << 4));
-                    const bf16x8 ba = expand_bits((b0 >> 4) | (b1 & 0xF0u));
+                for (int s = 0; s < SS; ++s) {
+                    const bf16x8 bv = __builtin_bit_cast(bf16x8, vops[(s * 16 + c) * 4 + q]);
+                    const bf16x8 ba = DICG ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
 #pragma unroll
                     for (int p = 0; p < PL; ++p)
 #pragma unroll
                         for (int j = 0; j < RW; ++j) {
-                            if (w + W * j < RT) {
-                                accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fc[p][j][s], bv, accv[j], 0, 0, 0);
-                                if (DICG)
-                                    acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fc[p][j][s], ba, acca[j], 0, 0, 0);
-                            }
+                            accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fc[p][j][s], bv, accv[j], 0, 0, 0);
+                            if (DICG)
+                                acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fc[p][j][s], ba, acca[j], 0, 0, 0);
                         }
                 }
-            }
-        } else {
-        bf16x8 cur[PL][RW];
-#pragma unroll
-        for (int p = 0; p < PL; ++p)
-#pragma unroll
-            for (int j = 0; j < RW; ++j) {
-                const int t = w + W * j;
-                if (t < RT) cur[p][j] = load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride);
-            }
-        for (int s = 0; s < S; ++s) {
-            bf16x8 nxt[PL][RW];
-            if (s + 1 < S) {
+            } else {
+                // Streaming: the next k-step's fragments load while the
+                // current one multiplies.
+                bf16x8 cur[PL][RW];
 #pragma unroll
                 for (int p = 0; p < PL; ++p)
 #pragma unroll
                     for (int j = 0; j < RW; ++j) {
                         const int t = w + W * j;
-                        if (t < RT)
-                            nxt[p][j] = load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride +
-                                                  static_cast<size_t>(s + 1) * kWave);
+                        cur[p][j] = t < RT ? load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride) : bf16x8{};
                     }
-            }
-            const unsigned int b0 = bits[(2 * s) * kWave + lane];
-            const unsigned int b1 = bits[(2 * s + 1) * kWave + lane];
-            const bf16x8 bv = expand_bits((b0 & 0xFu) | ((b1 & 0xFu) << 4));
-            const bf16x8 ba = expand_bits((b0 >> 4) | (b1 & 0xF0u));
 #pragma unroll
-            for (int p = 0; p < PL; ++p)
+                for (int s = 0; s < SS; ++s) {
+                    bf16x8 nxt[PL][RW];
+                    if (s + 1 < SS) {
 #pragma unroll
-                for (int j = 0; j < RW; ++j) {
-                    const int t = w + W * j;
-                    if (t < RT) {
-                        accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], bv, accv[j], 0, 0, 0);
-                        if (DICG) acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], ba, acca[j], 0, 0, 0);
+                        for (int p = 0; p < PL; ++p)
+#pragma unroll
+                            for (int j = 0; j < RW; ++j) {
+                                const int t = w + W * j;
+                                nxt[p][j] = t < RT ? load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride +
+                                                               static_cast<size_t>(s + 1) * kWave)
+                                                   : bf16x8{};
+                            }
+                    }
+                    const bf16x8 bv = __builtin_bit_cast(bf16x8, vops[(s * 16 + c) * 4 + q]);
+                    const bf16x8 ba = DICG ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
+#pragma unroll
+                    for (int p = 0; p < PL; ++p)
+#pragma unroll
+                        for (int j = 0; j < RW; ++j) {
+                            accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], bv, accv[j], 0, 0, 0);
+                            if (DICG) acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], ba, acca[j], 0, 0, 0);
+                        }
+                    if (s + 1 < SS) {
+#pragma unroll
+                        for (int p = 0; p < PL; ++p)
+#pragma unroll
+                            for (int j = 0; j < RW; ++j) cur[p][j] = nxt[p][j];
                     }
                 }
-            if (s + 1 < S) {
-#pragma unroll
-                for (int p = 0; p < PL; ++p)
-#pragma unroll
-                    for (int j = 0; j < RW; ++j) cur[p][j] = nxt[p][j];
             }
-        }
+        };
+        switch (S) {
+            case 1: products(std::integral_constant<int, 1>{}); break;
+            case 2: products(std::integral_constant<int, 2>{}); break;
+            case 3: products(std::integral_constant<int, 3>{}); break;
+            case 4: products(std::integral_constant<int, 4>{}); break;
+            case 5: products(std::integral_constant<int, 5>{}); break;
+            case 6: products(std::integral_constant<int, 6>{}); break;
+            case 7: products(std::integral_constant<int, 7>{}); break;
+            default: products(std::integral_constant<int, 8>{}); break;
         }
 
-        // Vertex values and step partials; Q d replaces Q v in accv (Q v for
-        // a column that is being initialised).
-        float fvp = 0.0f, fap = 0.0f, pgp = 0.0f, cvp = 0.0f, rr = 1.0f;
+        // Accumulators to the element layout: the wave's own LDS region,
+        // [array][j][column][row] floats (no other wave touches it).
+        float qv[E], qa[E];
 #pragma unroll
-        for (int e = 0; e < EW; ++e) {
-            const int t = w + W * (e >> 2);
-            const int row = 16 * t + 4 * q + (e & 3);
-            const float ls = t < RT ? lsc_lds[row] : 0.0f;
-            const float qv = accv[e >> 2][e & 3];
+        for (int j = 0; j < RW; ++j) {
+            float* pv = xch + ((0 * RW + j) * 16 + c) * 16 + 4 * q;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) pv[r] = accv[j][r];
+            if (DICG) {
+                float* pa = xch + ((1 * RW + j) * 16 + c) * 16 + 4 * q;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) pa[r] = acca[j][r];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RW; ++j) {
+            const float* pv = xch + ((0 * RW + j) * 16 + col) * 16 + RPT * m;
+            const float* pa = xch + ((1 * RW + j) * 16 + col) * 16 + RPT * m;
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                qv[RPT * j + r] = pv[r];
+                qa[RPT * j + r] = DICG ? pa[r] : 0.0f;
+            }
+        }
+
+        // Vertex values and step partials; Q d replaces Q v (Q v for a column
+        // that is being initialised).
+        float fvp = 0.0f, fap = 0.0f, pgp = 0.0f, cvp = 0.0f, rr = 1.0f;
+        float qd[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
             const float vf = ((vm >> e) & 1u) ? 1.0f : 0.0f;
-            fvp += vf * (hs * qv + ls);
-            const float g = gs * qx[e] + ls;
-            float d, qd;
+            fvp += vf * (hs * qv[e] + ls[e]);
+            float d, qde;
             if (DICG) {
                 const float af = ((am >> e) & 1u) ? 1.0f : 0.0f;
-                const float qa = acca[e >> 2][e & 3];
-                fap += af * (hs * qa + ls);
+                fap += af * (hs * qa[e] + ls[e]);
                 d = vf - af;
-                qd = qv - qa;
+                qde = qv[e] - qa[e];
                 // a - v is -1, 0 or 1 on a 0/1 box: the ratio needs no division.
-                const float dv = af - vf;
                 const float uf = ((ubm >> e) & 1u) ? 1.0f : 0.0f;
                 const float lf = ((lbm >> e) & 1u) ? 1.0f : 0.0f;
-                if (dv < 0.0f) rr = fminf(rr, uf - x[e]);
-                else if (dv > 0.0f) rr = fminf(rr, x[e] - lf);
+                const float lim = d > 0.0f ? uf - x[e] : (d < 0.0f ? x[e] - lf : 1.0f);
+                rr = fminf(rr, lim);
             } else {
                 d = vf - x[e];
-                qd = qv - qx[e];
+                qde = qv[e] - qx[e];
             }
-            pgp += -g * d;
-            cvp += gs * d * qd;
-            accv[e >> 2][e & 3] = ini ? qv : qd;
+            pgp += -g[e] * d;
+            cvp += gs * d * qde;
+            qd[e] = ini ? qv[e] : qde;
         }
-        // Column totals: 4 lanes of a column, then the tile's waves in order.
-        double part[kRed] = {col_sum(gp), col_sum(fp), col_sum(fvp), col_sum(fap),
-                             col_sum(pgp), col_sum(cvp), col_min(rr)};
-        if (q == 0) {
+        // Column totals: the column's lanes (DPP), then the waves in order.
+        float part[8] = {group_sum<G>(gp), group_sum<G>(fp), group_sum<G>(fvp), DICG ? group_sum<G>(fap) : 0.0f,
+                         group_sum<G>(pgp), group_sum<G>(cvp), DICG ? group_min<G>(rr) : 1.0f, 0.0f};
+        if (m == 0) {
+            float* o = red + (w * 16 + col) * 8;
 #pragma unroll
-            for (int k = 0; k < kRed; ++k) red[(w * 16 + c) * kRed + k] = part[k];
+            for (int k = 0; k < 8; ++k) o[k] = part[k];
         }
         __syncthreads();
-        double tot[kRed];
+        float tot[7];
+        {
+            const float* o = red + col * 8;
 #pragma unroll
-        for (int k = 0; k < kRed; ++k) tot[k] = red[c * kRed + k];
-        for (int v = 1; v < W; ++v) {
-#pragma unroll
-            for (int k = 0; k < kRed - 1; ++k) tot[k] = tot[k] + red[(v * 16 + c) * kRed + k];
-            tot[kRed - 1] = fmin(tot[kRed - 1], red[(v * 16 + c) * kRed + kRed - 1]);
+            for (int k = 0; k < 7; ++k) tot[k] = o[k];
         }
-        const double gap = tot[0];
-        const double fx = tot[1] + P.kappa;
-        const double fv = tot[2] + P.kappa;
-        const double fa = DICG ? tot[3] + P.kappa : kNone;
-        const double pg = tot[4];
-        const double cv = tot[5];
-        const double gmax = DICG ? fmax(0.0, fmin(tot[6], 1.0)) : 1.0;
+        for (int v = 1; v < W; ++v) {
+            const float* o = red + (v * 16 + col) * 8;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) tot[k] = tot[k] + o[k];
+            tot[6] = fminf(tot[6], o[6]);
+        }
+        const float gap = tot[0];
+        const double fx = static_cast<double>(tot[1]) + P.kappa;
+        const double fv = static_cast<double>(tot[2]) + P.kappa;
+        const double fa = DICG ? static_cast<double>(tot[3]) + P.kappa : kNone;
+        const float gmax = DICG ? fmaxf(0.0f, fminf(tot[6], 1.0f)) : 1.0f;
 
         bool retire = false;
         float gm = 0.0f;
@@ -441,9 +510,9 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
                 bestm = am;
             }
             const double tol = fmax(eps, static_cast<double>(P.rel_floor) * (1.0 + fabs(fx)));
-            const bool stop = (gap >= 0.0 && gap < tol) || (fx - gap >= cutoff);
+            const bool stop = (gap >= 0.0f && static_cast<double>(gap) < tol) || (fx - gap >= cutoff);
             if (!stop) {
-                gm = static_cast<float>(quad_exact_step(pg, cv, gmax));
+                gm = quad_exact_step(tot[4], tot[5], gmax);
                 step = true;
             }
             retire = stop || it >= P.max_iters;
@@ -451,19 +520,16 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
         // x += gamma d, Q x += gamma Q d for stepping columns; Q x = Q x0 for
         // columns being initialised.
 #pragma unroll
-        for (int e = 0; e < EW; ++e) {
-            const float qd = accv[e >> 2][e & 3];
-            if (ini) {
-                qx[e] = qd;
-                x[e] = ((x0m >> e) & 1u) ? 1.0f : 0.0f;
-            } else if (step) {
-                const float vf = ((vm >> e) & 1u) ? 1.0f : 0.0f;
-                const float d = DICG ? vf - (((am >> e) & 1u) ? 1.0f : 0.0f) : vf - x[e];
-                const float uf = ((ubm >> e) & 1u) ? 1.0f : 0.0f;
-                const float lf = ((lbm >> e) & 1u) ? 1.0f : 0.0f;
-                x[e] = fminf(fmaxf(fmaf(gm, d, x[e]), lf), uf);
-                qx[e] = fmaf(gm, qd, qx[e]);
-            }
+        for (int e = 0; e < E; ++e) {
+            const float vf = ((vm >> e) & 1u) ? 1.0f : 0.0f;
+            const float d = DICG ? vf - (((am >> e) & 1u) ? 1.0f : 0.0f) : vf - x[e];
+            const float uf = ((ubm >> e) & 1u) ? 1.0f : 0.0f;
+            const float lf = ((lbm >> e) & 1u) ? 1.0f : 0.0f;
+            const float nx = fminf(fmaxf(fmaf(gm, d, x[e]), lf), uf);
+            const float nq = fmaf(gm, qd[e], qx[e]);
+            const float x0 = ((x0m >> e) & 1u) ? 1.0f : 0.0f;
+            x[e] = ini ? x0 : (step ? nx : x[e]);
+            qx[e] = ini ? qd[e] : (step ? nq : qx[e]);
         }
         if (ini) phase = kPhaseActive;
 
@@ -471,15 +537,15 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
             double* xn = P.x + static_cast<size_t>(node) * n;
             double* bvn = P.best_v + static_cast<size_t>(node) * n;
 #pragma unroll
-            for (int e = 0; e < EW; ++e) {
-                const int t = w + W * (e >> 2);
-                const int row = 16 * t + 4 * q + (e & 3);
+            for (int e = 0; e < E; ++e) {
+                const int t = w + W * (e / RPT);
+                const int row = 16 * t + RPT * m + e % RPT;
                 if (t < RT && row < n) {
                     xn[row] = static_cast<double>(x[e]);
                     if (best < kNone) bvn[row] = ((bestm >> e) & 1u) ? 1.0 : 0.0;
                 }
             }
-            if (w == 0 && q == 0) {
+            if (w == 0 && m == 0) {
                 P.iters[node] = initial + it;
                 P.status[node] = GB_NODE_INITIAL_POINT_SUPPLIED;
                 P.best_val[node] = best;
@@ -543,14 +609,17 @@ int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double 
 }
 
 int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void* stream) {
+    if (p.cols < 1 || p.cols > 16 || rw < 1 || rw > kMixedMaxRowTilesPerWave || p.rt < 2 || p.rt > 16)
+        return static_cast<int>(hipErrorInvalidValue);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(mixed_reset_kernel, dim3(1), dim3(1), 0, st, p.counter);
     const int waves = (p.rt + rw - 1) / rw;
+    const int rpt = mixed_rows_per_lane(p.cols);
     const dim3 grid(blocks), block(waves * kWave);
-    const size_t lds = mixed_lds_layout(p.rt, waves).total;
-#define GB_MIXED_ONE(RW, PL, DI)                                                                \
+    const size_t lds = mixed_lds_layout(p.rt, waves, rw).total;
+#define GB_MIXED_ONE(RW, PL, DI, RPT)                                                           \
     do {                                                                                        \
-        auto k = relax_mixed_kernel<RW, PL, DI>;                                                \
+        auto k = relax_mixed_kernel<RW, PL, DI, RPT>;                                           \
         if (lds > 64 * 1024) {                                                                  \
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),                \
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
@@ -558,14 +627,20 @@ int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void
         }                                                                                       \
         hipLaunchKernelGGL(k, grid, block, lds, st, p);                                         \
     } while (0)
+#define GB_MIXED_RPT(RW, PL, DI)                          \
+    do {                                                  \
+        if (rpt == 1) GB_MIXED_ONE(RW, PL, DI, 1);        \
+        else if (rpt == 2) GB_MIXED_ONE(RW, PL, DI, 2);   \
+        else GB_MIXED_ONE(RW, PL, DI, 4);                 \
+    } while (0)
 #define GB_MIXED(RW)                                                     \
     case RW:                                                             \
         if (p.planes == 1) {                                             \
-            if (dicg) GB_MIXED_ONE(RW, 1, true);                         \
-            else GB_MIXED_ONE(RW, 1, false);                             \
+            if (dicg) GB_MIXED_RPT(RW, 1, true);                         \
+            else GB_MIXED_RPT(RW, 1, false);                             \
         } else {                                                         \
-            if (dicg) GB_MIXED_ONE(RW, 3, true);                         \
-            else GB_MIXED_ONE(RW, 3, false);                             \
+            if (dicg) GB_MIXED_RPT(RW, 3, true);                         \
+            else GB_MIXED_RPT(RW, 3, false);                             \
         }                                                                \
         break;
     switch (rw) {
@@ -577,6 +652,7 @@ int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void
             return static_cast<int>(hipErrorInvalidValue);
     }
 #undef GB_MIXED
+#undef GB_MIXED_RPT
 #undef GB_MIXED_ONE
     return static_cast<int>(hipGetLastError());
 }
