@@ -54,35 +54,89 @@ struct WaveNode {
     long long columns;  // column updates performed (work accounting)
     long long gemvs;    // direct products performed
 
-    // y = Q z for the lane's rows; z given in registers.
-    __device__ void gemv(const double (&z)[NT], double (&y)[NT]) {
+    // One sweep over Q for NV = 1 or 2 vectors staged in LDS (za, zb):
+    // ya = Q za, yb = Q zb for the lane's rows.  k runs outermost in batches
+    // of kBatch columns over all of the lane's rows, so kBatch * NT loads
+    // are in flight per round trip.  Every output is the
+    // FMA chain over k ascending, whatever NV and kBatch.
+    static constexpr int kBatch = NT >= 8 ? 2 : (NT >= 4 ? 4 : 8);
+    __device__ void load_batch(int k0, double (&qq)[kBatch][NT]) {
         const int n = P.n;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int j = lane + kWave * t;
-            if (j < n) zs[j] = z[t];
-        }
-        wave_sync();
+        for (int u = 0; u < kBatch; ++u)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int j = lane + kWave * t;
-            double acc = 0.0;
-            if (j < n) {
-                const double* q = Q + j;
-                int k = 0;
-                for (; k + 8 <= n; k += 8) {
-                    double qq[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) qq[u] = q[(size_t)(k + u) * n];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) acc = fma(qq[u], zs[k + u], acc);
-                }
-                for (; k < n; ++k) acc = fma(q[(size_t)k * n], zs[k], acc);
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                const int k = k0 + u;
+                qq[u][t] = (j < n && k < n) ? Q[(size_t)k * n + j] : 0.0;
             }
-            y[t] = acc;
+    }
+    template <int NV>
+    __device__ void sweep(const double* za, const double* zb, double (&ya)[NT], double (&yb)[NT]) {
+        const int n = P.n;
+        double acc[NV][NT];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[v][t] = 0.0;
+        const int batches = (n + kBatch - 1) / kBatch;
+        for (int b = 0; b < batches; ++b) {
+            double qq[kBatch][NT];
+            load_batch(b * kBatch, qq);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int k = b * kBatch + u;
+                if (k < n) {
+                    const double z0 = za[k];
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[0][t] = fma(qq[u][t], z0, acc[0][t]);
+                    if constexpr (NV == 2) {
+                        const double z1 = zb[k];
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) acc[NV - 1][t] = fma(qq[u][t], z1, acc[NV - 1][t]);
+                    }
+                }
+            }
         }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bool in = lane + kWave * t < n;
+            ya[t] = in ? acc[0][t] : 0.0;
+            if constexpr (NV == 2) yb[t] = in ? acc[NV - 1][t] : 0.0;
+        }
+    }
+
+    __device__ void stage(double* dst, const double (&z)[NT]) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int j = lane + kWave * t;
+            if (j < P.n) dst[j] = z[t];
+        }
+    }
+
+    // y = Q z for the lane's rows; z given in registers.
+    __device__ void gemv(const double (&z)[NT], double (&y)[NT]) {
+        stage(zs, z);
+        wave_sync();
+        sweep<1>(zs, zs, y, y);
         wave_sync();
         ++gemvs;
+    }
+
+    // y1 = Q z1 and y2 = Q z2 in one sweep over Q (the change-list buffer
+    // stages z2; no update is in progress).
+    __device__ void gemv2(const double (&z1)[NT], double (&y1)[NT], const double (&z2)[NT], double (&y2)[NT]) {
+        if constexpr (NT >= 8) {  // n > 256: two sweeps keep the state in registers
+            gemv(z1, y1);
+            gemv(z2, y2);
+            return;
+        }
+        stage(zs, z1);
+        stage(dlist, z2);
+        wave_sync();
+        sweep<2>(zs, dlist, y1, y2);
+        wave_sync();
+        gemvs += 2;
     }
 
     // Changed columns are compacted into this wave's LDS list (k ascending)
@@ -236,19 +290,6 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
             }
         };
 
-        // A candidate from an earlier phase (the mixed-precision pass) is
-        // re-scored exactly before this node's own candidates.
-        if (P.rescore && P.best_val[node] < kNone) {
-            double z[NT], qz[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const int j = lane + kWave * t;
-                z[t] = (j < n) ? bvn[j] : 0.0;
-            }
-            W.gemv(z, qz);
-            score(z, qz);
-        }
-
         // Starting point: LMO vertex of the artificial direction (ub for a
         // box, since it is negative) or the supplied / warm-start iterate.
 #pragma unroll
@@ -256,7 +297,21 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
             const int j = lane + kWave * t;
             x[t] = (j < n) ? ((initial == 0 && !supplied) ? ub[t] : xn[j]) : 0.0;
         }
-        W.gemv(x, qx);
+        // A candidate from an earlier phase (the mixed-precision pass) is
+        // re-scored exactly before this node's own candidates; its product
+        // shares the sweep over Q with Q x.
+        if (P.rescore && P.best_val[node] < kNone) {
+            double z[NT], qz[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                z[t] = (j < n) ? bvn[j] : 0.0;
+            }
+            W.gemv2(z, qz, x, qx);
+            score(z, qz);
+        } else {
+            W.gemv(x, qx);
+        }
         auto grad = [&]() {
 #pragma unroll
             for (int t = 0; t < NT; ++t) g[t] = P.gs * qx[t] + lsc[t];
@@ -265,10 +320,29 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) out[t] = (g[t] > 0.0) ? lb[t] : ub[t];
         };
+        // In-face away vertex at x (lmo.cuh:278-295 face tolerances).
+        auto away = [&](double (&out)[NT]) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const double ut = atol + rtol * fabs(ub[t]);
+                const double lt = atol + rtol * fabs(lb[t]);
+                if (fabs(x[t] - ub[t]) <= ut) out[t] = ub[t];
+                else if (fabs(x[t] - lb[t]) <= lt) out[t] = lb[t];
+                else out[t] = (-g[t] > 0.0) ? lb[t] : ub[t];
+            }
+        };
         grad();
         fwd(v);
-        W.gemv(v, qv);
+        // DICG: the first away vertex depends only on x and g, so its
+        // product shares the sweep with Q v (it is used if the node takes a
+        // first step).
         bool have_a = false;
+        if (P.dicg) {
+            away(a);
+            W.gemv2(v, qv, a, qa);
+        } else {
+            W.gemv(v, qv);
+        }
         int iterations = initial;
         double gap = 0.0;
         int it = 0;
@@ -299,20 +373,16 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
 
             double d[NT], qd[NT], gmax;
             if (P.dicg) {
-                double an[NT];
+                // First step: a and Q a are those computed with Q v (x and g
+                // have not changed since).
+                if (have_a) {
+                    double an[NT];
+                    away(an);
+                    W.update(an, a, qa);
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const double ut = atol + rtol * fabs(ub[t]);
-                    const double lt = atol + rtol * fabs(lb[t]);
-                    if (fabs(x[t] - ub[t]) <= ut) an[t] = ub[t];
-                    else if (fabs(x[t] - lb[t]) <= lt) an[t] = lb[t];
-                    else an[t] = (-g[t] > 0.0) ? lb[t] : ub[t];
+                    for (int t = 0; t < NT; ++t) a[t] = an[t];
                 }
-                if (have_a) W.update(an, a, qa);
-                else W.gemv(an, qa);
                 have_a = true;
-#pragma unroll
-                for (int t = 0; t < NT; ++t) a[t] = an[t];
                 score(a, qa);
                 double r = 1.0;
 #pragma unroll
@@ -360,7 +430,21 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
         // Q x and Q v of this iterate and vertex (recomputing them gives the
         // same bits), so they are reused.
         const bool fresh = it == 0;
-        if (!fresh) W.gemv(x, qx);
+        // The rounded candidate depends only on x: its product shares the
+        // sweep with the direct Q x.
+        double qc[NT];
+        auto rounded = [&](double (&out)[NT]) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) out[t] = fl[t] ? round(x[t]) : x[t];
+        };
+        if (P.flags) {
+            double cand[NT];
+            rounded(cand);
+            if (fresh) W.gemv(cand, qc);
+            else W.gemv2(x, qx, cand, qc);
+        } else if (!fresh) {
+            W.gemv(x, qx);
+        }
         grad();
         if (P.gout) {
             double* go = P.gout + (size_t)node * n;
@@ -389,10 +473,8 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
         uint8_t st = (fgap >= 0.0 && fgap < eps) ? 1 : 0;
         bool integral = true;
         if (P.flags) {
-            double cand[NT], qc[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) cand[t] = fl[t] ? round(x[t]) : x[t];
-            W.gemv(cand, qc);
+            double cand[NT];
+            rounded(cand);
             score(cand, qc);
         }
 #pragma unroll
