@@ -216,8 +216,14 @@ int most_fractional(const double* x, const std::vector<int>& iv, double tol) {
     return best;
 }
 
+// Node batch buffers.  The per-node arrays are views into one device block
+// and one pinned host block with the same layout, recomputed for each batch
+// size N by layout(): inputs first, then the arrays that are both input and
+// result, then the results, so a batch moves in one host-to-device copy
+// (lb .. bvv) and one device-to-host copy (x .. score).
 struct DeviceBatch {
     int cap = 0, n = 0;
+    bool learned_io = false;  // depth, pseudocost means and scores travel with the batch
     double *x = nullptr, *lb = nullptr, *ub = nullptr, *bv = nullptr;
     double *eps = nullptr, *gap = nullptr, *val = nullptr, *bvv = nullptr, *cut = nullptr;
     int* it = nullptr;
@@ -231,65 +237,117 @@ struct DeviceBatch {
     // depths, per-variable pseudocost means (up, down; < 0 = no history), Q
     // diagonal
     double *g = nullptr, *hg = nullptr;
-    float *feat = nullptr, *hfeat = nullptr, *score = nullptr, *hscore = nullptr;
+    float *feat = nullptr, *score = nullptr, *hscore = nullptr;
     int *depth = nullptr, *hdepth = nullptr;
     double *pc = nullptr, *hpc = nullptr, *qdiag = nullptr;
+    unsigned char *dpack = nullptr, *hpack = nullptr;
+    size_t in_bytes = 0, out_off = 0, out_bytes = 0;
 
+    struct Offsets {
+        size_t lb, ub, eps, cut, depth, pc, x, it, st, bvv, bv, gap, val, score, end;
+    };
+    Offsets offsets(int N) const {
+        Offsets f{};
+        size_t o = 0;
+        auto take = [&](size_t bytes) {
+            const size_t at = o;
+            o = (o + bytes + 255) & ~static_cast<size_t>(255);
+            return at;
+        };
+        const size_t xb = (size_t)N * n * 8;
+        f.lb = take(xb);
+        f.ub = take(xb);
+        f.eps = take((size_t)N * 8);
+        f.cut = take((size_t)N * 8);
+        f.depth = take(learned_io ? (size_t)N * 4 : 0);
+        f.pc = take(learned_io ? (size_t)n * 16 : 0);
+        f.x = take(xb);
+        f.it = take((size_t)N * 4);
+        f.st = take((size_t)N);
+        f.bvv = take((size_t)N * 8);
+        f.bv = take(xb);
+        f.gap = take((size_t)N * 8);
+        f.val = take((size_t)N * 8);
+        f.score = take(learned_io ? (size_t)N * n * 4 : 0);
+        f.end = o;
+        return f;
+    }
+
+    // Point every per-node array at the layout of an N-node batch.
+    void layout(int N) {
+        const Offsets f = offsets(N);
+        auto d = [&](size_t off) { return dpack + off; };
+        auto h = [&](size_t off) { return hpack + off; };
+        lb = reinterpret_cast<double*>(d(f.lb));
+        ub = reinterpret_cast<double*>(d(f.ub));
+        eps = reinterpret_cast<double*>(d(f.eps));
+        cut = reinterpret_cast<double*>(d(f.cut));
+        x = reinterpret_cast<double*>(d(f.x));
+        it = reinterpret_cast<int*>(d(f.it));
+        st = d(f.st);
+        bvv = reinterpret_cast<double*>(d(f.bvv));
+        bv = reinterpret_cast<double*>(d(f.bv));
+        gap = reinterpret_cast<double*>(d(f.gap));
+        val = reinterpret_cast<double*>(d(f.val));
+        hlb = reinterpret_cast<double*>(h(f.lb));
+        hub = reinterpret_cast<double*>(h(f.ub));
+        heps = reinterpret_cast<double*>(h(f.eps));
+        hcut = reinterpret_cast<double*>(h(f.cut));
+        hx = reinterpret_cast<double*>(h(f.x));
+        hit = reinterpret_cast<int*>(h(f.it));
+        hst = h(f.st);
+        hbvv = reinterpret_cast<double*>(h(f.bvv));
+        hbv = reinterpret_cast<double*>(h(f.bv));
+        hgap = reinterpret_cast<double*>(h(f.gap));
+        hval = reinterpret_cast<double*>(h(f.val));
+        if (learned_io) {
+            depth = reinterpret_cast<int*>(d(f.depth));
+            pc = reinterpret_cast<double*>(d(f.pc));
+            score = reinterpret_cast<float*>(d(f.score));
+            hdepth = reinterpret_cast<int*>(h(f.depth));
+            hpc = reinterpret_cast<double*>(h(f.pc));
+            hscore = reinterpret_cast<float*>(h(f.score));
+        }
+        in_bytes = f.bvv + (size_t)N * 8;
+        out_off = f.x;
+        out_bytes = f.end - f.x;
+    }
+    gb_status upload() {
+        GB_HIP(hipMemcpyAsync(dpack, hpack, in_bytes, hipMemcpyHostToDevice, stream));
+        return GB_OK;
+    }
+    gb_status download() {
+        GB_HIP(hipMemcpyAsync(hpack + out_off, dpack + out_off, out_bytes, hipMemcpyDeviceToHost, stream));
+        return GB_OK;
+    }
+
+    // Gradients and feature rows (device), pseudocost-free labelling buffers.
     gb_status alloc_features() {
         const size_t xn = (size_t)cap * n;
         GB_HIP(hipMalloc(&g, xn * 8));
         GB_HIP(hipHostMalloc(&hg, xn * 8, hipHostMallocDefault));
         GB_HIP(hipMalloc(&feat, xn * kMlpFeatures * 4));
-        GB_HIP(hipHostMalloc(&hfeat, xn * kMlpFeatures * 4, hipHostMallocDefault));
-        GB_HIP(hipMalloc(&score, xn * 4));
-        GB_HIP(hipHostMalloc(&hscore, xn * 4, hipHostMallocDefault));
-        GB_HIP(hipMalloc(&depth, (size_t)cap * 4));
-        GB_HIP(hipHostMalloc(&hdepth, (size_t)cap * 4, hipHostMallocDefault));
-        GB_HIP(hipMalloc(&pc, (size_t)n * 16));
-        GB_HIP(hipHostMalloc(&hpc, (size_t)n * 16, hipHostMallocDefault));
         GB_HIP(hipMalloc(&qdiag, (size_t)n * 8));
         return GB_OK;
     }
 
-    gb_status alloc(int capacity, int nv) {
+    gb_status alloc(int capacity, int nv, bool learned) {
         cap = capacity;
         n = nv;
-        const size_t xn = (size_t)cap * n;
-        GB_HIP(hipMalloc(&x, xn * 8));
-        GB_HIP(hipMalloc(&lb, xn * 8));
-        GB_HIP(hipMalloc(&ub, xn * 8));
-        GB_HIP(hipMalloc(&bv, xn * 8));
-        GB_HIP(hipMalloc(&eps, cap * 8));
-        GB_HIP(hipMalloc(&gap, cap * 8));
-        GB_HIP(hipMalloc(&val, cap * 8));
-        GB_HIP(hipMalloc(&bvv, cap * 8));
-        GB_HIP(hipMalloc(&cut, cap * 8));
-        GB_HIP(hipMalloc(&it, cap * 4));
-        GB_HIP(hipMalloc(&st, cap));
+        learned_io = learned;
+        const size_t bytes = offsets(cap).end;
+        GB_HIP(hipMalloc(&dpack, bytes));
+        GB_HIP(hipHostMalloc(&hpack, bytes, hipHostMallocDefault));
         GB_HIP(hipMalloc(&flags, n));
-        GB_HIP(hipHostMalloc(&hx, xn * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hlb, xn * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hub, xn * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hbv, xn * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&heps, cap * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hgap, cap * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hval, cap * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hbvv, cap * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hcut, cap * 8, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hit, cap * 4, hipHostMallocDefault));
-        GB_HIP(hipHostMalloc(&hst, cap, hipHostMallocDefault));
         GB_HIP(hipStreamCreate(&stream));
+        layout(cap);
         return GB_OK;
     }
     ~DeviceBatch() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : {(void*)x, (void*)lb, (void*)ub, (void*)bv, (void*)eps, (void*)gap,
-                        (void*)val, (void*)bvv, (void*)cut, (void*)it, (void*)st, (void*)flags,
-                        (void*)g, (void*)feat, (void*)score, (void*)depth, (void*)pc, (void*)qdiag})
+        for (void* p : {(void*)dpack, (void*)flags, (void*)g, (void*)feat, (void*)qdiag})
             if (p) (void)hipFree(p);
-        for (void* p : {(void*)hx, (void*)hlb, (void*)hub, (void*)hbv, (void*)heps, (void*)hgap,
-                        (void*)hval, (void*)hbvv, (void*)hcut, (void*)hit, (void*)hst, (void*)hg,
-                        (void*)hfeat, (void*)hscore, (void*)hdepth, (void*)hpc})
+        for (void* p : {(void*)hpack, (void*)hg})
             if (p) (void)hipHostFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -388,6 +446,7 @@ gb_status strong_branch_labels(const gb_objective_desc* o, const gb_lmo_desc* l,
     const double* plb = pool.lb(nd.slot);
     const double* pub = pool.ub(nd.slot);
     const double eps = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth + 1);
+    sb.layout(N);
     for (int c = 0; c < C; ++c)
         for (int d = 0; d < 2; ++d) {
             const int r = 2 * c + d;
@@ -402,16 +461,13 @@ gb_status strong_branch_labels(const gb_objective_desc* o, const gb_lmo_desc* l,
             sb.heps[r] = eps;
             sb.hit[r] = 0;
             sb.hst[r] = GB_NODE_INITIAL_POINT_SUPPLIED;
+            sb.hcut[r] = __builtin_inf();
+            sb.hbvv[r] = 1e300;
         }
-    const size_t xb = (size_t)N * n * 8;
-    GB_HIP(hipMemcpyAsync(sb.x, sb.hx, xb, hipMemcpyHostToDevice, sb.stream));
-    GB_HIP(hipMemcpyAsync(sb.lb, sb.hlb, xb, hipMemcpyHostToDevice, sb.stream));
-    GB_HIP(hipMemcpyAsync(sb.ub, sb.hub, xb, hipMemcpyHostToDevice, sb.stream));
-    GB_HIP(hipMemcpyAsync(sb.eps, sb.heps, N * 8, hipMemcpyHostToDevice, sb.stream));
-    GB_HIP(hipMemcpyAsync(sb.it, sb.hit, N * 4, hipMemcpyHostToDevice, sb.stream));
-    GB_HIP(hipMemcpyAsync(sb.st, sb.hst, N, hipMemcpyHostToDevice, sb.stream));
+    gb_status st = sb.upload();
+    if (st != GB_OK) return st;
     const gb_fast_options fo{kFastRefresh, 4, 0, nullptr, nullptr};
-    gb_status st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, sb.x, sb.lb, sb.ub, sb.eps,
+    st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, sb.x, sb.lb, sb.ub, sb.eps,
                                     sb.it, sb.st, sb.gap, sb.val, sb.bv, sb.bvv, sb.flags,
                                     s->integer_tol, nullptr, &fo, sb.stream);
     if (st != GB_OK) return st;
@@ -420,9 +476,8 @@ gb_status strong_branch_labels(const gb_objective_desc* o, const gb_lmo_desc* l,
     const int ge = launch_batched_gradient(o->Qt, o->linear, o->grad_scale, o->linear_scale, n, 1, sb.g,
                                            sb.g + n, sb.stream);
     if (ge != 0) return hip_fail(static_cast<hipError_t>(ge), __FILE__, __LINE__);
-    GB_HIP(hipMemcpyAsync(sb.hgap, sb.gap, N * 8, hipMemcpyDeviceToHost, sb.stream));
-    GB_HIP(hipMemcpyAsync(sb.hval, sb.val, N * 8, hipMemcpyDeviceToHost, sb.stream));
-    GB_HIP(hipMemcpyAsync(sb.hst, sb.st, N, hipMemcpyDeviceToHost, sb.stream));
+    st = sb.download();
+    if (st != GB_OK) return st;
     GB_HIP(hipMemcpyAsync(sb.hg, sb.g + n, (size_t)n * 8, hipMemcpyDeviceToHost, sb.stream));
     GB_HIP(hipStreamSynchronize(sb.stream));
     const double big = 1e3 * (1.0 + std::fabs(fval));
@@ -468,10 +523,10 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     const std::vector<int> iv(pr->int_vars, pr->int_vars + pr->n_int_vars);
     const int B = s->gpu_batch_size;
 
-    DeviceBatch db;
-    st = db.alloc(B, n);
-    if (st != GB_OK) return st;
     const bool learned = s->branch_rule == GB_BRANCH_LEARNED && s->learned_model != nullptr;
+    DeviceBatch db;
+    st = db.alloc(B, n, learned);
+    if (st != GB_OK) return st;
     if (learned || lab) {
         st = db.alloc_features();
         if (st != GB_OK) return st;
@@ -479,7 +534,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     // Child batch of the strong-branching labeller (2 children per candidate).
     DeviceBatch sb;
     if (lab) {
-        st = sb.alloc(2 * n, n);
+        st = sb.alloc(2 * n, n, false);
         if (st != GB_OK) return st;
         st = sb.alloc_features();
         if (st != GB_OK) return st;
@@ -573,6 +628,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         }
         if (batch.empty()) { final_bound = inc; finished = true; break; }
         const int N = (int)batch.size();
+        db.layout(N);
         for (int i = 0; i < N; ++i) {
             const HostNode& nd = batch[i];
             const double* nlb = pool.lb(nd.slot);
@@ -583,31 +639,28 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             db.heps[i] = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth);
             db.hit[i] = nd.riters;
             db.hst[i] = nd.has_rx ? GB_NODE_INITIAL_POINT_SUPPLIED : 0;
+            db.hcut[i] = inc;
+            db.hbvv[i] = 1e300;
             if (learned) db.hdepth[i] = nd.depth;
             active.insert(nd.bound);
         }
+        if (learned) {
+            for (int j = 0; j < n; ++j) {
+                db.hpc[j] = pc.up_cnt[j] ? pc.up_sum[j] / pc.up_cnt[j] : -1.0;
+                db.hpc[n + j] = pc.dn_cnt[j] ? pc.dn_sum[j] / pc.dn_cnt[j] : -1.0;
+            }
+        }
         const auto r0 = Clock::now();
-        const size_t xb = (size_t)N * n * 8;
-        GB_HIP(hipMemcpyAsync(db.x, db.hx, xb, hipMemcpyHostToDevice, db.stream));
-        GB_HIP(hipMemcpyAsync(db.lb, db.hlb, xb, hipMemcpyHostToDevice, db.stream));
-        GB_HIP(hipMemcpyAsync(db.ub, db.hub, xb, hipMemcpyHostToDevice, db.stream));
-        GB_HIP(hipMemcpyAsync(db.eps, db.heps, N * 8, hipMemcpyHostToDevice, db.stream));
-        GB_HIP(hipMemcpyAsync(db.it, db.hit, N * 4, hipMemcpyHostToDevice, db.stream));
-        GB_HIP(hipMemcpyAsync(db.st, db.hst, N, hipMemcpyHostToDevice, db.stream));
+        // One copy in (bounds, tolerances, cutoffs, warm starts, counters,
+        // learned inputs), one copy out after the kernels.
+        st = db.upload();
+        if (st != GB_OK) return st;
         if (s->form == 1) {
-            for (int i = 0; i < N; ++i) db.hcut[i] = inc;
-            GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
             const gb_fast_options fo{kFastRefresh, 4, 0, nullptr, learned ? db.g : nullptr};
             st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
                                   s->integer_tol, db.cut, &fo, db.stream);
         } else if (s->form == 2) {
-            for (int i = 0; i < N; ++i) {
-                db.hcut[i] = inc;
-                db.hbvv[i] = 1e300;
-            }
-            GB_HIP(hipMemcpyAsync(db.cut, db.hcut, N * 8, hipMemcpyHostToDevice, db.stream));
-            GB_HIP(hipMemcpyAsync(db.bvv, db.hbvv, N * 8, hipMemcpyHostToDevice, db.stream));
             if (s->autotune && !tuned && N >= 64) {
                 st = gb_mixed_autotune(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                        db.eps, db.it, db.st, db.flags, s->integer_tol, db.cut, &tune, nullptr,
@@ -626,13 +679,6 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                                   s->integer_tol, s->gpu_block_threads, db.stream);
         }
         if (st != GB_OK) return st;
-        GB_HIP(hipMemcpyAsync(db.hx, db.x, xb, hipMemcpyDeviceToHost, db.stream));
-        GB_HIP(hipMemcpyAsync(db.hbv, db.bv, xb, hipMemcpyDeviceToHost, db.stream));
-        GB_HIP(hipMemcpyAsync(db.hgap, db.gap, N * 8, hipMemcpyDeviceToHost, db.stream));
-        GB_HIP(hipMemcpyAsync(db.hval, db.val, N * 8, hipMemcpyDeviceToHost, db.stream));
-        GB_HIP(hipMemcpyAsync(db.hbvv, db.bvv, N * 8, hipMemcpyDeviceToHost, db.stream));
-        GB_HIP(hipMemcpyAsync(db.hit, db.it, N * 4, hipMemcpyDeviceToHost, db.stream));
-        GB_HIP(hipMemcpyAsync(db.hst, db.st, N, hipMemcpyDeviceToHost, db.stream));
         if (learned) {
             // Scores of every (node, variable) on the device, returned with
             // the batch: gradient (from the relaxation kernels, or one
@@ -642,20 +688,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                                                        n, N, db.x, db.g, db.stream);
                 if (ge != 0) return hip_fail(static_cast<hipError_t>(ge), __FILE__, __LINE__);
             }
-            for (int j = 0; j < n; ++j) {
-                db.hpc[j] = pc.up_cnt[j] ? pc.up_sum[j] / pc.up_cnt[j] : -1.0;
-                db.hpc[n + j] = pc.dn_cnt[j] ? pc.dn_sum[j] / pc.dn_cnt[j] : -1.0;
-            }
-            GB_HIP(hipMemcpyAsync(db.pc, db.hpc, (size_t)n * 16, hipMemcpyHostToDevice, db.stream));
-            GB_HIP(hipMemcpyAsync(db.depth, db.hdepth, (size_t)N * 4, hipMemcpyHostToDevice, db.stream));
             const int fe = launch_candidate_features(n, N, db.x, db.g, db.lb, db.ub, db.depth, db.gap, db.val,
                                                      db.pc, db.pc + n, db.qdiag, o->grad_scale, db.flags,
                                                      s->integer_tol, db.feat, db.stream);
             if (fe != 0) return hip_fail(static_cast<hipError_t>(fe), __FILE__, __LINE__);
             st = gb_brancher_predict(s->learned_model, db.feat, N * n, db.score, db.stream);
             if (st != GB_OK) return st;
-            GB_HIP(hipMemcpyAsync(db.hscore, db.score, (size_t)N * n * 4, hipMemcpyDeviceToHost, db.stream));
         }
+        st = db.download();
+        if (st != GB_OK) return st;
         GB_HIP(hipStreamSynchronize(db.stream));
         relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
 
