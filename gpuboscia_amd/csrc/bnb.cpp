@@ -18,7 +18,6 @@
 #include <limits>
 #include <memory>
 #include <queue>
-#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -223,7 +222,8 @@ int most_fractional(const double* x, const std::vector<int>& iv, double tol) {
 // (lb .. bvv) and one device-to-host copy (x .. score).
 struct DeviceBatch {
     int cap = 0, n = 0;
-    bool learned_io = false;  // depth, pseudocost means and scores travel with the batch
+    bool learned_io = false;  // depth, pseudocost means and chosen candidates travel with the batch
+    int topk_k = 0;           // candidates returned per node
     double *x = nullptr, *lb = nullptr, *ub = nullptr, *bv = nullptr;
     double *eps = nullptr, *gap = nullptr, *val = nullptr, *bvv = nullptr, *cut = nullptr;
     int* it = nullptr;
@@ -237,14 +237,15 @@ struct DeviceBatch {
     // depths, per-variable pseudocost means (up, down; < 0 = no history), Q
     // diagonal
     double *g = nullptr, *hg = nullptr;
-    float *feat = nullptr, *score = nullptr, *hscore = nullptr;
+    float *feat = nullptr, *score = nullptr;
+    int *topk = nullptr, *htopk = nullptr;
     int *depth = nullptr, *hdepth = nullptr;
     double *pc = nullptr, *hpc = nullptr, *qdiag = nullptr;
     unsigned char *dpack = nullptr, *hpack = nullptr;
     size_t in_bytes = 0, out_off = 0, out_bytes = 0;
 
     struct Offsets {
-        size_t lb, ub, eps, cut, depth, pc, x, it, st, bvv, bv, gap, val, score, end;
+        size_t lb, ub, eps, cut, depth, pc, x, it, st, bvv, bv, gap, val, topk, end;
     };
     Offsets offsets(int N) const {
         Offsets f{};
@@ -268,7 +269,7 @@ struct DeviceBatch {
         f.bv = take(xb);
         f.gap = take((size_t)N * 8);
         f.val = take((size_t)N * 8);
-        f.score = take(learned_io ? (size_t)N * n * 4 : 0);
+        f.topk = take(learned_io ? (size_t)N * topk_k * 4 : 0);
         f.end = o;
         return f;
     }
@@ -303,10 +304,10 @@ struct DeviceBatch {
         if (learned_io) {
             depth = reinterpret_cast<int*>(d(f.depth));
             pc = reinterpret_cast<double*>(d(f.pc));
-            score = reinterpret_cast<float*>(d(f.score));
+            topk = reinterpret_cast<int*>(d(f.topk));
             hdepth = reinterpret_cast<int*>(h(f.depth));
             hpc = reinterpret_cast<double*>(h(f.pc));
-            hscore = reinterpret_cast<float*>(h(f.score));
+            htopk = reinterpret_cast<int*>(h(f.topk));
         }
         in_bytes = f.bvv + (size_t)N * 8;
         out_off = f.x;
@@ -327,14 +328,16 @@ struct DeviceBatch {
         GB_HIP(hipMalloc(&g, xn * 8));
         GB_HIP(hipHostMalloc(&hg, xn * 8, hipHostMallocDefault));
         GB_HIP(hipMalloc(&feat, xn * kMlpFeatures * 4));
+        GB_HIP(hipMalloc(&score, xn * 4));
         GB_HIP(hipMalloc(&qdiag, (size_t)n * 8));
         return GB_OK;
     }
 
-    gb_status alloc(int capacity, int nv, bool learned) {
+    gb_status alloc(int capacity, int nv, bool learned, int k = 0) {
         cap = capacity;
         n = nv;
         learned_io = learned;
+        topk_k = k;
         const size_t bytes = offsets(cap).end;
         GB_HIP(hipMalloc(&dpack, bytes));
         GB_HIP(hipHostMalloc(&hpack, bytes, hipHostMallocDefault));
@@ -345,7 +348,7 @@ struct DeviceBatch {
     }
     ~DeviceBatch() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : {(void*)dpack, (void*)flags, (void*)g, (void*)feat, (void*)qdiag})
+        for (void* p : {(void*)dpack, (void*)flags, (void*)g, (void*)feat, (void*)score, (void*)qdiag})
             if (p) (void)hipFree(p);
         for (void* p : {(void*)hpack, (void*)hg})
             if (p) (void)hipHostFree(p);
@@ -525,7 +528,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
 
     const bool learned = s->branch_rule == GB_BRANCH_LEARNED && s->learned_model != nullptr;
     DeviceBatch db;
-    st = db.alloc(B, n, learned);
+    st = db.alloc(B, n, learned, s->branch_k);
     if (st != GB_OK) return st;
     if (learned || lab) {
         st = db.alloc_features();
@@ -563,8 +566,6 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     }
     gb_mixed_options tune{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
     bool tuned = false;
-    std::vector<int> score_off, cand_var, order_buf;
-    std::vector<float> cand_score;
     if (lab) GB_HIP(hipMemcpy(sb.flags, hflags.data(), n, hipMemcpyHostToDevice));
 
     std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
@@ -589,12 +590,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     long long fw_iters = 0;
     double final_bound = -kInf, relax_s = 0.0;
     bool tol_hit = false, time_hit = false, finished = false;
-    std::multiset<double> active;
+    // Bounds of the batch nodes not yet post-processed: nodes leave in batch
+    // order, so the smallest is the suffix minimum at the first open position.
+    std::vector<double> open_min;
+    size_t open_pos = 0;
 
     auto live_bound = [&]() {
         double b = inc;
         if (!q.empty()) b = std::min(b, q.top().bound);
-        if (!active.empty()) b = std::min(b, *active.begin());
+        if (open_pos < open_min.size()) b = std::min(b, open_min[open_pos]);
         return b;
     };
     auto finish_check = [&](bool batch_remaining) {
@@ -642,8 +646,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             db.hcut[i] = inc;
             db.hbvv[i] = 1e300;
             if (learned) db.hdepth[i] = nd.depth;
-            active.insert(nd.bound);
         }
+        open_min.resize(N);
+        for (int i = N - 1; i >= 0; --i)
+            open_min[i] = i + 1 < N ? std::min(batch[i].bound, open_min[i + 1]) : batch[i].bound;
+        open_pos = 0;
         if (learned) {
             for (int j = 0; j < n; ++j) {
                 db.hpc[j] = pc.up_cnt[j] ? pc.up_sum[j] / pc.up_cnt[j] : -1.0;
@@ -694,38 +701,19 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             if (fe != 0) return hip_fail(static_cast<hipError_t>(fe), __FILE__, __LINE__);
             st = gb_brancher_predict(s->learned_model, db.feat, N * n, db.score, db.stream);
             if (st != GB_OK) return st;
+            const int te = launch_topk_candidates(n, N, db.x, db.flags, s->integer_tol, db.score, db.st,
+                                                  s->branch_k, db.topk, db.stream);
+            if (te != 0) return hip_fail(static_cast<hipError_t>(te), __FILE__, __LINE__);
         }
         st = db.download();
         if (st != GB_OK) return st;
         GB_HIP(hipStreamSynchronize(db.stream));
         relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
 
-        // Learned scores of each node's fractional candidates (flat arrays,
-        // reused across batches).
-        if (learned) {
-            score_off.assign(N + 1, 0);
-            cand_var.clear();
-            cand_score.clear();
-            for (int i = 0; i < N; ++i) {
-                score_off[i] = (int)cand_var.size();
-                if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
-                    const double* xi = db.hx + (size_t)i * n;
-                    for (int v : iv) {
-                        const double frac = xi[v] - std::floor(xi[v]);
-                        if (std::min(frac, 1.0 - frac) > s->integer_tol) {
-                            cand_var.push_back(v);
-                            cand_score.push_back(db.hscore[(size_t)i * n + v]);
-                        }
-                    }
-                }
-            }
-            score_off[N] = (int)cand_var.size();
-        }
-
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
         for (int i = 0; i < N && !finished; ++i) {
             HostNode nd = batch[i];
-            active.erase(active.find(nd.bound));
+            open_pos = (size_t)i + 1;
             const int remaining = N - i - 1;
             if (db.hst[i] == GB_NODE_INFEASIBLE) {
                 pool.release(nd.slot);
@@ -806,19 +794,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                             if (score[c] > score[bi]) bi = c;
                         vars.push_back(cand[bi]);
                     }
-                } else if (learned && score_off[i + 1] > score_off[i]) {
+                } else if (learned && db.htopk[(size_t)i * s->branch_k] >= 0) {
                     // the branch_k candidates with the highest predicted
-                    // strong-branching score (ties to the lower index), split
-                    // 2^k ways like branch_pow2
-                    const float* sc = cand_score.data() + score_off[i];
-                    const int* cv = cand_var.data() + score_off[i];
-                    const int cnt = score_off[i + 1] - score_off[i];
-                    order_buf.resize(cnt);
-                    for (int c = 0; c < cnt; ++c) order_buf[c] = c;
-                    const int k = std::min(s->branch_k, cnt);
-                    std::partial_sort(order_buf.begin(), order_buf.begin() + k, order_buf.end(),
-                                      [&](int a, int b) { return sc[a] > sc[b] || (sc[a] == sc[b] && cv[a] < cv[b]); });
-                    for (int c = 0; c < k; ++c) vars.push_back(cv[order_buf[c]]);
+                    // strong-branching score (ties to the lower index, chosen
+                    // on the device), split 2^k ways like branch_pow2
+                    const int* tk = db.htopk + (size_t)i * s->branch_k;
+                    for (int c = 0; c < s->branch_k && tk[c] >= 0; ++c) vars.push_back(tk[c]);
                 } else {
                     // GB_BRANCH_MOST_FRACTIONAL, and the learned rule without a model
                     const int v = most_fractional(xi, iv, s->integer_tol);

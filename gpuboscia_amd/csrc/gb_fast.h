@@ -41,5 +41,11 @@ int launch_candidate_features(int n, int N, const double* X, const double* G, co
                               const uint8_t* flags, double tol, float* F, void* stream);
 int launch_batched_gradient(const double* Qt, const double* lin, double gs, double ls, int n, int N,
                             const double* X, double* G, void* stream);
+// Per node the k best-scored fractional candidates (flags[j] set and
+// min(frac, 1 - frac) > tol), score descending with ties to the lower index,
+// -1 after the last; nodes with status >= GB_NODE_INTEGRAL_UNCONVERGED get
+// none.  out is [N][k].
+int launch_topk_candidates(int n, int N, const double* X, const uint8_t* flags, double tol, const float* score,
+                           const uint8_t* status, int k, int* out, void* stream);
 
 }  // namespace gb

@@ -589,6 +589,51 @@ __global__ void __launch_bounds__(64) candidate_features_kernel(
     }
 }
 
+// One wave per node: k rounds of a wave argmax over the candidates after
+// the previous pick in the order (score descending, index ascending).
+__global__ void __launch_bounds__(64) topk_candidates_kernel(int n, int N, const double* X, const uint8_t* flags,
+                                                             double tol, const float* score,
+                                                             const uint8_t* status, int k, int* out) {
+    const int node = blockIdx.x;
+    if (node >= N) return;
+    const int lane = threadIdx.x;
+    const size_t base = static_cast<size_t>(node) * n;
+    const bool open = status[node] < GB_NODE_INTEGRAL_UNCONVERGED;
+    float last_s = __builtin_inff();
+    int last_j = -1;
+    bool more = true;
+    for (int r = 0; r < k; ++r) {
+        float bs = -__builtin_inff();
+        int bj = 0x7FFFFFFF;
+        if (more && open) {
+            for (int j = lane; j < n; j += kWave) {
+                const double xj = X[base + j];
+                const double frac = xj - floor(xj);
+                if (!flags[j] || !(fmin(frac, 1.0 - frac) > tol)) continue;
+                const float sj = score[base + j];
+                const bool after = sj < last_s || (sj == last_s && j > last_j);
+                if (after && (sj > bs || (sj == bs && j < bj))) {
+                    bs = sj;
+                    bj = j;
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const float os = __shfl_xor(bs, off, kWave);
+            const int oj = __shfl_xor(bj, off, kWave);
+            if (os > bs || (os == bs && oj < bj)) {
+                bs = os;
+                bj = oj;
+            }
+        }
+        if (bj == 0x7FFFFFFF) more = false;
+        if (lane == 0) out[static_cast<size_t>(node) * k + r] = more ? bj : -1;
+        last_s = bs;
+        last_j = bj;
+    }
+}
+
 constexpr size_t kFastLdsBudget = 160 * 1024;
 
 }  // namespace
@@ -608,6 +653,14 @@ int launch_candidate_features(int n, int N, const double* X, const double* G, co
     if (N <= 0) return 0;
     hipLaunchKernelGGL(candidate_features_kernel, dim3(N), dim3(kWave), 0, reinterpret_cast<hipStream_t>(stream),
                        n, N, X, G, LB, UB, depth, gap, val, pcu, pcd, qdiag, gs, flags, tol, F);
+    return static_cast<int>(hipGetLastError());
+}
+
+int launch_topk_candidates(int n, int N, const double* X, const uint8_t* flags, double tol, const float* score,
+                           const uint8_t* status, int k, int* out, void* stream) {
+    if (N <= 0 || k <= 0) return 0;
+    hipLaunchKernelGGL(topk_candidates_kernel, dim3(N), dim3(kWave), 0, reinterpret_cast<hipStream_t>(stream),
+                       n, N, X, flags, tol, score, status, k, out);
     return static_cast<int>(hipGetLastError());
 }
 
