@@ -2,13 +2,14 @@
 //
 // Follows gpuboscia::solve / DeviceResidentHybridSolver
 // (gpuboscia.cuh:56-170, device_resident_solver.cuh:1184-2085) with one
-// deliberate change of schedule: one batch is in flight at a time and every
-// node of a completed batch is post-processed in batch order before the next
-// pop.  The reference overlaps up to num_streams batches on host threads,
-// which makes node counts and tie-breaking schedule dependent; a single
-// in-flight batch makes the search deterministic, so the GPU run and the CPU
-// oracle expand the same tree.  The frontier is a best-first heap ordered by
-// (dual_bound, id).
+// deliberate change of schedule: every node of a completed batch is
+// post-processed in batch order, on one host thread.  The plain form keeps one
+// batch in flight, so the GPU run and the CPU oracle expand the same tree; the
+// batched forms keep exactly two (the next batch is popped and launched before
+// the current one is post-processed).  The reference overlaps up to
+// num_streams batches on host threads, which makes node counts and
+// tie-breaking schedule dependent; a fixed alternation keeps the search
+// deterministic.  The frontier is a best-first heap ordered by (dual_bound, id).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,7 +18,6 @@
 #include <cstring>
 #include <limits>
 #include <memory>
-#include <queue>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,18 +34,33 @@ constexpr double kInf = std::numeric_limits<double>::infinity();
 // Node bounds and warm starts live in a chunked slab (3 n doubles per slot:
 // lb, ub, relaxation_x); heap entries are small records.  This keeps host
 // post-processing at a few hundred ns per node (no per-child allocation).
+// A slot is reference counted: children that warm-start from their parent's
+// iterate hold the parent's slot (whose relaxation_x is that iterate) until
+// they are staged.
 struct NodePool {
     int n = 0;
     static constexpr int kChunk = 4096;
     std::vector<std::unique_ptr<double[]>> chunks;
     std::vector<int> free_slots;
+    std::vector<int> refs;
     int next = 0;
     int alloc() {
-        if (!free_slots.empty()) { const int s = free_slots.back(); free_slots.pop_back(); return s; }
-        if (next % kChunk == 0) chunks.emplace_back(new double[(size_t)kChunk * 3 * n]);
-        return next++;
+        int s;
+        if (!free_slots.empty()) {
+            s = free_slots.back();
+            free_slots.pop_back();
+        } else {
+            if (next % kChunk == 0) chunks.emplace_back(new double[(size_t)kChunk * 3 * n]);
+            s = next++;
+            refs.push_back(0);
+        }
+        refs[s] = 1;
+        return s;
     }
-    void release(int slot) { free_slots.push_back(slot); }
+    void retain(int slot) { ++refs[slot]; }
+    void release(int slot) {
+        if (--refs[slot] == 0) free_slots.push_back(slot);
+    }
     double* lb(int slot) { return chunks[slot / kChunk].get() + (size_t)(slot % kChunk) * 3 * n; }
     double* ub(int slot) { return lb(slot) + n; }
     double* rx(int slot) { return lb(slot) + 2 * n; }
@@ -55,6 +70,7 @@ struct HostNode {
     int id = 0, parent = -1, depth = 0;
     int slot = -1;                   // NodePool slot holding lb, ub, relaxation_x
     bool has_rx = false;             // warm start present (relaxation_x non-empty)
+    int xslot = -1;                  // or: clip the relaxation_x of this slot (the parent's) to the box
     int riters = 0;                  // relaxation_iterations
     double bound = -1e300;           // dual_bound
     // single-variable branching record for pseudocosts
@@ -154,10 +170,45 @@ struct Labeler {
     int labeled = 0;
 };
 
-struct HeapOrder {
-    bool operator()(const HostNode& a, const HostNode& b) const {
-        return a.bound > b.bound || (a.bound == b.bound && a.id > b.id);
+// Best-first frontier ordered by (dual_bound, id): a binary heap of 16-byte
+// keys over node records kept in a recycled vector.
+class Frontier {
+  public:
+    bool empty() const { return heap_.empty(); }
+    size_t size() const { return heap_.size(); }
+    const HostNode& top() const { return recs_[heap_.front().rec]; }
+    void push(const HostNode& nd) {
+        int r;
+        if (!free_.empty()) {
+            r = free_.back();
+            free_.pop_back();
+            recs_[r] = nd;
+        } else {
+            r = (int)recs_.size();
+            recs_.push_back(nd);
+        }
+        heap_.push_back(Key{nd.bound, nd.id, r});
+        std::push_heap(heap_.begin(), heap_.end(), After());
     }
+    void pop() {
+        std::pop_heap(heap_.begin(), heap_.end(), After());
+        free_.push_back(heap_.back().rec);
+        heap_.pop_back();
+    }
+
+  private:
+    struct Key {
+        double bound;
+        int id, rec;
+    };
+    struct After {
+        bool operator()(const Key& a, const Key& b) const {
+            return a.bound > b.bound || (a.bound == b.bound && a.id > b.id);
+        }
+    };
+    std::vector<Key> heap_;
+    std::vector<HostNode> recs_;
+    std::vector<int> free_;
 };
 
 // gap_utils.h:14-62
@@ -527,12 +578,20 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     const int B = s->gpu_batch_size;
 
     const bool learned = s->branch_rule == GB_BRANCH_LEARNED && s->learned_model != nullptr;
-    DeviceBatch db;
-    st = db.alloc(B, n, learned, s->branch_k);
-    if (st != GB_OK) return st;
-    if (learned || lab) {
-        st = db.alloc_features();
+    // The batched forms keep two batches in flight (num_streams >= 2): the
+    // next batch is chosen, uploaded and launched before the current one is
+    // post-processed, so the host's tree work overlaps the GPU's relaxations.
+    // The plain form keeps the reference schedule (one batch at a time).
+    const bool pipelined = s->form != 0 && s->num_streams >= 2;
+    const int slots = pipelined ? 2 : 1;
+    DeviceBatch bufs[2];
+    for (int k = 0; k < slots; ++k) {
+        st = bufs[k].alloc(B, n, learned, s->branch_k);
         if (st != GB_OK) return st;
+        if (learned || lab) {
+            st = bufs[k].alloc_features();
+            if (st != GB_OK) return st;
+        }
     }
     // Child batch of the strong-branching labeller (2 children per candidate).
     DeviceBatch sb;
@@ -550,25 +609,38 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         qdiag.resize(n);
         GB_HIP(hipMemcpy2D(qdiag.data(), sizeof(double), o->Qt, (size_t)(n + 1) * sizeof(double),
                            sizeof(double), n, hipMemcpyDeviceToHost));
-        if (learned) GB_HIP(hipMemcpy(db.qdiag, qdiag.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        if (learned)
+            for (int k = 0; k < slots; ++k)
+                GB_HIP(hipMemcpy(bufs[k].qdiag, qdiag.data(), (size_t)n * 8, hipMemcpyHostToDevice));
     }
     std::vector<uint8_t> hflags(n, 0);
     for (int i : iv) hflags[i] = 1;
-    GB_HIP(hipMemcpy(db.flags, hflags.data(), n, hipMemcpyHostToDevice));
+    for (int k = 0; k < slots; ++k) GB_HIP(hipMemcpy(bufs[k].flags, hflags.data(), n, hipMemcpyHostToDevice));
     // Mixed-precision form: Q split into bf16 planes once per solve (one
     // plane for GB_PREC_BF16, three otherwise).
     std::unique_ptr<gb_mixed_q, gb_status (*)(gb_mixed_q*)> mq(nullptr, gb_mixed_destroy);
     if (s->form == 2) {
         gb_mixed_q* h = nullptr;
-        st = gb_mixed_prepare(o, s->precision == GB_PREC_BF16 ? 1 : 3, db.stream, &h);
+        st = gb_mixed_prepare(o, s->precision == GB_PREC_BF16 ? 1 : 3, bufs[0].stream, &h);
         if (st != GB_OK) return st;
         mq.reset(h);
+        GB_HIP(hipStreamSynchronize(bufs[0].stream));  // both slots' streams use the planes
     }
     gb_mixed_options tune{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
     bool tuned = false;
+    // Every exit drains both slots' streams before the mixed planes and the
+    // batch buffers are released (a second batch may still be in flight).
+    struct DrainOnExit {
+        DeviceBatch* b;
+        int count;
+        ~DrainOnExit() {
+            for (int k = 0; k < count; ++k)
+                if (b[k].stream) (void)hipStreamSynchronize(b[k].stream);
+        }
+    } drain{bufs, slots};
     if (lab) GB_HIP(hipMemcpy(sb.flags, hflags.data(), n, hipMemcpyHostToDevice));
 
-    std::priority_queue<HostNode, std::vector<HostNode>, HeapOrder> q;
+    Frontier q;
     NodePool pool;
     pool.n = n;
     {
@@ -590,15 +662,19 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     long long fw_iters = 0;
     double final_bound = -kInf, relax_s = 0.0;
     bool tol_hit = false, time_hit = false, finished = false;
-    // Bounds of the batch nodes not yet post-processed: nodes leave in batch
-    // order, so the smallest is the suffix minimum at the first open position.
-    std::vector<double> open_min;
-    size_t open_pos = 0;
+    // Bounds of the nodes of each slot's batch not yet post-processed: nodes
+    // leave in batch order, so the smallest is the suffix minimum at the
+    // slot's first open position (pos == size: nothing in flight).
+    std::vector<double> open_min[2];
+    size_t open_pos[2] = {0, 0};
+    std::vector<HostNode> batches[2];
+    auto in_flight = [&](int k) { return open_pos[k] < open_min[k].size(); };
 
     auto live_bound = [&]() {
         double b = inc;
         if (!q.empty()) b = std::min(b, q.top().bound);
-        if (open_pos < open_min.size()) b = std::min(b, open_min[open_pos]);
+        for (int k = 0; k < 2; ++k)
+            if (in_flight(k)) b = std::min(b, open_min[k][open_pos[k]]);
         return b;
     };
     auto finish_check = [&](bool batch_remaining) {
@@ -616,41 +692,59 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     new_inc ? 1 : 0);
     };
 
-    while (!finished) {
-        const double el = std::chrono::duration<double>(Clock::now() - t0).count();
-        if (el >= s->time_limit_seconds || (s->max_nodes > 0 && done_nodes >= s->max_nodes)) {
-            final_bound = live_bound();
-            time_hit = true;
-            break;
-        }
-        std::vector<HostNode> batch;
+    // Select the next batch from the queue into `slot`, upload it and queue
+    // its kernels and the result copy on the slot's stream (an empty queue
+    // leaves the slot idle).
+    auto launch_batch = [&](int slot) -> gb_status {
+        gb_status st = GB_OK;
+        std::vector<HostNode>& batch = batches[slot];
+        DeviceBatch& db = bufs[slot];
+        batch.clear();
+        open_min[slot].clear();
+        open_pos[slot] = 0;
         while (!q.empty() && (int)batch.size() < B) {
             HostNode nd = q.top();
             q.pop();
-            if (nd.bound >= inc) { pool.release(nd.slot); continue; }
+            if (nd.bound >= inc) {
+                pool.release(nd.slot);
+                if (nd.xslot >= 0) pool.release(nd.xslot);
+                continue;
+            }
             batch.push_back(nd);
         }
-        if (batch.empty()) { final_bound = inc; finished = true; break; }
+        if (batch.empty()) return GB_OK;
         const int N = (int)batch.size();
         db.layout(N);
         for (int i = 0; i < N; ++i) {
-            const HostNode& nd = batch[i];
+            HostNode& nd = batch[i];
             const double* nlb = pool.lb(nd.slot);
-            std::copy(nd.has_rx ? pool.rx(nd.slot) : nlb, (nd.has_rx ? pool.rx(nd.slot) : nlb) + n,
-                      db.hx + (size_t)i * n);
+            const double* nub = pool.ub(nd.slot);
+            double* hx = db.hx + (size_t)i * n;
+            const bool warm = nd.has_rx || nd.xslot >= 0;
+            if (nd.has_rx) {
+                std::copy(pool.rx(nd.slot), pool.rx(nd.slot) + n, hx);
+            } else if (nd.xslot >= 0) {
+                // Batched forms: the parent's iterate clipped to this box (any
+                // box point is feasible).
+                const double* px = pool.rx(nd.xslot);
+                for (int j = 0; j < n; ++j) hx[j] = std::min(std::max(px[j], nlb[j]), nub[j]);
+                pool.release(nd.xslot);
+                nd.xslot = -1;
+            } else {
+                std::copy(nlb, nlb + n, hx);
+            }
             std::copy(nlb, nlb + n, db.hlb + (size_t)i * n);
-            std::copy(pool.ub(nd.slot), pool.ub(nd.slot) + n, db.hub + (size_t)i * n);
+            std::copy(nub, nub + n, db.hub + (size_t)i * n);
             db.heps[i] = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth);
             db.hit[i] = nd.riters;
-            db.hst[i] = nd.has_rx ? GB_NODE_INITIAL_POINT_SUPPLIED : 0;
+            db.hst[i] = warm ? GB_NODE_INITIAL_POINT_SUPPLIED : 0;
             db.hcut[i] = inc;
             db.hbvv[i] = 1e300;
             if (learned) db.hdepth[i] = nd.depth;
         }
-        open_min.resize(N);
+        open_min[slot].resize(N);
         for (int i = N - 1; i >= 0; --i)
-            open_min[i] = i + 1 < N ? std::min(batch[i].bound, open_min[i + 1]) : batch[i].bound;
-        open_pos = 0;
+            open_min[slot][i] = i + 1 < N ? std::min(batch[i].bound, open_min[slot][i + 1]) : batch[i].bound;
         if (learned) {
             for (int j = 0; j < n; ++j) {
                 db.hpc[j] = pc.up_cnt[j] ? pc.up_sum[j] / pc.up_cnt[j] : -1.0;
@@ -707,14 +801,43 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         }
         st = db.download();
         if (st != GB_OK) return st;
-        GB_HIP(hipStreamSynchronize(db.stream));
         relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
+        return GB_OK;
+    };
+
+    int head = 0;
+    while (!finished) {
+        const double el = std::chrono::duration<double>(Clock::now() - t0).count();
+        if (el >= s->time_limit_seconds || (s->max_nodes > 0 && done_nodes >= s->max_nodes)) {
+            final_bound = live_bound();
+            time_hit = true;
+            break;
+        }
+        if (!in_flight(head)) {
+            st = launch_batch(head);
+            if (st != GB_OK) return st;
+            if (!in_flight(head)) { final_bound = inc; finished = true; break; }
+        }
+        const int other = 1 - head;
+        if (pipelined && !in_flight(other)) {
+            st = launch_batch(other);
+            if (st != GB_OK) return st;
+        }
+        DeviceBatch& db = bufs[head];
+        std::vector<HostNode>& batch = batches[head];
+        const int N = (int)batch.size();
+        const int queued_elsewhere = pipelined && in_flight(other) ? (int)batches[other].size() : 0;
+        {
+            const auto w0 = Clock::now();
+            GB_HIP(hipStreamSynchronize(db.stream));
+            relax_s += std::chrono::duration<double>(Clock::now() - w0).count();
+        }
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
         for (int i = 0; i < N && !finished; ++i) {
             HostNode nd = batch[i];
-            open_pos = (size_t)i + 1;
-            const int remaining = N - i - 1;
+            open_pos[head] = (size_t)i + 1;
+            const int remaining = N - i - 1 + queued_elsewhere;
             if (db.hst[i] == GB_NODE_INFEASIBLE) {
                 pool.release(nd.slot);
                 ++done_nodes;
@@ -814,6 +937,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 next_id += 8;  // device_resident_solver.cuh:1989
                 int nid = base;
                 const int k = (int)vars.size();
+                bool parent_x_kept = false;
                 for (int mask = 0; k > 0 && mask < (1 << k); ++mask) {
                     HostNode c;
                     c.id = nid++;
@@ -824,8 +948,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     c.slot = pool.alloc();
                     double* clb = pool.lb(c.slot);
                     double* cub = pool.ub(c.slot);
-                    std::copy(pool.lb(nd.slot), pool.lb(nd.slot) + n, clb);
-                    std::copy(pool.ub(nd.slot), pool.ub(nd.slot) + n, cub);
+                    std::copy(pool.lb(nd.slot), pool.lb(nd.slot) + 2 * n, clb);  // lb and ub
                     for (int b = 0; b < k; ++b) {
                         const int vi = vars[b];
                         if (mask & (1 << b)) clb[vi] = std::ceil(xi[vi]);
@@ -833,10 +956,14 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     }
                     if (s->form != 0) {
                         // Batched forms: the child starts from the parent's
-                        // iterate clipped to its box (any box point is feasible).
-                        double* crx = pool.rx(c.slot);
-                        for (int j = 0; j < n; ++j) crx[j] = std::min(std::max(xi[j], clb[j]), cub[j]);
-                        c.has_rx = true;
+                        // iterate, kept once in the parent's slot and clipped to
+                        // the child's box when the child is staged.
+                        if (!parent_x_kept) {
+                            std::copy(xi, xi + n, pool.rx(nd.slot));
+                            parent_x_kept = true;
+                        }
+                        pool.retain(nd.slot);
+                        c.xslot = nd.slot;
                     }
                     if (k == 1) {
                         const int vi = vars[0];
@@ -853,7 +980,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             progress(new_inc, remaining);
             finish_check(remaining > 0);
         }
-        if (!finished && q.empty()) { final_bound = inc; finished = true; }
+        if (!finished && q.empty() && !in_flight(other)) { final_bound = inc; finished = true; }
+        if (pipelined && in_flight(other)) head = other;
     }
     if (final_bound == -kInf) final_bound = q.empty() ? inc : std::min(inc, q.top().bound);
     res->objective = inc;

@@ -435,7 +435,8 @@ typedef enum {
 typedef struct {
     /* DeviceResidentConfig */
     int gpu_batch_size;       /* 1024 */
-    int num_streams;          /* 8 (accepted; one batch in flight per stream) */
+    int num_streams;          /* 8; forms 1 and 2 keep two node batches in flight when >= 2
+                                 (1: one batch at a time); the plain form always keeps one */
     int gpu_block_threads;    /* 0 = automatic; else a multiple of 64 */
     int gpu_max_iters;        /* 20000 */
     double gpu_eps;           /* 1e-3 */

@@ -37,7 +37,6 @@
 #include <limits>
 #include <map>
 #include <queue>
-#include <set>
 #include <thread>
 #include <vector>
 
@@ -1247,11 +1246,24 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
     double final_bound = -kInf;
     bool tol_hit = false, time_hit = false, finished = false;
     const int B = s->batch_size;
-    std::multiset<double> active;
+    // Two slots when the batched schedule keeps two batches in flight: the
+    // next batch is popped (against the incumbent of that moment) before the
+    // current one is post-processed, as gpuboscia_amd/csrc/bnb.cpp does.
+    const bool pipelined = s->form != 0 && s->num_streams >= 2;
+    struct Slot {
+        std::vector<BNode> batch;
+        std::vector<double> X, L, U, E, G, V, BV, BVV, open_min;
+        std::vector<int> IT;
+        std::vector<uint8_t> ST;
+        size_t pos = 0;
+        bool busy() const { return pos < open_min.size(); }
+    };
+    Slot slot[2];
     auto live_bound = [&]() {
         double b = inc;
         if (!q.empty()) b = std::min(b, q.top().bound);
-        if (!active.empty()) b = std::min(b, *active.begin());
+        for (const Slot& sl : slot)
+            if (sl.busy()) b = std::min(b, sl.open_min[sl.pos]);
         return b;
     };
     auto finish_check = [&](bool batch_remaining) {
@@ -1262,51 +1274,83 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
         }
         return false;
     };
+    // Pop up to B live nodes into a slot and relax them.
+    auto launch = [&](Slot& sl) {
+        sl.batch.clear();
+        sl.open_min.clear();
+        sl.pos = 0;
+        while (!q.empty() && (int)sl.batch.size() < B) {
+            BNode nd = q.top();
+            q.pop();
+            if (nd.bound >= inc) continue;
+            sl.batch.push_back(std::move(nd));
+        }
+        if (sl.batch.empty()) return;
+        const int N = (int)sl.batch.size();
+        sl.X.assign((size_t)N * n, 0.0);
+        sl.L.assign((size_t)N * n, 0.0);
+        sl.U.assign((size_t)N * n, 0.0);
+        sl.E.assign(N, 0.0);
+        sl.G.assign(N, 0.0);
+        sl.V.assign(N, 0.0);
+        sl.BV.assign((size_t)N * n, 0.0);
+        sl.BVV.assign(N, kNone);
+        sl.IT.assign(N, 0);
+        sl.ST.assign(N, 0);
+        for (int i = 0; i < N; ++i) {
+            const BNode& nd = sl.batch[i];
+            for (int j = 0; j < n; ++j) {
+                sl.X[(size_t)i * n + j] = nd.rx.empty() ? nd.lb[j] : nd.rx[j];
+                sl.L[(size_t)i * n + j] = nd.lb[j];
+                sl.U[(size_t)i * n + j] = nd.ub[j];
+            }
+            sl.E[i] = orc_depth_dependent_tolerance(s->eps, s->eps_decay, s->eps_min, nd.depth);
+            sl.IT[i] = nd.riters;
+            sl.ST[i] = nd.rx.empty() ? 0 : 0x80;
+        }
+        sl.open_min.resize(N);
+        for (int i = N - 1; i >= 0; --i)
+            sl.open_min[i] = i + 1 < N ? std::min(sl.batch[i].bound, sl.open_min[i + 1]) : sl.batch[i].bound;
+        if (s->form == 1) {
+            std::vector<double> CUT(N, inc);
+            orc_relax_fast_batch(o, a->dicg, N, s->max_iters, sl.X.data(), sl.L.data(), sl.U.data(), sl.E.data(),
+                                 sl.IT.data(), sl.ST.data(), sl.G.data(), sl.V.data(), sl.BV.data(),
+                                 sl.BVV.data(), flags.data(), s->integer_tol, CUT.data(), s->refresh, 0,
+                                 s->threads);
+        } else {
+            orc_relax_batch(o, a, l, N, s->max_iters, sl.X.data(), sl.L.data(), sl.U.data(), sl.E.data(),
+                            sl.IT.data(), sl.ST.data(), sl.G.data(), sl.V.data(), sl.BV.data(), sl.BVV.data(),
+                            flags.data(), s->integer_tol, s->lanes, s->threads);
+        }
+    };
+    int head = 0;
     while (!finished) {
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (el >= s->time_limit_seconds || (s->max_nodes > 0 && done_nodes >= s->max_nodes)) {
             final_bound = live_bound(); time_hit = true; break;
         }
-        std::vector<BNode> batch;
-        while (!q.empty() && (int)batch.size() < B) {
-            BNode nd = q.top();
-            q.pop();
-            if (nd.bound >= inc) continue;
-            batch.push_back(std::move(nd));
+        if (!slot[head].busy()) {
+            launch(slot[head]);
+            if (!slot[head].busy()) { final_bound = inc; finished = true; break; }
         }
-        if (batch.empty()) { final_bound = inc; finished = true; break; }
-        const int N = (int)batch.size();
-        std::vector<double> X((size_t)N * n), L((size_t)N * n), U((size_t)N * n), E(N), G(N), V(N),
-            BV((size_t)N * n, 0.0), BVV(N, kNone);
-        std::vector<int> IT(N);
-        std::vector<uint8_t> ST(N);
-        for (int i = 0; i < N; ++i) {
-            const BNode& nd = batch[i];
-            for (int j = 0; j < n; ++j) {
-                X[(size_t)i * n + j] = nd.rx.empty() ? nd.lb[j] : nd.rx[j];
-                L[(size_t)i * n + j] = nd.lb[j];
-                U[(size_t)i * n + j] = nd.ub[j];
-            }
-            E[i] = orc_depth_dependent_tolerance(s->eps, s->eps_decay, s->eps_min, nd.depth);
-            IT[i] = nd.riters;
-            ST[i] = nd.rx.empty() ? 0 : 0x80;
-            active.insert(nd.bound);
-        }
-        if (s->form == 1) {
-            std::vector<double> CUT(N, inc);
-            orc_relax_fast_batch(o, a->dicg, N, s->max_iters, X.data(), L.data(), U.data(), E.data(),
-                                 IT.data(), ST.data(), G.data(), V.data(), BV.data(), BVV.data(),
-                                 flags.data(), s->integer_tol, CUT.data(), s->refresh, 0, s->threads);
-        } else {
-            orc_relax_batch(o, a, l, N, s->max_iters, X.data(), L.data(), U.data(), E.data(), IT.data(),
-                            ST.data(), G.data(), V.data(), BV.data(), BVV.data(), flags.data(),
-                            s->integer_tol, s->lanes, s->threads);
-        }
-        for (int i = 0; i < N; ++i) fw_iters += IT[i] - batch[i].riters;
+        const int other = 1 - head;
+        if (pipelined && !slot[other].busy()) launch(slot[other]);
+        Slot& sl = slot[head];
+        const int N = (int)sl.batch.size();
+        const int elsewhere = slot[other].busy() ? (int)slot[other].batch.size() : 0;
+        const std::vector<double>& X = sl.X;
+        const std::vector<double>& G = sl.G;
+        const std::vector<double>& V = sl.V;
+        const std::vector<double>& BV = sl.BV;
+        const std::vector<double>& BVV = sl.BVV;
+        const std::vector<double>& E = sl.E;
+        const std::vector<int>& IT = sl.IT;
+        const std::vector<uint8_t>& ST = sl.ST;
+        for (int i = 0; i < N; ++i) fw_iters += IT[i] - sl.batch[i].riters;
         for (int i = 0; i < N && !finished; ++i) {
-            BNode nd = std::move(batch[i]);
-            active.erase(active.find(nd.bound));
-            const bool remaining = i + 1 < N;
+            BNode nd = std::move(sl.batch[i]);
+            sl.pos = (size_t)i + 1;
+            const bool remaining = i + 1 < N || elsewhere > 0;
             if (ST[i] == 2) { ++done_nodes; finish_check(remaining); continue; }
             const double node_eps = E[i];
             nd.bound = orc_certified_fw_lower_bound(nd.bound, V[i], G[i]);
@@ -1346,7 +1390,8 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
             ++done_nodes;
             finish_check(remaining);
         }
-        if (!finished && q.empty()) { final_bound = inc; finished = true; }
+        if (!finished && q.empty() && !slot[other].busy()) { final_bound = inc; finished = true; }
+        if (pipelined && slot[other].busy()) head = other;
     }
     r->objective = inc;
     r->dual_bound = final_bound;

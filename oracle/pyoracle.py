@@ -47,7 +47,7 @@ class BnbSettings(ctypes.Structure):
                 ("branch_k", ctypes.c_int), ("lanes", ctypes.c_int), ("threads", ctypes.c_int),
                 ("natural_lower_bound", ctypes.c_double),
                 ("time_limit_seconds", ctypes.c_double), ("max_nodes", ctypes.c_longlong),
-                ("form", ctypes.c_int), ("refresh", ctypes.c_int)]
+                ("form", ctypes.c_int), ("refresh", ctypes.c_int), ("num_streams", ctypes.c_int)]
 
 
 class BnbResult(ctypes.Structure):
@@ -160,14 +160,14 @@ def relax_batch(obj, a, lmo, x, lb, ub, eps, iterations, status, flags=None, max
 def bnb_solve(obj, a, lmo, lb, ub, int_vars, batch_size=1024, max_iters=20000, eps=1e-3,
               eps_decay=0.8, eps_min=1e-6, integer_tol=1e-6, abs_gap_tol=1e-6, rel_gap_tol=1e-2,
               branch_k=3, lanes=64, threads=1, natural_lower_bound=float("-inf"),
-              time_limit_seconds=float("inf"), max_nodes=0, form=0, refresh=64):
+              time_limit_seconds=float("inf"), max_nodes=0, form=0, refresh=64, num_streams=8):
     lib = load()
     n = len(lb)
     lb_a, ub_a = f64(lb), f64(ub)
     iv = np.ascontiguousarray(int_vars, dtype=np.int32)
     s = BnbSettings(batch_size, max_iters, eps, eps_decay, eps_min, integer_tol, abs_gap_tol,
                     rel_gap_tol, branch_k, lanes, threads, natural_lower_bound,
-                    time_limit_seconds, max_nodes, form, refresh)
+                    time_limit_seconds, max_nodes, form, refresh, num_streams)
     r = BnbResult()
     xo = np.zeros(n)
     lib.orc_bnb_solve(ctypes.byref(obj), ctypes.byref(a), ctypes.byref(lmo), n, p(lb_a), p(ub_a),
