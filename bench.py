@@ -19,9 +19,10 @@ statuses (fill kernels) and relaxes the batch again.
              2 n^2 per FW iteration / its launch time, against 2.5 PFLOP/s
   cpu_baseline  the CPU oracle (FP64 restatement of the plain form) on a
              bounded sample of the same nodes, 16 host threads
-  time_to_optimum  gb_solve to the proven optimum (abs = rel gap 1e-6) on the
-             tractable binary-QP sizes: plain vs batched vs mixed vs
-             learned + mixed
+  time_to_optimum  gb_solve to the proven optimum (abs = rel gap 1e-6) on
+             binary QPs of n = 20, 40, 60, 80, 100 (three seeds each): plain
+             vs batched vs mixed vs learned + mixed, and the geometric-mean
+             speedups over plain per n
 
 Prints ONE JSON line.
 
@@ -190,7 +191,7 @@ def cpu_baseline(args, Q, c, LB, UB, EPS):
                       f"in {dt:.2f} s (plain-form algorithm, FP64)"}
 
 
-def solve_row(gb, Q, c, n, form, learned=None):
+def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0):
     import torch
     dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
     dc = torch.from_numpy(c).cuda()
@@ -199,6 +200,7 @@ def solve_row(gb, Q, c, n, form, learned=None):
     s.abs_gap_tol = 1e-6
     s.rel_gap_tol = 1e-6
     s.time_limit_seconds = 120.0
+    s.max_nodes = max_nodes
     s.form = form
     if form == 2:
         # three bf16 planes (FP32-class products): fewer FP64 polish
@@ -224,7 +226,13 @@ def time_to_optimum(args):
     if scorer is not None:
         forms.append(("learned_mixed", 2, scorer))
     out = []
-    for n, rows, seeds in ((20, 30, (1, 2, 3)), (40, 60, (1, 2, 3)), (60, 90, (1, 2, 3))):
+    for n, rows, seeds in ((20, 30, (1, 2, 3)), (40, 60, (1, 2, 3)), (60, 90, (1, 2, 3)),
+                           (80, 120, (1, 2, 3)), (100, 150, (1, 2, 3))):
+        # an untimed, node-capped solve per form and size first: the first
+        # launches of the kernels of this size
+        Qw, cw, _ = instances.binary_qp(n, seed=9, rows=rows)
+        for key, form, learned in forms:
+            solve_row(gb, Qw, cw, n, form, learned, max_nodes=64)
         for seed in seeds:
             Q, c, sha = instances.binary_qp(n, seed=seed, rows=rows)
             row = {"n": n, "seed": seed, "sha256": sha[:16]}
@@ -335,10 +343,18 @@ def main():
     if not args.no_tto:
         tto = time_to_optimum(args)
         result["time_to_optimum"] = tto
+        gm = {}
         for key in ("batched", "mixed", "learned_mixed"):
+            per_n = {}
+            for nn in sorted({r["n"] for r in tto}):
+                sp = [r[f"speedup_{key}"] for r in tto if f"speedup_{key}" in r and r["n"] == nn]
+                if sp:
+                    per_n[str(nn)] = round(float(np.exp(np.mean(np.log(sp)))), 2)
             sp = [r[f"speedup_{key}"] for r in tto if f"speedup_{key}" in r and r["n"] >= 40]
             if sp:
-                result[f"time_to_optimum_speedup_geomean_n40_60_{key}"] = float(np.exp(np.mean(np.log(sp))))
+                per_n["n40_100"] = round(float(np.exp(np.mean(np.log(sp)))), 2)
+            gm[key] = per_n
+        result["time_to_optimum_speedup_geomean"] = gm
         result["time_to_optimum_all_same_optimum"] = all(r["same_optimum"] for r in tto)
     print(json.dumps(result), flush=True)
 
