@@ -18,6 +18,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -373,14 +374,36 @@ struct DeviceBatch {
         return GB_OK;
     }
 
+    // Buffers grow and never shrink: a batch set is reused by later solves
+    // (see BufferPool), so only a larger capacity or n allocates.
+    size_t pack_bytes = 0, flags_bytes = 0, feat_elems = 0, qdiag_elems = 0;
+
     // Gradients and feature rows (device), pseudocost-free labelling buffers.
     gb_status alloc_features() {
         const size_t xn = (size_t)cap * n;
-        GB_HIP(hipMalloc(&g, xn * 8));
-        GB_HIP(hipHostMalloc(&hg, xn * 8, hipHostMallocDefault));
-        GB_HIP(hipMalloc(&feat, xn * kMlpFeatures * 4));
-        GB_HIP(hipMalloc(&score, xn * 4));
-        GB_HIP(hipMalloc(&qdiag, (size_t)n * 8));
+        if (xn > feat_elems) {
+            if (stream) GB_HIP(hipStreamSynchronize(stream));
+            for (void* p : {(void*)g, (void*)feat, (void*)score})
+                if (p) (void)hipFree(p);
+            if (hg) (void)hipHostFree(hg);
+            g = nullptr;
+            hg = nullptr;
+            feat = nullptr;
+            score = nullptr;
+            feat_elems = 0;
+            GB_HIP(hipMalloc(&g, xn * 8));
+            GB_HIP(hipHostMalloc(&hg, xn * 8, hipHostMallocDefault));
+            GB_HIP(hipMalloc(&feat, xn * kMlpFeatures * 4));
+            GB_HIP(hipMalloc(&score, xn * 4));
+            feat_elems = xn;
+        }
+        if ((size_t)n > qdiag_elems) {
+            if (qdiag) (void)hipFree(qdiag);
+            qdiag = nullptr;
+            qdiag_elems = 0;
+            GB_HIP(hipMalloc(&qdiag, (size_t)n * 8));
+            qdiag_elems = (size_t)n;
+        }
         return GB_OK;
     }
 
@@ -389,11 +412,26 @@ struct DeviceBatch {
         n = nv;
         learned_io = learned;
         topk_k = k;
+        if (!stream) GB_HIP(hipStreamCreate(&stream));
         const size_t bytes = offsets(cap).end;
-        GB_HIP(hipMalloc(&dpack, bytes));
-        GB_HIP(hipHostMalloc(&hpack, bytes, hipHostMallocDefault));
-        GB_HIP(hipMalloc(&flags, n));
-        GB_HIP(hipStreamCreate(&stream));
+        if (bytes > pack_bytes) {
+            GB_HIP(hipStreamSynchronize(stream));
+            if (dpack) (void)hipFree(dpack);
+            if (hpack) (void)hipHostFree(hpack);
+            dpack = nullptr;
+            hpack = nullptr;
+            pack_bytes = 0;
+            GB_HIP(hipMalloc(&dpack, bytes));
+            GB_HIP(hipHostMalloc(&hpack, bytes, hipHostMallocDefault));
+            pack_bytes = bytes;
+        }
+        if ((size_t)n > flags_bytes) {
+            if (flags) (void)hipFree(flags);
+            flags = nullptr;
+            flags_bytes = 0;
+            GB_HIP(hipMalloc(&flags, n));
+            flags_bytes = (size_t)n;
+        }
         layout(cap);
         return GB_OK;
     }
@@ -405,6 +443,50 @@ struct DeviceBatch {
             if (p) (void)hipHostFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
+};
+
+// Batch buffer sets (two pipeline slots and the labeller's child batch, with
+// their streams and pinned staging) are kept per device and reused by later
+// solves: creating streams and pinning host memory per solve cost
+// milliseconds now and then.  A solve takes an idle set of its device (or a
+// new one) and returns it when it ends; concurrent solves use different sets.
+// Sets live until the process exits.
+struct SolveBuffers {
+    int device = -1;
+    DeviceBatch bufs[2];
+    DeviceBatch sb;
+};
+
+class BufferPool {
+  public:
+    explicit BufferPool(int device) {
+        std::lock_guard<std::mutex> lock(mu());
+        auto& idle = sets();
+        for (size_t i = 0; i < idle.size(); ++i)
+            if (idle[i]->device == device) {
+                set_ = idle[i];
+                idle.erase(idle.begin() + (long)i);
+                return;
+            }
+        set_ = new SolveBuffers();
+        set_->device = device;
+    }
+    ~BufferPool() {
+        std::lock_guard<std::mutex> lock(mu());
+        sets().push_back(set_);
+    }
+    SolveBuffers& get() { return *set_; }
+
+  private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<SolveBuffers*>& sets() {
+        static auto* v = new std::vector<SolveBuffers*>();
+        return *v;
+    }
+    SolveBuffers* set_ = nullptr;
 };
 
 gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, const gb_lmo_desc* l,
@@ -584,7 +666,10 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     // The plain form keeps the reference schedule (one batch at a time).
     const bool pipelined = s->form != 0 && s->num_streams >= 2;
     const int slots = pipelined ? 2 : 1;
-    DeviceBatch bufs[2];
+    int device = 0;
+    GB_HIP(hipGetDevice(&device));
+    BufferPool pooled(device);
+    DeviceBatch* bufs = pooled.get().bufs;
     for (int k = 0; k < slots; ++k) {
         st = bufs[k].alloc(B, n, learned, s->branch_k);
         if (st != GB_OK) return st;
@@ -594,7 +679,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         }
     }
     // Child batch of the strong-branching labeller (2 children per candidate).
-    DeviceBatch sb;
+    DeviceBatch& sb = pooled.get().sb;
     if (lab) {
         st = sb.alloc(2 * n, n, false);
         if (st != GB_OK) return st;
