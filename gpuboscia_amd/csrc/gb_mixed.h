@@ -56,12 +56,17 @@ int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double 
 //  xch         per wave, Q v and Q a of its tiles as [array][j][column][row]
 //              floats (2 rw KB per wave);
 //  red         per-column partial sums of each wave, 8 floats;
-//  fetch, ok   fetched node ids, per-wave scope flags.
-// At most 8 + 8 + 32 + 8 KB for n <= 256.
+//  fetch, ok   fetched node ids, per-wave scope flags;
+//  qf          (optional) the Q fragments, planes x rt x rt/2 KB.
+// At most 8 + 8 + 32 + 8 KB for n <= 256, plus the fragments when copied.
 struct MixedLds {
-    size_t vops, aops, xch, red, fetch, ok, total;
+    size_t vops, aops, xch, red, fetch, ok, qf, total;
 };
-__host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw) {
+constexpr size_t kMixedLdsBudget = 160 * 1024;  // LDS of one CU
+__host__ __device__ inline size_t mixed_q_bytes(int rt, int planes) {
+    return static_cast<size_t>(planes) * rt * (rt / 2) * 1024;
+}
+__host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw, size_t q_bytes) {
     MixedLds L{};
     size_t o = 0;
     L.vops = o;
@@ -76,6 +81,9 @@ __host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw) 
     o += sizeof(int) * 16;
     L.ok = o;
     o += sizeof(int) * static_cast<size_t>(waves) * 16;
+    o = (o + 15) & ~static_cast<size_t>(15);
+    L.qf = o;
+    o += q_bytes;
     L.total = o;
     return L;
 }

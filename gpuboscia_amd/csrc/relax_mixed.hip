@@ -169,7 +169,14 @@ __global__ void mixed_prepare_kernel(const double* Qt, const double* lin, double
 
 constexpr int kPhaseDone = 0, kPhaseInit = 1, kPhaseActive = 2, kPhaseFetch = 3;
 
-template <int RW, int PL, bool DICG, int RPT>
+// The wave's Q fragments are the same in every pass: where they fit beside
+// the rest of the state (32..96 VGPRs for n <= 256) they are loaded once and
+// kept in registers for the kernel's lifetime.
+constexpr bool mixed_caches_q(int rw, int pl) { return (pl == 1 && rw <= 2) || (pl == 3 && rw == 1); }
+
+// QL: otherwise the tile copies all fragments into LDS once (when they fit)
+// and reads them from there each pass instead of from L2.
+template <int RW, int PL, bool DICG, int RPT, bool QL>
 __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) relax_mixed_kernel(MixedParams P) {
     constexpr int G = 16 / RPT;  // lanes of a column in the element layout
     constexpr int E = RW * RPT;  // elements per lane: rows 16 t_j + RPT m + r, e = RPT j + r
@@ -178,7 +185,7 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
     const int RT = P.rt;
     const int S = RT / 2;
     const int W = blockDim.x / kWave;
-    const MixedLds L = mixed_lds_layout(RT, W, RW);
+    const MixedLds L = mixed_lds_layout(RT, W, RW, QL ? mixed_q_bytes(RT, PL) : 0);
     uint4* vops = reinterpret_cast<uint4*>(mixed_smem + L.vops);
     uint4* aops = reinterpret_cast<uint4*>(mixed_smem + L.aops);
     float* red = reinterpret_cast<float*>(mixed_smem + L.red);
@@ -191,6 +198,11 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
     for (int i = threadIdx.x; i < S * 64; i += blockDim.x) {
         vops[i] = make_uint4(0u, 0u, 0u, 0u);
         aops[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    const uint4* qf = reinterpret_cast<const uint4*>(mixed_smem + L.qf);
+    if constexpr (QL) {
+        uint4* dst = reinterpret_cast<uint4*>(mixed_smem + L.qf);
+        for (int i = threadIdx.x; i < PL * RT * S * kWave; i += blockDim.x) dst[i] = P.frag[i];
     }
     const int c = lane & 15;   // MFMA layout: column c, rows 4 q + r
     const int q = lane >> 4;
@@ -211,10 +223,8 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
         x[e] = 0.0f;
         qx[e] = 0.0f;
     }
-    // The wave's Q fragments are the same in every pass: where they fit
-    // beside the rest of the state (32..128 VGPRs for n <= 256) they are
-    // loaded once and kept in registers for the kernel's lifetime.
-    constexpr bool kCache = (PL == 1 && RW != 3) || (PL == 3 && RW == 1);
+    constexpr bool kCache = mixed_caches_q(RW, PL);
+    static_assert(!(kCache && QL), "register-cached fragments need no LDS copy");
     constexpr int kSMax = 8;
     bf16x8 fc[kCache ? PL : 1][kCache ? RW : 1][kCache ? kSMax : 1];
     if (kCache) {
@@ -372,7 +382,9 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
 #pragma unroll
                     for (int j = 0; j < RW; ++j) {
                         const int t = w + W * j;
-                        cur[p][j] = t < RT ? load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride) : bf16x8{};
+                        cur[p][j] = t >= RT ? bf16x8{}
+                                    : QL ? __builtin_bit_cast(bf16x8, qf[(static_cast<size_t>(p) * RT + t) * tstride + lane])
+                                         : load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride);
                     }
 #pragma unroll
                 for (int s = 0; s < SS; ++s) {
@@ -383,9 +395,10 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
 #pragma unroll
                             for (int j = 0; j < RW; ++j) {
                                 const int t = w + W * j;
-                                nxt[p][j] = t < RT ? load_frag(fbase + (static_cast<size_t>(p) * RT + t) * tstride +
-                                                               static_cast<size_t>(s + 1) * kWave)
-                                                   : bf16x8{};
+                                const size_t at = (static_cast<size_t>(p) * RT + t) * tstride +
+                                                  static_cast<size_t>(s + 1) * kWave;
+                                nxt[p][j] = t >= RT ? bf16x8{}
+                                            : QL ? __builtin_bit_cast(bf16x8, qf[at + lane]) : load_frag(fbase + at);
                             }
                     }
                     const bf16x8 bv = __builtin_bit_cast(bf16x8, vops[(s * 16 + c) * 4 + q]);
@@ -608,6 +621,28 @@ int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double 
     return static_cast<int>(hipGetLastError());
 }
 
+template <typename K>
+int start_mixed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, const MixedParams& p) {
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, p);
+    return 0;
+}
+
+template <int RW, int PL, bool DI, int RPT>
+int launch_mixed_shape(const MixedParams& p, dim3 grid, dim3 block, int waves, hipStream_t st) {
+    const size_t base = mixed_lds_layout(p.rt, waves, RW, 0).total;
+    if constexpr (!mixed_caches_q(RW, PL)) {
+        const size_t with_q = mixed_lds_layout(p.rt, waves, RW, mixed_q_bytes(p.rt, PL)).total;
+        if (with_q <= kMixedLdsBudget)
+            return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, true>, grid, block, with_q, st, p);
+    }
+    return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false>, grid, block, base, st, p);
+}
+
 int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void* stream) {
     if (p.cols < 1 || p.cols > 16 || rw < 1 || rw > kMixedMaxRowTilesPerWave || p.rt < 2 || p.rt > 16)
         return static_cast<int>(hipErrorInvalidValue);
@@ -616,22 +651,12 @@ int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void
     const int waves = (p.rt + rw - 1) / rw;
     const int rpt = mixed_rows_per_lane(p.cols);
     const dim3 grid(blocks), block(waves * kWave);
-    const size_t lds = mixed_lds_layout(p.rt, waves, rw).total;
-#define GB_MIXED_ONE(RW, PL, DI, RPT)                                                           \
-    do {                                                                                        \
-        auto k = relax_mixed_kernel<RW, PL, DI, RPT>;                                           \
-        if (lds > 64 * 1024) {                                                                  \
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),                \
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
-            if (e != hipSuccess) return (int)e;                                                 \
-        }                                                                                       \
-        hipLaunchKernelGGL(k, grid, block, lds, st, p);                                         \
-    } while (0)
-#define GB_MIXED_RPT(RW, PL, DI)                          \
-    do {                                                  \
-        if (rpt == 1) GB_MIXED_ONE(RW, PL, DI, 1);        \
-        else if (rpt == 2) GB_MIXED_ONE(RW, PL, DI, 2);   \
-        else GB_MIXED_ONE(RW, PL, DI, 4);                 \
+    int e = 0;
+#define GB_MIXED_RPT(RW, PL, DI)                                                        \
+    do {                                                                                \
+        if (rpt == 1) e = launch_mixed_shape<RW, PL, DI, 1>(p, grid, block, waves, st);  \
+        else if (rpt == 2) e = launch_mixed_shape<RW, PL, DI, 2>(p, grid, block, waves, st); \
+        else e = launch_mixed_shape<RW, PL, DI, 4>(p, grid, block, waves, st);          \
     } while (0)
 #define GB_MIXED(RW)                                                     \
     case RW:                                                             \
@@ -653,7 +678,7 @@ int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void
     }
 #undef GB_MIXED
 #undef GB_MIXED_RPT
-#undef GB_MIXED_ONE
+    if (e != 0) return e;
     return static_cast<int>(hipGetLastError());
 }
 
