@@ -115,7 +115,7 @@ class Batch:
         elif form == "batched":
             gb.relax_batch_fast(self.obj, self.alg, self.lmo, self.N, budget, self.x, self.lb, self.ub,
                                 self.eps, self.it, self.st, self.gap, self.val, self.bv, self.bvv, self.flags,
-                                1e-6, None, 64, 4, 0, s, self.stats)
+                                1e-6, None, 64, 0, 0, s, self.stats)
         else:
             planes = 3 if form == "mixed_fp32" else 1
             if self.mq.get(planes) is None:

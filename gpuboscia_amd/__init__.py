@@ -256,7 +256,7 @@ def relax_batch(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_ite
 
 def relax_batch_fast(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb,
                      ub, eps, iterations, status, gap, value, best_vertex, best_vertex_val,
-                     flags=None, candidate_tol=1e-6, cutoff=None, refresh=64, waves_per_block=4,
+                     flags=None, candidate_tol=1e-6, cutoff=None, refresh=64, waves_per_block=0,
                      blocks=0, stream=None, stats=None):
     """gb_relax_batch_fast on torch device tensors (stats: optional int64 [N,2])."""
     lib = load()
@@ -357,7 +357,7 @@ class RelaxGraph:
 
     def __init__(self, obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb, ub,
                  eps, iterations, status, gap, value, best_vertex, best_vertex_val, flags=None,
-                 candidate_tol=1e-6, cutoff=None, refresh=64, waves_per_block=4, blocks=0):
+                 candidate_tol=1e-6, cutoff=None, refresh=64, waves_per_block=0, blocks=0):
         lib = load()
         self._keep = (obj, alg, lmo, x, lb, ub, eps, iterations, status, gap, value, best_vertex,
                       best_vertex_val, flags, cutoff)

@@ -602,7 +602,7 @@ gb_status strong_branch_labels(const gb_objective_desc* o, const gb_lmo_desc* l,
         }
     gb_status st = sb.upload();
     if (st != GB_OK) return st;
-    const gb_fast_options fo{kFastRefresh, 4, 0, nullptr, nullptr};
+    const gb_fast_options fo{kFastRefresh, 0, 0, nullptr, nullptr};
     st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, sb.x, sb.lb, sb.ub, sb.eps,
                                     sb.it, sb.st, sb.gap, sb.val, sb.bv, sb.bvv, sb.flags,
                                     s->integer_tol, nullptr, &fo, sb.stream);
@@ -842,7 +842,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         st = db.upload();
         if (st != GB_OK) return st;
         if (s->form == 1) {
-            const gb_fast_options fo{kFastRefresh, 4, 0, nullptr, learned ? db.g : nullptr};
+            const gb_fast_options fo{kFastRefresh, 0, 0, nullptr, learned ? db.g : nullptr};
             st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
                                   s->integer_tol, db.cut, &fo, db.stream);

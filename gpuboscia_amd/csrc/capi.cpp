@@ -234,9 +234,9 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
             || !best_vertex_val)
         return fail(GB_ERR_INVALID_ARGUMENT, "null node buffer");
-    gb_fast_options op{64, 4, 0, nullptr, nullptr};
+    gb_fast_options op{64, 0, 0, nullptr, nullptr};
     if (opt) op = *opt;
-    if (op.waves_per_block < 1 || op.waves_per_block > 4 || op.refresh_interval < 0 || op.blocks < 0)
+    if (op.waves_per_block < 0 || op.waves_per_block > 4 || op.refresh_interval < 0 || op.blocks < 0)
         return fail(GB_ERR_INVALID_ARGUMENT, "invalid batched-form options");
     if (!counter) {
         st = counter_get(&counter);
@@ -271,12 +271,7 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     P.stats = op.stats;
     P.gout = op.gradient;
     P.rescore = rescore ? 1 : 0;
-    int blocks = op.blocks;
-    if (blocks == 0) {
-        const int needed = (N + op.waves_per_block - 1) / op.waves_per_block;
-        blocks = needed < kMaxFastBlocks ? needed : kMaxFastBlocks;
-    }
-    const int e = launch_relax_fast(P, op.waves_per_block, blocks, stream);
+    const int e = launch_relax_fast(P, op.waves_per_block, op.blocks, stream);
     if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
     return GB_OK;
 }

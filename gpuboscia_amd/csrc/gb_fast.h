@@ -34,6 +34,7 @@ struct FastParams {
     int rescore;            // re-score best_v[i] in FP64 first when best_val[i] < 1e300
 };
 
+constexpr int kMaxFastBlocks = 2048;  // persistent workgroups of the batched form (blocks = 0: automatic)
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream);
 int launch_candidate_features(int n, int N, const double* X, const double* G, const double* LB,
                               const double* UB, const int* depth, const double* gap, const double* val,

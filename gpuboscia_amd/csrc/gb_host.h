@@ -14,7 +14,6 @@ constexpr int kMaxSimplexN = 2048;       // sort/scan arrays in LDS
 constexpr int kMaxBirkhoffOrder = 256;   // Hungarian state in LDS
 constexpr size_t kMaxLdsBytes = 160 * 1024;
 constexpr int kMaxFastN = 512;           // batched form: 8 coordinates per lane
-constexpr int kMaxFastBlocks = 2048;     // persistent workgroups of the batched form
 constexpr int kFastRefresh = 64;         // direct Q x recompute interval of the batched form
 
 extern thread_local std::string g_last_error;

@@ -160,7 +160,7 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
         // FP64-certified bound, and resumes as a continuation).
         const int fp64_iters = op.fp64_max_iters > 0 ? op.fp64_max_iters
                                                      : (max_iters / 16 > 1 ? max_iters / 16 : 1);
-        const gb_fast_options fo{64, 4, 0, op.fp64_stats, op.gradient};
+        const gb_fast_options fo{64, 0, 0, op.fp64_stats, op.gradient};
         return relax_batch_fast(objective, algorithm, lmo, N, fp64_iters, x, lb, ub, eps, iterations,
                                 status, gap, value, best_vertex, best_vertex_val, integer_flags,
                                 candidate_tol, cutoff, &fo, stream, nullptr, true);

@@ -500,6 +500,379 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
     }
 }
 
+// The same relaxation with one node per workgroup and its rows split over
+// NT waves: wave w holds row j = lane + 64 w (the t = w slice of
+// relax_fast_kernel's lane).  Each row's operations and each product's FMA
+// order over k are those of relax_fast_kernel.  A reduction gathers every
+// wave's per-row factors through LDS, and every wave then evaluates the same
+// lane chain over t and the same wave tree, so results are bit-identical to
+// relax_fast_kernel's and all waves take the same branches.  With one row
+// per lane a wave keeps 8 Q columns in flight per L2 round trip, and every
+// node of a batch of up to a few thousand is in flight at once.
+constexpr int kRowSums = 3;  // fma-chain sums per exchange
+
+// Every node of a batch is meant to be in flight at once (the launcher uses
+// this layout only then), so the kernel asks for four waves per SIMD.
+template <int NT>
+__global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams P) {
+    extern __shared__ __align__(16) double lds_fast[];
+    const int lane = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane(wave_id());
+    const int n = P.n;
+    const int np = P.n_pad;
+    const int j = lane + kWave * w;  // this lane's row
+    const bool in = j < n;
+    const double* Q = P.Qt;
+    double* zs = lds_fast;             // staged vector (np)
+    double* zs2 = zs + np;             // second staged vector (np)
+    double* dlist = zs2 + np;          // changed-column coefficients (np)
+    int* klist = reinterpret_cast<int*>(dlist + np);  // changed columns (np ints)
+    double* xf = dlist + np + np / 2;  // 2 buffers x NT x 64 x (2 kRowSums + 1)
+    constexpr int kSlot = 2 * kRowSums + 1;
+    int* xok = reinterpret_cast<int*>(xf + 2 * NT * kWave * kSlot);  // 2 buffers x NT x 64 x 2
+    int* misc = xok + 2 * NT * kWave * 2;                            // counts[NT], node
+    const double hs = 0.5 * P.gs;
+    const double lsc_scale = P.gs * P.ls;
+    const double atol = 1e-6, rtol = 1e-4;
+    int xbuf = 0;
+
+    // K fma-chain sums of (a_k, b_k) over the rows j = lane + 64 t < n, t
+    // ascending, each followed by the wave tree; optionally a min chain from
+    // 1.0 (entries +inf where a row has none) and two AND flags.
+    struct Ex {
+        double a[kRowSums], b[kRowSums], mn;
+        bool ok, ok2;
+        double sum[kRowSums], min;
+        bool all_ok, all_ok2;
+    };
+    auto exchange = [&](Ex& e, int K, bool with_min, bool with_ok) {
+        double* X = xf + (size_t)xbuf * NT * kWave * kSlot;
+        int* O = xok + xbuf * NT * kWave * 2;
+        double* mine = X + ((size_t)w * kWave + lane) * kSlot;
+#pragma unroll
+        for (int k = 0; k < kRowSums; ++k)
+            if (k < K) {
+                mine[2 * k] = e.a[k];
+                mine[2 * k + 1] = e.b[k];
+            }
+        if (with_min) mine[2 * kRowSums] = e.mn;
+        if (with_ok) {
+            O[(w * kWave + lane) * 2] = e.ok ? 1 : 0;
+            O[(w * kWave + lane) * 2 + 1] = e.ok2 ? 1 : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kRowSums; ++k)
+            if (k < K) {
+                double acc = 0.0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const double* s = X + ((size_t)t * kWave + lane) * kSlot;
+                    if (lane + kWave * t < n) acc = fma(s[2 * k], s[2 * k + 1], acc);
+                }
+                e.sum[k] = wave_sum_all(acc);
+            }
+        if (with_min) {
+            double r = 1.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) r = fmin(r, X[((size_t)t * kWave + lane) * kSlot + 2 * kRowSums]);
+            e.min = wave_min_all(r);
+        }
+        if (with_ok) {
+            bool o = true, o2 = true;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                o = o && O[(t * kWave + lane) * 2] != 0;
+                o2 = o2 && O[(t * kWave + lane) * 2 + 1] != 0;
+            }
+            e.all_ok = wave_all(o);
+            e.all_ok2 = wave_all(o2);
+        }
+        xbuf ^= 1;
+    };
+
+    long long columns = 0, gemvs = 0;
+    constexpr int kB = 8;  // Q columns in flight per round trip
+    // ya = Q za (and yb = Q zb) for this lane's row; za, zb staged in LDS.
+    auto sweep = [&](bool two, double& ya, double& yb) {
+        double acc0 = 0.0, acc1 = 0.0;
+        for (int k0 = 0; k0 < n; k0 += kB) {
+            double qq[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) qq[u] = (in && k0 + u < n) ? Q[(size_t)(k0 + u) * n + j] : 0.0;
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const int k = k0 + u;
+                if (k < n) {
+                    acc0 = fma(qq[u], zs[k], acc0);
+                    if (two) acc1 = fma(qq[u], zs2[k], acc1);
+                }
+            }
+        }
+        ya = in ? acc0 : 0.0;
+        if (two) yb = in ? acc1 : 0.0;
+    };
+    auto gemv = [&](double z, double& y) {
+        if (in) zs[j] = z;
+        __syncthreads();
+        double unused = 0.0;
+        sweep(false, y, unused);
+        __syncthreads();
+        ++gemvs;
+    };
+    auto gemv2 = [&](double z1, double& y1, double z2, double& y2) {
+        if (in) {
+            zs[j] = z1;
+            zs2[j] = z2;
+        }
+        __syncthreads();
+        sweep(true, y1, y2);
+        __syncthreads();
+        gemvs += 2;
+    };
+    // y += sum over changed k (znew_k != zold_k), k ascending, of
+    // Q[:,k] (znew_k - zold_k): the list is compacted in wave order.
+    auto update = [&](double znew, double zold, double& y) {
+        const bool ch = in && znew != zold;
+        const unsigned long long mask = __ballot(ch);
+        if (lane == 0) misc[w] = __popcll(mask);
+        __syncthreads();
+        int off = 0, count = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int c = misc[t];
+            if (t < w) off += c;
+            count += c;
+        }
+        if (ch) {
+            const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
+            klist[pos] = j;
+            dlist[pos] = znew - zold;
+        }
+        __syncthreads();
+        for (int g0 = 0; g0 < count; g0 += kB) {
+            double c[kB], dd[kB];
+#pragma unroll
+            for (int g = 0; g < kB; ++g) {
+                const bool live = g0 + g < count;
+                const int k = live ? klist[g0 + g] : 0;
+                dd[g] = live ? dlist[g0 + g] : 0.0;
+                c[g] = (live && in) ? Q[(size_t)k * n + j] : 0.0;
+            }
+#pragma unroll
+            for (int g = 0; g < kB; ++g)
+                if (g0 + g < count && in) y = fma(c[g], dd[g], y);
+        }
+        __syncthreads();
+        columns += count;
+    };
+
+    while (true) {
+        if (threadIdx.x == 0) misc[NT] = atomicAdd(P.counter, 1);
+        __syncthreads();
+        const int node = misc[NT];
+        if (node >= P.N) break;
+        columns = 0;
+        gemvs = 0;
+
+        const double* lbn = P.lb + (size_t)node * n;
+        const double* ubn = P.ub + (size_t)node * n;
+        double* xn = P.x + (size_t)node * n;
+        double* bvn = P.best_v + (size_t)node * n;
+        const double eps = P.eps[node];
+        const double cutoff = P.cutoff ? P.cutoff[node] : __builtin_inf();
+        const int initial = P.iters[node];
+        const bool supplied = (P.status[node] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0;
+
+        const double lb = in ? lbn[j] : 0.0;
+        const double ub = in ? ubn[j] : 0.0;
+        const double lsc = (in && P.lin) ? lsc_scale * P.lin[j] : 0.0;
+        const bool fl = in && P.flags && P.flags[j];
+        Ex e{};
+        e.ok = !(in && (isnan(lb) || isnan(ub) || lb > ub || (fl && ceil(lb) > floor(ub))));
+        e.ok2 = true;
+        exchange(e, 0, false, true);
+        if (!e.all_ok) {
+            if (threadIdx.x == 0) {
+                P.status[node] = GB_NODE_INFEASIBLE;
+                P.value[node] = __builtin_inf();
+                P.gap[node] = 0.0;
+                P.best_val[node] = kNone;
+                P.iters[node] = initial;
+            }
+            continue;
+        }
+
+        double best = kNone;
+        // Score factors of an integral vertex z with known Q z; the sum goes
+        // in slot `k` of an exchange, the feasibility flag in ok.
+        auto score_in = [&](Ex& ex, int k, double z, double qz) {
+            ex.ok = !(in && (!isfinite(z) || z < lb - P.ctol || z > ub + P.ctol || (fl && z != round(z))));
+            ex.a[k] = z;
+            ex.b[k] = fma(hs, qz, lsc);
+        };
+        auto score_out = [&](const Ex& ex, int k, double z) {
+            if (!ex.all_ok) return;
+            const double f = ex.sum[k] + P.kappa;
+            if (!isfinite(f)) return;
+            if (f < best) {
+                best = f;
+                if (in) bvn[j] = z;
+            }
+        };
+
+        double x = in ? ((initial == 0 && !supplied) ? ub : xn[j]) : 0.0;
+        double qx = 0.0, v, qv = 0.0, a = 0.0, qa = 0.0, g;
+        if (P.rescore && P.best_val[node] < kNone) {
+            const double z = in ? bvn[j] : 0.0;
+            double qz = 0.0;
+            gemv2(z, qz, x, qx);
+            Ex s{};
+            score_in(s, 0, z, qz);
+            s.ok2 = true;
+            exchange(s, 1, false, true);
+            score_out(s, 0, z);
+        } else {
+            gemv(x, qx);
+        }
+        auto grad = [&]() { g = P.gs * qx + lsc; };
+        auto fwd = [&]() { return (g > 0.0) ? lb : ub; };
+        auto away = [&]() {
+            const double ut = atol + rtol * fabs(ub);
+            const double lt = atol + rtol * fabs(lb);
+            if (fabs(x - ub) <= ut) return ub;
+            if (fabs(x - lb) <= lt) return lb;
+            return (-g > 0.0) ? lb : ub;
+        };
+        grad();
+        v = fwd();
+        bool have_a = false;
+        if (P.dicg) {
+            a = away();
+            gemv2(v, qv, a, qa);
+        } else {
+            gemv(v, qv);
+        }
+        int iterations = initial;
+        int it = 0;
+        for (; it < P.max_iters; ++it) {
+            iterations = initial + it + 1;
+            if (it > 0) {
+                grad();
+                const double vn = fwd();
+                update(vn, v, qv);
+                v = vn;
+            }
+            // score(v), gap and f(x) in one exchange
+            Ex s{};
+            score_in(s, 0, v, qv);
+            s.ok2 = true;
+            s.a[1] = g;
+            s.b[1] = x - v;
+            s.a[2] = x;
+            s.b[2] = fma(hs, qx, lsc);
+            exchange(s, 3, false, true);
+            score_out(s, 0, v);
+            const double gap = s.sum[1];
+            const double fx = s.sum[2] + P.kappa;
+            if (gap >= 0.0 && gap < eps) break;
+            if (fx - gap >= cutoff) break;
+
+            double d, qd, gmax;
+            Ex r{};
+            if (P.dicg) {
+                if (have_a) {
+                    const double an = away();
+                    update(an, a, qa);
+                    a = an;
+                }
+                have_a = true;
+                // score(a), the DICG max step and the step sums in one exchange
+                score_in(r, 0, a, qa);
+                r.ok2 = true;
+                const double dv = a - v;  // direction a - v as the reference
+                r.mn = !in ? __builtin_inf()
+                       : dv < 0.0 ? (ub - x) / -dv
+                       : dv > 0.0 ? (x - lb) / dv
+                                  : __builtin_inf();
+                d = v - a;
+                qd = qv - qa;
+                r.a[1] = -g;
+                r.b[1] = d;
+                r.a[2] = P.gs * d;
+                r.b[2] = qd;
+                exchange(r, 3, true, true);
+                score_out(r, 0, a);
+                gmax = fmax(0.0, fmin(r.min, 1.0));
+            } else {
+                d = v - x;
+                qd = qv - qx;
+                r.a[1] = -g;
+                r.b[1] = d;
+                r.a[2] = P.gs * d;
+                r.b[2] = qd;
+                // slot 0 unused: a zero pair adds nothing to a chain that is not read
+                r.a[0] = 0.0;
+                r.b[0] = 0.0;
+                exchange(r, 3, false, false);
+                gmax = 1.0;
+            }
+            const double gamma = quad_exact(r.sum[1], r.sum[2], gmax);
+            x = fma(gamma, d, x);
+            qx = fma(gamma, qd, qx);
+            if (P.refresh > 0 && (it + 1) % P.refresh == 0) gemv(x, qx);
+        }
+
+        // Direct FP64 certificate at the returned iterate (see relax_fast_kernel).
+        const bool fresh = it == 0;
+        double qc = 0.0;
+        const double cand = fl ? round(x) : x;
+        if (P.flags) {
+            if (fresh) gemv(cand, qc);
+            else gemv2(x, qx, cand, qc);
+        } else if (!fresh) {
+            gemv(x, qx);
+        }
+        grad();
+        if (P.gout && in) P.gout[(size_t)node * n + j] = g;
+        if (!fresh) {
+            v = fwd();
+            gemv(v, qv);
+        }
+        Ex s{};
+        score_in(s, 0, v, qv);
+        s.ok2 = true;
+        s.a[1] = g;
+        s.b[1] = x - v;
+        s.a[2] = x;
+        s.b[2] = fma(hs, qx, lsc);
+        exchange(s, 3, false, true);
+        score_out(s, 0, v);
+        const double fgap = s.sum[1];
+        const double fval = s.sum[2] + P.kappa;
+        uint8_t st = (fgap >= 0.0 && fgap < eps) ? 1 : 0;
+        Ex c{};
+        if (P.flags) score_in(c, 0, cand, qc);
+        else c.ok = true;
+        c.ok2 = !(in && (!isfinite(x) || (fl && fabs(x - round(x)) > P.ctol)));
+        exchange(c, P.flags ? 1 : 0, false, true);
+        if (P.flags) score_out(c, 0, cand);
+        if (in) xn[j] = x;
+        if (c.all_ok2) st = st ? GB_NODE_INTEGRAL_CONVERGED : GB_NODE_INTEGRAL_UNCONVERGED;
+        if (threadIdx.x == 0) {
+            P.value[node] = fval;
+            P.gap[node] = fgap;
+            P.status[node] = st;
+            P.iters[node] = iterations;
+            P.best_val[node] = best;
+            if (P.stats) {
+                P.stats[2 * (size_t)node] = columns;
+                P.stats[2 * (size_t)node + 1] = gemvs;
+            }
+        }
+    }
+}
+
 __global__ void reset_counter_kernel(int* counter) { *counter = 0; }
 
 // G[i] = s_g (Q x_i) + s_g s_l c for a batch of iterates: one wave per node,
@@ -664,9 +1037,54 @@ int launch_topk_candidates(int n, int N, const double* X, const uint8_t* flags, 
     return static_cast<int>(hipGetLastError());
 }
 
+// LDS of relax_fast_rows_kernel: two staged vectors, the changed-column
+// list, the exchange buffers, counts and the node id.
+static size_t rows_lds_bytes(int n_pad, int nt) {
+    const size_t doubles = (size_t)n_pad * 7 / 2 + (size_t)2 * nt * kWave * (2 * kRowSums + 1);
+    const size_t ints = (size_t)2 * nt * kWave * 2 + nt + 1;
+    return doubles * 8 + ints * 4;
+}
+
+// Nodes relax_fast_rows_kernel<NT> keeps in flight on this device.
+template <int NT>
+static int rows_capacity(size_t lds) {
+    static int cached[2] = {-1, -1};  // [device 0 / other]: blocks per CU x CUs
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    int& c = cached[dev == 0 ? 0 : 1];
+    if (c < 0 || dev != 0) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_fast_rows_kernel<NT>, NT * kWave, lds) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0;
+        c = per_cu * cus;
+    }
+    return c;
+}
+
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, p.counter);
+    if (waves_per_block == 0 && p.n > 64) {
+        // One node per workgroup when the whole batch is in flight at once;
+        // otherwise the one-wave-per-node layout keeps more nodes resident.
+        const int nt = p.n <= 128 ? 2 : (p.n <= 256 ? 4 : 8);
+        const size_t lds = rows_lds_bytes(p.n_pad, nt);
+        const int cap = nt == 2 ? rows_capacity<2>(lds) : nt == 4 ? rows_capacity<4>(lds) : rows_capacity<8>(lds);
+        if (p.N <= cap) {
+            const dim3 grid(p.N < blocks || blocks <= 0 ? p.N : blocks), block(nt * kWave);
+            if (nt == 2) hipLaunchKernelGGL(relax_fast_rows_kernel<2>, grid, block, lds, st, p);
+            else if (nt == 4) hipLaunchKernelGGL(relax_fast_rows_kernel<4>, grid, block, lds, st, p);
+            else hipLaunchKernelGGL(relax_fast_rows_kernel<8>, grid, block, lds, st, p);
+            return (int)hipGetLastError();
+        }
+    }
+    if (waves_per_block == 0) waves_per_block = 4;
+    if (blocks <= 0) {
+        const int needed = (p.N + waves_per_block - 1) / waves_per_block;
+        blocks = needed < kMaxFastBlocks ? needed : kMaxFastBlocks;
+    }
     // per wave: 2 n_pad doubles + n_pad ints (n_pad even keeps 8-byte alignment)
     const size_t wave_lds = (size_t)waves_per_block * p.n_pad * 5 / 2 * sizeof(double);
     const size_t q_lds = (size_t)p.n * p.n * sizeof(double);

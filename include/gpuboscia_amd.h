@@ -160,7 +160,9 @@ gb_status gb_relax_batch(const gb_objective_desc* objective,
 /* Options of the batched-form relaxation (gb_relax_batch_fast). */
 typedef struct {
     int refresh_interval;     /* recompute Q x directly every k iterations (0 = never) */
-    int waves_per_block;      /* nodes resident per workgroup (1..4) */
+    int waves_per_block;      /* 1..4: nodes resident per workgroup, one wave each;
+                                 0: one node per workgroup with its rows split over
+                                 ceil(n/64) waves (n > 64; n <= 64 uses 4) */
     int blocks;               /* persistent workgroups (0 = automatic) */
     long long* stats;         /* optional device [N][2]: Q-column updates and direct
                                  n x n products per node (work accounting), or NULL */
