@@ -16,9 +16,12 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -210,6 +213,72 @@ class Frontier {
     std::vector<Key> heap_;
     std::vector<HostNode> recs_;
     std::vector<int> free_;
+};
+
+// A fixed set of host worker threads for the per-node loops whose iterations
+// are independent (staging copies, branching scans, child bounds): each call
+// splits [0, count) into contiguous parts, the caller runs part 0 and waits
+// for the rest.  Results do not depend on the thread count.  One call at a
+// time: a concurrent caller runs its loop inline.  Created on first use and
+// kept for the life of the process.
+class HostThreads {
+  public:
+    static HostThreads& get() {
+        static HostThreads* p = new HostThreads();
+        return *p;
+    }
+    template <typename F>
+    void parallel_for(int count, F&& f) {
+        const int parts = std::min((int)workers_.size() + 1, count / kMinPerPart);
+        std::unique_lock<std::mutex> call(call_mu_, std::try_to_lock);
+        if (parts <= 1 || !call.owns_lock()) {
+            for (int i = 0; i < count; ++i) f(i);
+            return;
+        }
+        std::function<void(int)> body = [&](int p) {
+            const int lo = (int)((long long)count * p / parts);
+            const int hi = (int)((long long)count * (p + 1) / parts);
+            for (int i = lo; i < hi; ++i) f(i);
+        };
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            body_ = &body;
+            parts_ = parts;
+            next_ = 1;
+            left_ = parts - 1;
+        }
+        cv_.notify_all();
+        body(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return left_ == 0; });
+        body_ = nullptr;
+    }
+
+  private:
+    static constexpr int kMinPerPart = 64;
+    HostThreads() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nw = hw > 1 ? (int)std::min(7u, hw - 1) : 0;
+        for (int t = 0; t < nw; ++t) std::thread([this] { loop(); }).detach();
+        workers_.resize(nw);
+    }
+    void loop() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return body_ != nullptr && next_ < parts_; });
+            const int p = next_++;
+            std::function<void(int)>* b = body_;
+            lk.unlock();
+            (*b)(p);
+            lk.lock();
+            if (--left_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::vector<char> workers_;  // one entry per worker thread
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    std::function<void(int)>* body_ = nullptr;
+    int parts_ = 0, next_ = 0, left_ = 0;
 };
 
 // gap_utils.h:14-62
@@ -800,8 +869,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         if (batch.empty()) return GB_OK;
         const int N = (int)batch.size();
         db.layout(N);
-        for (int i = 0; i < N; ++i) {
-            HostNode& nd = batch[i];
+        // Staging copies on the host threads (the pool is only read here).
+        HostThreads::get().parallel_for(N, [&](int i) {
+            const HostNode& nd = batch[i];
             const double* nlb = pool.lb(nd.slot);
             const double* nub = pool.ub(nd.slot);
             double* hx = db.hx + (size_t)i * n;
@@ -813,8 +883,6 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 // box point is feasible).
                 const double* px = pool.rx(nd.xslot);
                 for (int j = 0; j < n; ++j) hx[j] = std::min(std::max(px[j], nlb[j]), nub[j]);
-                pool.release(nd.xslot);
-                nd.xslot = -1;
             } else {
                 std::copy(nlb, nlb + n, hx);
             }
@@ -826,7 +894,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             db.hcut[i] = inc;
             db.hbvv[i] = 1e300;
             if (learned) db.hdepth[i] = nd.depth;
-        }
+        });
+        for (HostNode& nd : batch)
+            if (nd.xslot >= 0) {
+                pool.release(nd.xslot);
+                nd.xslot = -1;
+            }
         open_min[slot].resize(N);
         for (int i = N - 1; i >= 0; --i)
             open_min[slot][i] = i + 1 < N ? std::min(batch[i].bound, open_min[slot][i + 1]) : batch[i].bound;
@@ -890,6 +963,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         return GB_OK;
     };
 
+    struct ChildTask {
+        int child, parent, row, mask, k, var0;
+    };
+    struct XTask {
+        int slot, row;
+    };
+    std::vector<ChildTask> child_tasks;
+    std::vector<XTask> x_tasks;
+    std::vector<int> deferred, task_vars, scan_vars;
     int head = 0;
     while (!finished) {
         const double el = std::chrono::duration<double>(Clock::now() - t0).count();
@@ -919,6 +1001,30 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         }
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
+        // Branching variables of the rules that depend only on x, on the host
+        // threads; child bounds and iterates are copied after the sequential
+        // pass (parent slots are released only then).
+        const bool prescan = s->branch_rule == GB_BRANCH_POW2 || s->branch_rule == GB_BRANCH_MOST_FRACTIONAL ||
+                             (s->branch_rule == GB_BRANCH_LEARNED && !learned);
+        const int bk = s->branch_rule == GB_BRANCH_POW2 ? s->branch_k : 1;
+        if (prescan) {
+            scan_vars.assign((size_t)N * bk, -1);
+            HostThreads::get().parallel_for(N, [&](int i) {
+                if (db.hst[i] >= GB_NODE_INTEGRAL_UNCONVERGED) return;
+                const double* xr = db.hx + (size_t)i * n;
+                int* out = scan_vars.data() + (size_t)i * bk;
+                if (s->branch_rule == GB_BRANCH_POW2) {
+                    const std::vector<int> v = top_k_infeasible(xr, iv, s->branch_k, s->integer_tol);
+                    for (size_t c = 0; c < v.size(); ++c) out[c] = v[c];
+                } else {
+                    out[0] = most_fractional(xr, iv, s->integer_tol);
+                }
+            });
+        }
+        child_tasks.clear();
+        x_tasks.clear();
+        deferred.clear();
+        task_vars.clear();
         for (int i = 0; i < N && !finished; ++i) {
             HostNode nd = batch[i];
             open_pos[head] = (size_t)i + 1;
@@ -954,7 +1060,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             const double* xi = db.hx + (size_t)i * n;
             if (db.hst[i] >= GB_NODE_INTEGRAL_UNCONVERGED && !cert) {
                 // device_resident_solver.cuh:1970-1980: keep the node live.
-                std::copy(xi, xi + n, pool.rx(nd.slot));
+                x_tasks.push_back({nd.slot, i});
                 nd.has_rx = true;
                 nd.riters = db.hit[i];
                 q.push(nd);
@@ -985,8 +1091,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 }
             } else if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
                 std::vector<int> vars;
-                if (s->branch_rule == GB_BRANCH_POW2) {
-                    vars = top_k_infeasible(xi, iv, s->branch_k, s->integer_tol);
+                if (prescan) {
+                    const int* sv = scan_vars.data() + (size_t)i * bk;
+                    for (int c = 0; c < bk && sv[c] >= 0; ++c) vars.push_back(sv[c]);
                 } else if (s->branch_rule == GB_BRANCH_CALLBACK) {
                     std::vector<int> cand;
                     for (int v : iv) {
@@ -1009,7 +1116,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     const int* tk = db.htopk + (size_t)i * s->branch_k;
                     for (int c = 0; c < s->branch_k && tk[c] >= 0; ++c) vars.push_back(tk[c]);
                 } else {
-                    // GB_BRANCH_MOST_FRACTIONAL, and the learned rule without a model
+                    // the learned rule where no candidate was scored
                     const int v = most_fractional(xi, iv, s->integer_tol);
                     if (v >= 0) vars.push_back(v);
                 }
@@ -1022,6 +1129,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 next_id += 8;  // device_resident_solver.cuh:1989
                 int nid = base;
                 const int k = (int)vars.size();
+                const int var0 = (int)task_vars.size();
+                task_vars.insert(task_vars.end(), vars.begin(), vars.end());
                 bool parent_x_kept = false;
                 for (int mask = 0; k > 0 && mask < (1 << k); ++mask) {
                     HostNode c;
@@ -1031,20 +1140,13 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     c.bound = nd.bound;
                     if (!(c.bound < inc)) continue;
                     c.slot = pool.alloc();
-                    double* clb = pool.lb(c.slot);
-                    double* cub = pool.ub(c.slot);
-                    std::copy(pool.lb(nd.slot), pool.lb(nd.slot) + 2 * n, clb);  // lb and ub
-                    for (int b = 0; b < k; ++b) {
-                        const int vi = vars[b];
-                        if (mask & (1 << b)) clb[vi] = std::ceil(xi[vi]);
-                        else cub[vi] = std::floor(xi[vi]);
-                    }
+                    child_tasks.push_back({c.slot, nd.slot, i, mask, k, var0});
                     if (s->form != 0) {
                         // Batched forms: the child starts from the parent's
                         // iterate, kept once in the parent's slot and clipped to
                         // the child's box when the child is staged.
                         if (!parent_x_kept) {
-                            std::copy(xi, xi + n, pool.rx(nd.slot));
+                            x_tasks.push_back({nd.slot, i});
                             parent_x_kept = true;
                         }
                         pool.retain(nd.slot);
@@ -1060,11 +1162,30 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     q.push(c);
                 }
             }
-            pool.release(nd.slot);
+            deferred.push_back(nd.slot);
             ++done_nodes;
             progress(new_inc, remaining);
             finish_check(remaining > 0);
         }
+        // Child bounds (the parent's with the branched variables rounded) and
+        // kept iterates, on the host threads; then the parents' slots go back.
+        HostThreads::get().parallel_for((int)child_tasks.size(), [&](int t) {
+            const ChildTask& ct = child_tasks[t];
+            const double* xr = db.hx + (size_t)ct.row * n;
+            double* clb = pool.lb(ct.child);
+            double* cub = pool.ub(ct.child);
+            std::copy(pool.lb(ct.parent), pool.lb(ct.parent) + 2 * n, clb);  // lb and ub
+            for (int b = 0; b < ct.k; ++b) {
+                const int vi = task_vars[ct.var0 + b];
+                if (ct.mask & (1 << b)) clb[vi] = std::ceil(xr[vi]);
+                else cub[vi] = std::floor(xr[vi]);
+            }
+        });
+        HostThreads::get().parallel_for((int)x_tasks.size(), [&](int t) {
+            const double* xr = db.hx + (size_t)x_tasks[t].row * n;
+            std::copy(xr, xr + n, pool.rx(x_tasks[t].slot));
+        });
+        for (int sl : deferred) pool.release(sl);
         if (!finished && q.empty() && !in_flight(other)) { final_bound = inc; finished = true; }
         if (pipelined && in_flight(other)) head = other;
     }
