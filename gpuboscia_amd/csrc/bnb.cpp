@@ -174,8 +174,10 @@ struct Labeler {
     int labeled = 0;
 };
 
-// Best-first frontier ordered by (dual_bound, id): a binary heap of 16-byte
-// keys over node records kept in a recycled vector.
+// Best-first frontier ordered by (dual_bound, id): a 4-ary heap of 16-byte
+// keys over node records kept in a recycled vector.  (bound, id) is a strict
+// order (ids are unique), so nodes leave in exactly that order whatever the
+// heap shape.
 class Frontier {
   public:
     bool empty() const { return heap_.empty(); }
@@ -191,13 +193,36 @@ class Frontier {
             r = (int)recs_.size();
             recs_.push_back(nd);
         }
-        heap_.push_back(Key{nd.bound, nd.id, r});
-        std::push_heap(heap_.begin(), heap_.end(), After());
+        const Key key{nd.bound, nd.id, r};
+        size_t i = heap_.size();
+        heap_.push_back(key);
+        while (i > 0) {
+            const size_t parent = (i - 1) / 4;
+            if (!before(key, heap_[parent])) break;
+            heap_[i] = heap_[parent];
+            i = parent;
+        }
+        heap_[i] = key;
     }
     void pop() {
-        std::pop_heap(heap_.begin(), heap_.end(), After());
-        free_.push_back(heap_.back().rec);
+        free_.push_back(heap_.front().rec);
+        const Key last = heap_.back();
         heap_.pop_back();
+        const size_t m = heap_.size();
+        if (m == 0) return;
+        size_t i = 0;
+        for (;;) {
+            const size_t c0 = 4 * i + 1;
+            if (c0 >= m) break;
+            size_t best = c0;
+            const size_t ce = std::min(c0 + 4, m);
+            for (size_t c = c0 + 1; c < ce; ++c)
+                if (before(heap_[c], heap_[best])) best = c;
+            if (!before(heap_[best], last)) break;
+            heap_[i] = heap_[best];
+            i = best;
+        }
+        heap_[i] = last;
     }
 
   private:
@@ -205,11 +230,9 @@ class Frontier {
         double bound;
         int id, rec;
     };
-    struct After {
-        bool operator()(const Key& a, const Key& b) const {
-            return a.bound > b.bound || (a.bound == b.bound && a.id > b.id);
-        }
-    };
+    static bool before(const Key& a, const Key& b) {
+        return a.bound < b.bound || (a.bound == b.bound && a.id < b.id);
+    }
     std::vector<Key> heap_;
     std::vector<HostNode> recs_;
     std::vector<int> free_;
