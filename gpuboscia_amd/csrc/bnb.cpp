@@ -174,15 +174,18 @@ struct Labeler {
     int labeled = 0;
 };
 
-// Best-first frontier ordered by (dual_bound, id): a 4-ary heap of 16-byte
-// keys over node records kept in a recycled vector.  (bound, id) is a strict
+// Best-first frontier ordered by (dual_bound, id).  (bound, id) is a strict
 // order (ids are unique), so nodes leave in exactly that order whatever the
-// heap shape.
+// structure: a 4-ary heap of 16-byte keys holds every key below a threshold
+// (at most a few ten thousand, cache resident), an unsorted vector the rest;
+// when the heap runs empty the smallest keys of the vector are selected
+// (nth_element) into it, and when it grows too large its largest keys move to
+// the vector.  Node records live in a recycled vector.
 class Frontier {
   public:
-    bool empty() const { return heap_.empty(); }
-    size_t size() const { return heap_.size(); }
-    const HostNode& top() const { return recs_[heap_.front().rec]; }
+    bool empty() const { return hot_.empty(); }
+    size_t size() const { return hot_.size() + cold_.size(); }
+    const HostNode& top() const { return recs_[hot_.front().rec]; }
     void push(const HostNode& nd) {
         int r;
         if (!free_.empty()) {
@@ -194,21 +197,43 @@ class Frontier {
             recs_.push_back(nd);
         }
         const Key key{nd.bound, nd.id, r};
-        size_t i = heap_.size();
-        heap_.push_back(key);
-        while (i > 0) {
-            const size_t parent = (i - 1) / 4;
-            if (!before(key, heap_[parent])) break;
-            heap_[i] = heap_[parent];
-            i = parent;
+        if (!cold_.empty() && !before(key, lim_)) {
+            cold_.push_back(key);
+            return;
         }
-        heap_[i] = key;
+        heap_push(key);
+        if (hot_.size() > 2 * kHot) spill();
     }
     void pop() {
-        free_.push_back(heap_.front().rec);
-        const Key last = heap_.back();
-        heap_.pop_back();
-        const size_t m = heap_.size();
+        free_.push_back(hot_.front().rec);
+        heap_pop();
+        if (hot_.empty() && !cold_.empty()) refill();
+    }
+
+  private:
+    static constexpr size_t kHot = 1 << 15;
+    struct Key {
+        double bound;
+        int id, rec;
+    };
+    static bool before(const Key& a, const Key& b) {
+        return a.bound < b.bound || (a.bound == b.bound && a.id < b.id);
+    }
+    void heap_push(const Key& key) {
+        size_t i = hot_.size();
+        hot_.push_back(key);
+        while (i > 0) {
+            const size_t parent = (i - 1) / 4;
+            if (!before(key, hot_[parent])) break;
+            hot_[i] = hot_[parent];
+            i = parent;
+        }
+        hot_[i] = key;
+    }
+    void heap_pop() {
+        const Key last = hot_.back();
+        hot_.pop_back();
+        const size_t m = hot_.size();
         if (m == 0) return;
         size_t i = 0;
         for (;;) {
@@ -217,23 +242,56 @@ class Frontier {
             size_t best = c0;
             const size_t ce = std::min(c0 + 4, m);
             for (size_t c = c0 + 1; c < ce; ++c)
-                if (before(heap_[c], heap_[best])) best = c;
-            if (!before(heap_[best], last)) break;
-            heap_[i] = heap_[best];
+                if (before(hot_[c], hot_[best])) best = c;
+            if (!before(hot_[best], last)) break;
+            hot_[i] = hot_[best];
             i = best;
         }
-        heap_[i] = last;
+        hot_[i] = last;
     }
-
-  private:
-    struct Key {
-        double bound;
-        int id, rec;
-    };
-    static bool before(const Key& a, const Key& b) {
-        return a.bound < b.bound || (a.bound == b.bound && a.id < b.id);
+    void sift_down(size_t i) {
+        const size_t m = hot_.size();
+        const Key key = hot_[i];
+        for (;;) {
+            const size_t c0 = 4 * i + 1;
+            if (c0 >= m) break;
+            size_t best = c0;
+            const size_t ce = std::min(c0 + 4, m);
+            for (size_t c = c0 + 1; c < ce; ++c)
+                if (before(hot_[c], hot_[best])) best = c;
+            if (!before(hot_[best], key)) break;
+            hot_[i] = hot_[best];
+            i = best;
+        }
+        hot_[i] = key;
     }
-    std::vector<Key> heap_;
+    void heapify() {  // bottom-up, O(size)
+        if (hot_.size() < 2) return;
+        for (size_t i = (hot_.size() - 2) / 4 + 1; i-- > 0;) sift_down(i);
+    }
+    // Keep the kHot smallest keys in the heap, the rest in the vector.
+    void spill() {
+        std::nth_element(hot_.begin(), hot_.begin() + kHot, hot_.end(), before);
+        lim_ = *std::min_element(hot_.begin() + kHot, hot_.end(), before);
+        cold_.insert(cold_.end(), hot_.begin() + kHot, hot_.end());
+        hot_.resize(kHot);
+        heapify();
+    }
+    // Move the kHot smallest keys of the vector into the (empty) heap.
+    void refill() {
+        if (cold_.size() > kHot) {
+            std::nth_element(cold_.begin(), cold_.begin() + kHot, cold_.end(), before);
+            hot_.assign(cold_.begin(), cold_.begin() + kHot);
+            cold_.erase(cold_.begin(), cold_.begin() + kHot);
+            lim_ = *std::min_element(cold_.begin(), cold_.end(), before);
+        } else {
+            hot_.swap(cold_);
+            cold_.clear();
+        }
+        heapify();
+    }
+    std::vector<Key> hot_, cold_;
+    Key lim_{0.0, 0, 0};  // every heap key is before lim_, no vector key is (when the vector is non-empty)
     std::vector<HostNode> recs_;
     std::vector<int> free_;
 };
@@ -281,7 +339,7 @@ class HostThreads {
     static constexpr int kMinPerPart = 64;
     HostThreads() {
         const unsigned hw = std::thread::hardware_concurrency();
-        const int nw = hw > 1 ? (int)std::min(7u, hw - 1) : 0;
+        const int nw = hw > 1 ? (int)std::min(15u, hw - 1) : 0;
         for (int t = 0; t < nw; ++t) std::thread([this] { loop(); }).detach();
         workers_.resize(nw);
     }
@@ -344,6 +402,49 @@ std::vector<int> top_k_infeasible(const double* x, const std::vector<int>& iv, i
     std::vector<int> r(m);
     for (int i = 0; i < m; ++i) r[i] = c[i].idx;
     return r;
+}
+
+// top_k_infeasible with the same result, without the allocation and the
+// full sort when the k largest distances are distinct and none ties the k-th
+// from below (then the sorted prefix is unique); otherwise the full sort of
+// top_k_infeasible decides the order of equal distances.  `cand` is scratch.
+int top_k_infeasible_into(const double* x, const std::vector<int>& iv, int k, double tol, int* out,
+                          std::vector<std::pair<int, double>>& cand) {
+    cand.clear();
+    int best_i[16];
+    double best_d[16];
+    int m = 0;
+    for (int i : iv) {
+        const double frac = x[i] - std::floor(x[i]);
+        const double dist = std::min(frac, 1.0 - frac);
+        if (!(dist > tol)) continue;
+        cand.push_back({i, dist});
+        if (m < k || dist > best_d[m - 1]) {
+            int p = m < k ? m++ : m - 1;
+            while (p > 0 && best_d[p - 1] < dist) {
+                best_d[p] = best_d[p - 1];
+                best_i[p] = best_i[p - 1];
+                --p;
+            }
+            best_d[p] = dist;
+            best_i[p] = i;
+        }
+    }
+    bool tie = false;
+    for (int c = 1; c < m && !tie; ++c) tie = best_d[c] == best_d[c - 1];
+    if (!tie && m == k)
+        for (const auto& cd : cand)
+            if (cd.second == best_d[m - 1] && cd.first != best_i[m - 1]) {
+                tie = true;
+                break;
+            }
+    if (tie) {
+        const std::vector<int> r = top_k_infeasible(x, iv, k, tol);
+        for (size_t c = 0; c < r.size(); ++c) out[c] = r[c];
+        return (int)r.size();
+    }
+    for (int c = 0; c < m; ++c) out[c] = best_i[c];
+    return m;
 }
 
 // Single most fractional variable (branching.h:24-36), ignoring variables
@@ -1034,11 +1135,13 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             scan_vars.assign((size_t)N * bk, -1);
             HostThreads::get().parallel_for(N, [&](int i) {
                 if (db.hst[i] >= GB_NODE_INTEGRAL_UNCONVERGED) return;
+                // pruned below whatever incumbent the pass finds (it only improves)
+                if (!(certified_fw_lower_bound(batch[i].bound, db.hval[i], db.hgap[i]) < inc)) return;
                 const double* xr = db.hx + (size_t)i * n;
                 int* out = scan_vars.data() + (size_t)i * bk;
                 if (s->branch_rule == GB_BRANCH_POW2) {
-                    const std::vector<int> v = top_k_infeasible(xr, iv, s->branch_k, s->integer_tol);
-                    for (size_t c = 0; c < v.size(); ++c) out[c] = v[c];
+                    thread_local std::vector<std::pair<int, double>> cand;
+                    top_k_infeasible_into(xr, iv, s->branch_k, s->integer_tol, out, cand);
                 } else {
                     out[0] = most_fractional(xr, iv, s->integer_tol);
                 }
