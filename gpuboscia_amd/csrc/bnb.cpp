@@ -605,7 +605,9 @@ struct DeviceBatch {
         n = nv;
         learned_io = learned;
         topk_k = k;
-        if (!stream) GB_HIP(hipStreamCreate(&stream));
+        // each batch set on a hardware queue of its own, so the two batches
+        // in flight overlap on the device
+        if (!stream) GB_HIP(create_dedicated_stream(&stream));
         const size_t bytes = offsets(cap).end;
         if (bytes > pack_bytes) {
             GB_HIP(hipStreamSynchronize(stream));

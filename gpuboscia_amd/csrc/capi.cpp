@@ -1,6 +1,7 @@
 // C-ABI entry points: descriptor validation, workspace management and the
 // FP64 batch relaxation launch (gb_relax_batch).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <atomic>
@@ -216,6 +217,23 @@ gb_status counter_get(int** out) {
     const unsigned slot = g_counter_next[dev].fetch_add(1u) % kCounterSlots;
     *out = g_counter[dev] + static_cast<size_t>(slot) * kCounterStride;
     return GB_OK;
+}
+
+hipError_t create_dedicated_stream(hipStream_t* out) {
+    // The runtime gives a stream with a CU mask a hardware queue of its own
+    // instead of the least-used queue of its shared pool.
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if (cus > 0) {
+        std::vector<uint32_t> mask(static_cast<size_t>(cus + 31) / 32, 0u);
+        for (int c = 0; c < cus; ++c) mask[c / 32] |= 1u << (c % 32);
+        if (hipExtStreamCreateWithCUMask(out, static_cast<uint32_t>(mask.size()), mask.data()) == hipSuccess)
+            return hipSuccess;
+        (void)hipGetLastError();
+    }
+    return hipStreamCreate(out);
 }
 
 gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* a,

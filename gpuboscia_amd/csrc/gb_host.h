@@ -38,6 +38,12 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
                            const gb_fast_options* opt, void* stream, int* counter = nullptr,
                            bool rescore = false);
 gb_status counter_get(int** out);
+// A stream on a hardware queue of its own (all CUs enabled), for work that
+// must run concurrently with the library's other streams: plain streams
+// share a small pool of hardware queues with every stream in the process
+// (e.g. a framework's stream pool), and two streams on one queue run in
+// order.  Falls back to a plain stream if the runtime refuses.
+hipError_t create_dedicated_stream(hipStream_t* out);
 
 #define GB_HIP(call)                                                   \
     do {                                                               \

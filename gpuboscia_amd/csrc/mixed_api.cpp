@@ -355,7 +355,7 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
         while ((int)F.streams.size() < concurrency) {
             hipStream_t s = nullptr;
             hipEvent_t e = nullptr;
-            GB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            GB_HIP(create_dedicated_stream(&s));  // batches run concurrently
             GB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             F.streams.push_back(s);
             F.done.push_back(e);

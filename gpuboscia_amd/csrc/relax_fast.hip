@@ -21,6 +21,10 @@
 //     final step (device_resident_solver.cuh:588-621).
 // Vertex and rounded-candidate scoring follow drs_integer_value and the
 // rounding heuristic (device_resident_solver.cuh:297-335).
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "gb_device.h"
 #include "gb_fast.h"
 
@@ -1045,22 +1049,24 @@ static size_t rows_lds_bytes(int n_pad, int nt) {
     return doubles * 8 + ints * 4;
 }
 
-// Nodes relax_fast_rows_kernel<NT> keeps in flight on this device.
+// Nodes relax_fast_rows_kernel<NT> keeps in flight on this device with
+// `lds` bytes of LDS per workgroup (occupancy x CUs), cached per device and
+// LDS size.
 template <int NT>
 static int rows_capacity(size_t lds) {
-    static int cached[2] = {-1, -1};  // [device 0 / other]: blocks per CU x CUs
+    static std::mutex mu;
+    static std::map<std::pair<int, size_t>, int> cached;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    int& c = cached[dev == 0 ? 0 : 1];
-    if (c < 0 || dev != 0) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_fast_rows_kernel<NT>, NT * kWave, lds) !=
-                hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            return 0;
-        c = per_cu * cus;
-    }
-    return c;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = cached.find({dev, lds});
+    if (it != cached.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_fast_rows_kernel<NT>, NT * kWave, lds) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return cached[{dev, lds}] = per_cu * cus;
 }
 
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream) {
