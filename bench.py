@@ -17,8 +17,8 @@ statuses (fill kernels) and relaxes the batch again.
   scale      the mixed form on 16384 nodes (16 batches in flight)
   roofline   the BF16 MFMA kernel (relax_mixed_kernel): algorithmic flops
              2 n^2 per FW iteration / its launch time, against 2.5 PFLOP/s
-  cpu_baseline  the CPU oracle (FP64 restatement of the plain form) on a
-             bounded sample of the same nodes, 16 host threads
+  cpu_baseline  the CPU oracle (FP64 restatement of the plain form) on the
+             same 1024 nodes and budget (about 8 s), 16 host threads
   time_to_optimum  gb_solve to the proven optimum (abs = rel gap 1e-6) on
              binary QPs of n = 20, 40, 60, 80, 100 (three seeds each): plain
              vs batched vs mixed vs learned + mixed, and the geometric-mean
@@ -171,8 +171,8 @@ def traffic_bytes():
 
 
 def cpu_baseline(args, Q, c, LB, UB, EPS):
-    """The CPU oracle (FP64 restatement of the plain form) on a bounded sample
-    of the same node batch, node-parallel on host threads."""
+    """The CPU oracle (FP64 restatement of the plain form) on the node batch
+    (or its first --cpu-nodes nodes), node-parallel on host threads."""
     from oracle import pyoracle as orc
     threads = max(1, min(16, os.cpu_count() or 1))
     sample = min(len(LB), max(threads, args.cpu_nodes))
@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--nodes", type=int, default=1024)
     ap.add_argument("--n", type=int, default=200)
     ap.add_argument("--budget", type=int, default=200)
-    ap.add_argument("--cpu-nodes", type=int, default=32)
+    ap.add_argument("--cpu-nodes", type=int, default=1024)
     ap.add_argument("--scale-nodes", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tto", action="store_true")
