@@ -59,7 +59,8 @@ constexpr int kWorkVectors = 6;  // g, v, a, trial_x, trial_g, qap residual
 // LDS layout of the FP64 kernel (bytes; offsets 16-byte aligned).
 struct LdsLayout {
     size_t red, ired, vec, sg, sidx, scap, spre, elig, fixed;
-    size_t fcol, frow, rows, cols, hu, hv, minv, hp, way, used, misc;
+    size_t fcol, frow, rows, cols, hu, hp, way, misc;
+    size_t hcost, hcost_bytes;  // staged residual Hungarian costs (0 bytes: not staged)
     size_t total;
 };
 
@@ -90,12 +91,16 @@ inline LdsLayout lds_layout(int obj_kind, int lmo_kind, int n, int order) {
         L.rows = take((size_t)m * sizeof(int));
         L.cols = take((size_t)m * sizeof(int));
         L.hu = take((size_t)(m + 1) * sizeof(double));
-        L.hv = take((size_t)(m + 1) * sizeof(double));
-        L.minv = take((size_t)(m + 1) * sizeof(double));
         L.hp = take((size_t)(m + 1) * sizeof(int));
         L.way = take((size_t)(m + 1) * sizeof(int));
-        L.used = take((size_t)(m + 1));
         L.misc = take(8 * sizeof(int));
+        // The residual cost matrix (at most m x m) in LDS when it fits
+        // beside the rest of the layout in one CU's 160 KB.
+        const size_t cb = (size_t)m * m * sizeof(double);
+        if (off + cb <= 160 * 1024) {
+            L.hcost = take(cb);
+            L.hcost_bytes = cb;
+        }
     }
     L.total = off;
     return L;

@@ -27,12 +27,10 @@ struct Lds {
     int* rows;
     int* cols;
     double* hu;
-    double* hv;
-    double* minv;
     int* hp;
     int* way;
-    uint8_t* used;
     int* misc;
+    double* hcost;  // residual Hungarian costs (null: read from global)
 };
 
 __device__ __forceinline__ int next_pow2(int v) { int r = 1; while (r < v) r <<= 1; return r; }
@@ -460,18 +458,75 @@ struct Lmo<GB_LMO_PORTFOLIO> {
     }
 };
 
+// (value, index) minimum as in `better`, without branches.
+__device__ __forceinline__ MinPair better_sel(MinPair a, MinPair b) {
+    const bool take = (b.i >= 0) & ((a.i < 0) | (b.v < a.v) | ((b.v == a.v) & (b.i < a.i)));
+    return take ? b : a;
+}
+
+// Wave-wide reductions by DPP within rows of 16 lanes (quad permutes,
+// half-row and row mirrors) and lane reads across the 4 rows; results are
+// wave-uniform.
+template <int CTRL>
+__device__ __forceinline__ int dpp_int(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+    return __hiloint2double(dpp_int<CTRL>(__double2hiint(v)), dpp_int<CTRL>(__double2loint(v)));
+}
+__device__ __forceinline__ double lane_double(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_min_double(double v) {
+    v = fmin(v, dpp_double<0xB1>(v));   // lane ^ 1
+    v = fmin(v, dpp_double<0x4E>(v));   // lane ^ 2
+    v = fmin(v, dpp_double<0x141>(v));  // mirror within 8
+    v = fmin(v, dpp_double<0x140>(v));  // mirror within 16
+    return fmin(fmin(lane_double(v, 0), lane_double(v, 16)), fmin(lane_double(v, 32), lane_double(v, 48)));
+}
+__device__ __forceinline__ int wave_min_int(int v) {
+    v = min(v, dpp_int<0xB1>(v));
+    v = min(v, dpp_int<0x4E>(v));
+    v = min(v, dpp_int<0x141>(v));
+    v = min(v, dpp_int<0x140>(v));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// (value, index) argmin of wave 0 over one candidate per lane (index < 0:
+// none; values are never NaN): the smallest value, then the lowest index
+// among the lanes holding it (+0 and -0 tie, as `better` compares them),
+// and that entry's own value.  Equal to wave_argmin.  Lane l holds indices
+// congruent to l + 1 modulo 64.
+__device__ __forceinline__ MinPair wave_argmin_2p(MinPair p) {
+    const double inf = __builtin_inf();
+    const double vm = wave_min_double(p.i >= 0 ? p.v : inf);
+    if (!(vm < inf)) return MinPair{kNone, -1};
+    const int im = wave_min_int(p.i >= 0 && p.v == vm ? p.i : 0x7fffffff);
+    return MinPair{lane_double(p.v, (im - 1) & (kWave - 1)), im};
+}
+
 // BirkhoffLMO (lmo.cuh:918-1419).  Forced edges (lb > 1/2, or face entries
 // >= 1 - 1e-9) are fixed; the residual assignment is solved by the
-// shortest-augmenting-path Hungarian method on wave 0 alone: each lane owns
-// columns j = lane+1, lane+65, ...; the column choice is an exact
+// shortest-augmenting-path Hungarian method (hungarian_cpu,
+// lmo.cuh:1063-1171) on wave 0: lane l owns the residual columns
+// j = l + 1 + 64 t and keeps their minv, v potential, used flag and path
+// link in registers; the residual costs are staged in LDS by the whole
+// workgroup when they fit (forbidden entries as +inf, which no comparison
+// accepts, exactly as skipping them).  The column choice is an exact
 // (value, index) argmin, so ties resolve to the lowest index as in
 // hungarian_cpu (lmo.cuh:1143-1146).  No block barrier inside the solve.
 template <>
 struct Lmo<GB_LMO_BIRKHOFF> {
-    // Runs on wave 0; returns a wave-uniform validity flag.
-    __device__ static bool hungarian(const RelaxParams& P, const Lds& s, const double* cost,
-                                     const double* lb, const double* ub, const double* face,
-                                     double* out) {
+    // Wave 0: forced edges into `out` and the residual rows and columns
+    // (s.rows, s.cols, k = s.misc[0]).  Returns a wave-uniform flag; false
+    // for an invalid forced set (with the forced writes the reference makes
+    // before it fails) or a non-square residual.
+    __device__ static bool forced_edges(const RelaxParams& P, const Lds& s, const double* lb,
+                                        const double* ub, const double* face, double* out) {
         const int m = P.lorder;
         const int lane = threadIdx.x;
         for (int i = lane; i < m; i += kWave) { s.fcol[i] = -1; s.frow[i] = -1; }
@@ -526,62 +581,167 @@ struct Lmo<GB_LMO_BIRKHOFF> {
             s.misc[1] = (k == kc) ? 1 : 0;
         }
         wave_sync();
-        const int k = s.misc[0];
-        if (!s.misc[1]) return false;
-        for (int j = lane; j <= k; j += kWave) { s.hu[j] = 0.0; s.hv[j] = 0.0; s.hp[j] = 0; }
+        return s.misc[1] != 0;
+    }
+
+    // Wave 0: assignment of the k residual rows (T = ceil(k / 64) columns
+    // per lane; STAGED: costs in LDS at hcost_off).  Lane l keeps, for its
+    // columns j, minv, the v potential, the used flag, the path link, the
+    // matched row hp[j] and that row's u potential in registers; the row of
+    // the next column and its potential are read from the owning lane, so a
+    // search step touches LDS only for the cost row.  LDS arrays are
+    // addressed from the kernel's dynamic LDS (offsets by value), and the
+    // row potentials are written back to LDS once per row.  Returns a
+    // wave-uniform validity flag.
+    template <int T, bool STAGED>
+    __device__ __noinline__ static bool assign(int m, int k, unsigned hu_off, unsigned hp_off, unsigned way_off,
+                                               unsigned rows_off, unsigned cols_off, unsigned hcost_off,
+                                               const double* cost, const double* ub, const double* face,
+                                               double* out) {
+        extern __shared__ __align__(16) unsigned char lds_raw[];
+        double* hu = reinterpret_cast<double*>(lds_raw + hu_off);
+        int* hp = reinterpret_cast<int*>(lds_raw + hp_off);
+        int* wy = reinterpret_cast<int*>(lds_raw + way_off);
+        const int* rows = reinterpret_cast<const int*>(lds_raw + rows_off);
+        const int* cols = reinterpret_cast<const int*>(lds_raw + cols_off);
+        const double* hcost = reinterpret_cast<const double*>(lds_raw + hcost_off);
+        const int lane = threadIdx.x;
+        const double inf = __builtin_inf();
+        double minv[T], hv[T], uj[T];
+        int way[T], hpj[T];
+        bool used[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) { hv[t] = 0.0; way[t] = 0; }
+        for (int j = lane; j <= k; j += kWave) { hu[j] = 0.0; hp[j] = 0; }
         wave_sync();
         for (int i = 1; i <= k; ++i) {
-            if (lane == 0) s.hp[0] = i;
-            for (int j = lane; j <= k; j += kWave) { s.minv[j] = kNone; s.used[j] = 0; }
-            wave_sync();
-            int cur = 0;
+            if (lane == 0) hp[0] = i;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int j = kWave * t + lane + 1;
+                minv[t] = kNone;
+                used[t] = false;
+                hpj[t] = j <= k ? hp[j] : 0;
+                uj[t] = hpj[t] > 0 ? hu[hpj[t]] : 0.0;
+            }
+            double ui = hu[i];  // u of row i (column 0, the root of the tree)
+            int cur = 0, row = i;
+            double urow = ui;
             while (true) {
-                if (lane == 0) s.used[cur] = 1;
-                wave_sync();
-                const int row = s.hp[cur];
-                const size_t rbase = (size_t)s.rows[row - 1] * m;
-                const double urow = s.hu[row];
+#pragma unroll
+                for (int t = 0; t < T; ++t)
+                    if (kWave * t + lane + 1 == cur) used[t] = true;
+                const double* crow = STAGED ? hcost + (size_t)(row - 1) * k : nullptr;
+                const size_t rbase = STAGED ? 0 : (size_t)rows[row - 1] * m;
                 MinPair best{kNone, -1};
-                for (int j = lane + 1; j <= k; j += kWave) {
-                    if (s.used[j]) continue;
-                    const size_t e = rbase + s.cols[j - 1];
-                    if (ub[e] >= 0.5 && (!face || face[e] > 1e-9)) {
-                        const double c = cost[e] - urow - s.hv[j];
-                        if (c < s.minv[j]) { s.minv[j] = c; s.way[j] = cur; }
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const int j = kWave * t + lane + 1;
+                    if (j <= k && !used[t]) {
+                        double cr;
+                        if (STAGED) {
+                            cr = crow[j - 1];
+                        } else {
+                            const size_t e = rbase + cols[j - 1];
+                            cr = (ub[e] >= 0.5 && (!face || face[e] > 1e-9)) ? cost[e] : inf;
+                        }
+                        const double c = cr - urow - hv[t];
+                        if (c < minv[t]) { minv[t] = c; way[t] = cur; }
+                        if (minv[t] < kNone) best = better_sel(best, MinPair{minv[t], j});
                     }
-                    if (s.minv[j] < kNone) best = better(best, MinPair{s.minv[j], j});
                 }
-                best = wave_argmin(best);
+                best = wave_argmin_2p(best);
                 if (best.i < 0 || !isfinite(best.v)) { wave_sync(); return false; }
                 const double delta = best.v;
-                for (int j = lane; j <= k; j += kWave) {
-                    if (s.used[j]) { s.hu[s.hp[j]] = s.hu[s.hp[j]] + delta; s.hv[j] = s.hv[j] - delta; }
-                    else s.minv[j] = s.minv[j] - delta;
+                ui = ui + delta;
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    if (kWave * t + lane + 1 <= k) {
+                        if (used[t]) {
+                            uj[t] = uj[t] + delta;
+                            hv[t] = hv[t] - delta;
+                        } else {
+                            minv[t] = minv[t] - delta;
+                        }
+                    }
                 }
-                wave_sync();
                 cur = best.i;
-                if (s.hp[cur] == 0) break;
+                const int owner = (cur - 1) & (kWave - 1), tc = (cur - 1) / kWave;
+                int hsel = hpj[0];
+                double usel = uj[0];
+#pragma unroll
+                for (int t = 1; t < T; ++t)
+                    if (tc == t) { hsel = hpj[t]; usel = uj[t]; }
+                const int next = __builtin_amdgcn_readlane(hsel, owner);
+                if (next == 0) break;
+                row = next;
+                urow = lane_double(usel, owner);
             }
+            // row potentials back to LDS, then augment along the recorded path
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int j = kWave * t + lane + 1;
+                if (j <= k) {
+                    if (hpj[t] > 0) hu[hpj[t]] = uj[t];
+                    wy[j] = way[t];
+                }
+            }
+            if (lane == 0) hu[i] = ui;
+            wave_sync();
             if (lane == 0) {
-                do { const int prev = s.way[cur]; s.hp[cur] = s.hp[prev]; cur = prev; } while (cur != 0);
+                do { const int prev = wy[cur]; hp[cur] = hp[prev]; cur = prev; } while (cur != 0);
             }
             wave_sync();
         }
-        for (int j = lane + 1; j <= k; j += kWave) out[(size_t)s.rows[s.hp[j] - 1] * m + s.cols[j - 1]] = 1.0;
+        for (int j = lane + 1; j <= k; j += kWave) out[(size_t)rows[hp[j] - 1] * m + cols[j - 1]] = 1.0;
         wave_sync();
         return true;
     }
+
     __device__ static bool solve(const RelaxParams& P, const Lds& s, const double* cost,
                                  const double* lb, const double* ub, const double* face, double* out) {
         for (int e = threadIdx.x; e < P.n; e += blockDim.x) out[e] = 0.0;
         __syncthreads();
         if (P.n != P.lorder * P.lorder) return false;
         if (threadIdx.x < kWave) {
-            const bool ok = hungarian(P, s, cost, lb, ub, face, out);
+            const bool ok = forced_edges(P, s, lb, ub, face, out);
             if (threadIdx.x == 0) s.misc[2] = ok ? 1 : 0;
         }
         __syncthreads();
-        const bool ok = s.misc[2] != 0;
+        bool ok = s.misc[2] != 0;
+        if (ok) {
+            const int m = P.lorder;
+            const int k = s.misc[0];
+            if (s.hcost) {
+                const double inf = __builtin_inf();
+                for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+                    const int r = e / k, c = e - r * k;
+                    const size_t E = (size_t)s.rows[r] * m + s.cols[c];
+                    s.hcost[e] = (ub[E] >= 0.5 && (!face || face[E] > 1e-9)) ? cost[E] : inf;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x < kWave) {
+                extern __shared__ __align__(16) unsigned char lds_raw[];
+                auto off = [&](const void* p) {
+                    return (unsigned)(reinterpret_cast<const unsigned char*>(p) - lds_raw);
+                };
+                const unsigned o_hu = off(s.hu), o_hp = off(s.hp), o_wy = off(s.way), o_r = off(s.rows),
+                               o_c = off(s.cols), o_q = s.hcost ? off(s.hcost) : 0u;
+#define GB_ASSIGN(TT)                                                                            \
+    (s.hcost ? assign<TT, true>(m, k, o_hu, o_hp, o_wy, o_r, o_c, o_q, cost, ub, face, out)       \
+             : assign<TT, false>(m, k, o_hu, o_hp, o_wy, o_r, o_c, o_q, cost, ub, face, out))
+                bool r;
+                if (k <= kWave) r = GB_ASSIGN(1);
+                else if (k <= 2 * kWave) r = GB_ASSIGN(2);
+                else if (k <= 3 * kWave) r = GB_ASSIGN(3);
+                else r = GB_ASSIGN(4);
+#undef GB_ASSIGN
+                if (threadIdx.x == 0) s.misc[2] = r ? 1 : 0;
+            }
+            __syncthreads();
+            ok = s.misc[2] != 0;
+        }
         __syncthreads();
         return ok;
     }
@@ -732,12 +892,10 @@ __global__ void __launch_bounds__(1024) relax_fp64_kernel(RelaxParams P, LdsLayo
     s.rows = reinterpret_cast<int*>(lds_raw + LL.rows);
     s.cols = reinterpret_cast<int*>(lds_raw + LL.cols);
     s.hu = reinterpret_cast<double*>(lds_raw + LL.hu);
-    s.hv = reinterpret_cast<double*>(lds_raw + LL.hv);
-    s.minv = reinterpret_cast<double*>(lds_raw + LL.minv);
     s.hp = reinterpret_cast<int*>(lds_raw + LL.hp);
     s.way = reinterpret_cast<int*>(lds_raw + LL.way);
-    s.used = lds_raw + LL.used;
     s.misc = reinterpret_cast<int*>(lds_raw + LL.misc);
+    s.hcost = LL.hcost_bytes ? reinterpret_cast<double*>(lds_raw + LL.hcost) : nullptr;
 
     using O = Obj<OBJ>;
     using L = Lmo<LMO>;

@@ -118,6 +118,24 @@ def test_qap_birkhoff_parity():
         compare(prob, alg, LB, UB, np.full(N, 1e-6), np.ones(n, np.uint8), 40, label="qap")
 
 
+@pytest.mark.parametrize("m", [72, 136, 200])
+def test_qap_birkhoff_parity_large(m):
+    """Hungarian solves with 2, 3 and 4 residual columns per lane; the
+    residual costs are staged in LDS up to m = 136 and read from global
+    memory at m = 200.  The initial vertex's cost (g_art) makes every
+    permutation optimal and the feasibility check's zero cost ties every
+    entry, so the tie rule is exercised as well as generic gradients."""
+    A, B, perm, _ = instances.qap_instance(m, seed=m)
+    prob = Problem("qap", A=A, B=B, lmo="birkhoff", order=m)
+    n, N = m * m, 3
+    rng = np.random.default_rng(m)
+    LB, UB = np.zeros((N, n)), np.ones((N, n))
+    LB[1, 5] = 1.0                                    # forced edge (0, 5)
+    UB[2, rng.choice(n, n // 10, replace=False)] = 0.0  # forbidden edges
+    compare(prob, {"dicg": True, "ls": gb.LS_QUADRATIC_EXACT}, LB, UB, np.full(N, 1e-6),
+            np.ones(n, np.uint8), 3, label=f"qap{m}")
+
+
 def test_infeasible_and_mixed_batch():
     Q, c, _ = instances.binary_qp(20, seed=2, rows=30)
     prob = Problem("dense", Q, c)
