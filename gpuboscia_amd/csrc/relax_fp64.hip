@@ -103,35 +103,72 @@ struct Obj<GB_OBJ_DENSE_QUADRATIC> {
 };
 
 // QAP f(P) = ||A P - P B||_F^2 on an order-m Birkhoff polytope.  R = AP - PB
-// is written to the node's residual vector; grad = 2 (A'R - R B').
+// is written to the node's residual vector; grad = 2 (A'R - R B').  Each
+// thread computes blocks of kBI x kBJ outputs, loading kBI + kBJ values of
+// each operand per k for 2 kBI kBJ FMAs; every output still accumulates its
+// products in k order with fma.  Block edges clamp the loads and skip the
+// stores.
 template <>
 struct Obj<GB_OBJ_QAP> {
-    __device__ static void residual(const RelaxParams& P, double* R, const double* X) {
-        const int m = P.order;
-        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-            const int i = e / m, j = e - i * m;
-            double ap = 0.0, pb = 0.0;
+    static constexpr int kBI = 2, kBJ = 4;
+
+    // out = 2 (L M' - M N') style products of one pass over the blocks:
+    // RES: out = A X - X B (ap - pb);  GRAD: out = 2 (A' R - R B').
+    // Out of line with operands by value, so the blocked loop has the
+    // registers to itself.
+    template <bool GRAD>
+    __device__ __noinline__ static void blocks(int m, const double* A, const double* B, const double* X,
+                                               double* out) {
+        const int nbi = (m + kBI - 1) / kBI, nbj = (m + kBJ - 1) / kBJ;
+        for (int blk = threadIdx.x; blk < nbi * nbj; blk += blockDim.x) {
+            const int i0 = (blk / nbj) * kBI, j0 = (blk % nbj) * kBJ;
+            int ii[kBI], jj[kBJ];
+#pragma unroll
+            for (int a = 0; a < kBI; ++a) ii[a] = min(i0 + a, m - 1);
+#pragma unroll
+            for (int b = 0; b < kBJ; ++b) jj[b] = min(j0 + b, m - 1);
+            double s1[kBI][kBJ], s2[kBI][kBJ];
+#pragma unroll
+            for (int a = 0; a < kBI; ++a)
+#pragma unroll
+                for (int b = 0; b < kBJ; ++b) { s1[a][b] = 0.0; s2[a][b] = 0.0; }
+#pragma unroll 2
             for (int k = 0; k < m; ++k) {
-                ap = fma(P.A[i * m + k], X[k * m + j], ap);
-                pb = fma(X[i * m + k], P.B[k * m + j], pb);
+                double l1[kBI], l2[kBI], r1[kBJ], r2[kBJ];
+#pragma unroll
+                for (int a = 0; a < kBI; ++a) {
+                    if (GRAD) { l1[a] = A[k * m + ii[a]]; l2[a] = X[ii[a] * m + k]; }   // A[k][i], R[i][k]
+                    else { l1[a] = A[ii[a] * m + k]; l2[a] = X[ii[a] * m + k]; }        // A[i][k], X[i][k]
+                }
+#pragma unroll
+                for (int b = 0; b < kBJ; ++b) {
+                    if (GRAD) { r1[b] = X[k * m + jj[b]]; r2[b] = B[jj[b] * m + k]; }   // R[k][j], B[j][k]
+                    else { r1[b] = X[k * m + jj[b]]; r2[b] = B[k * m + jj[b]]; }        // X[k][j], B[k][j]
+                }
+#pragma unroll
+                for (int a = 0; a < kBI; ++a)
+#pragma unroll
+                    for (int b = 0; b < kBJ; ++b) {
+                        s1[a][b] = fma(l1[a], r1[b], s1[a][b]);
+                        s2[a][b] = fma(l2[a], r2[b], s2[a][b]);
+                    }
             }
-            R[e] = ap - pb;
+#pragma unroll
+            for (int a = 0; a < kBI; ++a)
+#pragma unroll
+                for (int b = 0; b < kBJ; ++b)
+                    if (i0 + a < m && j0 + b < m)
+                        out[(i0 + a) * m + j0 + b] = GRAD ? 2.0 * (s1[a][b] - s2[a][b]) : s1[a][b] - s2[a][b];
         }
+    }
+    __device__ static void residual(const RelaxParams& P, double* R, const double* X) {
+        blocks<false>(P.order, P.A, P.B, X, R);
         __syncthreads();
     }
     __device__ static void grad(const RelaxParams& P, const Lds&, double* R, const double* x,
                                 double* g) {
         residual(P, R, x);
-        const int m = P.order;
-        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-            const int i = e / m, j = e - i * m;
-            double atr = 0.0, rbt = 0.0;
-            for (int k = 0; k < m; ++k) {
-                atr = fma(P.A[k * m + i], R[k * m + j], atr);
-                rbt = fma(R[i * m + k], P.B[j * m + k], rbt);
-            }
-            g[e] = 2.0 * (atr - rbt);
-        }
+        blocks<true>(P.order, P.A, P.B, R, g);
         __syncthreads();
     }
     __device__ static double value(const RelaxParams& P, const Lds& s, double* R, const double* x) {
