@@ -191,6 +191,71 @@ def cpu_baseline(args, Q, c, LB, UB, EPS):
                       f"in {dt:.2f} s (plain-form algorithm, FP64)"}
 
 
+def hetero_batches(args):
+    """Heterogeneous batch: args.c3_instances seeded binary QPs with n
+    log-uniform in [32, 512] (instances.batched_sizes), box LMO, args.c3_nodes
+    open nodes each (node_batch), relaxed by ONE gb_relax_instances call
+    over 8 streams: the mixed form (3 bf16 planes, FP64 certification) for
+    n <= 256 and the batched FP64 form above.  Budget args.budget iterations
+    per node; the HIP events on the calling stream bracket each call."""
+    import torch
+
+    import gpuboscia_amd as gb
+    from gpuboscia_amd import instances
+    sizes = instances.batched_sizes(args.c3_instances, seed=1)
+    N = args.c3_nodes
+    f64 = dict(dtype=torch.float64, device="cuda")
+    stream = torch.cuda.Stream()
+    entries, host = [], []
+    for k, n in enumerate(sizes):
+        n = int(n)
+        Q, c, _ = instances.binary_qp(n, seed=100000 + k)
+        LB, UB, EPS = node_batch(n, N, seed=200000 + k)
+        dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
+        dc = torch.from_numpy(c).cuda()
+        obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
+        e = {"obj": obj, "lmo": gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), "keep": (dQt, dc),
+             "mixed": gb.MixedQ(obj, 3, stream.cuda_stream) if n <= 256 else None,
+             "x": torch.from_numpy(LB).cuda(), "lb": torch.from_numpy(LB).cuda(), "ub": torch.from_numpy(UB).cuda(),
+             "eps": torch.from_numpy(EPS).cuda(), "iterations": torch.zeros(N, dtype=torch.int32, device="cuda"),
+             "status": torch.zeros(N, dtype=torch.uint8, device="cuda"), "gap": torch.zeros(N, **f64),
+             "value": torch.zeros(N, **f64), "best_vertex": torch.zeros((N, n), **f64),
+             "best_vertex_val": torch.full((N,), 1e300, **f64),
+             "flags": torch.ones(n, dtype=torch.uint8, device="cuda"), "cutoff": None}
+        entries.append(e)
+        host.append(LB)
+    alg = gb.algorithm(dicg=True)
+
+    def reset():
+        for e, LB in zip(entries, host):
+            e["x"].copy_(torch.from_numpy(LB))
+            e["iterations"].zero_()
+            e["status"].zero_()
+            e["best_vertex_val"].fill_(1e300)
+
+    times, iters = [], 0
+    with torch.cuda.stream(stream):
+        for step in range(args.c3_steps + 1):
+            reset()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            gb.relax_instances(entries, alg, args.budget, concurrency=8, stream=stream.cuda_stream)
+            b.record(stream)
+            stream.synchronize()
+            if step > 0:  # the first call is the warm-up
+                times.append(a.elapsed_time(b) / 1e3)
+        iters = int(sum(int(e["iterations"].sum().item()) for e in entries))
+        nodes = len(entries) * N
+        # statuses 1 and 4: fractional / integral, converged to the node tolerance
+        converged = int(sum(int(((e["status"] == 1) | (e["status"] == 4)).sum().item()) for e in entries))
+    t = float(np.mean(times))
+    return {"instances": len(entries), "nodes_per_instance": N, "n_min": int(sizes.min()),
+            "n_max": int(sizes.max()), "n_mixed_form": int((sizes <= 256).sum()), "budget": args.budget,
+            "call_ms": t * 1e3, "fw_iterations_per_call": iters, "value": iters / t, "unit": "FW iterations/s",
+            "nodes_per_s": nodes / t, "converged_nodes": converged}
+
+
 def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0):
     import torch
     dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
@@ -262,6 +327,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tto", action="store_true")
     ap.add_argument("--no-scale", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-instances", type=int, default=256)
+    ap.add_argument("--c3-nodes", type=int, default=256)
+    ap.add_argument("--c3-steps", type=int, default=3)
     ap.add_argument("--forms", default="plain,batched,mixed,mixed_fp32,mixed_pass1",
                     help="forms to time (the profiled runs time one)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -338,6 +407,11 @@ def main():
                            "batched": {"value": sc["batched"]["value"], "launch_ms": sc["batched"]["launch_ms"]}}
         say(f"scale {args.scale_nodes}: mixed {sc['mixed']['value']:.4g} batched {sc['batched']['value']:.4g}")
         del sb
+    if not args.no_c3:
+        result["heterogeneous_batch"] = hetero_batches(args)
+        h = result["heterogeneous_batch"]
+        say(f"heterogeneous batch: {h['instances']} instances x {h['nodes_per_instance']} nodes, n {h['n_min']}.."
+            f"{h['n_max']}: {h['call_ms']:.1f} ms per call, {h['value']:.4g} FW it/s, {h['nodes_per_s']:.4g} nodes/s")
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, Q, c, LB, UB, EPS)
     if not args.no_tto:

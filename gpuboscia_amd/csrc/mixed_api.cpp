@@ -364,19 +364,31 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
         hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
         GB_HIP(hipEventRecord(F.start, caller));
         const int k = concurrency < count ? concurrency : count;
+        int cus = 0;
+        GB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         for (int j = 0; j < k; ++j) GB_HIP(hipStreamWaitEvent(F.streams[j], F.start, 0));
         for (int i = 0; i < count; ++i) {
             const gb_instance_batch& b = batches[i];
             void* s = F.streams[i % k];
             gb_status st;
-            if (b.mixed)
+            if (b.mixed) {
+                // k entries share the CUs: widen the tiles so that together
+                // they cover the chip once, within the default shape's
+                // rows-per-lane class (same arithmetic, so each entry still
+                // equals its solo call).
+                gb_mixed_options mo{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
+                const int def = mixed_default_shape(b.mixed->rt, b.N).cols;
+                const int cap = 4 * mixed_rows_per_lane(def);
+                const int want = cus > 0 ? (int)(((long long)b.N * k + cus - 1) / cus) : def;
+                mo.columns = want < def ? def : (want > cap ? cap : want);
                 st = relax_mixed_impl(b.mixed, b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
-                                      b.integer_flags, candidate_tol, b.cutoff, nullptr, s);
-            else
+                                      b.integer_flags, candidate_tol, b.cutoff, &mo, s);
+            } else {
                 st = relax_batch_fast(b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
                                       b.integer_flags, candidate_tol, b.cutoff, nullptr, s);
+            }
             if (st != GB_OK) return st;
         }
         for (int j = 0; j < k; ++j) {

@@ -116,3 +116,10 @@ def portfolio(assets: int = 100, seed: int = 1, periods: int = 600, factors: int
     c = np.zeros(n)
     c[:m] = -lam * mu
     return Q, c, Sigma, mu, sha256(Q, c)
+
+
+def batched_sizes(count: int = 256, seed: int = 1, lo: int = 32, hi: int = 512) -> np.ndarray:
+    """Sizes of the heterogeneous-batch configuration: n_i = round(exp U(ln lo,
+    ln hi)), i < count; instance i is binary_qp(n_i, seed * 100000 + i)."""
+    u = uniform(seed, 21, count, math.log(lo), math.log(hi))
+    return np.rint(np.exp(u)).astype(np.int64)
