@@ -195,8 +195,8 @@ def hetero_batches(args):
     """Heterogeneous batch: args.c3_instances seeded binary QPs with n
     log-uniform in [32, 512] (instances.batched_sizes), box LMO, args.c3_nodes
     open nodes each (node_batch), relaxed by ONE gb_relax_instances call
-    over 8 streams: the mixed form (3 bf16 planes, FP64 certification) for
-    n <= 256 and the batched FP64 form above.  Budget args.budget iterations
+    over 8 streams in the mixed form (args.c3_planes bf16 planes of Q on MFMA
+    with FP32 accumulation, then the FP64 certification pass).  Budget args.budget iterations
     per node; the HIP events on the calling stream bracket each call."""
     import torch
 
@@ -215,7 +215,7 @@ def hetero_batches(args):
         dc = torch.from_numpy(c).cuda()
         obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
         e = {"obj": obj, "lmo": gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), "keep": (dQt, dc),
-             "mixed": gb.MixedQ(obj, 3, stream.cuda_stream) if n <= 256 else None,
+             "mixed": gb.MixedQ(obj, args.c3_planes, stream.cuda_stream),
              "x": torch.from_numpy(LB).cuda(), "lb": torch.from_numpy(LB).cuda(), "ub": torch.from_numpy(UB).cuda(),
              "eps": torch.from_numpy(EPS).cuda(), "iterations": torch.zeros(N, dtype=torch.int32, device="cuda"),
              "status": torch.zeros(N, dtype=torch.uint8, device="cuda"), "gap": torch.zeros(N, **f64),
@@ -251,7 +251,7 @@ def hetero_batches(args):
         converged = int(sum(int(((e["status"] == 1) | (e["status"] == 4)).sum().item()) for e in entries))
     t = float(np.mean(times))
     return {"instances": len(entries), "nodes_per_instance": N, "n_min": int(sizes.min()),
-            "n_max": int(sizes.max()), "n_mixed_form": int((sizes <= 256).sum()), "budget": args.budget,
+            "n_max": int(sizes.max()), "precision": f"{args.c3_planes} bf16 plane(s) of Q + FP64 certification", "budget": args.budget,
             "call_ms": t * 1e3, "fw_iterations_per_call": iters, "value": iters / t, "unit": "FW iterations/s",
             "nodes_per_s": nodes / t, "converged_nodes": converged}
 
@@ -331,6 +331,7 @@ def main():
     ap.add_argument("--c3-instances", type=int, default=256)
     ap.add_argument("--c3-nodes", type=int, default=256)
     ap.add_argument("--c3-steps", type=int, default=3)
+    ap.add_argument("--c3-planes", type=int, default=3, choices=(1, 3))
     ap.add_argument("--forms", default="plain,batched,mixed,mixed_fp32,mixed_pass1",
                     help="forms to time (the profiled runs time one)")
     ap.add_argument("--copy-probe", action="store_true",

@@ -733,9 +733,9 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     if (s->form == 1 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > kMaxFastN))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "the batched form supports dense objectives over a box with n <= 512");
-    if (s->form == 2 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 256))
+    if (s->form == 2 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 512))
         return fail(GB_ERR_INVALID_ARGUMENT,
-                    "the mixed-precision form supports dense objectives over a box with n <= 256");
+                    "the mixed-precision form supports dense objectives over a box with n <= 512");
     return GB_OK;
 }
 

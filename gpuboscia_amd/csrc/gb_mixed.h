@@ -8,7 +8,10 @@
 
 namespace gb {
 
-constexpr int kMixedMaxN = 256;   // 16 row tiles of 16: x, Qx, Qv, Qa in registers
+constexpr int kMixedMaxN = 512;   // 32 row tiles of 16: x, Qx, Qv, Qa in registers
+// Up to 16 row tiles any tile shape; 17..32 row tiles run 4 row tiles per
+// wave (at most 8 waves) with the fragments streamed from L2.
+constexpr int kMixedSmallRowTiles = 16;
 constexpr int kMixedMaxPlanes = 3;
 
 // Q in MFMA A-fragment order, split into bf16 planes q = q0 + q1 + q2:
@@ -58,7 +61,8 @@ int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double 
 //  red         per-column partial sums of each wave, 8 floats;
 //  fetch, ok   fetched node ids, per-wave scope flags;
 //  qf          (optional) the Q fragments, planes x rt x rt/2 KB.
-// At most 8 + 8 + 32 + 8 KB for n <= 256, plus the fragments when copied.
+// At most 8 + 8 + 32 + 8 KB for n <= 256 and 16 + 16 + 64 + 4 KB for n <= 512,
+// plus the fragments when copied.
 struct MixedLds {
     size_t vops, aops, xch, red, fetch, ok, qf, total;
 };

@@ -29,8 +29,8 @@ gb_status check_mixed_scope(const gb_mixed_q* q, const gb_objective_desc* o, con
 MixedShape mixed_default_shape(int rt, int N) {
     MixedShape sh;
     // Row split (measured on MI355X at n = 60 and 200): one row tile per
-    // wave up to 8 waves, two beyond.
-    sh.rw = rt <= 8 ? 1 : 2;
+    // wave up to 8 waves, two up to 16 row tiles, four beyond (n > 256).
+    sh.rw = rt <= 8 ? 1 : (rt <= kMixedSmallRowTiles ? 2 : 4);
     // Columns: enough tiles to cover the 256 CUs of the chip.
     const int want = (N + 255) / 256;
     sh.cols = want < 1 ? 1 : (want > 16 ? 16 : want);
@@ -61,7 +61,7 @@ extern "C" gb_status gb_mixed_prepare(const gb_objective_desc* objective, int pl
     if (!objective || objective->kind != GB_OBJ_DENSE_QUADRATIC || !objective->Qt)
         return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision form needs a dense quadratic objective");
     if (objective->n <= 0 || objective->n > kMixedMaxN)
-        return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision form supports 1 <= n <= 256");
+        return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision form supports 1 <= n <= 512");
     if (planes != 1 && planes != kMixedMaxPlanes)
         return fail(GB_ERR_INVALID_ARGUMENT, "planes must be 1 or 3");
     auto* q = new gb_mixed_q();
@@ -146,8 +146,10 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
             MixedShape sh = mixed_default_shape(q->rt, N);
             if (op.row_tiles_per_wave) sh.rw = op.row_tiles_per_wave;
             if (op.columns) sh.cols = op.columns;
-            // one row tile per wave supports up to 8 waves (n <= 128)
+            // one row tile per wave supports up to 8 waves (n <= 128); more
+            // than 16 row tiles (n > 256) run four per wave
             if (sh.rw == 1 && q->rt > 8) sh.rw = 2;
+            if (q->rt > kMixedSmallRowTiles) sh.rw = 4;
             if (op.blocks) sh.blocks = op.blocks;
             else sh.blocks = (N + sh.cols - 1) / sh.cols;
             P.cols = sh.cols;
@@ -266,6 +268,7 @@ extern "C" gb_status gb_mixed_autotune(const gb_mixed_q* q, const gb_objective_d
         double best_t = 1e300;
         for (int rw = 1; rw <= kMixedMaxRowTilesPerWave; ++rw) {
             if (rw == 1 && q->rt > 8) continue;
+            if (rw != 4 && q->rt > kMixedSmallRowTiles) continue;
             for (int cols : {2, 4, 8, 16}) {
                 op.row_tiles_per_wave = rw;
                 op.columns = cols;

@@ -175,9 +175,12 @@ constexpr int kPhaseDone = 0, kPhaseInit = 1, kPhaseActive = 2, kPhaseFetch = 3;
 constexpr bool mixed_caches_q(int rw, int pl) { return (pl == 1 && rw <= 2) || (pl == 3 && rw == 1); }
 
 // QL: otherwise the tile copies all fragments into LDS once (when they fit)
-// and reads them from there each pass instead of from L2.
-template <int RW, int PL, bool DICG, int RPT, bool QL>
-__global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) relax_mixed_kernel(MixedParams P) {
+// and reads them from there each pass instead of from L2.  BIG: 17..32 row
+// tiles (n up to 512), 4 row tiles per wave, up to 16 k-steps.
+template <int RW, int PL, bool DICG, int RPT, bool QL, bool BIG = false>
+__global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)))
+    relax_mixed_kernel(MixedParams P) {
+    static_assert(!BIG || (RW == 4 && !QL), "wide tiles run 4 row tiles per wave from L2");
     constexpr int G = 16 / RPT;  // lanes of a column in the element layout
     constexpr int E = RW * RPT;  // elements per lane: rows 16 t_j + RPT m + r, e = RPT j + r
     constexpr unsigned long long kGroupMask = (1ull << G) - 1ull;
@@ -419,15 +422,58 @@ __global__ void __launch_bounds__(RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)) rel
                 }
             }
         };
-        switch (S) {
-            case 1: products(std::integral_constant<int, 1>{}); break;
-            case 2: products(std::integral_constant<int, 2>{}); break;
-            case 3: products(std::integral_constant<int, 3>{}); break;
-            case 4: products(std::integral_constant<int, 4>{}); break;
-            case 5: products(std::integral_constant<int, 5>{}); break;
-            case 6: products(std::integral_constant<int, 6>{}); break;
-            case 7: products(std::integral_constant<int, 7>{}); break;
-            default: products(std::integral_constant<int, 8>{}); break;
+        if constexpr (BIG) {
+            // 9..16 k-steps: a runtime loop (one k-step of fragments in
+            // flight ahead), so the unrolled chains do not hold every
+            // step's addresses in registers.
+            const uint4* fp[PL][RW];
+#pragma unroll
+            for (int p = 0; p < PL; ++p)
+#pragma unroll
+                for (int j = 0; j < RW; ++j) {
+                    const int t = w + W * j;
+                    fp[p][j] = fbase + (static_cast<size_t>(p) * RT + (t < RT ? t : 0)) * tstride;
+                }
+            bf16x8 cur[PL][RW];
+#pragma unroll
+            for (int p = 0; p < PL; ++p)
+#pragma unroll
+                for (int j = 0; j < RW; ++j) cur[p][j] = w + W * j < RT ? load_frag(fp[p][j]) : bf16x8{};
+#pragma unroll 1
+            for (int s = 0; s < S; ++s) {
+                bf16x8 nxt[PL][RW];
+                const bool more = s + 1 < S;
+#pragma unroll
+                for (int p = 0; p < PL; ++p)
+#pragma unroll
+                    for (int j = 0; j < RW; ++j)
+                        nxt[p][j] = (more && w + W * j < RT) ? load_frag(fp[p][j] + static_cast<size_t>(s + 1) * kWave)
+                                                             : bf16x8{};
+                const bf16x8 bv = __builtin_bit_cast(bf16x8, vops[(s * 16 + c) * 4 + q]);
+                const bf16x8 ba = DICG ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
+#pragma unroll
+                for (int p = 0; p < PL; ++p)
+#pragma unroll
+                    for (int j = 0; j < RW; ++j) {
+                        accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], bv, accv[j], 0, 0, 0);
+                        if (DICG) acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], ba, acca[j], 0, 0, 0);
+                    }
+#pragma unroll
+                for (int p = 0; p < PL; ++p)
+#pragma unroll
+                    for (int j = 0; j < RW; ++j) cur[p][j] = nxt[p][j];
+            }
+        } else {
+            switch (S) {
+                case 1: products(std::integral_constant<int, 1>{}); break;
+                case 2: products(std::integral_constant<int, 2>{}); break;
+                case 3: products(std::integral_constant<int, 3>{}); break;
+                case 4: products(std::integral_constant<int, 4>{}); break;
+                case 5: products(std::integral_constant<int, 5>{}); break;
+                case 6: products(std::integral_constant<int, 6>{}); break;
+                case 7: products(std::integral_constant<int, 7>{}); break;
+                default: products(std::integral_constant<int, 8>{}); break;
+            }
         }
 
         // Accumulators to the element layout: the wave's own LDS region,
@@ -635,6 +681,10 @@ int start_mixed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, con
 template <int RW, int PL, bool DI, int RPT>
 int launch_mixed_shape(const MixedParams& p, dim3 grid, dim3 block, int waves, hipStream_t st) {
     const size_t base = mixed_lds_layout(p.rt, waves, RW, 0).total;
+    if constexpr (RW == 4) {
+        if (p.rt > kMixedSmallRowTiles)
+            return start_mixed(relax_mixed_kernel<4, PL, DI, RPT, false, true>, grid, block, base, st, p);
+    }
     if constexpr (!mixed_caches_q(RW, PL)) {
         const size_t with_q = mixed_lds_layout(p.rt, waves, RW, mixed_q_bytes(p.rt, PL)).total;
         if (with_q <= kMixedLdsBudget)
@@ -644,7 +694,8 @@ int launch_mixed_shape(const MixedParams& p, dim3 grid, dim3 block, int waves, h
 }
 
 int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void* stream) {
-    if (p.cols < 1 || p.cols > 16 || rw < 1 || rw > kMixedMaxRowTilesPerWave || p.rt < 2 || p.rt > 16)
+    if (p.cols < 1 || p.cols > 16 || rw < 1 || rw > kMixedMaxRowTilesPerWave || p.rt < 2 ||
+        p.rt > 2 * kMixedSmallRowTiles || (p.rt > kMixedSmallRowTiles && rw != 4))
         return static_cast<int>(hipErrorInvalidValue);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(mixed_reset_kernel, dim3(1), dim3(1), 0, st, p.counter);
@@ -698,6 +749,14 @@ int launch_mixed_products(const MixedQ& q, const double* V, int N, double* Y, vo
         GB_PROD(12)
         GB_PROD(14)
         GB_PROD(16)
+        GB_PROD(18)
+        GB_PROD(20)
+        GB_PROD(22)
+        GB_PROD(24)
+        GB_PROD(26)
+        GB_PROD(28)
+        GB_PROD(30)
+        GB_PROD(32)
         default:
             return static_cast<int>(hipErrorInvalidValue);
     }

@@ -199,7 +199,7 @@ gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
 
 /* Q of a dense objective prepared for the BF16 MFMA products: split into
    `planes` bf16 planes (1: ~2^-8 relative per entry, 3: ~2^-24) in MFMA
-   fragment order, plus s_g s_l c in FP32.  n <= 256.  Asynchronous on
+   fragment order, plus s_g s_l c in FP32.  n <= 512.  Asynchronous on
    `stream`; the objective's Qt / linear must stay valid until it completes. */
 typedef struct gb_mixed_q gb_mixed_q;
 gb_status gb_mixed_prepare(const gb_objective_desc* objective, int planes, void* stream,
@@ -257,7 +257,7 @@ gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
  * Heterogeneous batches (no reference counterpart; the reference fixes n at
  * compile time, device_resident_solver.cuh:362, :1217-1218): independent
  * node batches of different problems and sizes relaxed in one call.  Each
- * entry is relaxed like gb_relax_batch_mixed (when `mixed` is set, n <= 256)
+ * entry is relaxed like gb_relax_batch_mixed (when `mixed` is set, n <= 512)
  * or gb_relax_batch_fast (n <= 512), with the entry's own buffers, on one of
  * `concurrency` internal streams (1..8) forked from and joined back into
  * `stream`, so small batches of different instances share the GPU.  Every

@@ -52,7 +52,7 @@ def results(e):
 def test_instances_equal_solo_runs(concurrency):
     import torch
     sizes = [(32, 40, 1), (45, 17, 3), (64, 100, 3), (100, 64, 1), (150, 33, 3), (200, 80, 1), (256, 20, 3),
-             (300, 50, 0), (420, 25, 0), (512, 16, 0)]
+             (300, 50, 1), (420, 25, 3), (512, 16, 0)]
     alg = gb.algorithm(dicg=True)
     budget = 250
     together = [make_entry(n, N, 100 + k, planes) for k, (n, N, planes) in enumerate(sizes)]

@@ -25,7 +25,7 @@ def dense(Q, c):
     return obj, (dQt, dc)
 
 
-@pytest.mark.parametrize("n", [7, 40, 200, 256])
+@pytest.mark.parametrize("n", [7, 40, 200, 256, 300, 512])
 @pytest.mark.parametrize("planes", [1, 3])
 def test_products_within_error_bound(n, planes):
     import torch
@@ -86,7 +86,8 @@ def run_mixed(mq, obj, dicg, LB, UB, X, ST, IT, eps, budget, fp64_iters, cutoff=
 
 
 @pytest.mark.parametrize("n,planes,dicg,cut", [(40, 3, True, False), (130, 1, True, True),
-                                               (200, 3, False, False), (256, 3, True, True)])
+                                               (200, 3, False, False), (256, 3, True, True),
+                                               (300, 1, True, False), (512, 3, True, True)])
 def test_fp64_pass_matches_oracle_from_mixed_output(n, planes, dicg, cut):
     from oracle import pyoracle as orc
     N, budget = 70, 400
@@ -216,6 +217,46 @@ def test_autotuner_picks_a_valid_shape_and_leaves_inputs_untouched():
                              best_vertex=p1["best_vertex"].copy(), best_vertex_val=p1["best_vertex_val"].copy())
     assert_bit_equal(full, o, "tuned shape")
     del zeros
+
+
+@pytest.mark.parametrize("n,cols,planes", [(300, 8, 1), (300, 16, 3), (450, 8, 3)])
+def test_wide_tiles_fp64_pass_matches_oracle(n, cols, planes):
+    """n > 256 (more than 16 row tiles: four per wave, fragments from L2) at
+    two and four rows per lane: the FP64 pass equals the oracle started from
+    the mixed pass's own output, and the mixed pass reaches the same
+    objective values as the one-row-per-lane shape (FP32 sums in another
+    order; values agree to the bf16 product error)."""
+    from types import SimpleNamespace
+    from oracle import pyoracle as orc
+    N, budget = 60, 300
+    Q, c, _ = instances.binary_qp(n, seed=41)
+    obj, keep = dense(Q, c)
+    mq = gb.MixedQ(obj, planes)
+    LB, UB, eps = node_set(n, N, 9)
+    opt = SimpleNamespace(row_tiles_per_wave=4, columns=cols, fp64_max_iters=budget // 16)
+    say(f"[mixed] wide n={n} cols={cols} planes={planes}")
+    p1 = run_mixed_opts(mq, obj, LB, UB, eps, budget, -1, opt)
+    assert np.all(p1["x"] >= LB) and np.all(p1["x"] <= UB) and np.all(p1["iterations"] >= 1)
+    full = run_mixed_opts(mq, obj, LB, UB, eps, budget, opt.fp64_max_iters, opt)
+    o = orc.relax_fast_batch(orc.dense_objective(Q, c, 2.0, 0.5, 0.0), True, p1["x"].copy(), LB.copy(),
+                             UB.copy(), eps.copy(), p1["iterations"].astype(np.int32).copy(), p1["status"].copy(),
+                             flags=np.ones(n, np.uint8), max_iters=opt.fp64_max_iters, refresh=64, threads=8,
+                             best_vertex=p1["best_vertex"].copy(), best_vertex_val=p1["best_vertex_val"].copy())
+    assert_bit_equal(full, o, "wide tiles")
+    ref = run_mixed_opts(mq, obj, LB, UB, eps, budget, -1, SimpleNamespace(row_tiles_per_wave=4, columns=4,
+                                                                           fp64_max_iters=-1))
+
+    def f(X):
+        return np.einsum("ij,jk,ik->i", X, Q, X) + X @ c
+
+    fw, fr = f(p1["x"]), f(ref["x"])
+    dev_rel = float(np.max(np.abs(fw - fr) / (1.0 + np.abs(fr))))
+    conv = np.isin(full["status"], (1, 4))
+    say(f"[mixed] wide converged {conv.sum()}/{N}, objective vs one row per lane {dev_rel:.2e}")
+    assert dev_rel < 1e-3
+    if planes == 3:  # one bf16 plane limits the reachable gap at this budget
+        assert conv.sum() > 0
+    mq.close()
 
 
 def run_mixed_opts(mq, obj, LB, UB, eps, budget, fp64_iters, opt):
