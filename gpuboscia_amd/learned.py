@@ -79,10 +79,10 @@ def fit(X, T, epochs=30, batch=1024, lr=2e-3, seed=0, progress=False):
                "sigma": sigma.tolist()}
 
 
-def solve_with(Q, c, rule, brancher=None, time_limit=120.0, form=2):
+def solve_with(Q, c, rule, brancher=None, time_limit=120.0, form=2, max_nodes=0):
     n = Q.shape[0]
     obj, keep = _objective(Q, c)
-    s = _settings(time_limit, form=form)
+    s = _settings(time_limit, max_nodes=max_nodes, form=form)
     s.branch_rule = rule
     if brancher is not None:
         s.learned_model = brancher.handle
@@ -112,11 +112,50 @@ def evaluate(brancher, n, seeds, time_limit=120.0, form=2):
     return rows
 
 
+def evaluate_budget(brancher, n, seeds, max_nodes, time_limit=120.0, form=2):
+    """For sizes whose searches do not finish: the learned rule and the rule
+    without it (branch_pow2 on the most fractional variables, same tree
+    shape) each get the same node budget on the same instances; the better
+    rule ends with the higher global dual bound (both bounds are FP64-
+    certified, so each is a valid lower bound on the optimum)."""
+    rows = []
+    for k, seed in enumerate(seeds):
+        if k % 128 == 0:
+            print(f"[learned] budget evaluation, instance {k}", flush=True)
+        Q, c, sha = instances.binary_qp(n, seed=seed)
+        mf = solve_with(Q, c, gb.BRANCH_POW2, None, time_limit, form, max_nodes)
+        ln = solve_with(Q, c, gb.BRANCH_LEARNED, brancher, time_limit, form, max_nodes)
+        rows.append({"seed": seed, "sha256": sha[:16], "mf_nodes": mf["nodes_processed"],
+                     "learned_nodes": ln["nodes_processed"], "mf_seconds": mf["wall"],
+                     "learned_seconds": ln["wall"], "mf_bound": mf["dual_bound"], "learned_bound": ln["dual_bound"],
+                     "mf_objective": mf["objective"], "learned_objective": ln["objective"]})
+    return rows
+
+
+def budget_summary(rows):
+    """Bound comparison at equal node budgets: the gap of each rule is measured
+    against the better incumbent of the two runs."""
+    mb = np.array([r["mf_bound"] for r in rows])
+    lb = np.array([r["learned_bound"] for r in rows])
+    inc = np.minimum([r["mf_objective"] for r in rows], [r["learned_objective"] for r in rows])
+    gm, gl = inc - mb, inc - lb
+    ok = (gm > 0) & (gl > 0)
+    return {"instances": len(rows), "learned_bound_higher": int((lb > mb).sum()),
+            "learned_bound_lower": int((lb < mb).sum()), "mean_bound_gain": float(np.mean(lb - mb)),
+            "gap_ratio_geomean": float(np.exp(np.mean(np.log(gl[ok] / gm[ok])))) if ok.any() else None,
+            "mf_seconds": float(sum(r["mf_seconds"] for r in rows)),
+            "learned_seconds": float(sum(r["learned_seconds"] for r in rows))}
+
+
 def train_main(argv=None):
     """Trains a scorer on solver-generated strong-branching labels and writes it
     (weights + feature normalization, plain arrays) to --out:
 
       python -m gpuboscia_amd.learned --n 40 --train 3072 --out scorer.npz
+
+    Sizes whose searches cannot be proven are compared at a node budget:
+
+      python -m gpuboscia_amd.learned --n 200 --train 0 --scorer scorer.npz --test 1024 --node-budget 20000
     """
     import argparse
     import json
@@ -127,15 +166,40 @@ def train_main(argv=None):
     ap.add_argument("--labeled-nodes", type=int, default=16)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--node-budget", type=int, default=0,
+                    help="evaluate at this node budget per search (for sizes that cannot be proven)")
+    ap.add_argument("--scorer", help="evaluate this saved scorer instead of training one (--train 0)")
+    ap.add_argument("--out", help="where to write the trained scorer")
     a = ap.parse_args(argv)
-    t0 = time.perf_counter()
-    X, T = generate(a.n, seeds=range(1000, 1000 + a.train), labeled_nodes=a.labeled_nodes, progress=True)
-    t1 = time.perf_counter()
-    b, info = fit(X, T, epochs=a.epochs, lr=a.lr, progress=True)
-    t2 = time.perf_counter()
-    b.save(a.out)
-    rows = evaluate(b, a.n, seeds=range(50000, 50000 + a.test)) if a.test > 0 else []
+    if a.train > 0 and not a.out:
+        ap.error("--out is required when training")
+    if a.train == 0 and not a.scorer:
+        ap.error("--scorer is required with --train 0")
+    if a.train == 0:
+        b = gb.Brancher.from_file(a.scorer)
+        if a.node_budget > 0:
+            rows = evaluate_budget(b, a.n, range(50000, 50000 + a.test), a.node_budget)
+            summary = {"n": a.n, "node_budget": a.node_budget, **budget_summary(rows)}
+            print(json.dumps(summary), flush=True)
+            return summary
+        rows = evaluate(b, a.n, seeds=range(50000, 50000 + a.test))
+        info = {"rows": 0, "initial_loss": None, "final_loss": None}
+        t0 = t1 = t2 = time.perf_counter()
+    else:
+        t0 = time.perf_counter()
+        X, T = generate(a.n, seeds=range(1000, 1000 + a.train), labeled_nodes=a.labeled_nodes, progress=True)
+        t1 = time.perf_counter()
+        b, info = fit(X, T, epochs=a.epochs, lr=a.lr, progress=True)
+        t2 = time.perf_counter()
+        b.save(a.out)
+        if a.node_budget > 0 and a.test > 0:
+            rows = evaluate_budget(b, a.n, range(50000, 50000 + a.test), a.node_budget)
+            summary = {"n": a.n, "train_instances": a.train, "rows": info["rows"],
+                       "label_seconds": round(t1 - t0, 2), "fit_seconds": round(t2 - t1, 2),
+                       "final_loss": info["final_loss"], "node_budget": a.node_budget, **budget_summary(rows)}
+            print(json.dumps(summary), flush=True)
+            return summary
+        rows = evaluate(b, a.n, seeds=range(50000, 50000 + a.test)) if a.test > 0 else []
     mf = np.array([r["mf_nodes"] for r in rows], dtype=float)
     ln = np.array([r["learned_nodes"] for r in rows], dtype=float)
     summary = {"n": a.n, "train_instances": a.train, "rows": info["rows"], "label_seconds": round(t1 - t0, 2),
