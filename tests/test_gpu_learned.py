@@ -107,3 +107,22 @@ def test_scorer_save_and_restore(tmp_path):
     r.predict(dX, y2)
     torch.cuda.synchronize()
     assert torch.equal(y1, y2)
+
+
+def test_node_budget_comparison_at_n200():
+    """The committed n = 200 scorer against pow2 at an equal node budget:
+    both searches stop at the budget (plus at most one batch in flight on
+    each pipeline slot) with certified bounds below their incumbents."""
+    import os
+    from gpuboscia_amd import learned
+    path = os.path.join(os.path.dirname(gb.__file__), "scorer_binary_qp_n200.npz")
+    b = gb.Brancher.from_file(path)
+    budget = 3000
+    rows = learned.evaluate_budget(b, 200, range(50000, 50003), budget)
+    for r in rows:
+        say(f"[learned] n=200 seed {r['seed']}: bound mf {r['mf_bound']:.6f} learned {r['learned_bound']:.6f}")
+        for key in ("mf", "learned"):
+            assert budget <= r[f"{key}_nodes"] <= budget + 2 * 1024
+            assert np.isfinite(r[f"{key}_bound"]) and r[f"{key}_bound"] <= r[f"{key}_objective"] + 1e-9
+    s = learned.budget_summary(rows)
+    assert s["instances"] == 3 and s["learned_bound_higher"] + s["learned_bound_lower"] <= 3
