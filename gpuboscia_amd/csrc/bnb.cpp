@@ -485,7 +485,7 @@ struct DeviceBatch {
     float *feat = nullptr, *score = nullptr;
     int *topk = nullptr, *htopk = nullptr;
     int *depth = nullptr, *hdepth = nullptr;
-    double *pc = nullptr, *hpc = nullptr, *qdiag = nullptr;
+    double *pc = nullptr, *hpc = nullptr;
     unsigned char *dpack = nullptr, *hpack = nullptr;
     size_t in_bytes = 0, out_off = 0, out_bytes = 0;
 
@@ -569,7 +569,7 @@ struct DeviceBatch {
 
     // Buffers grow and never shrink: a batch set is reused by later solves
     // (see BufferPool), so only a larger capacity or n allocates.
-    size_t pack_bytes = 0, flags_bytes = 0, feat_elems = 0, qdiag_elems = 0;
+    size_t pack_bytes = 0, flags_bytes = 0, feat_elems = 0;
 
     // Gradients and feature rows (device), pseudocost-free labelling buffers.
     gb_status alloc_features() {
@@ -589,13 +589,6 @@ struct DeviceBatch {
             GB_HIP(hipMalloc(&feat, xn * kMlpFeatures * 4));
             GB_HIP(hipMalloc(&score, xn * 4));
             feat_elems = xn;
-        }
-        if ((size_t)n > qdiag_elems) {
-            if (qdiag) (void)hipFree(qdiag);
-            qdiag = nullptr;
-            qdiag_elems = 0;
-            GB_HIP(hipMalloc(&qdiag, (size_t)n * 8));
-            qdiag_elems = (size_t)n;
         }
         return GB_OK;
     }
@@ -632,7 +625,7 @@ struct DeviceBatch {
     }
     ~DeviceBatch() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : {(void*)dpack, (void*)flags, (void*)g, (void*)feat, (void*)score, (void*)qdiag})
+        for (void* p : {(void*)dpack, (void*)flags, (void*)g, (void*)feat, (void*)score})
             if (p) (void)hipFree(p);
         for (void* p : {(void*)hpack, (void*)hg})
             if (p) (void)hipHostFree(p);
@@ -883,15 +876,13 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     }
     Pseudocosts pc;
     pc.init(n);
-    // Q diagonal on the host (feature estimate of branch gains)
+    // Q diagonal on the host for the labeller's features (feature estimate
+    // of branch gains); the device features read it from Qt directly.
     std::vector<double> qdiag;
-    if ((learned || lab) && o->kind == GB_OBJ_DENSE_QUADRATIC) {
+    if (lab && o->kind == GB_OBJ_DENSE_QUADRATIC) {
         qdiag.resize(n);
         GB_HIP(hipMemcpy2D(qdiag.data(), sizeof(double), o->Qt, (size_t)(n + 1) * sizeof(double),
                            sizeof(double), n, hipMemcpyDeviceToHost));
-        if (learned)
-            for (int k = 0; k < slots; ++k)
-                GB_HIP(hipMemcpy(bufs[k].qdiag, qdiag.data(), (size_t)n * 8, hipMemcpyHostToDevice));
     }
     std::vector<uint8_t> hflags(n, 0);
     for (int i : iv) hflags[i] = 1;
@@ -1074,7 +1065,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 if (ge != 0) return hip_fail(static_cast<hipError_t>(ge), __FILE__, __LINE__);
             }
             const int fe = launch_candidate_features(n, N, db.x, db.g, db.lb, db.ub, db.depth, db.gap, db.val,
-                                                     db.pc, db.pc + n, db.qdiag, o->grad_scale, db.flags,
+                                                     db.pc, db.pc + n, o->Qt, (size_t)n + 1, o->grad_scale, db.flags,
                                                      s->integer_tol, db.feat, db.stream);
             if (fe != 0) return hip_fail(static_cast<hipError_t>(fe), __FILE__, __LINE__);
             st = gb_brancher_predict(s->learned_model, db.feat, N * n, db.score, db.stream);

@@ -38,8 +38,8 @@ constexpr int kMaxFastBlocks = 2048;  // persistent workgroups of the batched fo
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream);
 int launch_candidate_features(int n, int N, const double* X, const double* G, const double* LB,
                               const double* UB, const int* depth, const double* gap, const double* val,
-                              const double* pcu, const double* pcd, const double* qdiag, double gs,
-                              const uint8_t* flags, double tol, float* F, void* stream);
+                              const double* pcu, const double* pcd, const double* qdiag, size_t qstride,
+                              double gs, const uint8_t* flags, double tol, float* F, void* stream);
 int launch_batched_gradient(const double* Qt, const double* lin, double gs, double ls, int n, int N,
                             const double* X, double* G, void* stream);
 // Per node the k best-scored fractional candidates (flags[j] set and

@@ -897,6 +897,7 @@ __global__ void __launch_bounds__(64) batched_gradient_kernel(const double* Qt, 
 // Feature rows of the learned scorer for every (node, variable) of a batch,
 // row node * n + j; rows of variables that are not fractional candidates
 // are zero.  Same definitions as the host rows of the label generator:
+// Q_jj is read as qdiag[j * qstride] (Qt itself with qstride = n + 1).
 // 0 x_j, 1 min(frac, 1 - frac), 2 g_j / max|g|, 3 / 4 up / down gain
 // (pseudocost mean gain per unit, or 1/2 s_g Q_jj d without history for
 // the variable, times the distance d moved) over its mean across the node's
@@ -905,7 +906,7 @@ __global__ void __launch_bounds__(64) batched_gradient_kernel(const double* Qt, 
 __global__ void __launch_bounds__(64) candidate_features_kernel(
     int n, int N, const double* X, const double* G, const double* LB, const double* UB, const int* depth,
     const double* gap, const double* val, const double* pcu, const double* pcd, const double* qdiag,
-    double gs, const uint8_t* flags, double tol, float* F) {
+    size_t qstride, double gs, const uint8_t* flags, double tol, float* F) {
     const int node = blockIdx.x;
     if (node >= N) return;
     const int lane = threadIdx.x;
@@ -920,7 +921,7 @@ __global__ void __launch_bounds__(64) candidate_features_kernel(
     };
     auto gains = [&](int j, double& gu, double& gd) {
         const double du = ceil(x[j]) - x[j], dd = x[j] - floor(x[j]);
-        const double curv = 0.5 * gs * qdiag[j];
+        const double curv = 0.5 * gs * qdiag[static_cast<size_t>(j) * qstride];
         gu = (pcu[j] >= 0.0 ? pcu[j] : curv * du) * du;
         gd = (pcd[j] >= 0.0 ? pcd[j] : curv * dd) * dd;
     };
@@ -1025,11 +1026,11 @@ int launch_batched_gradient(const double* Qt, const double* lin, double gs, doub
 
 int launch_candidate_features(int n, int N, const double* X, const double* G, const double* LB,
                               const double* UB, const int* depth, const double* gap, const double* val,
-                              const double* pcu, const double* pcd, const double* qdiag, double gs,
-                              const uint8_t* flags, double tol, float* F, void* stream) {
+                              const double* pcu, const double* pcd, const double* qdiag, size_t qstride,
+                              double gs, const uint8_t* flags, double tol, float* F, void* stream) {
     if (N <= 0) return 0;
     hipLaunchKernelGGL(candidate_features_kernel, dim3(N), dim3(kWave), 0, reinterpret_cast<hipStream_t>(stream),
-                       n, N, X, G, LB, UB, depth, gap, val, pcu, pcd, qdiag, gs, flags, tol, F);
+                       n, N, X, G, LB, UB, depth, gap, val, pcu, pcd, qdiag, qstride, gs, flags, tol, F);
     return static_cast<int>(hipGetLastError());
 }
 
