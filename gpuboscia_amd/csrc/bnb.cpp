@@ -41,10 +41,17 @@ constexpr double kInf = std::numeric_limits<double>::infinity();
 // A slot is reference counted: children that warm-start from their parent's
 // iterate hold the parent's slot (whose relaxation_x is that iterate) until
 // they are staged.
+// Slots of node data.  Each holds the node's bounds and its relaxation x
+// (n doubles).  The bounds are 2n doubles, or, in compact mode (a problem
+// whose bounds are all 0 or 1 and whose branching rounds fractional values
+// only, so every node bound stays 0 or 1), 2n bytes: child creation and
+// staging then move a byte per bound instead of eight.
 struct NodePool {
     int n = 0;
+    bool compact = false;
     static constexpr int kChunk = 4096;
     std::vector<std::unique_ptr<double[]>> chunks;
+    std::vector<std::unique_ptr<uint8_t[]>> bchunks;  // compact: lb, ub bytes
     std::vector<int> free_slots;
     std::vector<int> refs;
     int next = 0;
@@ -54,7 +61,10 @@ struct NodePool {
             s = free_slots.back();
             free_slots.pop_back();
         } else {
-            if (next % kChunk == 0) chunks.emplace_back(new double[(size_t)kChunk * 3 * n]);
+            if (next % kChunk == 0) {
+                chunks.emplace_back(new double[(size_t)kChunk * (compact ? 1 : 3) * n]);
+                if (compact) bchunks.emplace_back(new uint8_t[(size_t)kChunk * 2 * n]);
+            }
             s = next++;
             refs.push_back(0);
         }
@@ -65,9 +75,15 @@ struct NodePool {
     void release(int slot) {
         if (--refs[slot] == 0) free_slots.push_back(slot);
     }
+    // double bounds (not compact)
     double* lb(int slot) { return chunks[slot / kChunk].get() + (size_t)(slot % kChunk) * 3 * n; }
     double* ub(int slot) { return lb(slot) + n; }
-    double* rx(int slot) { return lb(slot) + 2 * n; }
+    double* rx(int slot) {
+        return compact ? chunks[slot / kChunk].get() + (size_t)(slot % kChunk) * n : lb(slot) + 2 * n;
+    }
+    // byte bounds (compact): lb8, then ub8
+    uint8_t* lb8(int slot) { return bchunks[slot / kChunk].get() + (size_t)(slot % kChunk) * 2 * n; }
+    uint8_t* ub8(int slot) { return lb8(slot) + n; }
 };
 
 struct HostNode {
@@ -915,11 +931,25 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     NodePool pool;
     pool.n = n;
     {
+        bool zero_one = s->branch_rule != GB_BRANCH_CUSTOM && !lab;
+        // +0.0 or 1.0 exactly (a -0.0 bound would come back as +0.0)
+        auto zo = [](double v) { return v == 1.0 || (v == 0.0 && !std::signbit(v)); };
+        for (int j = 0; j < n && zero_one; ++j) zero_one = zo(pr->lb[j]) && zo(pr->ub[j]);
+        pool.compact = zero_one;
+    }
+    {
         HostNode root;
         root.id = 0;
         root.slot = pool.alloc();
-        std::copy(pr->lb, pr->lb + n, pool.lb(root.slot));
-        std::copy(pr->ub, pr->ub + n, pool.ub(root.slot));
+        if (pool.compact) {
+            for (int j = 0; j < n; ++j) {
+                pool.lb8(root.slot)[j] = pr->lb[j] == 1.0 ? 1 : 0;
+                pool.ub8(root.slot)[j] = pr->ub[j] == 1.0 ? 1 : 0;
+            }
+        } else {
+            std::copy(pr->lb, pr->lb + n, pool.lb(root.slot));
+            std::copy(pr->ub, pr->ub + n, pool.ub(root.slot));
+        }
         if (s->root_initial_point && s->root_initial_point_size == n) {
             std::copy(s->root_initial_point, s->root_initial_point + n, pool.rx(root.slot));
             root.has_rx = true;
@@ -989,8 +1019,21 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         // Staging copies on the host threads (the pool is only read here).
         HostThreads::get().parallel_for(N, [&](int i) {
             const HostNode& nd = batch[i];
-            const double* nlb = pool.lb(nd.slot);
-            const double* nub = pool.ub(nd.slot);
+            if (pool.compact) {
+                const uint8_t* l8 = pool.lb8(nd.slot);
+                const uint8_t* u8 = pool.ub8(nd.slot);
+                double* hl = db.hlb + (size_t)i * n;
+                double* hu = db.hub + (size_t)i * n;
+                for (int j = 0; j < n; ++j) {
+                    hl[j] = l8[j];
+                    hu[j] = u8[j];
+                }
+            } else {
+                std::copy(pool.lb(nd.slot), pool.lb(nd.slot) + n, db.hlb + (size_t)i * n);
+                std::copy(pool.ub(nd.slot), pool.ub(nd.slot) + n, db.hub + (size_t)i * n);
+            }
+            const double* nlb = db.hlb + (size_t)i * n;
+            const double* nub = db.hub + (size_t)i * n;
             double* hx = db.hx + (size_t)i * n;
             const bool warm = nd.has_rx || nd.xslot >= 0;
             if (nd.has_rx) {
@@ -1003,8 +1046,6 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             } else {
                 std::copy(nlb, nlb + n, hx);
             }
-            std::copy(nlb, nlb + n, db.hlb + (size_t)i * n);
-            std::copy(nub, nub + n, db.hub + (size_t)i * n);
             db.heps[i] = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth);
             db.hit[i] = nd.riters;
             db.hst[i] = warm ? GB_NODE_INITIAL_POINT_SUPPLIED : 0;
@@ -1291,6 +1332,17 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         HostThreads::get().parallel_for((int)child_tasks.size(), [&](int t) {
             const ChildTask& ct = child_tasks[t];
             const double* xr = db.hx + (size_t)ct.row * n;
+            if (pool.compact) {
+                // branched values are fractional, so ceil is 1 and floor 0
+                uint8_t* cl8 = pool.lb8(ct.child);
+                std::copy(pool.lb8(ct.parent), pool.lb8(ct.parent) + 2 * n, cl8);  // lb and ub
+                for (int b = 0; b < ct.k; ++b) {
+                    const int vi = task_vars[ct.var0 + b];
+                    if (ct.mask & (1 << b)) cl8[vi] = 1;
+                    else cl8[n + vi] = 0;
+                }
+                return;
+            }
             double* clb = pool.lb(ct.child);
             double* cub = pool.ub(ct.child);
             std::copy(pool.lb(ct.parent), pool.lb(ct.parent) + 2 * n, clb);  // lb and ub
