@@ -202,6 +202,7 @@ class Frontier {
     bool empty() const { return hot_.empty(); }
     size_t size() const { return hot_.size() + cold_.size(); }
     const HostNode& top() const { return recs_[hot_.front().rec]; }
+    double top_bound() const { return hot_.front().bound; }  // without touching the record
     void push(const HostNode& nd) {
         int r;
         if (!free_.empty()) {
@@ -973,7 +974,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
 
     auto live_bound = [&]() {
         double b = inc;
-        if (!q.empty()) b = std::min(b, q.top().bound);
+        if (!q.empty()) b = std::min(b, q.top_bound());
         for (int k = 0; k < 2; ++k)
             if (in_flight(k)) b = std::min(b, open_min[k][open_pos[k]]);
         return b;
@@ -1129,7 +1130,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     };
     std::vector<ChildTask> child_tasks;
     std::vector<XTask> x_tasks;
-    std::vector<int> deferred, task_vars, scan_vars;
+    std::vector<int> deferred, task_vars, scan_vars, vars;  // vars: one node's branching variables
     int head = 0;
     while (!finished) {
         const double el = std::chrono::duration<double>(Clock::now() - t0).count();
@@ -1250,7 +1251,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     q.push(c);
                 }
             } else if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
-                std::vector<int> vars;
+                vars.clear();
                 if (prescan) {
                     const int* sv = scan_vars.data() + (size_t)i * bk;
                     for (int c = 0; c < bk && sv[c] >= 0; ++c) vars.push_back(sv[c]);
