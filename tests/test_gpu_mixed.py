@@ -185,6 +185,30 @@ def test_mixed_bnb_matches_plain_optimum(n, seed):
     assert abs(res[1]["objective"] - res[2]["objective"]) <= 1e-9 * max(1.0, abs(res[1]["objective"]))
 
 
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_byte_bounds_give_the_same_search(form):
+    """0/1 problems keep node bounds as bytes; lower bounds of -0.0 (not
+    eligible) keep them as doubles.  Both searches are the same tree."""
+    n = 40
+    Q, c, _ = instances.binary_qp(n, seed=3)
+    obj, keep = dense(Q, c)
+    res = []
+    for lb0 in (0.0, -0.0):
+        s = gb.default_settings()
+        s.abs_gap_tol = 1e-6
+        s.rel_gap_tol = 1e-6
+        s.form = form
+        s.time_limit_seconds = 60.0
+        r = gb.solve(n, [lb0] * n, [1.0] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), s)
+        say(f"[bounds] form={form} lb={lb0!r}: {r['objective']:.12f} nodes {r['nodes_processed']}")
+        res.append(r)
+    a, b = res
+    assert not a["reached_time_limit"] and not b["reached_time_limit"]
+    for key in ("objective", "dual_bound", "nodes_processed", "fw_iterations"):
+        assert a[key] == b[key], key
+    assert np.array_equal(np.asarray(a["x"]), np.asarray(b["x"]))
+
+
 def test_autotuner_picks_a_valid_shape_and_leaves_inputs_untouched():
     from oracle import pyoracle as orc
     import torch
