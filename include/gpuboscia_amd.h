@@ -261,7 +261,9 @@ gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
  * or gb_relax_batch_fast (n <= 512), with the entry's own buffers, on one of
  * `concurrency` internal streams (1..8) forked from and joined back into
  * `stream`, so small batches of different instances share the GPU.  Every
- * entry's results equal those of its solo call.
+ * entry's results equal those of its solo call.  After one call on the same
+ * buffers (which makes the library's allocations) a call can be captured in
+ * a HIP graph on `stream` and replayed.
  */
 typedef struct {
     const gb_objective_desc* objective;

@@ -79,3 +79,45 @@ def test_instances_reject_bad_entries():
     e["obj"] = gb.ObjectiveDesc(gb.OBJ_DENSE, 40, 0, 0, 2.0, 0.5, 0.0, 0, 0, 0)  # no Qt
     with pytest.raises(gb.InvalidArgument):
         gb.relax_instances([e], gb.algorithm(), 10)
+
+
+def test_instances_captured_call_replays_match_eager():
+    """One gb_relax_instances call captured in a HIP graph (after a warm-up
+    call has made its allocations): each replay, with the inputs refilled by
+    eager work and a synchronisation in between, equals the eager call."""
+    import torch
+    sizes = [(48, 40, 1), (150, 33, 3), (300, 20, 1), (420, 16, 0)]
+    alg = gb.algorithm(dicg=True)
+    budget = 150
+    entries = [make_entry(n, N, 300 + k, planes) for k, (n, N, planes) in enumerate(sizes)]
+    init = [(e["x"].clone(), e["iterations"].clone(), e["status"].clone()) for e in entries]
+
+    def refill():
+        for e, (x0, it0, st0) in zip(entries, init):
+            e["x"].copy_(x0)
+            e["iterations"].copy_(it0)
+            e["status"].copy_(st0)
+            e["best_vertex_val"].fill_(1e300)
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        refill()
+        gb.relax_instances(entries, alg, budget, concurrency=4, stream=s.cuda_stream)
+    s.synchronize()
+    eager = [results(e) for e in entries]
+    with torch.cuda.stream(s):
+        refill()
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        gb.relax_instances(entries, alg, budget, concurrency=4, stream=s.cuda_stream)
+    s.synchronize()
+    for rep in range(3):
+        with torch.cuda.stream(s):
+            refill()
+        s.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        say(f"[instances] captured replay {rep}")
+        for k, e in enumerate(entries):
+            assert_bit_equal(results(e), eager[k], f"replay {rep} instance {k} n={sizes[k][0]}")
