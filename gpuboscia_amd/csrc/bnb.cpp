@@ -192,37 +192,46 @@ struct Labeler {
 
 // Best-first frontier ordered by (dual_bound, id).  (bound, id) is a strict
 // order (ids are unique), so nodes leave in exactly that order whatever the
-// structure: a 4-ary heap of 16-byte keys holds every key below a threshold
+// structure.  Entries are families: nodes with one bound and consecutive ids
+// (the children of one branching), which leave one by one; no other node's
+// key can fall between two members of a family, so a family stays the
+// smallest entry until its last member has left, and costs one heap push and
+// one pop.  A 4-ary heap of 16-byte keys holds every key below a threshold
 // (at most a few ten thousand, cache resident), an unsorted vector the rest;
 // when the heap runs empty the smallest keys of the vector are selected
 // (nth_element) into it, and when it grows too large its largest keys move to
-// the vector.  Node records live in a recycled vector.
+// the vector.  Family records live in recycled blocks.
 class Frontier {
   public:
     bool empty() const { return hot_.empty(); }
-    size_t size() const { return hot_.size() + cold_.size(); }
-    const HostNode& top() const { return recs_[hot_.front().rec]; }
+    size_t size() const { return nodes_; }
+    const HostNode& top() const {
+        const Family& f = fams_[hot_.front().fam];
+        return recs_[f.start + f.pos];
+    }
     double top_bound() const { return hot_.front().bound; }  // without touching the record
-    void push(const HostNode& nd) {
-        int r;
-        if (!free_.empty()) {
-            r = free_.back();
-            free_.pop_back();
-            recs_[r] = nd;
-        } else {
-            r = (int)recs_.size();
-            recs_.push_back(nd);
+    void push(const HostNode& nd) { push_family(&nd, 1); }
+    // nds[0 .. count): consecutive runs with one bound and ids nds[0].id + i
+    // become one entry each.
+    void push_family(const HostNode* nds, int count) {
+        while (count > 0) {
+            int run = 1;
+            while (run < count && nds[run].bound == nds[0].bound && nds[run].id == nds[0].id + run) ++run;
+            push_run(nds, run);
+            nds += run;
+            count -= run;
         }
-        const Key key{nd.bound, nd.id, r};
-        if (!cold_.empty() && !before(key, lim_)) {
-            cold_.push_back(key);
-            return;
-        }
-        heap_push(key);
-        if (hot_.size() > 2 * kHot) spill();
     }
     void pop() {
-        free_.push_back(hot_.front().rec);
+        Key& k = hot_.front();
+        Family& f = fams_[k.fam];
+        --nodes_;
+        if (++f.pos < f.count) {
+            k.id = recs_[f.start + f.pos].id;  // the next member: still the smallest key
+            return;
+        }
+        free_blocks_[f.count].push_back(f.start);
+        free_fams_.push_back(k.fam);
         heap_pop();
         if (hot_.empty() && !cold_.empty()) refill();
     }
@@ -231,10 +240,42 @@ class Frontier {
     static constexpr size_t kHot = 1 << 15;
     struct Key {
         double bound;
-        int id, rec;
+        int id, fam;
+    };
+    struct Family {
+        int start = 0, count = 0, pos = 0;
     };
     static bool before(const Key& a, const Key& b) {
         return a.bound < b.bound || (a.bound == b.bound && a.id < b.id);
+    }
+    void push_run(const HostNode* nds, int count) {
+        int f;
+        if (!free_fams_.empty()) {
+            f = free_fams_.back();
+            free_fams_.pop_back();
+        } else {
+            f = (int)fams_.size();
+            fams_.emplace_back();
+        }
+        if ((size_t)count >= free_blocks_.size()) free_blocks_.resize(count + 1);
+        int at;
+        if (!free_blocks_[count].empty()) {
+            at = free_blocks_[count].back();
+            free_blocks_[count].pop_back();
+        } else {
+            at = (int)recs_.size();
+            recs_.resize(recs_.size() + count);
+        }
+        std::copy(nds, nds + count, recs_.begin() + at);
+        fams_[f] = Family{at, count, 0};
+        nodes_ += count;
+        const Key key{nds[0].bound, nds[0].id, f};
+        if (!cold_.empty() && !before(key, lim_)) {
+            cold_.push_back(key);
+            return;
+        }
+        heap_push(key);
+        if (hot_.size() > 2 * kHot) spill();
     }
     void heap_push(const Key& key) {
         size_t i = hot_.size();
@@ -310,7 +351,10 @@ class Frontier {
     std::vector<Key> hot_, cold_;
     Key lim_{0.0, 0, 0};  // every heap key is before lim_, no vector key is (when the vector is non-empty)
     std::vector<HostNode> recs_;
-    std::vector<int> free_;
+    std::vector<Family> fams_;
+    std::vector<int> free_fams_;
+    std::vector<std::vector<int>> free_blocks_;  // record blocks by size
+    size_t nodes_ = 0;
 };
 
 // A fixed set of host worker threads for the per-node loops whose iterations
@@ -1131,6 +1175,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     std::vector<ChildTask> child_tasks;
     std::vector<XTask> x_tasks;
     std::vector<int> deferred, task_vars, scan_vars, vars;  // vars: one node's branching variables
+    std::vector<HostNode> kids;                             // one node's children, pushed as a family
     int head = 0;
     while (!finished) {
         const double el = std::chrono::duration<double>(Clock::now() - t0).count();
@@ -1291,6 +1336,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 int nid = base;
                 const int k = (int)vars.size();
                 const int var0 = (int)task_vars.size();
+                kids.clear();
                 task_vars.insert(task_vars.end(), vars.begin(), vars.end());
                 bool parent_x_kept = false;
                 for (int mask = 0; k > 0 && mask < (1 << k); ++mask) {
@@ -1320,8 +1366,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         c.bfrac = c.bdir ? std::ceil(xi[vi]) - xi[vi] : xi[vi] - std::floor(xi[vi]);
                         c.pbound = nd.bound;
                     }
-                    q.push(c);
+                    kids.push_back(c);
                 }
+                q.push_family(kids.data(), (int)kids.size());
             }
             deferred.push_back(nd.slot);
             ++done_nodes;
@@ -1361,7 +1408,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         if (!finished && q.empty() && !in_flight(other)) { final_bound = inc; finished = true; }
         if (pipelined && in_flight(other)) head = other;
     }
-    if (final_bound == -kInf) final_bound = q.empty() ? inc : std::min(inc, q.top().bound);
+    if (final_bound == -kInf) final_bound = q.empty() ? inc : std::min(inc, q.top_bound());
     res->objective = inc;
     res->dual_bound = final_bound;
     res->nodes_processed = done_nodes;
