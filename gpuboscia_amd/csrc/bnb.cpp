@@ -525,9 +525,15 @@ int most_fractional(const double* x, const std::vector<int>& iv, double tol) {
 // and one pinned host block with the same layout, recomputed for each batch
 // size N by layout(): inputs first, then the arrays that are both input and
 // result, then the results, so a batch moves in one host-to-device copy
-// (lb .. bvv) and one device-to-host copy (x .. score).
+// (bounds .. bvv) and one device-to-host copy (x .. topk, and the best
+// vertices for small batches; in large ones they stay on the device and the
+// host fetches one when it improves the incumbent).  In compact mode (0/1
+// bounds, large batches) the bounds travel as bytes and are widened into
+// device-only double arrays by a kernel.
 struct DeviceBatch {
     int cap = 0, n = 0;
+    bool compact = false;
+    bool bv_out = true;  // best vertices in the result copy (else fetched when they improve the incumbent)
     bool learned_io = false;  // depth, pseudocost means and chosen candidates travel with the batch
     int topk_k = 0;           // candidates returned per node
     double *x = nullptr, *lb = nullptr, *ub = nullptr, *bv = nullptr;
@@ -547,11 +553,12 @@ struct DeviceBatch {
     int *topk = nullptr, *htopk = nullptr;
     int *depth = nullptr, *hdepth = nullptr;
     double *pc = nullptr, *hpc = nullptr;
+    uint8_t *lb8 = nullptr, *hlb8 = nullptr;  // compact: lb bytes then ub bytes, [2][N][n]
     unsigned char *dpack = nullptr, *hpack = nullptr;
     size_t in_bytes = 0, out_off = 0, out_bytes = 0;
 
     struct Offsets {
-        size_t lb, ub, eps, cut, depth, pc, x, it, st, bvv, bv, gap, val, topk, end;
+        size_t lb8, lb, ub, eps, cut, depth, pc, x, it, st, bvv, gap, val, topk, out_end, bv, dlb, dub, end;
     };
     Offsets offsets(int N) const {
         Offsets f{};
@@ -562,8 +569,9 @@ struct DeviceBatch {
             return at;
         };
         const size_t xb = (size_t)N * n * 8;
-        f.lb = take(xb);
-        f.ub = take(xb);
+        f.lb8 = take(compact ? (size_t)2 * N * n : 0);
+        f.lb = take(compact ? 0 : xb);
+        f.ub = take(compact ? 0 : xb);
         f.eps = take((size_t)N * 8);
         f.cut = take((size_t)N * 8);
         f.depth = take(learned_io ? (size_t)N * 4 : 0);
@@ -572,10 +580,15 @@ struct DeviceBatch {
         f.it = take((size_t)N * 4);
         f.st = take((size_t)N);
         f.bvv = take((size_t)N * 8);
-        f.bv = take(xb);
         f.gap = take((size_t)N * 8);
         f.val = take((size_t)N * 8);
         f.topk = take(learned_io ? (size_t)N * topk_k * 4 : 0);
+        if (bv_out) f.bv = take(xb);
+        f.out_end = o;
+        // device only
+        if (!bv_out) f.bv = take(xb);
+        f.dlb = take(compact ? xb : 0);
+        f.dub = take(compact ? xb : 0);
         f.end = o;
         return f;
     }
@@ -585,8 +598,10 @@ struct DeviceBatch {
         const Offsets f = offsets(N);
         auto d = [&](size_t off) { return dpack + off; };
         auto h = [&](size_t off) { return hpack + off; };
-        lb = reinterpret_cast<double*>(d(f.lb));
-        ub = reinterpret_cast<double*>(d(f.ub));
+        lb = reinterpret_cast<double*>(d(compact ? f.dlb : f.lb));
+        ub = reinterpret_cast<double*>(d(compact ? f.dub : f.ub));
+        lb8 = compact ? d(f.lb8) : nullptr;
+        hlb8 = compact ? h(f.lb8) : nullptr;
         eps = reinterpret_cast<double*>(d(f.eps));
         cut = reinterpret_cast<double*>(d(f.cut));
         x = reinterpret_cast<double*>(d(f.x));
@@ -596,15 +611,15 @@ struct DeviceBatch {
         bv = reinterpret_cast<double*>(d(f.bv));
         gap = reinterpret_cast<double*>(d(f.gap));
         val = reinterpret_cast<double*>(d(f.val));
-        hlb = reinterpret_cast<double*>(h(f.lb));
-        hub = reinterpret_cast<double*>(h(f.ub));
+        hlb = compact ? nullptr : reinterpret_cast<double*>(h(f.lb));
+        hub = compact ? nullptr : reinterpret_cast<double*>(h(f.ub));
         heps = reinterpret_cast<double*>(h(f.eps));
         hcut = reinterpret_cast<double*>(h(f.cut));
         hx = reinterpret_cast<double*>(h(f.x));
         hit = reinterpret_cast<int*>(h(f.it));
         hst = h(f.st);
         hbvv = reinterpret_cast<double*>(h(f.bvv));
-        hbv = reinterpret_cast<double*>(h(f.bv));
+        hbv = bv_out ? reinterpret_cast<double*>(h(f.bv)) : nullptr;
         hgap = reinterpret_cast<double*>(h(f.gap));
         hval = reinterpret_cast<double*>(h(f.val));
         if (learned_io) {
@@ -617,7 +632,7 @@ struct DeviceBatch {
         }
         in_bytes = f.bvv + (size_t)N * 8;
         out_off = f.x;
-        out_bytes = f.end - f.x;
+        out_bytes = f.out_end - f.x;
     }
     gb_status upload() {
         GB_HIP(hipMemcpyAsync(dpack, hpack, in_bytes, hipMemcpyHostToDevice, stream));
@@ -662,7 +677,12 @@ struct DeviceBatch {
         // each batch set on a hardware queue of its own, so the two batches
         // in flight overlap on the device
         if (!stream) GB_HIP(create_dedicated_stream(&stream));
-        const size_t bytes = offsets(cap).end;
+        // large enough for either staging format (compact may change per batch)
+        const bool c0 = compact;
+        compact = false;
+        const size_t plain_bytes = offsets(cap).end;
+        compact = c0;
+        const size_t bytes = std::max(plain_bytes, offsets(cap).end);
         if (bytes > pack_bytes) {
             GB_HIP(hipStreamSynchronize(stream));
             if (dpack) (void)hipFree(dpack);
@@ -917,9 +937,19 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     const int slots = pipelined ? 2 : 1;
     int device = 0;
     GB_HIP(hipGetDevice(&device));
+    // 0/1 problems keep node bounds as bytes (every root bound +0 or 1, and
+    // branching rounds fractional values only, so every node bound stays 0
+    // or 1); the custom rule and the labeller work on double bounds.
+    bool zero_one = s->branch_rule != GB_BRANCH_CUSTOM && !lab;
+    {
+        // +0.0 or 1.0 exactly (a -0.0 bound would come back as +0.0)
+        auto zo = [](double v) { return v == 1.0 || (v == 0.0 && !std::signbit(v)); };
+        for (int j = 0; j < n && zero_one; ++j) zero_one = zo(pr->lb[j]) && zo(pr->ub[j]);
+    }
     BufferPool pooled(device);
     DeviceBatch* bufs = pooled.get().bufs;
     for (int k = 0; k < slots; ++k) {
+        bufs[k].compact = zero_one;
         st = bufs[k].alloc(B, n, learned, s->branch_k);
         if (st != GB_OK) return st;
         if (learned || lab) {
@@ -975,13 +1005,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     Frontier q;
     NodePool pool;
     pool.n = n;
-    {
-        bool zero_one = s->branch_rule != GB_BRANCH_CUSTOM && !lab;
-        // +0.0 or 1.0 exactly (a -0.0 bound would come back as +0.0)
-        auto zo = [](double v) { return v == 1.0 || (v == 0.0 && !std::signbit(v)); };
-        for (int j = 0; j < n && zero_one; ++j) zero_one = zo(pr->lb[j]) && zo(pr->ub[j]);
-        pool.compact = zero_one;
-    }
+    pool.compact = zero_one;
     {
         HostNode root;
         root.id = 0;
@@ -1060,36 +1084,55 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         }
         if (batch.empty()) return GB_OK;
         const int N = (int)batch.size();
+        // bytes on the bus when the batch is large enough to pay for the
+        // widening launch (about 16 K bound entries); doubles otherwise
+        db.compact = pool.compact && (size_t)N * n >= 16384;
+        db.bv_out = (size_t)N * n < 16384;
         db.layout(N);
         // Staging copies on the host threads (the pool is only read here).
         HostThreads::get().parallel_for(N, [&](int i) {
             const HostNode& nd = batch[i];
+            double* hx = db.hx + (size_t)i * n;
+            const bool warm = nd.has_rx || nd.xslot >= 0;
             if (pool.compact) {
                 const uint8_t* l8 = pool.lb8(nd.slot);
                 const uint8_t* u8 = pool.ub8(nd.slot);
-                double* hl = db.hlb + (size_t)i * n;
-                double* hu = db.hub + (size_t)i * n;
-                for (int j = 0; j < n; ++j) {
-                    hl[j] = l8[j];
-                    hu[j] = u8[j];
+                if (db.compact) {
+                    // bytes travel; the device widens them (0/1 values, exact)
+                    std::copy(l8, l8 + n, db.hlb8 + (size_t)i * n);
+                    std::copy(u8, u8 + n, db.hlb8 + (size_t)(N + i) * n);
+                } else {
+                    double* hl = db.hlb + (size_t)i * n;
+                    double* hu = db.hub + (size_t)i * n;
+                    for (int j = 0; j < n; ++j) {
+                        hl[j] = l8[j];
+                        hu[j] = u8[j];
+                    }
+                }
+                if (nd.has_rx) {
+                    std::copy(pool.rx(nd.slot), pool.rx(nd.slot) + n, hx);
+                } else if (nd.xslot >= 0) {
+                    const double* px = pool.rx(nd.xslot);
+                    for (int j = 0; j < n; ++j)
+                        hx[j] = std::min(std::max(px[j], (double)l8[j]), (double)u8[j]);
+                } else {
+                    for (int j = 0; j < n; ++j) hx[j] = l8[j];
                 }
             } else {
                 std::copy(pool.lb(nd.slot), pool.lb(nd.slot) + n, db.hlb + (size_t)i * n);
                 std::copy(pool.ub(nd.slot), pool.ub(nd.slot) + n, db.hub + (size_t)i * n);
-            }
-            const double* nlb = db.hlb + (size_t)i * n;
-            const double* nub = db.hub + (size_t)i * n;
-            double* hx = db.hx + (size_t)i * n;
-            const bool warm = nd.has_rx || nd.xslot >= 0;
-            if (nd.has_rx) {
-                std::copy(pool.rx(nd.slot), pool.rx(nd.slot) + n, hx);
-            } else if (nd.xslot >= 0) {
-                // Batched forms: the parent's iterate clipped to this box (any
-                // box point is feasible).
-                const double* px = pool.rx(nd.xslot);
-                for (int j = 0; j < n; ++j) hx[j] = std::min(std::max(px[j], nlb[j]), nub[j]);
-            } else {
-                std::copy(nlb, nlb + n, hx);
+                const double* nlb = db.hlb + (size_t)i * n;
+                const double* nub = db.hub + (size_t)i * n;
+                if (nd.has_rx) {
+                    std::copy(pool.rx(nd.slot), pool.rx(nd.slot) + n, hx);
+                } else if (nd.xslot >= 0) {
+                    // Batched forms: the parent's iterate clipped to this box (any
+                    // box point is feasible).
+                    const double* px = pool.rx(nd.xslot);
+                    for (int j = 0; j < n; ++j) hx[j] = std::min(std::max(px[j], nlb[j]), nub[j]);
+                } else {
+                    std::copy(nlb, nlb + n, hx);
+                }
             }
             db.heps[i] = depth_tolerance(s->gpu_eps, s->gpu_eps_decay, s->gpu_eps_min, nd.depth);
             db.hit[i] = nd.riters;
@@ -1117,6 +1160,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         // learned inputs), one copy out after the kernels.
         st = db.upload();
         if (st != GB_OK) return st;
+        if (db.compact) {
+            const int ee = launch_expand_bounds((size_t)N * n, db.lb8, db.lb8 + (size_t)N * n, db.lb, db.ub,
+                                               db.stream);
+            if (ee != 0) return hip_fail(static_cast<hipError_t>(ee), __FILE__, __LINE__);
+        }
         if (s->form == 1) {
             const gb_fast_options fo{kFastRefresh, 0, 0, nullptr, learned ? db.g : nullptr};
             st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
@@ -1252,7 +1300,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             const double bvv = db.hbvv[i];
             if (bvv < 1e299 && std::isfinite(bvv) && bvv < inc) {
                 inc = bvv;
-                inc_x.assign(db.hbv + (size_t)i * n, db.hbv + (size_t)(i + 1) * n);
+                if (db.hbv) {
+                    inc_x.assign(db.hbv + (size_t)i * n, db.hbv + (size_t)(i + 1) * n);
+                } else {
+                    // the batch's stream is idle here (its results were waited for)
+                    inc_x.resize(n);
+                    GB_HIP(hipMemcpyAsync(inc_x.data(), db.bv + (size_t)i * n, (size_t)n * 8,
+                                          hipMemcpyDeviceToHost, db.stream));
+                    GB_HIP(hipStreamSynchronize(db.stream));
+                }
                 new_inc = true;
             }
             const bool cert = std::isfinite(g) && g >= 0.0 && g < node_eps;

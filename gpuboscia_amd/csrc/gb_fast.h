@@ -46,6 +46,9 @@ int launch_batched_gradient(const double* Qt, const double* lin, double gs, doub
 // min(frac, 1 - frac) > tol), score descending with ties to the lower index,
 // -1 after the last; nodes with status >= GB_NODE_INTEGRAL_UNCONVERGED get
 // none.  out is [N][k].
+// Widens byte node bounds (0/1) into double arrays: lb[i] = lb8[i], ub[i] = ub8[i].
+int launch_expand_bounds(size_t count, const uint8_t* lb8, const uint8_t* ub8, double* lb, double* ub,
+                         void* stream);
 int launch_topk_candidates(int n, int N, const double* X, const uint8_t* flags, double tol, const float* score,
                            const uint8_t* status, int k, int* out, void* stream);
 

@@ -1034,6 +1034,26 @@ int launch_candidate_features(int n, int N, const double* X, const double* G, co
     return static_cast<int>(hipGetLastError());
 }
 
+// lb[i] = lb8[i], ub[i] = ub8[i] for i < count (0/1 node bounds sent as bytes).
+__global__ void __launch_bounds__(256) expand_bounds_kernel(size_t count, const uint8_t* lb8, const uint8_t* ub8,
+                                                            double* lb, double* ub) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        lb[i] = static_cast<double>(lb8[i]);
+        ub[i] = static_cast<double>(ub8[i]);
+    }
+}
+
+int launch_expand_bounds(size_t count, const uint8_t* lb8, const uint8_t* ub8, double* lb, double* ub,
+                         void* stream) {
+    if (count == 0) return 0;
+    const size_t want = (count + 255) / 256;
+    const unsigned blocks = static_cast<unsigned>(want < 2048 ? want : 2048);
+    hipLaunchKernelGGL(expand_bounds_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       count, lb8, ub8, lb, ub);
+    return static_cast<int>(hipGetLastError());
+}
+
 int launch_topk_candidates(int n, int N, const double* X, const uint8_t* flags, double tol, const float* score,
                            const uint8_t* status, int k, int* out, void* stream) {
     if (N <= 0 || k <= 0) return 0;
