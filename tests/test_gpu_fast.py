@@ -96,8 +96,10 @@ def test_fast_kernel_single_wave_blocks():
     assert_bit_equal(g1, o, "wpb1")
 
 
-@pytest.mark.parametrize("n,seed", [(20, 1), (20, 3), (40, 2), (60, 1)])
+@pytest.mark.parametrize("n,seed", [(20, 1), (20, 3), (40, 2), (60, 1), (80, 2)])
 def test_fast_bnb_matches_oracle(n, seed):
+    # n = 80: full 1024-node batches (byte bounds on the bus, best vertices
+    # fetched on incumbent improvements, sibling families in the frontier)
     from oracle import pyoracle as orc
     Q, c, sha = instances.binary_qp(n, seed=seed, rows=30 if n == 20 else None)
     say(f"[fast-bnb] n={n} seed={seed} sha={sha[:12]}")
