@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <condition_variable>
 #include <functional>
@@ -55,6 +56,9 @@ struct NodePool {
     std::vector<int> free_slots;
     std::vector<int> refs;
     int next = 0;
+    // device iterates: relaxation_x of slot s lives in the device store
+    // (dchunks) instead of a host chunk
+    const std::vector<double*>* dchunks = nullptr;
     int alloc() {
         int s;
         if (!free_slots.empty()) {
@@ -62,7 +66,7 @@ struct NodePool {
             free_slots.pop_back();
         } else {
             if (next % kChunk == 0) {
-                chunks.emplace_back(new double[(size_t)kChunk * (compact ? 1 : 3) * n]);
+                if (!(compact && dchunks)) chunks.emplace_back(new double[(size_t)kChunk * (compact ? 1 : 3) * n]);
                 if (compact) bchunks.emplace_back(new uint8_t[(size_t)kChunk * 2 * n]);
             }
             s = next++;
@@ -80,6 +84,10 @@ struct NodePool {
     double* ub(int slot) { return lb(slot) + n; }
     double* rx(int slot) {
         return compact ? chunks[slot / kChunk].get() + (size_t)(slot % kChunk) * n : lb(slot) + 2 * n;
+    }
+    // device address of slot's iterate (device iterates only)
+    unsigned long long drx(int slot) const {
+        return reinterpret_cast<unsigned long long>((*dchunks)[slot / kChunk] + (size_t)(slot % kChunk) * n);
     }
     // byte bounds (compact): lb8, then ub8
     uint8_t* lb8(int slot) { return bchunks[slot / kChunk].get() + (size_t)(slot % kChunk) * 2 * n; }
@@ -529,11 +537,15 @@ int most_fractional(const double* x, const std::vector<int>& iv, double tol) {
 // vertices for small batches; in large ones they stay on the device and the
 // host fetches one when it improves the incumbent).  In compact mode (0/1
 // bounds, large batches) the bounds travel as bytes and are widened into
-// device-only double arrays by a kernel.
+// device-only double arrays by a kernel.  With device iterates (xdev) x
+// crosses the bus in neither direction: the start points are gathered from
+// the device iterate store and the results scattered back into it, and only
+// the store addresses travel.
 struct DeviceBatch {
     int cap = 0, n = 0;
     bool compact = false;
     bool bv_out = true;  // best vertices in the result copy (else fetched when they improve the incumbent)
+    bool xdev = false;   // iterates stay on the device (gathered from and scattered to the iterate store)
     bool learned_io = false;  // depth, pseudocost means and chosen candidates travel with the batch
     int topk_k = 0;           // candidates returned per node
     double *x = nullptr, *lb = nullptr, *ub = nullptr, *bv = nullptr;
@@ -553,12 +565,16 @@ struct DeviceBatch {
     int *topk = nullptr, *htopk = nullptr;
     int *depth = nullptr, *hdepth = nullptr;
     double *pc = nullptr, *hpc = nullptr;
+    double *tkx = nullptr, *htkx = nullptr;   // x at the chosen candidates, [N][k]
     uint8_t *lb8 = nullptr, *hlb8 = nullptr;  // compact: lb bytes then ub bytes, [2][N][n]
+    // xdev: per node the store address of its start point and of its iterate
+    unsigned long long *xsrc = nullptr, *xdst = nullptr, *hxsrc = nullptr, *hxdst = nullptr;
     unsigned char *dpack = nullptr, *hpack = nullptr;
     size_t in_bytes = 0, out_off = 0, out_bytes = 0;
 
     struct Offsets {
-        size_t lb8, lb, ub, eps, cut, depth, pc, x, it, st, bvv, gap, val, topk, out_end, bv, dlb, dub, end;
+        size_t lb8, lb, ub, eps, cut, depth, pc, xsrc, xdst, x, it, st, bvv, gap, val, topk, tkx, out_end, bv, dlb,
+            dub, end;
     };
     Offsets offsets(int N) const {
         Offsets f{};
@@ -576,16 +592,20 @@ struct DeviceBatch {
         f.cut = take((size_t)N * 8);
         f.depth = take(learned_io ? (size_t)N * 4 : 0);
         f.pc = take(learned_io ? (size_t)n * 16 : 0);
-        f.x = take(xb);
+        f.xsrc = take(xdev ? (size_t)N * 8 : 0);
+        f.xdst = take(xdev ? (size_t)N * 8 : 0);
+        f.x = xdev ? 0 : take(xb);
         f.it = take((size_t)N * 4);
         f.st = take((size_t)N);
         f.bvv = take((size_t)N * 8);
         f.gap = take((size_t)N * 8);
         f.val = take((size_t)N * 8);
         f.topk = take(learned_io ? (size_t)N * topk_k * 4 : 0);
+        f.tkx = take(learned_io ? (size_t)N * topk_k * 8 : 0);
         if (bv_out) f.bv = take(xb);
         f.out_end = o;
         // device only
+        if (xdev) f.x = take(xb);
         if (!bv_out) f.bv = take(xb);
         f.dlb = take(compact ? xb : 0);
         f.dub = take(compact ? xb : 0);
@@ -615,7 +635,11 @@ struct DeviceBatch {
         hub = compact ? nullptr : reinterpret_cast<double*>(h(f.ub));
         heps = reinterpret_cast<double*>(h(f.eps));
         hcut = reinterpret_cast<double*>(h(f.cut));
-        hx = reinterpret_cast<double*>(h(f.x));
+        hx = xdev ? nullptr : reinterpret_cast<double*>(h(f.x));
+        xsrc = xdev ? reinterpret_cast<unsigned long long*>(d(f.xsrc)) : nullptr;
+        xdst = xdev ? reinterpret_cast<unsigned long long*>(d(f.xdst)) : nullptr;
+        hxsrc = xdev ? reinterpret_cast<unsigned long long*>(h(f.xsrc)) : nullptr;
+        hxdst = xdev ? reinterpret_cast<unsigned long long*>(h(f.xdst)) : nullptr;
         hit = reinterpret_cast<int*>(h(f.it));
         hst = h(f.st);
         hbvv = reinterpret_cast<double*>(h(f.bvv));
@@ -626,13 +650,15 @@ struct DeviceBatch {
             depth = reinterpret_cast<int*>(d(f.depth));
             pc = reinterpret_cast<double*>(d(f.pc));
             topk = reinterpret_cast<int*>(d(f.topk));
+            tkx = reinterpret_cast<double*>(d(f.tkx));
             hdepth = reinterpret_cast<int*>(h(f.depth));
             hpc = reinterpret_cast<double*>(h(f.pc));
             htopk = reinterpret_cast<int*>(h(f.topk));
+            htkx = reinterpret_cast<double*>(h(f.tkx));
         }
         in_bytes = f.bvv + (size_t)N * 8;
-        out_off = f.x;
-        out_bytes = f.out_end - f.x;
+        out_off = xdev ? f.it : f.x;
+        out_bytes = f.out_end - out_off;
     }
     gb_status upload() {
         GB_HIP(hipMemcpyAsync(dpack, hpack, in_bytes, hipMemcpyHostToDevice, stream));
@@ -677,12 +703,17 @@ struct DeviceBatch {
         // each batch set on a hardware queue of its own, so the two batches
         // in flight overlap on the device
         if (!stream) GB_HIP(create_dedicated_stream(&stream));
-        // large enough for either staging format (compact may change per batch)
-        const bool c0 = compact;
-        compact = false;
-        const size_t plain_bytes = offsets(cap).end;
+        // large enough for every staging format (compact may change per batch)
+        const bool c0 = compact, x0 = xdev;
+        size_t bytes = 0;
+        for (int c = 0; c < 2; ++c)
+            for (int d = 0; d < 2; ++d) {
+                compact = c != 0;
+                xdev = d != 0;
+                bytes = std::max(bytes, offsets(cap).end);
+            }
         compact = c0;
-        const size_t bytes = std::max(plain_bytes, offsets(cap).end);
+        xdev = x0;
         if (bytes > pack_bytes) {
             GB_HIP(hipStreamSynchronize(stream));
             if (dpack) (void)hipFree(dpack);
@@ -720,10 +751,36 @@ struct DeviceBatch {
 // milliseconds now and then.  A solve takes an idle set of its device (or a
 // new one) and returns it when it ends; concurrent solves use different sets.
 // Sets live until the process exits.
+// Device iterate store: chunks of NodePool::kChunk slots of n doubles, one
+// slot per node-pool slot, kept (like the batch buffers) for later solves.
+struct IterateStore {
+    std::vector<double*> chunks;
+    size_t chunk_bytes = 0;
+    // Chunks of `bytes` covering `slots` slots (only called with no work in
+    // flight on the store).
+    gb_status reserve(size_t slots, size_t bytes) {
+        if (bytes > chunk_bytes) {
+            for (double* p : chunks) (void)hipFree(p);
+            chunks.clear();
+            chunk_bytes = bytes;
+        }
+        while (chunks.size() * (size_t)NodePool::kChunk < slots) {
+            double* p = nullptr;
+            GB_HIP(hipMalloc(&p, chunk_bytes));
+            chunks.push_back(p);
+        }
+        return GB_OK;
+    }
+    ~IterateStore() {
+        for (double* p : chunks) (void)hipFree(p);
+    }
+};
+
 struct SolveBuffers {
     int device = -1;
     DeviceBatch bufs[2];
     DeviceBatch sb;
+    IterateStore xstore;
 };
 
 class BufferPool {
@@ -946,10 +1003,19 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         auto zo = [](double v) { return v == 1.0 || (v == 0.0 && !std::signbit(v)); };
         for (int j = 0; j < n && zero_one; ++j) zero_one = zo(pr->lb[j]) && zo(pr->ub[j]);
     }
+    // Learned searches over 0/1 boxes in the batched forms keep every node's
+    // iterate on the device: the learned rule takes its branching variables
+    // from the device, so the host never needs x (GPUBOSCIA_HOST_ITERATES=1
+    // stages iterates through the host instead; the search is the same).
+    const char* host_iterates = std::getenv("GPUBOSCIA_HOST_ITERATES");
+    const bool xdev = learned && zero_one && s->form != 0 && !(host_iterates && host_iterates[0] == '1');
     BufferPool pooled(device);
     DeviceBatch* bufs = pooled.get().bufs;
+    IterateStore& xstore = pooled.get().xstore;
+    const size_t xchunk_bytes = (size_t)NodePool::kChunk * n * 8;
     for (int k = 0; k < slots; ++k) {
         bufs[k].compact = zero_one;
+        bufs[k].xdev = xdev;
         st = bufs[k].alloc(B, n, learned, s->branch_k);
         if (st != GB_OK) return st;
         if (learned || lab) {
@@ -1006,6 +1072,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     NodePool pool;
     pool.n = n;
     pool.compact = zero_one;
+    if (xdev) {
+        st = xstore.reserve(1, xchunk_bytes);
+        if (st != GB_OK) return st;
+        pool.dchunks = &xstore.chunks;
+    }
     {
         HostNode root;
         root.id = 0;
@@ -1020,7 +1091,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             std::copy(pr->ub, pr->ub + n, pool.ub(root.slot));
         }
         if (s->root_initial_point && s->root_initial_point_size == n) {
-            std::copy(s->root_initial_point, s->root_initial_point + n, pool.rx(root.slot));
+            if (xdev)
+                GB_HIP(hipMemcpy(reinterpret_cast<double*>(pool.drx(root.slot)), s->root_initial_point,
+                                 (size_t)n * 8, hipMemcpyHostToDevice));
+            else
+                std::copy(s->root_initial_point, s->root_initial_point + n, pool.rx(root.slot));
             root.has_rx = true;
         }
         root.bound = s->natural_lower_bound;
@@ -1038,6 +1113,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     std::vector<double> open_min[2];
     size_t open_pos[2] = {0, 0};
     std::vector<HostNode> batches[2];
+    // device iterates: parents' slots read by each slot's gather, released
+    // once that batch has finished
+    std::vector<int> gathered[2];
     auto in_flight = [&](int k) { return open_pos[k] < open_min[k].size(); };
 
     auto live_bound = [&]() {
@@ -1089,11 +1167,21 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         db.compact = pool.compact && (size_t)N * n >= 16384;
         db.bv_out = (size_t)N * n < 16384;
         db.layout(N);
+        if (db.xdev) {
+            // store slots for every pool slot handed out so far
+            st = xstore.reserve((size_t)pool.next, xchunk_bytes);
+            if (st != GB_OK) return st;
+        }
         // Staging copies on the host threads (the pool is only read here).
         HostThreads::get().parallel_for(N, [&](int i) {
             const HostNode& nd = batch[i];
-            double* hx = db.hx + (size_t)i * n;
+            double* hx = db.xdev ? nullptr : db.hx + (size_t)i * n;
             const bool warm = nd.has_rx || nd.xslot >= 0;
+            if (db.xdev) {
+                // start point: own iterate as is, the parent's clipped (bit 0), or lb
+                db.hxsrc[i] = nd.has_rx ? pool.drx(nd.slot) : nd.xslot >= 0 ? (pool.drx(nd.xslot) | 1ull) : 0ull;
+                db.hxdst[i] = pool.drx(nd.slot);
+            }
             if (pool.compact) {
                 const uint8_t* l8 = pool.lb8(nd.slot);
                 const uint8_t* u8 = pool.ub8(nd.slot);
@@ -1109,7 +1197,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         hu[j] = u8[j];
                     }
                 }
-                if (nd.has_rx) {
+                if (db.xdev) {
+                    // gathered on the device
+                } else if (nd.has_rx) {
                     std::copy(pool.rx(nd.slot), pool.rx(nd.slot) + n, hx);
                 } else if (nd.xslot >= 0) {
                     const double* px = pool.rx(nd.xslot);
@@ -1143,7 +1233,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         });
         for (HostNode& nd : batch)
             if (nd.xslot >= 0) {
-                pool.release(nd.xslot);
+                if (db.xdev) gathered[slot].push_back(nd.xslot);  // the gather below reads it
+                else pool.release(nd.xslot);
                 nd.xslot = -1;
             }
         open_min[slot].resize(N);
@@ -1164,6 +1255,10 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             const int ee = launch_expand_bounds((size_t)N * n, db.lb8, db.lb8 + (size_t)N * n, db.lb, db.ub,
                                                db.stream);
             if (ee != 0) return hip_fail(static_cast<hipError_t>(ee), __FILE__, __LINE__);
+        }
+        if (db.xdev) {
+            const int ge = launch_gather_starts(n, N, db.xsrc, db.lb, db.ub, db.x, db.stream);
+            if (ge != 0) return hip_fail(static_cast<hipError_t>(ge), __FILE__, __LINE__);
         }
         if (s->form == 1) {
             const gb_fast_options fo{kFastRefresh, 0, 0, nullptr, learned ? db.g : nullptr};
@@ -1205,8 +1300,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             st = gb_brancher_predict(s->learned_model, db.feat, N * n, db.score, db.stream);
             if (st != GB_OK) return st;
             const int te = launch_topk_candidates(n, N, db.x, db.flags, s->integer_tol, db.score, db.st,
-                                                  s->branch_k, db.topk, db.stream);
+                                                  s->branch_k, db.topk, db.tkx, db.stream);
             if (te != 0) return hip_fail(static_cast<hipError_t>(te), __FILE__, __LINE__);
+        }
+        if (db.xdev) {
+            const int se = launch_scatter_iterates(n, N, db.xdst, db.x, db.stream);
+            if (se != 0) return hip_fail(static_cast<hipError_t>(se), __FILE__, __LINE__);
         }
         st = db.download();
         if (st != GB_OK) return st;
@@ -1223,6 +1322,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     std::vector<ChildTask> child_tasks;
     std::vector<XTask> x_tasks;
     std::vector<int> deferred, task_vars, scan_vars, vars;  // vars: one node's branching variables
+    std::vector<double> xrow;                               // one node's x fetched from the device
     std::vector<HostNode> kids;                             // one node's children, pushed as a family
     int head = 0;
     while (!finished) {
@@ -1251,6 +1351,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             GB_HIP(hipStreamSynchronize(db.stream));
             relax_s += std::chrono::duration<double>(Clock::now() - w0).count();
         }
+        for (int sl : gathered[head]) pool.release(sl);
+        gathered[head].clear();
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
         // Branching variables of the rules that depend only on x, on the host
@@ -1319,10 +1421,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 finish_check(remaining > 0);
                 continue;
             }
-            const double* xi = db.hx + (size_t)i * n;
+            // null with device iterates (the scatter kept x in the node's store slot)
+            const double* xi = db.hx ? db.hx + (size_t)i * n : nullptr;
             if (db.hst[i] >= GB_NODE_INTEGRAL_UNCONVERGED && !cert) {
                 // device_resident_solver.cuh:1970-1980: keep the node live.
-                x_tasks.push_back({nd.slot, i});
+                if (!db.xdev) x_tasks.push_back({nd.slot, i});
                 nd.has_rx = true;
                 nd.riters = db.hit[i];
                 q.push(nd);
@@ -1353,6 +1456,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 }
             } else if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
                 vars.clear();
+                const double* xr = xi;        // x of the node, where the host has it
+                const double* tkx = nullptr;  // x at the chosen candidates (learned rule)
                 if (prescan) {
                     const int* sv = scan_vars.data() + (size_t)i * bk;
                     for (int c = 0; c < bk && sv[c] >= 0; ++c) vars.push_back(sv[c]);
@@ -1377,9 +1482,18 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     // on the device), split 2^k ways like branch_pow2
                     const int* tk = db.htopk + (size_t)i * s->branch_k;
                     for (int c = 0; c < s->branch_k && tk[c] >= 0; ++c) vars.push_back(tk[c]);
+                    tkx = db.htkx + (size_t)i * s->branch_k;
                 } else {
                     // the learned rule where no candidate was scored
-                    const int v = most_fractional(xi, iv, s->integer_tol);
+                    if (!xr) {
+                        // device iterates: this node's row (the stream is idle)
+                        xrow.resize(n);
+                        GB_HIP(hipMemcpyAsync(xrow.data(), db.x + (size_t)i * n, (size_t)n * 8,
+                                              hipMemcpyDeviceToHost, db.stream));
+                        GB_HIP(hipStreamSynchronize(db.stream));
+                        xr = xrow.data();
+                    }
+                    const int v = most_fractional(xr, iv, s->integer_tol);
                     if (v >= 0) vars.push_back(v);
                 }
                 if (lab && lab->labeled < lab->max_nodes) {
@@ -1409,7 +1523,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         // iterate, kept once in the parent's slot and clipped to
                         // the child's box when the child is staged.
                         if (!parent_x_kept) {
-                            x_tasks.push_back({nd.slot, i});
+                            if (!db.xdev) x_tasks.push_back({nd.slot, i});
                             parent_x_kept = true;
                         }
                         pool.retain(nd.slot);
@@ -1419,7 +1533,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                         const int vi = vars[0];
                         c.bvar = vi;
                         c.bdir = mask & 1;
-                        c.bfrac = c.bdir ? std::ceil(xi[vi]) - xi[vi] : xi[vi] - std::floor(xi[vi]);
+                        const double xv = tkx ? tkx[0] : xr[vi];
+                        c.bfrac = c.bdir ? std::ceil(xv) - xv : xv - std::floor(xv);
                         c.pbound = nd.bound;
                     }
                     kids.push_back(c);
@@ -1435,7 +1550,6 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         // kept iterates, on the host threads; then the parents' slots go back.
         HostThreads::get().parallel_for((int)child_tasks.size(), [&](int t) {
             const ChildTask& ct = child_tasks[t];
-            const double* xr = db.hx + (size_t)ct.row * n;
             if (pool.compact) {
                 // branched values are fractional, so ceil is 1 and floor 0
                 uint8_t* cl8 = pool.lb8(ct.child);
@@ -1447,6 +1561,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 }
                 return;
             }
+            const double* xr = db.hx + (size_t)ct.row * n;
             double* clb = pool.lb(ct.child);
             double* cub = pool.ub(ct.child);
             std::copy(pool.lb(ct.parent), pool.lb(ct.parent) + 2 * n, clb);  // lb and ub

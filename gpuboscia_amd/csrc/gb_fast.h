@@ -42,14 +42,20 @@ int launch_candidate_features(int n, int N, const double* X, const double* G, co
                               double gs, const uint8_t* flags, double tol, float* F, void* stream);
 int launch_batched_gradient(const double* Qt, const double* lin, double gs, double ls, int n, int N,
                             const double* X, double* G, void* stream);
-// Per node the k best-scored fractional candidates (flags[j] set and
-// min(frac, 1 - frac) > tol), score descending with ties to the lower index,
-// -1 after the last; nodes with status >= GB_NODE_INTEGRAL_UNCONVERGED get
-// none.  out is [N][k].
 // Widens byte node bounds (0/1) into double arrays: lb[i] = lb8[i], ub[i] = ub8[i].
 int launch_expand_bounds(size_t count, const uint8_t* lb8, const uint8_t* ub8, double* lb, double* ub,
                          void* stream);
+// Per node the k best-scored fractional candidates (flags[j] set and
+// min(frac, 1 - frac) > tol), score descending with ties to the lower index,
+// -1 after the last; nodes with status >= GB_NODE_INTEGRAL_UNCONVERGED get
+// none.  out is [N][k]; xout (may be NULL) receives X at each candidate.
 int launch_topk_candidates(int n, int N, const double* X, const uint8_t* flags, double tol, const float* score,
-                           const uint8_t* status, int k, int* out, void* stream);
+                           const uint8_t* status, int k, int* out, double* xout, void* stream);
+// Device iterate store.  Gather: x[i] = the iterate at src[i] (bit 0 set:
+// clipped to [lb[i], ub[i]]), or lb[i] when src[i] is 0.  Scatter: node i's
+// x to the address dst[i] (0: skipped).  src, dst are [N] device addresses.
+int launch_gather_starts(int n, int N, const unsigned long long* src, const double* lb, const double* ub, double* x,
+                         void* stream);
+int launch_scatter_iterates(int n, int N, const unsigned long long* dst, const double* x, void* stream);
 
 }  // namespace gb

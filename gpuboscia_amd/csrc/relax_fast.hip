@@ -971,7 +971,8 @@ __global__ void __launch_bounds__(64) candidate_features_kernel(
 // the previous pick in the order (score descending, index ascending).
 __global__ void __launch_bounds__(64) topk_candidates_kernel(int n, int N, const double* X, const uint8_t* flags,
                                                              double tol, const float* score,
-                                                             const uint8_t* status, int k, int* out) {
+                                                             const uint8_t* status, int k, int* out,
+                                                             double* xout) {
     const int node = blockIdx.x;
     if (node >= N) return;
     const int lane = threadIdx.x;
@@ -1006,7 +1007,10 @@ __global__ void __launch_bounds__(64) topk_candidates_kernel(int n, int N, const
             }
         }
         if (bj == 0x7FFFFFFF) more = false;
-        if (lane == 0) out[static_cast<size_t>(node) * k + r] = more ? bj : -1;
+        if (lane == 0) {
+            out[static_cast<size_t>(node) * k + r] = more ? bj : -1;
+            if (xout) xout[static_cast<size_t>(node) * k + r] = more ? X[base + bj] : 0.0;
+        }
         last_s = bs;
         last_j = bj;
     }
@@ -1055,10 +1059,54 @@ int launch_expand_bounds(size_t count, const uint8_t* lb8, const uint8_t* ub8, d
 }
 
 int launch_topk_candidates(int n, int N, const double* X, const uint8_t* flags, double tol, const float* score,
-                           const uint8_t* status, int k, int* out, void* stream) {
+                           const uint8_t* status, int k, int* out, double* xout, void* stream) {
     if (N <= 0 || k <= 0) return 0;
     hipLaunchKernelGGL(topk_candidates_kernel, dim3(N), dim3(kWave), 0, reinterpret_cast<hipStream_t>(stream),
-                       n, N, X, flags, tol, score, status, k, out);
+                       n, N, X, flags, tol, score, status, k, out, xout);
+    return static_cast<int>(hipGetLastError());
+}
+
+// Start points from the device iterate store.  src[i] is the address of a
+// stored iterate of n doubles, with bit 0 set when it is to be clipped to
+// node i's box (a parent's iterate) and clear when it is copied as is (the
+// node's own iterate), or 0 for the node's lower bound.
+__global__ void __launch_bounds__(128) gather_starts_kernel(int n, const unsigned long long* src, const double* lb,
+                                                            const double* ub, double* x) {
+    const int node = blockIdx.x;
+    const unsigned long long a = src[node];
+    const size_t base = static_cast<size_t>(node) * n;
+    const double* p = reinterpret_cast<const double*>(a & ~1ull);
+    const bool clip = (a & 1ull) != 0ull;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double l = lb[base + j];
+        double v = l;
+        if (p) v = clip ? fmin(fmax(p[j], l), ub[base + j]) : p[j];
+        x[base + j] = v;
+    }
+}
+
+// Iterates into the store: dst[i] (or 0 to skip) receives node i's x.
+__global__ void __launch_bounds__(128) scatter_iterates_kernel(int n, const unsigned long long* dst,
+                                                               const double* x) {
+    const int node = blockIdx.x;
+    double* p = reinterpret_cast<double*>(dst[node]);
+    if (!p) return;
+    const size_t base = static_cast<size_t>(node) * n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) p[j] = x[base + j];
+}
+
+int launch_gather_starts(int n, int N, const unsigned long long* src, const double* lb, const double* ub, double* x,
+                         void* stream) {
+    if (N <= 0 || n <= 0) return 0;
+    hipLaunchKernelGGL(gather_starts_kernel, dim3(N), dim3(128), 0, reinterpret_cast<hipStream_t>(stream), n, src,
+                       lb, ub, x);
+    return static_cast<int>(hipGetLastError());
+}
+
+int launch_scatter_iterates(int n, int N, const unsigned long long* dst, const double* x, void* stream) {
+    if (N <= 0 || n <= 0) return 0;
+    hipLaunchKernelGGL(scatter_iterates_kernel, dim3(N), dim3(128), 0, reinterpret_cast<hipStream_t>(stream), n,
+                       dst, x);
     return static_cast<int>(hipGetLastError());
 }
 

@@ -126,3 +126,26 @@ def test_node_budget_comparison_at_n200():
             assert np.isfinite(r[f"{key}_bound"]) and r[f"{key}_bound"] <= r[f"{key}_objective"] + 1e-9
     s = learned.budget_summary(rows)
     assert s["instances"] == 3 and s["learned_bound_higher"] + s["learned_bound_lower"] <= 3
+
+
+@pytest.mark.parametrize("n,seed,form", [(30, 2, 2), (60, 1, 2), (80, 2, 2), (60, 3, 1)])
+def test_device_iterates_give_the_same_search(n, seed, form, monkeypatch):
+    """Learned searches keep every node's iterate on the device (gathered into
+    each batch's start points, scattered back after it); staging the iterates
+    through the host instead gives the same tree, bit for bit.  n = 30 runs
+    batches below the byte-bound size (double bounds on the bus), n = 80 full
+    batches with byte bounds."""
+    import os
+    from gpuboscia_amd import instances, learned
+    path = os.path.join(os.path.dirname(gb.__file__), "scorer_binary_qp_n40.npz")
+    b = gb.Brancher.from_file(path)
+    Q, c, _ = instances.binary_qp(n, seed=seed)
+    monkeypatch.delenv("GPUBOSCIA_HOST_ITERATES", raising=False)
+    dev = learned.solve_with(Q, c, gb.BRANCH_LEARNED, b, form=form)
+    monkeypatch.setenv("GPUBOSCIA_HOST_ITERATES", "1")
+    host = learned.solve_with(Q, c, gb.BRANCH_LEARNED, b, form=form)
+    say(f"[learned] n={n} seed={seed} form={form}: {dev['nodes_processed']} nodes "
+        f"device {dev['wall']:.4f} s, host {host['wall']:.4f} s")
+    assert not dev["reached_time_limit"]
+    for key in ("nodes_processed", "fw_iterations", "objective", "dual_bound"):
+        assert dev[key] == host[key], key
