@@ -48,6 +48,9 @@ struct MixedParams {
     int* counter;
     float rel_floor;        // stop once gap < max(eps, rel_floor (1 + |f|))
     long long* stats;       // optional [N]: iterations of the mixed phase
+    // optional [N][n]: Q x of the nodes with a supplied point and no
+    // iterations (launch_mixed_warm_products); such nodes start at x itself
+    const double* qx0;
 };
 
 int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double ls, int n, int rt,
@@ -107,6 +110,14 @@ MixedShape mixed_default_shape(int rt, int N);
 
 // rw: row tiles per wave (1..4); the tile runs ceil(rt / rw) waves.
 int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void* stream);
+// Shapes whose pass 1 starts a node with a supplied point at that point
+// (the others start at the box vertex nearest to it).
+constexpr bool mixed_warm_start(int rw, bool wide) { return !wide && rw != 3; }
+// out[i] = Q x[i] in FP64 (k-ordered FMA chains over Qt) for every node i
+// with status GB_NODE_INITIAL_POINT_SUPPLIED and iters[i] == 0; other rows
+// of out are left as they are.  n <= kMixedMaxN.
+int launch_mixed_warm_products(const double* Qt, int n, int N, const double* X, const uint8_t* status,
+                               const int* iters, double* out, void* stream);
 int launch_mixed_products(const MixedQ& q, const double* V, int N, double* Y, void* stream);
 
 }  // namespace gb

@@ -153,6 +153,15 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
             if (op.blocks) sh.blocks = op.blocks;
             else sh.blocks = (N + sh.cols - 1) / sh.cols;
             P.cols = sh.cols;
+            if (mixed_warm_start(sh.rw, q->rt > kMixedSmallRowTiles)) {
+                // Nodes with a supplied point start there: Q x in FP64 goes
+                // into their best-vertex rows, which pass 1 reads when it takes
+                // the node and overwrites when it retires it.
+                const int we = launch_mixed_warm_products(objective->Qt, q->n, N, x, status, iterations,
+                                                          best_vertex, stream);
+                if (we != 0) return hip_fail(static_cast<hipError_t>(we), __FILE__, __LINE__);
+                P.qx0 = best_vertex;
+            }
             const int e = launch_relax_mixed(P, algorithm->algorithm == GB_ALG_DICG, sh.rw, sh.blocks, stream);
             if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
         }

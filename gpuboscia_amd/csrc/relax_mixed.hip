@@ -184,6 +184,9 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
     constexpr int G = 16 / RPT;  // lanes of a column in the element layout
     constexpr int E = RW * RPT;  // elements per lane: rows 16 t_j + RPT m + r, e = RPT j + r
     constexpr unsigned long long kGroupMask = (1ull << G) - 1ull;
+    // Starting at a supplied point needs registers that the 3-tile and the
+    // wide (streamed) shapes do not have; their nodes start at a vertex.
+    constexpr bool kWarmStart = mixed_warm_start(RW, BIG);
     extern __shared__ __align__(16) unsigned char mixed_smem[];
     const int RT = P.rt;
     const int S = RT / 2;
@@ -266,12 +269,15 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             bool ok = true;
             uint32_t lm = 0, um = 0, xm = 0;
             int init_it = 0;
+            bool warm = false;
             if (loading) {
                 const double* lbn = P.lb + static_cast<size_t>(node) * n;
                 const double* ubn = P.ub + static_cast<size_t>(node) * n;
                 const double* xn = P.x + static_cast<size_t>(node) * n;
                 init_it = P.iters[node];
                 const bool from_x = init_it != 0 || (P.status[node] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0;
+                // a supplied point with Q x at hand: the column starts at x itself
+                warm = kWarmStart && P.qx0 != nullptr && from_x && init_it == 0;
 #pragma unroll 1
                 for (int e = 0; e < E; ++e) {
                     const int t = w + W * (e / RPT);
@@ -293,7 +299,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             bool all_ok = true;
             for (int v = 0; v < W; ++v) all_ok = all_ok && okw[v * 16 + col] != 0;
             if (loading && all_ok) {
-                phase = kPhaseInit;
+                phase = warm ? kPhaseActive : kPhaseInit;
                 lbm = lm;
                 ubm = um;
                 x0m = xm;
@@ -303,6 +309,25 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                 cutoff = P.cutoff ? P.cutoff[node] : __builtin_inf();
                 best = kNone;
                 bestm = 0;
+                if (kWarmStart && warm) {
+                    // x clipped to the box and Q x (the column's registers are
+                    // free while it fetches)
+                    const double* xn = P.x + static_cast<size_t>(node) * n;
+                    const double* qn = P.qx0 + static_cast<size_t>(node) * n;
+#pragma unroll
+                    for (int e = 0; e < E; ++e) {
+                        const int t = w + W * (e / RPT);
+                        const int row = 16 * t + RPT * m + e % RPT;
+                        const float lf = ((lm >> e) & 1u) ? 1.0f : 0.0f;
+                        const float uf = ((um >> e) & 1u) ? 1.0f : 0.0f;
+                        x[e] = 0.0f;
+                        qx[e] = 0.0f;
+                        if (t < RT && row < n) {
+                            x[e] = fminf(fmaxf(static_cast<float>(xn[row]), lf), uf);
+                            qx[e] = static_cast<float>(qn[row]);
+                        }
+                    }
+                }
             }
             __syncthreads();
         }
@@ -656,7 +681,52 @@ __global__ void __launch_bounds__(64) mixed_products_kernel(const uint4* frag, i
 
 __global__ void mixed_reset_kernel(int* counter) { *counter = 0; }
 
+// Q x for the nodes with a supplied point and no iterations, in FP64.  A
+// workgroup takes kWarmNodes nodes and one output row per thread, so Qt is
+// read once per kWarmNodes nodes; every output is the k-ordered FMA chain
+// sum_k Qt[k][j] x[k].
+constexpr int kWarmNodes = 8;
+__global__ void __launch_bounds__(256) mixed_warm_products_kernel(const double* Qt, int n, int N, const double* X,
+                                                                  const uint8_t* status, const int* iters,
+                                                                  double* out) {
+    __shared__ double xs[kWarmNodes][kMixedMaxN];
+    __shared__ int warm[kWarmNodes];
+    const int node0 = blockIdx.x * kWarmNodes;
+    if (threadIdx.x < kWarmNodes) {
+        const int i = node0 + static_cast<int>(threadIdx.x);
+        warm[threadIdx.x] = (i < N && (status[i] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0 && iters[i] == 0) ? 1 : 0;
+    }
+    __syncthreads();
+    for (int v = 0; v < kWarmNodes; ++v)
+        for (int k = threadIdx.x; k < n; k += blockDim.x)
+            xs[v][k] = warm[v] ? X[static_cast<size_t>(node0 + v) * n + k] : 0.0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double acc[kWarmNodes];
+#pragma unroll
+        for (int v = 0; v < kWarmNodes; ++v) acc[v] = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double q = Qt[static_cast<size_t>(k) * n + j];
+#pragma unroll
+            for (int v = 0; v < kWarmNodes; ++v) acc[v] = fma(q, xs[v][k], acc[v]);
+        }
+#pragma unroll
+        for (int v = 0; v < kWarmNodes; ++v)
+            if (warm[v]) out[static_cast<size_t>(node0 + v) * n + j] = acc[v];
+    }
+}
+
 }  // namespace
+
+int launch_mixed_warm_products(const double* Qt, int n, int N, const double* X, const uint8_t* status,
+                               const int* iters, double* out, void* stream) {
+    if (N <= 0 || n <= 0) return 0;
+    if (n > kMixedMaxN) return static_cast<int>(hipErrorInvalidValue);
+    const unsigned blocks = static_cast<unsigned>((N + kWarmNodes - 1) / kWarmNodes);
+    hipLaunchKernelGGL(mixed_warm_products_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       Qt, n, N, X, status, iters, out);
+    return static_cast<int>(hipGetLastError());
+}
 
 int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double ls, int n, int rt,
                          int planes, uint4* frag, float* lsc, void* stream) {

@@ -234,9 +234,12 @@ typedef struct {
  * batched kernel (gb_relax_batch_fast) continues from that iterate,
  * re-scores the vertex exactly, and returns FP64 gap, value and
  * certificate, so pruning decisions and incumbents are FP64.  Pass 1
- * starts a node at the box vertex nearest to its supplied point (the upper
- * bounds for a fresh node); nodes whose bounds are not all 0/1 and
- * continuations (iterations > 0 on entry) skip pass 1.  fp64_max_iters < 0
+ * starts a node at its supplied point (status
+ * GB_NODE_INITIAL_POINT_SUPPLIED), with Q x computed in FP64 into the
+ * node's best_vertex row first, or at its upper bounds; nodes whose bounds
+ * are not all 0/1 and continuations (iterations > 0 on entry) skip pass 1.
+ * best_vertex rows hold a vertex on return only where best_vertex_val is
+ * below 1e300.  fp64_max_iters < 0
  * runs pass 1 only (diagnostics).  Buffers as gb_relax_batch_fast;
  * best_vertex_val must be 1e300 on entry (as for the reference kernel).
  */
