@@ -700,7 +700,7 @@ struct DeviceBatch {
         n = nv;
         learned_io = learned;
         topk_k = k;
-        // each batch set on a hardware queue of its own, so the two batches
+        // each batch set on a hardware queue of its own, so the batches
         // in flight overlap on the device
         if (!stream) GB_HIP(create_dedicated_stream(&stream));
         // large enough for every staging format (compact may change per batch)
@@ -745,7 +745,7 @@ struct DeviceBatch {
     }
 };
 
-// Batch buffer sets (two pipeline slots and the labeller's child batch, with
+// Batch buffer sets (the pipeline slots and the labeller's child batch, with
 // their streams and pinned staging) are kept per device and reused by later
 // solves: creating streams and pinning host memory per solve cost
 // milliseconds now and then.  A solve takes an idle set of its device (or a
@@ -776,9 +776,11 @@ struct IterateStore {
     }
 };
 
+constexpr int kMaxSlots = 3;  // pipeline slots (batches in flight) of one search
+
 struct SolveBuffers {
     int device = -1;
-    DeviceBatch bufs[2];
+    DeviceBatch bufs[kMaxSlots];
     DeviceBatch sb;
     IterateStore xstore;
 };
@@ -986,12 +988,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     const int B = s->gpu_batch_size;
 
     const bool learned = s->branch_rule == GB_BRANCH_LEARNED && s->learned_model != nullptr;
-    // The batched forms keep two batches in flight (num_streams >= 2): the
-    // next batch is chosen, uploaded and launched before the current one is
-    // post-processed, so the host's tree work overlaps the GPU's relaxations.
-    // The plain form keeps the reference schedule (one batch at a time).
+    // The batched forms keep several batches in flight (num_streams >= 2):
+    // the next batches are chosen, uploaded and launched before the current
+    // one is post-processed, so the host's tree work overlaps the GPU's
+    // relaxations, and one batch's tail (its slowest nodes) overlaps the
+    // others.  Two slots for the batched FP64 form, three for the mixed form
+    // (and as many as num_streams allows).  The plain form keeps the
+    // reference schedule (one batch at a time).
     const bool pipelined = s->form != 0 && s->num_streams >= 2;
-    const int slots = pipelined ? 2 : 1;
+    const int slots = !pipelined ? 1 : std::min(s->form == 2 ? 3 : 2, std::min(s->num_streams, kMaxSlots));
     int device = 0;
     GB_HIP(hipGetDevice(&device));
     // 0/1 problems keep node bounds as bytes (every root bound +0 or 1, and
@@ -1052,11 +1057,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         st = gb_mixed_prepare(o, s->precision == GB_PREC_BF16 ? 1 : 3, bufs[0].stream, &h);
         if (st != GB_OK) return st;
         mq.reset(h);
-        GB_HIP(hipStreamSynchronize(bufs[0].stream));  // both slots' streams use the planes
+        GB_HIP(hipStreamSynchronize(bufs[0].stream));  // every slot's stream uses the planes
     }
     gb_mixed_options tune{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
     bool tuned = false;
-    // Every exit drains both slots' streams before the mixed planes and the
+    // Every exit drains every slot's stream before the mixed planes and the
     // batch buffers are released (a second batch may still be in flight).
     struct DrainOnExit {
         DeviceBatch* b;
@@ -1110,18 +1115,18 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     // Bounds of the nodes of each slot's batch not yet post-processed: nodes
     // leave in batch order, so the smallest is the suffix minimum at the
     // slot's first open position (pos == size: nothing in flight).
-    std::vector<double> open_min[2];
-    size_t open_pos[2] = {0, 0};
-    std::vector<HostNode> batches[2];
+    std::vector<double> open_min[kMaxSlots];
+    size_t open_pos[kMaxSlots] = {};
+    std::vector<HostNode> batches[kMaxSlots];
     // device iterates: parents' slots read by each slot's gather, released
     // once that batch has finished
-    std::vector<int> gathered[2];
+    std::vector<int> gathered[kMaxSlots];
     auto in_flight = [&](int k) { return open_pos[k] < open_min[k].size(); };
 
     auto live_bound = [&]() {
         double b = inc;
         if (!q.empty()) b = std::min(b, q.top_bound());
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < slots; ++k)
             if (in_flight(k)) b = std::min(b, open_min[k][open_pos[k]]);
         return b;
     };
@@ -1337,15 +1342,20 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             if (st != GB_OK) return st;
             if (!in_flight(head)) { final_bound = inc; finished = true; break; }
         }
-        const int other = 1 - head;
-        if (pipelined && !in_flight(other)) {
-            st = launch_batch(other);
-            if (st != GB_OK) return st;
+        // The other slots in launch order after head: idle ones take the next
+        // batches; their nodes count as open while head's are processed.
+        int queued_elsewhere = 0;
+        for (int d = 1; d < slots; ++d) {
+            const int k = (head + d) % slots;
+            if (!in_flight(k)) {
+                st = launch_batch(k);
+                if (st != GB_OK) return st;
+            }
+            if (in_flight(k)) queued_elsewhere += (int)batches[k].size();
         }
         DeviceBatch& db = bufs[head];
         std::vector<HostNode>& batch = batches[head];
         const int N = (int)batch.size();
-        const int queued_elsewhere = pipelined && in_flight(other) ? (int)batches[other].size() : 0;
         {
             const auto w0 = Clock::now();
             GB_HIP(hipStreamSynchronize(db.stream));
@@ -1576,8 +1586,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             std::copy(xr, xr + n, pool.rx(x_tasks[t].slot));
         });
         for (int sl : deferred) pool.release(sl);
-        if (!finished && q.empty() && !in_flight(other)) { final_bound = inc; finished = true; }
-        if (pipelined && in_flight(other)) head = other;
+        // next: the oldest batch in flight (slots are launched cyclically)
+        int next = -1;
+        for (int d = 1; d < slots && next < 0; ++d)
+            if (in_flight((head + d) % slots)) next = (head + d) % slots;
+        if (!finished && q.empty() && next < 0) { final_bound = inc; finished = true; }
+        if (next >= 0) head = next;
     }
     if (final_bound == -kInf) final_bound = q.empty() ? inc : std::min(inc, q.top_bound());
     res->objective = inc;
