@@ -319,10 +319,11 @@ def time_to_optimum(args):
     for n in sizes:
         rows, seeds = 3 * n // 2, (1, 2, 3)
         # an untimed, node-capped solve per form and size first: the first
-        # launches of the kernels of this size
+        # launches of the kernels of this size (full batches on every
+        # pipeline slot, so each kernel variant is loaded)
         Qw, cw, _ = instances.binary_qp(n, seed=9, rows=rows)
         for key, form, learned in forms:
-            solve_row(gb, Qw, cw, n, form, learned, max_nodes=64)
+            solve_row(gb, Qw, cw, n, form, learned, max_nodes=4096)
         for seed in seeds:
             Q, c, sha = instances.binary_qp(n, seed=seed, rows=rows)
             row = {"n": n, "seed": seed, "sha256": sha[:16]}

@@ -702,7 +702,8 @@ struct DeviceBatch {
         topk_k = k;
         // each batch set on a hardware queue of its own, so the batches
         // in flight overlap on the device
-        if (!stream) GB_HIP(create_dedicated_stream(&stream));
+        const bool new_stream = stream == nullptr;
+        if (new_stream) GB_HIP(create_dedicated_stream(&stream));
         // large enough for every staging format (compact may change per batch)
         const bool c0 = compact, x0 = xdev;
         size_t bytes = 0;
@@ -731,6 +732,13 @@ struct DeviceBatch {
             flags_bytes = 0;
             GB_HIP(hipMalloc(&flags, n));
             flags_bytes = (size_t)n;
+        }
+        if (new_stream) {
+            // The runtime creates a stream's hardware queue at its first
+            // dispatch (several milliseconds): pay that here, not inside the
+            // first search that fills this slot.
+            GB_HIP(hipMemsetAsync(flags, 0, (size_t)n, stream));
+            GB_HIP(hipStreamSynchronize(stream));
         }
         layout(cap);
         return GB_OK;
