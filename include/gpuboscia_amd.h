@@ -442,8 +442,9 @@ typedef enum {
 typedef struct {
     /* DeviceResidentConfig */
     int gpu_batch_size;       /* 1024 */
-    int num_streams;          /* 8; forms 1 and 2 keep two node batches in flight when >= 2
-                                 (1: one batch at a time); the plain form always keeps one */
+    int num_streams;          /* 8; with >= 2 the batched FP64 form keeps two node batches in
+                                 flight and the mixed form min(3, num_streams) (1: one batch
+                                 at a time); the plain form always keeps one */
     int gpu_block_threads;    /* 0 = automatic; else a multiple of 64 */
     int gpu_max_iters;        /* 20000 */
     double gpu_eps;           /* 1e-3 */
