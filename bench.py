@@ -21,7 +21,7 @@ statuses (fill kernels) and relaxes the batch again.
              same 1024 nodes and budget (about 8 s), 16 host threads
   time_to_optimum  gb_solve to the proven optimum (abs = rel gap 1e-6) on
              binary QPs of n = 20, 40, 60, 80, 100 (three seeds each): plain
-             vs batched vs mixed vs learned + mixed, and the geometric-mean
+             vs batched vs mixed vs learned + batched vs learned + mixed, and the geometric-mean
              speedups over plain per n
 
 Prints ONE JSON line.
@@ -305,12 +305,14 @@ def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0):
 def time_to_optimum(args):
     """gb_solve to the proven optimum (abs = rel gap 1e-6): plain form
     (reference defaults), batched FP64 form, mixed form (BF16 MFMA + FP64
-    certification) and learned + mixed, on the tractable binary-QP sizes."""
+    certification), and the learned rule with the batched and with the mixed
+    form, on the tractable binary-QP sizes."""
     import gpuboscia_amd as gb
     from gpuboscia_amd import instances
     scorer = gb.Brancher.from_file(SCORER) if os.path.exists(SCORER) else None
     forms = [("plain", 0, None), ("batched", 1, None), ("mixed", 2, None)]
     if scorer is not None:
+        forms.append(("learned_batched", 1, scorer))
         forms.append(("learned_mixed", 2, scorer))
     keep = set(args.tto_forms.split(","))
     forms = [f for f in forms if f[0] in keep]
@@ -361,7 +363,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tto", action="store_true")
     ap.add_argument("--tto-sizes", default="20,40,60,80,100", help="binary-QP sizes of time_to_optimum")
-    ap.add_argument("--tto-forms", default="plain,batched,mixed,learned_mixed",
+    ap.add_argument("--tto-forms", default="plain,batched,mixed,learned_batched,learned_mixed",
                     help="forms of time_to_optimum (speedups need plain)")
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
@@ -406,7 +408,7 @@ def main():
     result = {
         "metric": "FW iterations/s of the node relaxation on n=200 binary-QP nodes (mixed-precision form: BF16 "
                   "MFMA pass + FP64 certification); time-to-proven-optimum @gap 1e-6 plain vs batched / mixed / "
-                  "learned+mixed in time_to_optimum",
+                  "learned+batched / learned+mixed in time_to_optimum",
         "value": m["value"], "unit": "FW iterations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": m["wall_s"] / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
@@ -456,7 +458,7 @@ def main():
         tto = time_to_optimum(args)
         result["time_to_optimum"] = tto
         gm = {}
-        for key in ("batched", "mixed", "learned_mixed"):
+        for key in ("batched", "mixed", "learned_batched", "learned_mixed"):
             per_n = {}
             for nn in sorted({r["n"] for r in tto}):
                 sp = [r[f"speedup_{key}"] for r in tto if f"speedup_{key}" in r and r["n"] == nn]
