@@ -185,6 +185,35 @@ def test_mixed_bnb_matches_plain_optimum(n, seed):
     assert abs(res[1]["objective"] - res[2]["objective"]) <= 1e-9 * max(1.0, abs(res[1]["objective"]))
 
 
+def test_mixed_search_pipeline_slots():
+    """The mixed form keeps up to three batches in flight (num_streams >= 3),
+    two with num_streams = 2 and one with 1: every schedule proves the same
+    optimum, and a repeated search with the same schedule is the same tree."""
+    n, seed = 80, 2
+    Q, c, _ = instances.binary_qp(n, seed=seed)
+    obj, keep = dense(Q, c)
+    res = {}
+    for streams in (1, 2, 8, 8):
+        s = gb.default_settings()
+        s.abs_gap_tol = 1e-6
+        s.rel_gap_tol = 1e-6
+        s.form = 2
+        s.precision = gb.PREC_FP32
+        s.num_streams = streams
+        s.time_limit_seconds = 60.0
+        r = gb.solve(n, [0] * n, [1] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), s)
+        say(f"[mixed] streams={streams}: {r['objective']:.12f} nodes {r['nodes_processed']} "
+            f"{r['elapsed_seconds']:.4f}s")
+        assert not r["reached_time_limit"]
+        if streams in res:
+            for key in ("nodes_processed", "fw_iterations", "objective", "dual_bound"):
+                assert r[key] == res[streams][key], key
+        res[streams] = r
+    ref = res[1]["objective"]
+    for r in res.values():
+        assert abs(r["objective"] - ref) <= 1e-9 * max(1.0, abs(ref))
+
+
 @pytest.mark.parametrize("form", [0, 1, 2])
 def test_byte_bounds_give_the_same_search(form):
     """0/1 problems keep node bounds as bytes; lower bounds of -0.0 (not
