@@ -120,7 +120,7 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset",
             "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
             "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
-            "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances"]
+            "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances", "gb_shutdown"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -154,6 +154,8 @@ def load():
     lib.gb_relax_batch.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
                                    ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
         + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    lib.gb_shutdown.restype = ctypes.c_int
+    lib.gb_shutdown.argtypes = []
     lib.gb_relax_batch_fast.restype = ctypes.c_int
     lib.gb_relax_batch_fast.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
                                         ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
@@ -252,6 +254,11 @@ def relax_batch(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_ite
                               float(candidate_tol), int(block_threads),
                               ctypes.c_void_p(stream or 0))
     check(code)
+
+
+def shutdown():
+    """gb_shutdown: release the library's pooled device resources."""
+    check(load().gb_shutdown())
 
 
 def relax_batch_fast(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb,

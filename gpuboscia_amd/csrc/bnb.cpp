@@ -758,20 +758,27 @@ struct DeviceBatch {
 // solves: creating streams and pinning host memory per solve cost
 // milliseconds now and then.  A solve takes an idle set of its device (or a
 // new one) and returns it when it ends; concurrent solves use different sets.
-// Sets live until the process exits.
+// Sets live until gb_shutdown (or the process exit, which calls it).
 // Device iterate store: chunks of NodePool::kChunk slots of n doubles, one
 // slot per node-pool slot, kept (like the batch buffers) for later solves.
 struct IterateStore {
     std::vector<double*> chunks;
     size_t chunk_bytes = 0;
-    // Chunks of `bytes` covering `slots` slots (only called with no work in
-    // flight on the store).
+    // At the start of a solve (no work in flight on the store): chunks of at
+    // least `bytes`, covering `slots` slots.  Larger chunks than the store
+    // holds replace all of its chunks.
     gb_status reserve(size_t slots, size_t bytes) {
         if (bytes > chunk_bytes) {
             for (double* p : chunks) (void)hipFree(p);
             chunks.clear();
             chunk_bytes = bytes;
         }
+        return grow_to(slots);
+    }
+    // During a solve, while batches in flight read and write the existing
+    // chunks: only appends chunks of the current size, never frees one.
+    gb_status grow_to(size_t slots) {
+        if (chunk_bytes == 0) return fail(GB_ERR_RUNTIME, "iterate store used before reserve");
         while (chunks.size() * (size_t)NodePool::kChunk < slots) {
             double* p = nullptr;
             GB_HIP(hipMalloc(&p, chunk_bytes));
@@ -796,6 +803,7 @@ struct SolveBuffers {
 class BufferPool {
   public:
     explicit BufferPool(int device) {
+        ensure_shutdown_hook();
         std::lock_guard<std::mutex> lock(mu());
         auto& idle = sets();
         for (size_t i = 0; i < idle.size(); ++i)
@@ -812,6 +820,18 @@ class BufferPool {
         sets().push_back(set_);
     }
     SolveBuffers& get() { return *set_; }
+    // Frees the idle sets (their destructors wait for their streams).
+    static void release_idle() {
+        std::vector<SolveBuffers*> idle;
+        {
+            std::lock_guard<std::mutex> lock(mu());
+            idle.swap(sets());
+        }
+        for (SolveBuffers* b : idle) {
+            (void)hipSetDevice(b->device);
+            delete b;
+        }
+    }
 
   private:
     static std::mutex& mu() {
@@ -1181,8 +1201,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         db.bv_out = (size_t)N * n < 16384;
         db.layout(N);
         if (db.xdev) {
-            // store slots for every pool slot handed out so far
-            st = xstore.reserve((size_t)pool.next, xchunk_bytes);
+            // store slots for every pool slot handed out so far (other
+            // slots' batches may be using the store)
+            st = xstore.grow_to((size_t)pool.next);
             if (st != GB_OK) return st;
         }
         // Staging copies on the host threads (the pool is only read here).
@@ -1616,6 +1637,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
 }
 
 }  // namespace
+
+void release_buffer_sets() { BufferPool::release_idle(); }
+
 }  // namespace gb
 
 extern "C" void gb_default_settings(gb_settings* s) {

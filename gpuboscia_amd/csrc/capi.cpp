@@ -1,12 +1,10 @@
-// C-ABI entry points: descriptor validation, workspace management and the
-// FP64 batch relaxation launch (gb_relax_batch).
+// C-ABI entry points: descriptor validation and the FP64 batch relaxation
+// launches (gb_relax_batch, its allocation-free variant, the batched form).
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
 #include <cmath>
-#include <atomic>
 #include <cstring>
-#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -27,48 +25,6 @@ gb_status fail(gb_status code, const std::string& msg) {
 gb_status hip_fail(hipError_t e, const char* file, int line) {
     return fail(GB_ERR_DEVICE, std::string("HIP ") + file + ":" + std::to_string(line) + " - "
                                    + hipGetErrorString(e));
-}
-
-// Grow-only workspace for the per-node work vectors, one per (device,
-// stream): launches on different streams never share work vectors, and
-// launches on one stream are ordered.
-struct Workspace {
-    std::mutex mtx;
-    double* ptr = nullptr;
-    size_t bytes = 0;
-};
-static std::mutex g_ws_mtx;
-static std::map<std::pair<int, void*>, Workspace> g_ws;
-
-gb_status workspace_get(size_t bytes, void* stream, double** out) {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
-    Workspace* wp = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(g_ws_mtx);
-        wp = &g_ws[{dev, stream}];
-    }
-    Workspace& w = *wp;
-    std::lock_guard<std::mutex> lk(w.mtx);
-    if (w.bytes < bytes) {
-        if (w.ptr) {
-            // Work already queued on this stream may still use the old buffer.
-            e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
-            if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
-            e = hipFree(w.ptr);
-            if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
-            w.ptr = nullptr;
-            w.bytes = 0;
-        }
-        void* p = nullptr;
-        e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) return hip_fail(e, __FILE__, __LINE__);
-        w.ptr = static_cast<double*>(p);
-        w.bytes = bytes;
-    }
-    *out = w.ptr;
-    return GB_OK;
 }
 
 gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
@@ -124,11 +80,20 @@ int auto_block_threads(int n) {
     return wanted < 64 ? 64 : (wanted > 256 ? 256 : wanted);
 }
 
-gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* a,
-                           const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
-                           const double* ub, const double* eps, int* iterations, uint8_t* status,
-                           double* gap, double* value, double* best_vertex, double* best_vertex_val,
-                           const uint8_t* flags, double ctol, int block_threads, void* stream) {
+// Validation and launch parameters of the plain FP64 form (everything but
+// the work vectors, which the caller provides).
+struct Fp64Launch {
+    RelaxParams P{};
+    LdsLayout L{};
+    int threads = 0;
+    size_t work_bytes = 0;
+};
+
+static gb_status relax_fp64_setup(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                                  const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
+                                  const double* ub, const double* eps, int* iterations, uint8_t* status,
+                                  double* gap, double* value, double* best_vertex, double* best_vertex_val,
+                                  const uint8_t* flags, double ctol, int block_threads, Fp64Launch* out) {
     gb_status st = validate_descriptors(o, a, l);
     if (st != GB_OK) return st;
     if (N < 0 || max_iters < 0) return fail(GB_ERR_INVALID_ARGUMENT, "N and max_iters must be non-negative");
@@ -141,12 +106,11 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "gpu_block_threads must be zero or a positive multiple of 64 up to 1024");
     const int n = o->n;
-    const LdsLayout L = lds_layout(o->kind, l->kind, n, l->order);
-    if (L.total > kMaxLdsBytes) return fail(GB_ERR_INVALID_ARGUMENT, "problem exceeds the LDS budget");
-    double* work = nullptr;
-    st = workspace_get((size_t)N * kWorkVectors * n * sizeof(double), stream, &work);
-    if (st != GB_OK) return st;
-    RelaxParams P{};
+    out->L = lds_layout(o->kind, l->kind, n, l->order);
+    if (out->L.total > kMaxLdsBytes) return fail(GB_ERR_INVALID_ARGUMENT, "problem exceeds the LDS budget");
+    out->threads = block_threads;
+    out->work_bytes = (size_t)N * kWorkVectors * n * sizeof(double);
+    RelaxParams& P = out->P;
     P.obj_kind = o->kind;
     P.n = n;
     P.Qt = o->Qt;
@@ -185,37 +149,22 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
     P.best_val = best_vertex_val;
     P.flags = flags;
     P.ctol = ctol;
-    P.work = work;
-    const int e = launch_relax_fp64(P, block_threads, L.total, L, stream);
-    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
     return GB_OK;
 }
 
-// Device-resident node counters of the persistent batched kernel: a ring of
-// kCounterSlots counters per device, 64 bytes apart; each call takes the next
-// slot, so up to kCounterSlots launches may be in flight on one device (on
-// any streams) without sharing a counter.  Each launch resets its counter
-// with a kernel before use.
-constexpr int kCounterSlots = 1024;
-constexpr int kCounterStride = 16;  // ints
-static int* g_counter[64];
-static std::atomic<unsigned> g_counter_next[64];
-static std::mutex g_counter_mtx;
-
-gb_status counter_get(int** out) {
-    int dev = 0;
-    GB_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) return fail(GB_ERR_RUNTIME, "device index out of range");
-    {
-        std::lock_guard<std::mutex> lk(g_counter_mtx);
-        if (!g_counter[dev]) {
-            void* p = nullptr;
-            GB_HIP(hipMalloc(&p, sizeof(int) * kCounterSlots * kCounterStride));
-            g_counter[dev] = static_cast<int*>(p);
-        }
-    }
-    const unsigned slot = g_counter_next[dev].fetch_add(1u) % kCounterSlots;
-    *out = g_counter[dev] + static_cast<size_t>(slot) * kCounterStride;
+gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                           const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
+                           const double* ub, const double* eps, int* iterations, uint8_t* status,
+                           double* gap, double* value, double* best_vertex, double* best_vertex_val,
+                           const uint8_t* flags, double ctol, int block_threads, void* stream) {
+    Fp64Launch F;
+    gb_status st = relax_fp64_setup(o, a, l, N, max_iters, x, lb, ub, eps, iterations, status, gap, value,
+                                    best_vertex, best_vertex_val, flags, ctol, block_threads, &F);
+    if (st != GB_OK || N == 0) return st;
+    st = workspace_get(F.work_bytes, stream, &F.P.work);
+    if (st != GB_OK) return st;
+    const int e = launch_relax_fp64(F.P, F.threads, F.L.total, F.L, stream);
+    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
     return GB_OK;
 }
 
@@ -256,9 +205,11 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     if (opt) op = *opt;
     if (op.waves_per_block < 0 || op.waves_per_block > 4 || op.refresh_interval < 0 || op.blocks < 0)
         return fail(GB_ERR_INVALID_ARGUMENT, "invalid batched-form options");
+    ScopedCounter lent(stream);
     if (!counter) {
-        st = counter_get(&counter);
+        st = lent.acquire();
         if (st != GB_OK) return st;
+        counter = lent.get();
     }
     FastParams P{};
     P.n = o->n;

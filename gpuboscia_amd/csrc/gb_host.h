@@ -37,7 +37,40 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
                            const uint8_t* flags, double ctol, const double* cutoff,
                            const gb_fast_options* opt, void* stream, int* counter = nullptr,
                            bool rescore = false);
-gb_status counter_get(int** out);
+// A node counter of the persistent kernels lent to one call (resources.cpp):
+// no launch that may run at the same time uses it, and a counter lent while
+// the stream is being captured stays with the captured graph.
+struct CounterLease {
+    int device = -1;
+    int slot = -1;
+    bool capturing = false;
+    int* ptr = nullptr;
+};
+gb_status counter_acquire(void* stream, CounterLease* out);
+// Gives the counter back once the call has queued every launch that uses it
+// on `stream` (a completion event is recorded there).
+void counter_release(CounterLease& lease, void* stream);
+// Lends a counter for the scope of one call on one stream.
+class ScopedCounter {
+  public:
+    explicit ScopedCounter(void* stream) : stream_(stream) {}
+    ScopedCounter(const ScopedCounter&) = delete;
+    ScopedCounter& operator=(const ScopedCounter&) = delete;
+    ~ScopedCounter() { counter_release(lease_, stream_); }
+    gb_status acquire() { return counter_acquire(stream_, &lease_); }
+    int* get() const { return lease_.ptr; }
+
+  private:
+    void* stream_;
+    CounterLease lease_;
+};
+// Teardown hooks of gb_shutdown: the idle batch buffer sets of the
+// branch-and-bound driver (bnb.cpp) and the fork streams of
+// gb_relax_instances (mixed_api.cpp).
+void release_buffer_sets();
+void release_fork_streams();
+// Registers gb_shutdown to run at process exit (once; after HIP is up).
+void ensure_shutdown_hook();
 // A stream on a hardware queue of its own (all CUs enabled), for work that
 // must run concurrently with the library's other streams: plain streams
 // share a small pool of hardware queues with every stream in the process

@@ -101,7 +101,8 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
                            double* x, const double* lb, const double* ub, const double* eps, int* iterations,
                            uint8_t* status, double* gap, double* value, double* best_vertex,
                            double* best_vertex_val, const uint8_t* integer_flags, double candidate_tol,
-                           const double* cutoff, const gb_mixed_options* options, void* stream) {
+                           const double* cutoff, const gb_mixed_options* options, void* stream,
+                           int* counter) {
     {
         gb_status st = validate_descriptors(objective, algorithm, lmo);
         if (st != GB_OK) return st;
@@ -117,10 +118,15 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
         if (op.row_tiles_per_wave < 0 || op.row_tiles_per_wave > kMixedMaxRowTilesPerWave || op.columns < 0
                 || op.columns > 16 || op.blocks < 0 || !(op.rel_floor >= 0.0f))
             return fail(GB_ERR_INVALID_ARGUMENT, "invalid mixed-precision options");
-        if (max_iters > 0) {
-            int* counter = nullptr;
-            st = counter_get(&counter);
+        // One node counter serves both passes: they run in order on `stream`
+        // and each resets it before use.
+        ScopedCounter lent(stream);
+        if (!counter) {
+            st = lent.acquire();
             if (st != GB_OK) return st;
+            counter = lent.get();
+        }
+        if (max_iters > 0) {
             MixedParams P{};
             P.n = q->n;
             P.rt = q->rt;
@@ -174,7 +180,7 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
         const gb_fast_options fo{64, 0, 0, op.fp64_stats, op.gradient};
         return relax_batch_fast(objective, algorithm, lmo, N, fp64_iters, x, lb, ub, eps, iterations,
                                 status, gap, value, best_vertex, best_vertex_val, integer_flags,
-                                candidate_tol, cutoff, &fo, stream, nullptr, true);
+                                candidate_tol, cutoff, &fo, stream, counter, true);
     }
 }
 
@@ -192,7 +198,7 @@ extern "C" gb_status gb_relax_batch_mixed(const gb_mixed_q* q, const gb_objectiv
     try {
         return relax_mixed_impl(q, objective, algorithm, lmo, N, max_iters, x, lb, ub, eps, iterations, status,
                                 gap, value, best_vertex, best_vertex_val, integer_flags, candidate_tol, cutoff,
-                                options, stream);
+                                options, stream, nullptr);
     } catch (const std::exception& ex) {
         return fail(GB_ERR_RUNTIME, ex.what());
     }
@@ -258,7 +264,7 @@ extern "C" gb_status gb_mixed_autotune(const gb_mixed_q* q, const gb_objective_d
             if (r != GB_OK) return r;
             GB_HIP(hipEventRecord(S.e0, hs));
             r = relax_mixed_impl(q, objective, algorithm, lmo, N, max_iters, S.x, lb, ub, eps, S.it, S.st, S.gap,
-                                 S.val, S.bv, S.bvv, integer_flags, candidate_tol, cutoff, &op, stream);
+                                 S.val, S.bv, S.bvv, integer_flags, candidate_tol, cutoff, &op, stream, nullptr);
             if (r != GB_OK) return r;
             GB_HIP(hipEventRecord(S.e1, hs));
             GB_HIP(hipEventSynchronize(S.e1));
@@ -340,6 +346,23 @@ struct ForkStreams {
 ForkStreams g_fork[64];
 
 }  // namespace
+
+void release_fork_streams() {
+    for (int d = 0; d < 64; ++d) {
+        ForkStreams& F = g_fork[d];
+        std::lock_guard<std::mutex> lk(F.mtx);
+        if (F.streams.empty() && !F.start) continue;
+        (void)hipSetDevice(d);
+        for (hipStream_t s : F.streams) (void)hipStreamSynchronize(s);
+        for (hipStream_t s : F.streams) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : F.done) (void)hipEventDestroy(e);
+        if (F.start) (void)hipEventDestroy(F.start);
+        F.streams.clear();
+        F.done.clear();
+        F.start = nullptr;
+    }
+}
+
 }  // namespace gb
 
 extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batches,
@@ -362,6 +385,7 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
         int dev = 0;
         GB_HIP(hipGetDevice(&dev));
         if (dev < 0 || dev >= 64) return fail(GB_ERR_RUNTIME, "device index out of range");
+        ensure_shutdown_hook();
         ForkStreams& F = g_fork[dev];
         std::lock_guard<std::mutex> lk(F.mtx);
         while ((int)F.streams.size() < concurrency) {
@@ -376,12 +400,27 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
         hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
         GB_HIP(hipEventRecord(F.start, caller));
         const int k = concurrency < count ? concurrency : count;
+        // One node counter per fork stream: the entries of one fork stream
+        // run in order and each launch resets the counter before use.
+        std::vector<CounterLease> lease(k);
+        struct GiveBack {
+            std::vector<CounterLease>& l;
+            ForkStreams& f;
+            ~GiveBack() {
+                for (size_t j = 0; j < l.size(); ++j) counter_release(l[j], f.streams[j]);
+            }
+        } give_back{lease, F};
+        for (int j = 0; j < k; ++j) {
+            gb_status st = counter_acquire(F.streams[j], &lease[j]);
+            if (st != GB_OK) return st;
+        }
         int cus = 0;
         GB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         for (int j = 0; j < k; ++j) GB_HIP(hipStreamWaitEvent(F.streams[j], F.start, 0));
         for (int i = 0; i < count; ++i) {
             const gb_instance_batch& b = batches[i];
             void* s = F.streams[i % k];
+            int* counter = lease[i % k].ptr;
             gb_status st;
             if (b.mixed) {
                 // k entries share the CUs: widen the tiles so that together
@@ -395,11 +434,11 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
                 mo.columns = want < def ? def : (want > cap ? cap : want);
                 st = relax_mixed_impl(b.mixed, b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
-                                      b.integer_flags, candidate_tol, b.cutoff, &mo, s);
+                                      b.integer_flags, candidate_tol, b.cutoff, &mo, s, counter);
             } else {
                 st = relax_batch_fast(b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
-                                      b.integer_flags, candidate_tol, b.cutoff, nullptr, s);
+                                      b.integer_flags, candidate_tol, b.cutoff, nullptr, s, counter);
             }
             if (st != GB_OK) return st;
         }

@@ -16,9 +16,12 @@
  *
  * Device pointers are caller-owned, as in the reference (lmo.cuh:6-9,
  * heuristics.h:22).  Calls on one stream are ordered.  Library-owned
- * scratch (work vectors of gb_relax_batch, node counters of the batched
- * form) is kept per (device, stream) or per launch, so calls on different
- * streams may run concurrently.
+ * scratch is kept per (device, stream) (work vectors of gb_relax_batch) or
+ * lent per call (node counters of the batched and mixed forms, returned
+ * with an event recorded on the call's stream and lent again only after it
+ * has signalled), so calls on different streams may run concurrently.  A
+ * call captured into a HIP graph keeps its node counters for the graph's
+ * replays; replays of one graph must not overlap.
  */
 #ifndef GPUBOSCIA_AMD_H
 #define GPUBOSCIA_AMD_H
@@ -522,6 +525,16 @@ gb_status gb_branching_dataset(const gb_problem* problem, const gb_objective_des
 int gb_lmo_relaxation_feasible_cpu(const gb_lmo_desc* lmo, const uint8_t* is_int_host,
                                    const double* x, const double* lb,
                                    const double* ub, int n, double tolerance);
+
+/* Releases what the library holds on every device: pooled batch buffers and
+   their streams, fork streams, node counters and work vectors (no reference
+   counterpart; the reference frees its buffers in the solver's destructor,
+   device_resident_solver.cuh:1242-1245).  Waits for the library's work to
+   finish first.  It also runs by itself at process exit.  Call it only
+   while no other call into the library is running; later calls re-create
+   what they need.  Objects the caller owns (gb_mixed_q, gb_relax_graph,
+   gb_brancher) are not touched. */
+gb_status gb_shutdown(void);
 
 /* Thread-local message of the last failure. */
 const char* gb_last_error(void);
