@@ -120,7 +120,8 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_brancher_loss_grad", "gb_brancher_train", "gb_branching_dataset",
             "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
             "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
-            "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances", "gb_shutdown"]
+            "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances", "gb_shutdown",
+            "gb_relax_batch_noalloc"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -154,6 +155,8 @@ def load():
     lib.gb_relax_batch.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(AlgorithmDesc),
                                    ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
         + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    lib.gb_relax_batch_noalloc.restype = ctypes.c_int
+    lib.gb_relax_batch_noalloc.argtypes = lib.gb_relax_batch.argtypes
     lib.gb_shutdown.restype = ctypes.c_int
     lib.gb_shutdown.argtypes = []
     lib.gb_relax_batch_fast.restype = ctypes.c_int
@@ -253,6 +256,21 @@ def relax_batch(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_ite
                               _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
                               float(candidate_tol), int(block_threads),
                               ctypes.c_void_p(stream or 0))
+    check(code)
+
+
+def relax_batch_noalloc(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb, ub,
+                        eps, iterations, status, gap, value, best_vertex, best_vertex_val, flags=None,
+                        candidate_tol=1e-6, block_threads=0, stream=None):
+    """gb_relax_batch_noalloc: gb_relax_batch with its work vectors in the
+    library's static device array (no allocation)."""
+    lib = load()
+    code = lib.gb_relax_batch_noalloc(ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
+                                      int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
+                                      _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),
+                                      _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
+                                      float(candidate_tol), int(block_threads),
+                                      ctypes.c_void_p(stream or 0))
     check(code)
 
 

@@ -168,6 +168,30 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
     return GB_OK;
 }
 
+// The allocation-free variant: the work vectors live in the library's static
+// device array (relax_fp64.hip), whose address on the current device was
+// looked up once per device; nothing is allocated or freed here.
+static gb_status relax_batch_fp64_noalloc(const gb_objective_desc* o, const gb_algorithm_desc* a,
+                                          const gb_lmo_desc* l, int N, int max_iters, double* x,
+                                          const double* lb, const double* ub, const double* eps,
+                                          int* iterations, uint8_t* status, double* gap, double* value,
+                                          double* best_vertex, double* best_vertex_val, const uint8_t* flags,
+                                          double ctol, int block_threads, void* stream) {
+    Fp64Launch F;
+    gb_status st = relax_fp64_setup(o, a, l, N, max_iters, x, lb, ub, eps, iterations, status, gap, value,
+                                    best_vertex, best_vertex_val, flags, ctol, block_threads, &F);
+    if (st != GB_OK || N == 0) return st;
+    if (F.work_bytes > GB_NOALLOC_WORK_BYTES)
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "batch too large for the allocation-free relaxation: N * n * 48 bytes of work vectors "
+                    "exceed GB_NOALLOC_WORK_BYTES (" + std::to_string(GB_NOALLOC_WORK_BYTES) + ")");
+    const int se = static_work_address(&F.P.work);
+    if (se != 0) return hip_fail(static_cast<hipError_t>(se), __FILE__, __LINE__);
+    const int e = launch_relax_fp64(F.P, F.threads, F.L.total, F.L, stream);
+    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    return GB_OK;
+}
+
 hipError_t create_dedicated_stream(hipStream_t* out) {
     // The runtime gives a stream with a CU mask a hardware queue of its own
     // instead of the least-used queue of its shared pool.
@@ -353,6 +377,23 @@ extern "C" gb_status gb_relax_batch(const gb_objective_desc* objective,
         return relax_batch_fp64(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps,
                                 iterations, status, gap, value, best_vertex, best_vertex_val,
                                 integer_flags, candidate_tol, block_threads, stream);
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
+
+extern "C" gb_status gb_relax_batch_noalloc(const gb_objective_desc* objective,
+                                            const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,
+                                            int N, int max_iters, double* x, const double* lb,
+                                            const double* ub, const double* eps, int* iterations,
+                                            uint8_t* status, double* gap, double* value,
+                                            double* best_vertex, double* best_vertex_val,
+                                            const uint8_t* integer_flags, double candidate_tol,
+                                            int block_threads, void* stream) {
+    try {
+        return relax_batch_fp64_noalloc(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps,
+                                        iterations, status, gap, value, best_vertex, best_vertex_val,
+                                        integer_flags, candidate_tol, block_threads, stream);
     } catch (const std::exception& ex) {
         return fail(GB_ERR_RUNTIME, ex.what());
     }

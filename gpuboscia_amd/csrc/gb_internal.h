@@ -106,6 +106,11 @@ inline LdsLayout lds_layout(int obj_kind, int lmo_kind, int n, int order) {
     return L;
 }
 
+// Address, on the current device, of the static device array that holds the
+// work vectors of the allocation-free relaxation (GB_NOALLOC_WORK_BYTES;
+// relax_fp64.hip).  Looked up once per device.  Returns a hipError_t value.
+int static_work_address(double** out);
+
 // Launch the FP64 relaxation kernel (relax_fp64.hip).  Returns a hipError_t
 // value as int.
 int launch_relax_fp64(const RelaxParams& p, int block_threads, size_t lds_bytes,

@@ -6,10 +6,38 @@
 // gb_device.h.  Block-uniform scalars (gap, step, line-search state) are held
 // redundantly in every lane's registers instead of shared-memory scalars, so
 // each reduction needs two barriers and no broadcast.
+#include <mutex>
+
 #include "gb_device.h"
 #include "gb_internal.h"
 
 namespace gb {
+
+// Work vectors of the allocation-free entry point (gb_relax_batch_noalloc):
+// one file-scope device array per GPU, held for the life of the process.
+// Host code never passes its name to a kernel (on the host the name is a
+// stand-in, not a device address): static_work_address looks the address up
+// once per device and every launch uses that.
+static __device__ __align__(256) double scratch[GB_NOALLOC_WORK_BYTES / sizeof(double)];
+
+int static_work_address(double** out) {
+    constexpr int kDevices = 64;
+    static std::mutex mu;
+    static double* addr[kDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kDevices) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        addr[dev] = static_cast<double*>(p);
+    }
+    *out = addr[dev];
+    return (int)hipSuccess;
+}
 namespace {
 
 struct Lds {

@@ -160,6 +160,32 @@ gb_status gb_relax_batch(const gb_objective_desc* objective,
                          const uint8_t* integer_flags, double candidate_tol,
                          int block_threads, void* stream);
 
+/*
+ * Allocation-free variant of gb_relax_batch.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and results as gb_relax_batch, but the per-node
+ * work vectors live in a static device array of GB_NOALLOC_WORK_BYTES per
+ * GPU, held until the process ends, instead of a library allocation; the
+ * call allocates and frees nothing.  Limit: N * n * 48 bytes must not exceed
+ * GB_NOALLOC_WORK_BYTES (e.g. N = 1024 nodes up to n = 341), else
+ * GB_ERR_INVALID_ARGUMENT.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (different streams, or different host threads)
+ * race and must be serialised by the caller.  The array keeps the last
+ * call's contents; the kernel writes every work entry before reading it.
+ */
+#define GB_NOALLOC_WORK_BYTES (16u * 1024u * 1024u)
+gb_status gb_relax_batch_noalloc(const gb_objective_desc* objective,
+                                 const gb_algorithm_desc* algorithm,
+                                 const gb_lmo_desc* lmo,
+                                 int N, int max_iters,
+                                 double* x, const double* lb, const double* ub,
+                                 const double* eps, int* iterations, uint8_t* status,
+                                 double* gap, double* value,
+                                 double* best_vertex, double* best_vertex_val,
+                                 const uint8_t* integer_flags, double candidate_tol,
+                                 int block_threads, void* stream);
+
 /* Options of the batched-form relaxation (gb_relax_batch_fast). */
 typedef struct {
     int refresh_interval;     /* recompute Q x directly every k iterations (0 = never) */
