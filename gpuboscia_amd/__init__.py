@@ -121,7 +121,7 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
             "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
             "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances", "gb_shutdown",
-            "gb_relax_batch_noalloc"]
+            "gb_relax_batch_noalloc", "gb_lmo_compute_v"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -157,6 +157,8 @@ def load():
         + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
     lib.gb_relax_batch_noalloc.restype = ctypes.c_int
     lib.gb_relax_batch_noalloc.argtypes = lib.gb_relax_batch.argtypes
+    lib.gb_lmo_compute_v.restype = ctypes.c_int
+    lib.gb_lmo_compute_v.argtypes = [ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
     lib.gb_shutdown.restype = ctypes.c_int
     lib.gb_shutdown.argtypes = []
     lib.gb_relax_batch_fast.restype = ctypes.c_int
@@ -272,6 +274,12 @@ def relax_batch_noalloc(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N,
                                       float(candidate_tol), int(block_threads),
                                       ctypes.c_void_p(stream or 0))
     check(code)
+
+
+def lmo_compute_v(lmo: LmoDesc, n, N, direction, lb, ub, vertex, stream=None):
+    """gb_lmo_compute_v on torch float64 device tensors [N, n]."""
+    check(load().gb_lmo_compute_v(ctypes.byref(lmo), int(n), int(N), _ptr(direction), _ptr(lb), _ptr(ub),
+                                  _ptr(vertex), ctypes.c_void_p(stream or 0)))
 
 
 def shutdown():

@@ -27,6 +27,30 @@ gb_status hip_fail(hipError_t e, const char* file, int line) {
                                    + hipGetErrorString(e));
 }
 
+// LMO descriptor checks for n variables (obj_kind: the objective it is
+// paired with; a: the algorithm, may be NULL).
+static gb_status validate_lmo(const gb_lmo_desc* l, int n, int obj_kind, const gb_algorithm_desc* a) {
+    if (l->kind == GB_LMO_SIMPLEX) {
+        if (!l->is_int) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO requires is_int");
+        if (n > kMaxSimplexN) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO n exceeds the LDS limit");
+    } else if (l->kind == GB_LMO_BIRKHOFF) {
+        if (l->order <= 0 || l->order > kMaxBirkhoffOrder)
+            return fail(GB_ERR_INVALID_ARGUMENT, "Birkhoff order must be in [1, 256]");
+    } else if (l->kind == GB_LMO_PORTFOLIO) {
+        if (obj_kind != GB_OBJ_DENSE_QUADRATIC || n % 2 != 0 || n > kMaxSimplexN)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "portfolio LMO needs a dense objective over (x, z) with n = 2 * assets <= 2048");
+        if (!(l->cap > 0.0) || !(l->cardinality >= 0.0) || !(l->right_side > 0.0))
+            return fail(GB_ERR_INVALID_ARGUMENT, "portfolio LMO needs cap > 0, cardinality >= 0, right_side > 0");
+        if (a && a->algorithm == GB_ALG_DICG)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "the portfolio LMO has no in-face oracle: use the Frank-Wolfe algorithm");
+    } else if (l->kind != GB_LMO_BOX) {
+        return fail(GB_ERR_INVALID_ARGUMENT, "unknown LMO kind");
+    }
+    return GB_OK;
+}
+
 gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
                                const gb_lmo_desc* l) {
     if (!o || !a || !l) return fail(GB_ERR_INVALID_ARGUMENT, "null descriptor");
@@ -41,24 +65,8 @@ gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_de
     } else {
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown objective kind");
     }
-    if (l->kind == GB_LMO_SIMPLEX) {
-        if (!l->is_int) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO requires is_int");
-        if (o->n > kMaxSimplexN) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO n exceeds the LDS limit");
-    } else if (l->kind == GB_LMO_BIRKHOFF) {
-        if (l->order <= 0 || l->order > kMaxBirkhoffOrder)
-            return fail(GB_ERR_INVALID_ARGUMENT, "Birkhoff order must be in [1, 256]");
-    } else if (l->kind == GB_LMO_PORTFOLIO) {
-        if (o->kind != GB_OBJ_DENSE_QUADRATIC || o->n % 2 != 0 || o->n > kMaxSimplexN)
-            return fail(GB_ERR_INVALID_ARGUMENT,
-                        "portfolio LMO needs a dense objective over (x, z) with n = 2 * assets <= 2048");
-        if (!(l->cap > 0.0) || !(l->cardinality >= 0.0) || !(l->right_side > 0.0))
-            return fail(GB_ERR_INVALID_ARGUMENT, "portfolio LMO needs cap > 0, cardinality >= 0, right_side > 0");
-        if (a && a->algorithm == GB_ALG_DICG)
-            return fail(GB_ERR_INVALID_ARGUMENT,
-                        "the portfolio LMO has no in-face oracle: use the Frank-Wolfe algorithm");
-    } else if (l->kind != GB_LMO_BOX) {
-        return fail(GB_ERR_INVALID_ARGUMENT, "unknown LMO kind");
-    }
+    gb_status st = validate_lmo(l, o->n, o->kind, a);
+    if (st != GB_OK) return st;
     if (a->algorithm != GB_ALG_FRANK_WOLFE && a->algorithm != GB_ALG_DICG)
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown FW algorithm");
     if (a->line_search < GB_LS_OPEN_LOOP || a->line_search > GB_LS_QUADRATIC_EXACT)
@@ -394,6 +402,41 @@ extern "C" gb_status gb_relax_batch_noalloc(const gb_objective_desc* objective,
         return relax_batch_fp64_noalloc(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps,
                                         iterations, status, gap, value, best_vertex, best_vertex_val,
                                         integer_flags, candidate_tol, block_threads, stream);
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
+
+extern "C" gb_status gb_lmo_compute_v(const gb_lmo_desc* lmo, int n, int N, const double* direction,
+                                      const double* lb, const double* ub, double* vertex, void* stream) {
+    try {
+        if (!lmo) return fail(GB_ERR_INVALID_ARGUMENT, "null descriptor");
+        if (n <= 0 || N < 0) return fail(GB_ERR_INVALID_ARGUMENT, "n must be positive and N non-negative");
+        if (N == 0) return GB_OK;
+        if (!direction || !lb || !ub || !vertex) return fail(GB_ERR_INVALID_ARGUMENT, "null node buffer");
+        if (lmo->kind == GB_LMO_BIRKHOFF && lmo->order * lmo->order != n)
+            return fail(GB_ERR_INVALID_ARGUMENT, "Birkhoff LMO needs n == order^2");
+        gb_status st = validate_lmo(lmo, n, GB_OBJ_DENSE_QUADRATIC, nullptr);
+        if (st != GB_OK) return st;
+        const LdsLayout L = lds_layout(-1, lmo->kind, n, lmo->order);
+        if (L.total > kMaxLdsBytes) return fail(GB_ERR_INVALID_ARGUMENT, "problem exceeds the LDS budget");
+        RelaxParams P{};
+        P.obj_kind = -1;
+        P.n = n;
+        P.lmo_kind = lmo->kind;
+        P.rhs = lmo->right_side;
+        P.equality = lmo->equality;
+        P.is_int = lmo->is_int;
+        P.lorder = lmo->order;
+        P.pcap = lmo->cap;
+        P.pcard = lmo->cardinality;
+        P.N = N;
+        P.x = vertex;
+        P.lb = lb;
+        P.ub = ub;
+        const int e = launch_lmo_probe(P, auto_block_threads(n), L.total, L, direction, stream);
+        if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+        return GB_OK;
     } catch (const std::exception& ex) {
         return fail(GB_ERR_RUNTIME, ex.what());
     }

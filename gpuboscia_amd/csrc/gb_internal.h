@@ -111,6 +111,11 @@ inline LdsLayout lds_layout(int obj_kind, int lmo_kind, int n, int order) {
 // relax_fp64.hip).  Looked up once per device.  Returns a hipError_t value.
 int static_work_address(double** out);
 
+// Extreme points of the built-in LMOs for directions G ([N][n]) into p.x,
+// with bounds p.lb / p.ub (relax_fp64.hip).  Returns a hipError_t value.
+int launch_lmo_probe(const RelaxParams& p, int block_threads, size_t lds_bytes,
+                     const LdsLayout& L, const double* G, void* stream);
+
 // Launch the FP64 relaxation kernel (relax_fp64.hip).  Returns a hipError_t
 // value as int.
 int launch_relax_fp64(const RelaxParams& p, int block_threads, size_t lds_bytes,

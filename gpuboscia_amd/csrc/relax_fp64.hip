@@ -939,9 +939,7 @@ struct Node {
     }
 };
 
-template <int OBJ, int LMO>
-__global__ void __launch_bounds__(1024) relax_fp64_kernel(RelaxParams P, LdsLayout LL) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+__device__ __forceinline__ Lds lds_views(unsigned char* lds_raw, const LdsLayout& LL) {
     Lds s;
     s.red = reinterpret_cast<double*>(lds_raw + LL.red);
     s.ired = reinterpret_cast<int*>(lds_raw + LL.ired);
@@ -961,7 +959,25 @@ __global__ void __launch_bounds__(1024) relax_fp64_kernel(RelaxParams P, LdsLayo
     s.way = reinterpret_cast<int*>(lds_raw + LL.way);
     s.misc = reinterpret_cast<int*>(lds_raw + LL.misc);
     s.hcost = LL.hcost_bytes ? reinterpret_cast<double*>(lds_raw + LL.hcost) : nullptr;
+    return s;
+}
 
+// Extreme points of the built-in LMOs for given directions (one workgroup
+// per node, the compute_v of the relaxation kernel).
+template <int LMO>
+__global__ void __launch_bounds__(1024) lmo_probe_kernel(RelaxParams P, LdsLayout LL, const double* G) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const Lds s = lds_views(lds_raw, LL);
+    const int node = blockIdx.x;
+    if (node >= P.N) return;
+    const size_t off = (size_t)node * P.n;
+    Lmo<LMO>::compute_v(P, s, G + off, P.lb + off, P.ub + off, P.x + off);
+}
+
+template <int OBJ, int LMO>
+__global__ void __launch_bounds__(1024) relax_fp64_kernel(RelaxParams P, LdsLayout LL) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const Lds s = lds_views(lds_raw, LL);
     using O = Obj<OBJ>;
     using L = Lmo<LMO>;
     const int node = blockIdx.x;
@@ -1123,6 +1139,29 @@ __global__ void __launch_bounds__(1024) relax_fp64_kernel(RelaxParams P, LdsLayo
 }
 
 }  // namespace
+
+int launch_lmo_probe(const RelaxParams& p, int block_threads, size_t lds_bytes, const LdsLayout& L,
+                     const double* G, void* stream) {
+    const dim3 grid(p.N), block(block_threads);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+#define GB_LAUNCH(LMO)                                                                       \
+    do {                                                                                     \
+        auto k = lmo_probe_kernel<LMO>;                                                      \
+        if (lds_bytes > 64 * 1024) {                                                         \
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),             \
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);                 \
+            if (e != hipSuccess) return (int)e;                                              \
+        }                                                                                    \
+        hipLaunchKernelGGL(k, grid, block, lds_bytes, st, p, L, G);                          \
+        return (int)hipGetLastError();                                                       \
+    } while (0)
+    if (p.lmo_kind == GB_LMO_BOX) GB_LAUNCH(GB_LMO_BOX);
+    if (p.lmo_kind == GB_LMO_SIMPLEX) GB_LAUNCH(GB_LMO_SIMPLEX);
+    if (p.lmo_kind == GB_LMO_BIRKHOFF) GB_LAUNCH(GB_LMO_BIRKHOFF);
+    if (p.lmo_kind == GB_LMO_PORTFOLIO) GB_LAUNCH(GB_LMO_PORTFOLIO);
+#undef GB_LAUNCH
+    return (int)hipErrorInvalidValue;
+}
 
 int launch_relax_fp64(const RelaxParams& p, int block_threads, size_t lds_bytes, const LdsLayout& L,
                       void* stream) {

@@ -186,6 +186,20 @@ gb_status gb_relax_batch_noalloc(const gb_objective_desc* objective,
                                  const uint8_t* integer_flags, double candidate_tol,
                                  int block_threads, void* stream);
 
+/*
+ * Extreme points of a built-in LMO for N given directions: vertex[i] =
+ * argmin over the LMO's set with bounds lb[i], ub[i] of direction[i] . v,
+ * one workgroup per node, by the same device code the relaxation kernels
+ * call.  Replaces the LMO plugin's device compute_v (lmo.cuh:11-29) as the
+ * reference's own probes launch it (benchmarks/benchmark_lmo.cu:16-38,
+ * tests/test_pipeline.cu:193-299).  Device arrays [N][n]; an infeasible
+ * Birkhoff node returns its forced edges only (all zeros when none),
+ * like the reference's device LMO.  Asynchronous on `stream`.
+ */
+gb_status gb_lmo_compute_v(const gb_lmo_desc* lmo, int n, int N,
+                           const double* direction, const double* lb, const double* ub,
+                           double* vertex, void* stream);
+
 /* Options of the batched-form relaxation (gb_relax_batch_fast). */
 typedef struct {
     int refresh_interval;     /* recompute Q x directly every k iterations (0 = never) */

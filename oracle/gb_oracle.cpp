@@ -18,7 +18,17 @@
 //   box / simplex CPU DICG helpers .... tests/test_cpu.cpp:399-517 + oracle/_ref
 //   quadratic_exact_step .............. tests/test_cpu.cpp:543-552 + oracle/_ref
 //   box / simplex / Birkhoff forward LMO closed form, integer enumeration,
-//                                       brute-force permutations (tests/)
+//                                       brute-force permutations (tests/); the
+//                                       reference's LMO benchmark cases
+//                                       (benchmark_lmo.cu:173-214) against scipy LP /
+//                                       assignment optima; order-7 Birkhoff vs all
+//                                       5040 permutations (test_pipeline.cu:193-299)
+//   Hungarian tie order ............... all-zero costs give the identity, derived by
+//                                       hand from the strict-< column scan
+//   secant_search ..................... the reference's stall regression
+//                                       (test_pipeline.cu:1133-1205) and convergence
+//                                       matrix (:1062-1131); equals the exact step on
+//                                       quadratics (closed form)
 //   node relaxation ................... continuation KAT x=2/3, candidate 0.45 /
 //                                       0.09, exact-step 0.25, interior optimum
 //                                       (tests/test_pipeline.cu:1211-1441)
@@ -1474,4 +1484,48 @@ extern "C" double orc_enumerate_binary_qp(int n, const double* Q, const double* 
     if (x_out)
         for (int j = 0; j < n; ++j) x_out[j] = (double)((g.mask >> j) & 1ull);
     return g.f;
+}
+
+// ---------------------------------------------------------------------------
+// Secant line search, exposed for its closed-form pins: on a quadratic the
+// derivative along d is linear, so the secant root is the exact step.
+// ---------------------------------------------------------------------------
+extern "C" double orc_secant_search(const orc_objective* o, const orc_alg* a, int n, int lanes,
+                                    double gap, double max_gamma, const double* x, const double* dir) {
+    std::vector<double> tx(n), tg(n);
+    return secant_search(*o, *a, n, lanes, gap, max_gamma, x, dir, tx, tg);
+}
+
+// ---------------------------------------------------------------------------
+// The least-squares instance of the reference's equality-simplex convergence
+// matrix (tests/test_pipeline.cu:1068-1093): A (m x n) ~ N(0, 1/sqrt(m)) and
+// b (m) ~ N(0, 1) from std::mt19937(seed) through two
+// std::normal_distribution<double> objects, as that test draws them; then
+// Q = A^T A, linear = -A^T b, constant = b^T b / 2, accumulated in the
+// test's loop order with plain multiply-add (no contraction).  Outputs Qt
+// (Qt[k*n+j] = Q[j,k]), linear, constant.  The normal draws are those of the
+// C++ standard library this oracle is built with (libstdc++).
+// ---------------------------------------------------------------------------
+#include <random>
+
+extern "C" void orc_least_squares_instance(unsigned seed, int m, int n, double* Qt, double* linear,
+                                           double* constant) {
+    std::mt19937 rng(seed);
+    std::normal_distribution<double> a_dist(0.0, 1.0 / std::sqrt(static_cast<double>(m)));
+    std::normal_distribution<double> b_dist(0.0, 1.0);
+    std::vector<double> A((size_t)m * n), b(m);
+    for (double& v : A) v = a_dist(rng);
+    for (double& v : b) v = b_dist(rng);
+    std::vector<double> Q((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) linear[j] = 0.0;
+    for (int j = 0; j < n; ++j) {
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < m; ++i) Q[(size_t)j * n + k] += A[(size_t)i * n + j] * A[(size_t)i * n + k];
+        for (int i = 0; i < m; ++i) linear[j] -= A[(size_t)i * n + j] * b[i];
+    }
+    for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k) Qt[(size_t)k * n + j] = Q[(size_t)j * n + k];
+    double c = 0.0;
+    for (double v : b) c += 0.5 * v * v;
+    *constant = c;
 }

@@ -171,6 +171,18 @@ int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc_lmo* l,
 double orc_enumerate_binary_qp(int n, const double* Q, const double* c,
                                int threads, double* x_out);
 
+/* Secant line search of one node (algorithms.cuh:121-258): the step along
+   dir from x, given the FW gap (phi(0) = -gap) and the maximum step. */
+double orc_secant_search(const orc_objective* o, const orc_alg* a, int n, int lanes,
+                         double gap, double max_gamma, const double* x, const double* dir);
+
+/* Least-squares instance 0.5 ||Ax - b||^2 of the reference's convergence
+   matrix (tests/test_pipeline.cu:1068-1093), drawn with std::mt19937(seed)
+   and std::normal_distribution<double> as that test does: Qt (n*n), linear
+   (n), constant. */
+void orc_least_squares_instance(unsigned seed, int m, int n, double* Qt, double* linear,
+                                double* constant);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,7 +75,8 @@ def load():
                           ("orc_depth_dependent_tolerance", ctypes.c_double),
                           ("orc_box_dicg_maximum_step_cpu", ctypes.c_double),
                           ("orc_unit_simplex_dicg_maximum_step_cpu", ctypes.c_double),
-                          ("orc_enumerate_binary_qp", ctypes.c_double)]:
+                          ("orc_enumerate_binary_qp", ctypes.c_double),
+                          ("orc_secant_search", ctypes.c_double)]:
             getattr(_lib, name).restype = res
         _lib.orc_quadratic_exact_step.argtypes = [ctypes.c_double] * 3
         _lib.orc_certified_fw_lower_bound.argtypes = [ctypes.c_double] * 3
@@ -91,7 +92,12 @@ def f64(a):
 
 
 def p(a):
-    return ctypes.c_void_p(0 if a is None else a.ctypes.data)
+    """Address of a numpy array's data for a ctypes call.  The pointer object
+    keeps the array alive, so a temporary array (p(f64(-g))) stays valid for
+    the whole call."""
+    ptr = ctypes.c_void_p(0 if a is None else a.ctypes.data)
+    ptr._keep = a
+    return ptr
 
 
 def dense_objective(Q, c=None, grad_scale=2.0, linear_scale=0.5, constant=0.0, keep=None):
@@ -207,3 +213,21 @@ def relax_fast_batch(obj, dicg, x, lb, ub, eps, iterations, status, flags=None, 
                              ctypes.c_double(candidate_tol), p(cut), refresh, int(rescore), threads)
     return {"x": x, "gap": gap, "value": value, "iterations": iterations, "status": status,
             "best_vertex": bv, "best_vertex_val": bvv}
+
+
+def secant_search(obj, a, x, d, gap, max_gamma, lanes=64):
+    """orc_secant_search: secant step along d from x (algorithms.cuh:121-258)."""
+    lib = load()
+    x, d = f64(x), f64(d)
+    return lib.orc_secant_search(ctypes.byref(obj), ctypes.byref(a), len(x), lanes, ctypes.c_double(gap),
+                                 ctypes.c_double(max_gamma), p(x), p(d))
+
+
+def least_squares_instance(seed, m, n):
+    """Qt, linear, constant of the reference's convergence-matrix instance."""
+    lib = load()
+    Qt = np.zeros(n * n)
+    lin = np.zeros(n)
+    k = ctypes.c_double()
+    lib.orc_least_squares_instance(ctypes.c_uint(seed), m, n, p(Qt), p(lin), ctypes.byref(k))
+    return Qt.reshape(n, n), lin, k.value
