@@ -121,7 +121,7 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
             "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
             "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances", "gb_shutdown",
-            "gb_relax_batch_noalloc", "gb_lmo_compute_v"]
+            "gb_relax_batch_noalloc", "gb_lmo_compute_v", "gb_mixed_warm_products"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -159,6 +159,8 @@ def load():
     lib.gb_relax_batch_noalloc.argtypes = lib.gb_relax_batch.argtypes
     lib.gb_lmo_compute_v.restype = ctypes.c_int
     lib.gb_lmo_compute_v.argtypes = [ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
+    lib.gb_mixed_warm_products.restype = ctypes.c_int
+    lib.gb_mixed_warm_products.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.c_int] + [ctypes.c_void_p] * 5
     lib.gb_shutdown.restype = ctypes.c_int
     lib.gb_shutdown.argtypes = []
     lib.gb_relax_batch_fast.restype = ctypes.c_int
@@ -280,6 +282,12 @@ def lmo_compute_v(lmo: LmoDesc, n, N, direction, lb, ub, vertex, stream=None):
     """gb_lmo_compute_v on torch float64 device tensors [N, n]."""
     check(load().gb_lmo_compute_v(ctypes.byref(lmo), int(n), int(N), _ptr(direction), _ptr(lb), _ptr(ub),
                                   _ptr(vertex), ctypes.c_void_p(stream or 0)))
+
+
+def mixed_warm_products(obj: ObjectiveDesc, N, x, status, iterations, out, stream=None):
+    """gb_mixed_warm_products on torch device tensors."""
+    check(load().gb_mixed_warm_products(ctypes.byref(obj), int(N), _ptr(x), _ptr(status), _ptr(iterations),
+                                        _ptr(out), ctypes.c_void_p(stream or 0)))
 
 
 def shutdown():

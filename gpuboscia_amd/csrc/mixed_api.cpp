@@ -94,6 +94,21 @@ extern "C" gb_status gb_mixed_products(const gb_mixed_q* q, const double* V, int
     return GB_OK;
 }
 
+extern "C" gb_status gb_mixed_warm_products(const gb_objective_desc* objective, int N, const double* x,
+                                            const uint8_t* status, const int* iterations, double* out,
+                                            void* stream) {
+    if (!objective || objective->kind != GB_OBJ_DENSE_QUADRATIC || !objective->Qt)
+        return fail(GB_ERR_INVALID_ARGUMENT, "warm products need a dense quadratic objective");
+    if (objective->n <= 0 || objective->n > kMixedMaxN)
+        return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision form supports 1 <= n <= 512");
+    if (N < 0 || (N > 0 && (!x || !status || !iterations || !out)))
+        return fail(GB_ERR_INVALID_ARGUMENT, "invalid warm-product buffers");
+    if (N == 0) return GB_OK;
+    const int e = launch_mixed_warm_products(objective->Qt, objective->n, N, x, status, iterations, out, stream);
+    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    return GB_OK;
+}
+
 namespace gb {
 
 gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objective,

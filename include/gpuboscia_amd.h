@@ -254,6 +254,15 @@ gb_status gb_mixed_destroy(gb_mixed_q* q);
    the mixed form against FP64. */
 gb_status gb_mixed_products(const gb_mixed_q* q, const double* V, int N, double* Y, void* stream);
 
+/* The FP64 products that start pass 1 of gb_relax_batch_mixed at supplied
+   points (diagnostic; no reference counterpart): out[i] = Q x[i], each entry
+   one k-ordered FMA chain over Qt, for every node i with status
+   GB_NODE_INITIAL_POINT_SUPPLIED and iterations[i] == 0; other rows of out
+   are left as they are.  Dense objective, n <= 512; device arrays. */
+gb_status gb_mixed_warm_products(const gb_objective_desc* objective, int N, const double* x,
+                                 const uint8_t* status, const int* iterations, double* out,
+                                 void* stream);
+
 typedef struct {
     int row_tiles_per_wave;   /* 16-row tiles per wave (1..4; 0 = automatic); a tile
                                  (workgroup) runs ceil(n_pad / 16 / this) waves */
@@ -277,10 +286,13 @@ typedef struct {
  * batched kernel (gb_relax_batch_fast) continues from that iterate,
  * re-scores the vertex exactly, and returns FP64 gap, value and
  * certificate, so pruning decisions and incumbents are FP64.  Pass 1
- * starts a node at its supplied point (status
- * GB_NODE_INITIAL_POINT_SUPPLIED), with Q x computed in FP64 into the
- * node's best_vertex row first, or at its upper bounds; nodes whose bounds
- * are not all 0/1 and continuations (iterations > 0 on entry) skip pass 1.
+ * starts a node with a supplied point (status GB_NODE_INITIAL_POINT_SUPPLIED)
+ * at that point, with Q x computed in FP64 into the node's best_vertex row
+ * first, when the tile shape allows it: row_tiles_per_wave != 3 and
+ * n <= 256 (at most 16 row tiles of 16).  Other shapes start such a node at
+ * the box vertex nearest to its point.  A node without a supplied point
+ * starts at its upper bounds.  Nodes whose bounds are not all 0/1 and
+ * continuations (iterations > 0 on entry) skip pass 1.
  * best_vertex rows hold a vertex on return only where best_vertex_val is
  * below 1e300.  fp64_max_iters < 0
  * runs pass 1 only (diagnostics).  Buffers as gb_relax_batch_fast;
