@@ -192,39 +192,47 @@ def cpu_baseline(args, Q, c, LB, UB, EPS):
 
 
 def hetero_batches(args):
-    """Heterogeneous batch: args.c3_instances seeded binary QPs with n
-    log-uniform in [32, 512] (instances.batched_sizes), box LMO, args.c3_nodes
-    open nodes each (node_batch), relaxed by ONE gb_relax_instances call
-    over 8 streams in the mixed form (args.c3_planes bf16 planes of Q on MFMA
-    with FP32 accumulation, then the FP64 certification pass).  Budget args.budget iterations
-    per node; the HIP events on the calling stream bracket each call."""
+    """Heterogeneous batch at its configured shape: args.c3_instances seeded
+    binary QPs with n log-uniform in [32, 512] (instances.batched_sizes),
+    each with a box or a bounded-simplex LMO (sum x <= ceil(n / 4)) chosen
+    by seed (instances.batched_lmos) and args.c3_nodes open nodes
+    (instances.open_nodes), relaxed by ONE gb_relax_instances call over 8
+    streams: box instances in the mixed form (args.c3_planes bf16 planes of
+    Q on MFMA with FP32 accumulation, then the FP64 certification pass),
+    bounded-simplex instances in the plain FP64 form; DICG with the exact
+    step, budget args.budget iterations per node.  HIP events on the calling
+    stream bracket each call; the same call captured as one HIP graph is
+    replayed on refilled inputs."""
     import torch
 
     import gpuboscia_amd as gb
     from gpuboscia_amd import instances
     sizes = instances.batched_sizes(args.c3_instances, seed=1)
+    lmos = instances.batched_lmos(args.c3_instances, seed=1, sizes=sizes)
     N = args.c3_nodes
     f64 = dict(dtype=torch.float64, device="cuda")
     stream = torch.cuda.Stream()
     entries, host = [], []
-    for k, n in enumerate(sizes):
+    for k, (n, (kind, K)) in enumerate(zip(sizes, lmos)):
         n = int(n)
         Q, c, _ = instances.binary_qp(n, seed=100000 + k)
-        LB, UB, EPS = node_batch(n, N, seed=200000 + k)
+        LB, UB, EPS = instances.open_nodes(n, N, 200000 + k, card=K)
         dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
         dc = torch.from_numpy(c).cuda()
+        flags = torch.ones(n, dtype=torch.uint8, device="cuda")
         obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
-        e = {"obj": obj, "lmo": gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), "keep": (dQt, dc),
-             "mixed": gb.MixedQ(obj, args.c3_planes, stream.cuda_stream),
+        lmo = (gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0) if kind == "box"
+               else gb.LmoDesc(gb.LMO_SIMPLEX, float(K), 0, flags.data_ptr(), 0))
+        e = {"obj": obj, "lmo": lmo, "keep": (dQt, dc), "kind": kind,
+             "mixed": gb.MixedQ(obj, args.c3_planes, stream.cuda_stream) if kind == "box" else None,
              "x": torch.from_numpy(LB).cuda(), "lb": torch.from_numpy(LB).cuda(), "ub": torch.from_numpy(UB).cuda(),
              "eps": torch.from_numpy(EPS).cuda(), "iterations": torch.zeros(N, dtype=torch.int32, device="cuda"),
              "status": torch.zeros(N, dtype=torch.uint8, device="cuda"), "gap": torch.zeros(N, **f64),
              "value": torch.zeros(N, **f64), "best_vertex": torch.zeros((N, n), **f64),
-             "best_vertex_val": torch.full((N,), 1e300, **f64),
-             "flags": torch.ones(n, dtype=torch.uint8, device="cuda"), "cutoff": None}
+             "best_vertex_val": torch.full((N,), 1e300, **f64), "flags": flags, "cutoff": None}
         entries.append(e)
         host.append(LB)
-    alg = gb.algorithm(dicg=True)
+    alg = gb.algorithm(dicg=True, line_search=gb.LS_QUADRATIC_EXACT)
 
     def reset():
         for e, LB in zip(entries, host):
@@ -233,7 +241,10 @@ def hetero_batches(args):
             e["status"].zero_()
             e["best_vertex_val"].fill_(1e300)
 
-    times, iters = [], 0
+    def count_iters(kind=None):
+        return int(sum(int(e["iterations"].sum().item()) for e in entries if kind in (None, e["kind"])))
+
+    times = []
     with torch.cuda.stream(stream):
         for step in range(args.c3_steps + 1):
             reset()
@@ -245,7 +256,8 @@ def hetero_batches(args):
             stream.synchronize()
             if step > 0:  # the first call is the warm-up
                 times.append(a.elapsed_time(b) / 1e3)
-        iters = int(sum(int(e["iterations"].sum().item()) for e in entries))
+        iters = count_iters()
+        iters_box, iters_simplex = count_iters("box"), count_iters("simplex")
         nodes = len(entries) * N
         # the same call captured once as a HIP graph, replayed on refilled inputs
         g = torch.cuda.CUDAGraph()
@@ -265,17 +277,22 @@ def hetero_batches(args):
             b.record(stream)
             stream.synchronize()
             replay.append(a.elapsed_time(b) / 1e3)
-        iters_replay = int(sum(int(e["iterations"].sum().item()) for e in entries))
+        iters_replay = count_iters()
         del g
         # statuses 1 and 4: fractional / integral, converged to the node tolerance
         converged = int(sum(int(((e["status"] == 1) | (e["status"] == 4)).sum().item()) for e in entries))
     t = float(np.mean(times))
-    return {"instances": len(entries), "nodes_per_instance": N, "n_min": int(sizes.min()),
-            "n_max": int(sizes.max()), "precision": f"{args.c3_planes} bf16 plane(s) of Q + FP64 certification", "budget": args.budget,
+    kinds = [e["kind"] for e in entries]
+    return {"instances": len(entries), "box_instances": kinds.count("box"),
+            "simplex_instances": kinds.count("simplex"), "nodes_per_instance": N, "n_min": int(sizes.min()),
+            "n_max": int(sizes.max()),
+            "precision": f"box: {args.c3_planes} bf16 plane(s) of Q + FP64 certification; "
+                         "bounded simplex: plain FP64 form",
+            "algorithm": "DICG, exact quadratic step", "budget": args.budget,
             "call_ms": t * 1e3, "captured_replay_ms": float(np.mean(replay)) * 1e3,
             "replay_iterations_equal": iters_replay == iters,
-            "fw_iterations_per_call": iters, "value": iters / t, "unit": "FW iterations/s",
-            "nodes_per_s": nodes / t, "converged_nodes": converged}
+            "fw_iterations_per_call": iters, "fw_iterations_box": iters_box, "fw_iterations_simplex": iters_simplex,
+            "value": iters / t, "unit": "FW iterations/s", "nodes_per_s": nodes / t, "converged_nodes": converged}
 
 
 def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0):
@@ -368,7 +385,7 @@ def main():
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-instances", type=int, default=256)
-    ap.add_argument("--c3-nodes", type=int, default=256)
+    ap.add_argument("--c3-nodes", type=int, default=1024)
     ap.add_argument("--c3-steps", type=int, default=3)
     ap.add_argument("--c3-planes", type=int, default=3, choices=(1, 3))
     ap.add_argument("--forms", default="plain,batched,mixed,mixed_fp32,mixed_pass1",

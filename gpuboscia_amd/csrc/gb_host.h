@@ -46,7 +46,10 @@ struct CounterLease {
     bool capturing = false;
     int* ptr = nullptr;
 };
-gb_status counter_acquire(void* stream, CounterLease* out);
+// capture_stream: the stream whose capture status decides (default: stream
+// itself); a fork stream that will join the caller's capture passes the
+// caller's stream, so its counter stays with the graph.
+gb_status counter_acquire(void* stream, CounterLease* out, void* capture_stream = nullptr);
 // Gives the counter back once the call has queued every launch that uses it
 // on `stream` (a completion event is recorded there).
 void counter_release(CounterLease& lease, void* stream);

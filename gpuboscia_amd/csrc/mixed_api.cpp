@@ -3,6 +3,7 @@
 // polishes, certifies and re-scores), and the product diagnostic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -360,6 +361,15 @@ struct ForkStreams {
 };
 ForkStreams g_fork[64];
 
+// Form of a gb_relax_instances entry: 2 mixed, 1 batched FP64 (dense over a
+// box, n <= 512), 0 plain FP64 (everything else).
+int instance_form(const gb_instance_batch& b) {
+    if (b.mixed) return 2;
+    if (b.objective->kind == GB_OBJ_DENSE_QUADRATIC && b.lmo->kind == GB_LMO_BOX && b.objective->n <= kMaxFastN)
+        return 1;
+    return 0;
+}
+
 }  // namespace
 
 void release_fork_streams() {
@@ -426,11 +436,28 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
             }
         } give_back{lease, F};
         for (int j = 0; j < k; ++j) {
-            gb_status st = counter_acquire(F.streams[j], &lease[j]);
+            gb_status st = counter_acquire(F.streams[j], &lease[j], caller);
             if (st != GB_OK) return st;
         }
         int cus = 0;
         GB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        // Entries of the plain FP64 form use their fork stream's work
+        // vectors: size them for the largest such entry first, so that no
+        // entry grows them (and waits on its stream) mid-call.
+        {
+            std::vector<size_t> need(k, 0);
+            for (int i = 0; i < count; ++i) {
+                const gb_instance_batch& b = batches[i];
+                if (instance_form(b) == 0)
+                    need[i % k] = std::max(need[i % k], (size_t)b.N * kWorkVectors * b.objective->n * sizeof(double));
+            }
+            for (int j = 0; j < k; ++j)
+                if (need[j]) {
+                    double* w = nullptr;
+                    gb_status st = workspace_get(need[j], F.streams[j], &w);
+                    if (st != GB_OK) return st;
+                }
+        }
         for (int j = 0; j < k; ++j) GB_HIP(hipStreamWaitEvent(F.streams[j], F.start, 0));
         for (int i = 0; i < count; ++i) {
             const gb_instance_batch& b = batches[i];
@@ -450,10 +477,14 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
                 st = relax_mixed_impl(b.mixed, b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
                                       b.integer_flags, candidate_tol, b.cutoff, &mo, s, counter);
-            } else {
+            } else if (instance_form(b) == 1) {
                 st = relax_batch_fast(b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
                                       b.integer_flags, candidate_tol, b.cutoff, nullptr, s, counter);
+            } else {
+                st = relax_batch_fp64(b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
+                                      b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
+                                      b.integer_flags, candidate_tol, 0, s);
             }
             if (st != GB_OK) return st;
         }

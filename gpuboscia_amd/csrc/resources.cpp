@@ -89,12 +89,12 @@ static bool stream_capturing(void* stream) {
     return cs != hipStreamCaptureStatusNone;
 }
 
-gb_status counter_acquire(void* stream, CounterLease* out) {
+gb_status counter_acquire(void* stream, CounterLease* out, void* capture_stream) {
     int dev = 0;
     GB_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return fail(GB_ERR_RUNTIME, "device index out of range");
     ensure_shutdown_hook();
-    const bool capturing = stream_capturing(stream);
+    const bool capturing = stream_capturing(capture_stream ? capture_stream : stream);
     CounterPool& P = g_counters[dev];
     std::lock_guard<std::mutex> lk(P.mtx);
     auto signalled = [](hipEvent_t e) {

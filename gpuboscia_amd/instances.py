@@ -123,3 +123,44 @@ def batched_sizes(count: int = 256, seed: int = 1, lo: int = 32, hi: int = 512) 
     ln hi)), i < count; instance i is binary_qp(n_i, seed * 100000 + i)."""
     u = uniform(seed, 21, count, math.log(lo), math.log(hi))
     return np.rint(np.exp(u)).astype(np.int64)
+
+
+def batched_lmos(count: int = 256, seed: int = 1, sizes=None):
+    """LMO of each heterogeneous-batch instance, chosen by seed: ("box", 0)
+    or ("simplex", K), the bounded simplex {x in [0,1]^n, sum x <= K} with
+    K = ceil(n / 4) (about half of the instances each)."""
+    sizes = batched_sizes(count, seed) if sizes is None else sizes
+    pick = uniform(seed, 22, count)
+    out = []
+    for i, n in enumerate(sizes):
+        if pick[i] < 0.5:
+            out.append(("box", 0))
+        else:
+            out.append(("simplex", int(math.ceil(int(n) / 4))))
+    return out
+
+
+def open_nodes(n: int, N: int, seed: int, card: int = 0):
+    """N open B&B nodes of a binary problem: node 0 is the root, the others
+    carry random fixings of up to 30 % of the variables (lower bound 1 or
+    upper bound 0; depth = number of fixings).  With a cardinality limit
+    card > 0 (bounded simplex sum x <= card) at most card - 1 variables of a
+    node are fixed at 1, so most nodes are feasible; the rest are fixed at 0.
+    Returns LB, UB [N][n] and the per-node FW tolerance
+    max(1e-6, 1e-3 * 0.8^depth)."""
+    u = uniform(seed, 23, 3 * N * n).reshape(3, N, n)
+    frac = uniform(seed, 24, N) * 0.3
+    fix = u[0] < frac[:, None]
+    one = u[1] < 0.5
+    fix[0] = False
+    LB = np.where(fix & one, 1.0, 0.0)
+    UB = np.where(fix & ~one, 0.0, 1.0)
+    if card > 0:
+        # keep the first card - 1 ones of each node (in index order), fix the others at 0
+        ones = np.cumsum(LB, axis=1)
+        drop = (LB == 1.0) & (ones > card - 1)
+        LB[drop] = 0.0
+        UB[drop] = 0.0
+    depth = fix.sum(axis=1)
+    eps = np.maximum(1e-6, 1e-3 * 0.8 ** depth)
+    return LB, UB, eps

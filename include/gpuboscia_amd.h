@@ -315,8 +315,11 @@ gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
  * Heterogeneous batches (no reference counterpart; the reference fixes n at
  * compile time, device_resident_solver.cuh:362, :1217-1218): independent
  * node batches of different problems and sizes relaxed in one call.  Each
- * entry is relaxed like gb_relax_batch_mixed (when `mixed` is set, n <= 512)
- * or gb_relax_batch_fast (n <= 512), with the entry's own buffers, on one of
+ * entry is relaxed like gb_relax_batch_mixed (when `mixed` is set: dense
+ * objective, box LMO, n <= 512), like gb_relax_batch_fast (other box
+ * entries of a dense objective, n <= 512), or like gb_relax_batch (every
+ * other entry, e.g. a bounded simplex: FP64, the algorithm's own line
+ * search, cutoff unused), with the entry's own buffers, on one of
  * `concurrency` internal streams (1..8) forked from and joined back into
  * `stream`, so small batches of different instances share the GPU.  Every
  * entry's results equal those of its solo call.  After one call on the same

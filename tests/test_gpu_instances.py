@@ -18,11 +18,24 @@ def say(*a):
     sys.stdout.flush()
 
 
-def make_entry(n, N, seed, mixed_planes):
+def make_entry(n, N, seed, mixed_planes, card=0):
+    """card > 0: a bounded-simplex entry (sum x <= card), relaxed in the plain FP64 form."""
     import torch
     Q, c, _ = instances.binary_qp(n, seed=seed)
     dQt, dc = dev(np.ascontiguousarray(Q.T)), dev(c)
     obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
+    if card > 0:
+        LB, UB, eps = instances.open_nodes(n, N, seed, card=card)
+        f64 = dict(dtype=torch.float64, device="cuda")
+        flags = dev(np.ones(n, np.uint8))
+        return {"obj": obj, "lmo": gb.LmoDesc(gb.LMO_SIMPLEX, float(card), 0, flags.data_ptr(), 0),
+                "keep": (dQt, dc), "mixed": None,
+                "x": dev(LB.copy()), "lb": dev(LB), "ub": dev(UB), "eps": dev(eps),
+                "iterations": torch.zeros(N, dtype=torch.int32, device="cuda"),
+                "status": torch.zeros(N, dtype=torch.uint8, device="cuda"),
+                "gap": torch.zeros(N, **f64), "value": torch.zeros(N, **f64),
+                "best_vertex": torch.zeros((N, n), **f64), "best_vertex_val": torch.full((N,), 1e300, **f64),
+                "flags": flags, "cutoff": None}
     rng = np.random.default_rng(seed)
     LB, UB = np.zeros((N, n)), np.ones((N, n))
     for i in range(1, N):
@@ -121,3 +134,49 @@ def test_instances_captured_call_replays_match_eager():
         say(f"[instances] captured replay {rep}")
         for k, e in enumerate(entries):
             assert_bit_equal(results(e), eager[k], f"replay {rep} instance {k} n={sizes[k][0]}")
+
+
+def test_instances_captured_call_with_simplex_entries():
+    """Box entries (mixed and batched forms) and bounded-simplex entries
+    (plain FP64 form) in one captured gb_relax_instances call: every replay,
+    with refills in between, equals the eager call."""
+    import torch
+    specs = [(48, 40, 1, 0), (150, 33, 3, 38), (300, 20, 0, 0), (64, 25, 0, 16), (420, 16, 3, 0)]
+    alg = gb.algorithm(dicg=True, line_search=gb.LS_QUADRATIC_EXACT)
+    budget = 150
+    entries = [make_entry(n, N, 500 + k, planes, card) for k, (n, N, planes, card) in enumerate(specs)]
+    init = [(e["x"].clone(), e["iterations"].clone(), e["status"].clone()) for e in entries]
+
+    def refill():
+        for e, (x0, it0, st0) in zip(entries, init):
+            e["x"].copy_(x0)
+            e["iterations"].copy_(it0)
+            e["status"].copy_(st0)
+            e["best_vertex_val"].fill_(1e300)
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        refill()
+        gb.relax_instances(entries, alg, budget, concurrency=3, stream=s.cuda_stream)
+    s.synchronize()
+    eager = [results(e) for e in entries]
+    with torch.cuda.stream(s):
+        refill()
+    s.synchronize()
+    say("[instances] capturing a call with bounded-simplex entries")
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        gb.relax_instances(entries, alg, budget, concurrency=3, stream=s.cuda_stream)
+    s.synchronize()
+    for rep in range(3):
+        with torch.cuda.stream(s):
+            refill()
+        s.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        # eager calls between replays use counters of their own
+        gb.relax_instances(entries[:1], alg, budget, concurrency=1)
+        torch.cuda.synchronize()
+        say(f"[instances] captured replay {rep} with simplex entries")
+        for k, e in enumerate(entries[1:], start=1):
+            assert_bit_equal(results(e), eager[k], f"replay {rep} instance {k} n={specs[k][0]}")
