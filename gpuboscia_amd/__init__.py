@@ -81,8 +81,36 @@ VALIDATOR_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp, _dp, ct
                                 ctypes.c_double)
 BRANCH_SCORE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int,
                                    ctypes.c_int, _dp)
-BRANCH_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
-                             ctypes.c_int, _dp, _dp, ctypes.c_int)
+
+
+class Node(ctypes.Structure):
+    """gb_node (node.h:9-33): host arrays of n_vars doubles."""
+    _fields_ = [("id", ctypes.c_int), ("parent_id", ctypes.c_int), ("depth", ctypes.c_int),
+                ("lb", _dp), ("ub", _dp), ("relaxation_x", _dp), ("relaxation_iterations", ctypes.c_int),
+                ("dual_bound", ctypes.c_double)]
+
+
+BRANCH_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Node), _ip, _dp, ctypes.c_int, _ip,
+                             ctypes.c_int, ctypes.c_void_p)
+
+
+class SchedulingEvent(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("slot_id", ctypes.c_int), ("generation", ctypes.c_ulonglong),
+                ("n_nodes", ctypes.c_int)]
+
+
+SCHEDULING_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(SchedulingEvent))
+SCHED_DISPATCHED, SCHED_COMPLETION_DETACHED, SCHED_POSTPROCESS_STARTED = 0, 1, 2
+
+
+class SolverStats(ctypes.Structure):
+    _fields_ = [("incumbent_value", ctypes.c_double), ("dual_bound", ctypes.c_double),
+                ("nodes_processed", ctypes.c_int), ("last_gpu_iterations", ctypes.c_int),
+                ("last_gpu_gap", ctypes.c_double), ("last_gpu_converged", ctypes.c_int),
+                ("reached_tolerance", ctypes.c_int), ("reached_time_limit", ctypes.c_int),
+                ("effective_gpu_batch_size", ctypes.c_int), ("effective_num_streams", ctypes.c_int),
+                ("paired_gpu_batches", ctypes.c_int), ("fw_iterations", ctypes.c_longlong),
+                ("elapsed_seconds", ctypes.c_double), ("relax_seconds", ctypes.c_double)]
 
 
 class Settings(ctypes.Structure):
@@ -121,7 +149,10 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
             "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
             "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances", "gb_shutdown",
-            "gb_relax_batch_noalloc", "gb_lmo_compute_v", "gb_mixed_warm_products"]
+            "gb_relax_batch_noalloc", "gb_lmo_compute_v", "gb_mixed_warm_products",
+            "gb_children_push", "gb_solver_create", "gb_solver_destroy", "gb_solver_push",
+            "gb_solver_set_scheduling_callback", "gb_solver_set_progress_callback", "gb_solver_solve",
+            "gb_solver_get_stats", "gb_solver_incumbent_x"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -161,6 +192,20 @@ def load():
     lib.gb_lmo_compute_v.argtypes = [ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
     lib.gb_mixed_warm_products.restype = ctypes.c_int
     lib.gb_mixed_warm_products.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.c_int] + [ctypes.c_void_p] * 5
+    lib.gb_children_push.argtypes = [ctypes.c_void_p, ctypes.POINTER(Node)]
+    lib.gb_solver_create.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(LmoDesc),
+                                     ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_void_p)]
+    lib.gb_solver_destroy.argtypes = [ctypes.c_void_p]
+    lib.gb_solver_push.argtypes = [ctypes.c_void_p, ctypes.POINTER(Node)]
+    lib.gb_solver_set_scheduling_callback.argtypes = [ctypes.c_void_p, SCHEDULING_FN, ctypes.c_void_p]
+    lib.gb_solver_set_progress_callback.argtypes = [ctypes.c_void_p, PROGRESS_FN, ctypes.c_void_p]
+    lib.gb_solver_solve.argtypes = [ctypes.c_void_p]
+    lib.gb_solver_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(SolverStats)]
+    lib.gb_solver_incumbent_x.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int)]
+    for name in ("gb_children_push", "gb_solver_create", "gb_solver_destroy", "gb_solver_push",
+                 "gb_solver_set_scheduling_callback", "gb_solver_set_progress_callback", "gb_solver_solve",
+                 "gb_solver_get_stats", "gb_solver_incumbent_x"):
+        getattr(lib, name).restype = ctypes.c_int
     lib.gb_shutdown.restype = ctypes.c_int
     lib.gb_shutdown.argtypes = []
     lib.gb_relax_batch_fast.restype = ctypes.c_int
@@ -556,6 +601,96 @@ def solve(n, lb, ub, int_vars, obj: ObjectiveDesc, lmo: LmoDesc, settings: Setti
             "reached_tolerance": bool(res.reached_tolerance),
             "reached_time_limit": bool(res.reached_time_limit),
             "fw_iterations": res.fw_iterations, "relax_seconds": res.relax_seconds}
+
+
+class Solver:
+    """gb_solver_*: the low-level solver object (DeviceResidentHybridSolver)
+    over host lists; `events` collects the scheduling events of each solve
+    as (type, slot_id, generation, n_nodes)."""
+
+    def __init__(self, n, lb, ub, int_vars, obj: ObjectiveDesc, lmo: LmoDesc, settings: Settings | None = None,
+                 branch=None):
+        """branch: optional Python BranchStrategy (parent dict, next_id, x list,
+        int_vars) -> (children as dicts with id, parent_id, depth, lb, ub,
+        dual_bound; new next_id)."""
+        lib = load()
+        self.n = n
+        self._s = settings if settings is not None else default_settings()
+        self._lb = (ctypes.c_double * n)(*[float(v) for v in lb])
+        self._ub = (ctypes.c_double * n)(*[float(v) for v in ub])
+        iv = list(int_vars)
+        self._iv = (ctypes.c_int * max(1, len(iv)))(*iv)
+        self._prob = Problem(n, self._lb, self._ub, self._iv, len(iv))
+        self._obj, self._lmo = obj, lmo
+        self._keep = []
+        self.events = []
+        if branch is not None:
+            def tramp(user, parent, next_id, x, nn, ivp, niv, children):
+                p = parent.contents
+                view = {"id": p.id, "parent_id": p.parent_id, "depth": p.depth,
+                        "lb": [p.lb[j] for j in range(nn)], "ub": [p.ub[j] for j in range(nn)],
+                        "relaxation_x": None if not p.relaxation_x else [p.relaxation_x[j] for j in range(nn)],
+                        "relaxation_iterations": p.relaxation_iterations, "dual_bound": p.dual_bound}
+                kids, nid = branch(view, next_id[0], [x[j] for j in range(nn)], [ivp[t] for t in range(niv)])
+                next_id[0] = nid
+                for k in kids:
+                    clb = (ctypes.c_double * nn)(*k["lb"])
+                    cub = (ctypes.c_double * nn)(*k["ub"])
+                    node = Node(k["id"], k["parent_id"], k["depth"], clb, cub, None, 0, k["dual_bound"])
+                    if lib.gb_children_push(children, ctypes.byref(node)) != GB_OK:
+                        return 1
+                return 0
+            self._branch_cb = BRANCH_FN(tramp)
+            self._s.branch_rule = BRANCH_CUSTOM
+            self._s.branch = self._branch_cb
+        h = ctypes.c_void_p()
+        check(lib.gb_solver_create(ctypes.byref(self._prob), ctypes.byref(obj), ctypes.byref(lmo),
+                                   ctypes.byref(self._s), ctypes.byref(h)))
+        self.handle = h
+
+        def on_event(user, ev):
+            e = ev.contents
+            self.events.append((e.type, e.slot_id, e.generation, e.n_nodes))
+        self._sched_cb = SCHEDULING_FN(on_event)
+        check(lib.gb_solver_set_scheduling_callback(self.handle, self._sched_cb, None))
+
+    def push(self, id, parent_id, depth, lb, ub, dual_bound=-1e300, relaxation_x=None, relaxation_iterations=0):
+        n = self.n
+        lb_a = (ctypes.c_double * len(lb))(*[float(v) for v in lb])
+        ub_a = (ctypes.c_double * len(ub))(*[float(v) for v in ub])
+        rx = None if relaxation_x is None else (ctypes.c_double * len(relaxation_x))(*relaxation_x)
+        if len(lb) != n or len(ub) != n or (rx is not None and len(relaxation_x) != n):
+            raise InvalidArgument(GB_ERR_INVALID_ARGUMENT, "invalid node buffer dimensions or resume state")
+        self._keep.append((lb_a, ub_a, rx))
+        node = Node(id, parent_id, depth, lb_a, ub_a, rx, relaxation_iterations, dual_bound)
+        check(load().gb_solver_push(self.handle, ctypes.byref(node)))
+
+    def solve(self):
+        self.events.clear()
+        check(load().gb_solver_solve(self.handle))
+        self._keep.clear()
+
+    def stats(self):
+        st = SolverStats()
+        check(load().gb_solver_get_stats(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in SolverStats._fields_}
+
+    def incumbent_x(self):
+        x = (ctypes.c_double * self.n)()
+        has = ctypes.c_int()
+        check(load().gb_solver_incumbent_x(self.handle, x, ctypes.byref(has)))
+        return [x[j] for j in range(self.n)] if has.value else []
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().gb_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def inf():

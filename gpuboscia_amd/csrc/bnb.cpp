@@ -31,6 +31,17 @@
 #include "gb_host.h"
 #include "gb_mlp.h"
 
+// Children a BranchStrategy callback returns (gb_children_push).
+struct gb_children {
+    struct Child {
+        int id, parent, depth;
+        double bound;
+        std::vector<double> lb, ub;
+    };
+    int n = 0;
+    std::vector<Child> kids;
+};
+
 namespace gb {
 namespace {
 
@@ -106,6 +117,26 @@ struct HostNode {
     int bdir = 0;                    // 0 = down (ub = floor), 1 = up (lb = ceil)
     double bfrac = 0.0;              // distance moved by the branching bound
     double pbound = 0.0;             // parent's certified bound
+};
+
+// A node pushed on a solver object (gb_solver_push), with its own bounds and
+// iterate.
+struct InitNode {
+    int id = 0, parent = -1, depth = 0;
+    std::vector<double> lb, ub, rx;
+    int riters = 0;
+    double bound = -1e300;
+};
+
+// What a solver object adds to a solve: its pushed nodes, its scheduling
+// observer, and the accessors it reports (device_resident_solver.cuh:1280-1298).
+struct SolveHooks {
+    const std::vector<InitNode>* nodes = nullptr;
+    gb_scheduling_fn sched = nullptr;
+    void* sched_user = nullptr;
+    int last_iters = 0;
+    double last_gap = std::numeric_limits<double>::infinity();
+    bool last_conv = false;
 };
 
 // Pseudocosts: mean certified-bound gain per unit of fractionality removed,
@@ -556,7 +587,8 @@ struct DeviceBatch {
     double *heps = nullptr, *hgap = nullptr, *hval = nullptr, *hbvv = nullptr, *hcut = nullptr;
     int* hit = nullptr;
     uint8_t* hst = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the stream the batch runs on: its own, or another set's
+    hipStream_t own = nullptr;     // the stream this set created (destroyed with it)
     // learned branching / labelling: gradients, feature rows, scores, node
     // depths, per-variable pseudocost means (up, down; < 0 = no history), Q
     // diagonal
@@ -701,9 +733,11 @@ struct DeviceBatch {
         learned_io = learned;
         topk_k = k;
         // each batch set on a hardware queue of its own, so the batches
-        // in flight overlap on the device
-        const bool new_stream = stream == nullptr;
-        if (new_stream) GB_HIP(create_dedicated_stream(&stream));
+        // in flight overlap on the device (a solve may then make it share
+        // another set's stream instead)
+        const bool new_stream = own == nullptr;
+        if (new_stream) GB_HIP(create_dedicated_stream(&own));
+        stream = own;
         // large enough for every staging format (compact may change per batch)
         const bool c0 = compact, x0 = xdev;
         size_t bytes = 0;
@@ -749,7 +783,7 @@ struct DeviceBatch {
             if (p) (void)hipFree(p);
         for (void* p : {(void*)hpack, (void*)hg})
             if (p) (void)hipHostFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (own) (void)hipStreamDestroy(own);
     }
 };
 
@@ -1001,13 +1035,15 @@ gb_status strong_branch_labels(const gb_objective_desc* o, const gb_lmo_desc* l,
 }
 
 gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_lmo_desc* l,
-                     const gb_settings* s, gb_result* res, Labeler* lab = nullptr) {
+                     const gb_settings* s, gb_result* res, Labeler* lab = nullptr, SolveHooks* hk = nullptr) {
     gb_status st = validate_settings(pr, o, l, s);
     if (st != GB_OK) return st;
     st = validate_descriptors(o, &s->algorithm, l);
     if (st != GB_OK) return st;
-    st = validate_root(pr, l, s);
-    if (st != GB_OK) return st;
+    if (!(hk && hk->nodes)) {
+        st = validate_root(pr, l, s);
+        if (st != GB_OK) return st;
+    }
 
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
@@ -1023,8 +1059,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     // others.  Two slots for the batched FP64 form, three for the mixed form
     // (and as many as num_streams allows).  The plain form keeps the
     // reference schedule (one batch at a time).
+    //
+    // Otherwise (the plain form, or one stream) the slots share one stream,
+    // as the reference's single stream slot: when a batch completes, the
+    // next full batch is popped and launched before the completed one is
+    // post-processed (device_resident_solver.cuh:1585-1596), else the
+    // next batch, full or not, follows the post-processing.
     const bool pipelined = s->form != 0 && s->num_streams >= 2;
-    const int slots = !pipelined ? 1 : std::min(s->form == 2 ? 3 : 2, std::min(s->num_streams, kMaxSlots));
+    const bool shared = !pipelined;
+    const int slots = shared ? 2 : std::min(s->form == 2 ? 3 : 2, std::min(s->num_streams, kMaxSlots));
     int device = 0;
     GB_HIP(hipGetDevice(&device));
     // 0/1 problems keep node bounds as bytes (every root bound +0 or 1, and
@@ -1035,6 +1078,9 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         // +0.0 or 1.0 exactly (a -0.0 bound would come back as +0.0)
         auto zo = [](double v) { return v == 1.0 || (v == 0.0 && !std::signbit(v)); };
         for (int j = 0; j < n && zero_one; ++j) zero_one = zo(pr->lb[j]) && zo(pr->ub[j]);
+        if (hk && hk->nodes)
+            for (const InitNode& in : *hk->nodes)
+                for (int j = 0; j < n && zero_one; ++j) zero_one = zo(in.lb[j]) && zo(in.ub[j]);
     }
     // Learned searches over 0/1 boxes in the batched forms keep every node's
     // iterate on the device: the learned rule takes its branching variables
@@ -1056,6 +1102,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             if (st != GB_OK) return st;
         }
     }
+    if (shared)
+        for (int k = 1; k < slots; ++k) bufs[k].stream = bufs[0].stream;
     // Child batch of the strong-branching labeller (2 children per candidate).
     DeviceBatch& sb = pooled.get().sb;
     if (lab) {
@@ -1110,29 +1158,48 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         if (st != GB_OK) return st;
         pool.dchunks = &xstore.chunks;
     }
-    {
-        HostNode root;
-        root.id = 0;
-        root.slot = pool.alloc();
+    // The initial nodes: the root of the problem (gpuboscia.cuh:86-96), or
+    // the nodes pushed on a solver object.
+    auto add_initial = [&](int id, int parent, int depth, const double* lb, const double* ub, const double* rx,
+                           int riters, double bound) -> gb_status {
+        HostNode nd;
+        nd.id = id;
+        nd.parent = parent;
+        nd.depth = depth;
+        nd.slot = pool.alloc();
         if (pool.compact) {
             for (int j = 0; j < n; ++j) {
-                pool.lb8(root.slot)[j] = pr->lb[j] == 1.0 ? 1 : 0;
-                pool.ub8(root.slot)[j] = pr->ub[j] == 1.0 ? 1 : 0;
+                pool.lb8(nd.slot)[j] = lb[j] == 1.0 ? 1 : 0;
+                pool.ub8(nd.slot)[j] = ub[j] == 1.0 ? 1 : 0;
             }
         } else {
-            std::copy(pr->lb, pr->lb + n, pool.lb(root.slot));
-            std::copy(pr->ub, pr->ub + n, pool.ub(root.slot));
+            std::copy(lb, lb + n, pool.lb(nd.slot));
+            std::copy(ub, ub + n, pool.ub(nd.slot));
         }
-        if (s->root_initial_point && s->root_initial_point_size == n) {
+        if (rx) {
             if (xdev)
-                GB_HIP(hipMemcpy(reinterpret_cast<double*>(pool.drx(root.slot)), s->root_initial_point,
-                                 (size_t)n * 8, hipMemcpyHostToDevice));
+                GB_HIP(hipMemcpy(reinterpret_cast<double*>(pool.drx(nd.slot)), rx, (size_t)n * 8,
+                                 hipMemcpyHostToDevice));
             else
-                std::copy(s->root_initial_point, s->root_initial_point + n, pool.rx(root.slot));
-            root.has_rx = true;
+                std::copy(rx, rx + n, pool.rx(nd.slot));
+            nd.has_rx = true;
         }
-        root.bound = s->natural_lower_bound;
-        q.push(root);
+        nd.riters = riters;
+        nd.bound = bound;
+        q.push(nd);
+        return GB_OK;
+    };
+    if (hk && hk->nodes) {
+        for (const InitNode& in : *hk->nodes) {
+            st = add_initial(in.id, in.parent, in.depth, in.lb.data(), in.ub.data(),
+                             in.rx.empty() ? nullptr : in.rx.data(), in.riters, in.bound);
+            if (st != GB_OK) return st;
+        }
+    } else {
+        const bool has_rx = s->root_initial_point && s->root_initial_point_size == n;
+        st = add_initial(0, -1, 0, pr->lb, pr->ub, has_rx ? s->root_initial_point : nullptr, 0,
+                         s->natural_lower_bound);
+        if (st != GB_OK) return st;
     }
     double inc = kInf;
     std::vector<double> inc_x;
@@ -1176,6 +1243,15 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     // Select the next batch from the queue into `slot`, upload it and queue
     // its kernels and the result copy on the slot's stream (an empty queue
     // leaves the slot idle).
+    // Scheduling observer (device_resident_solver.cuh:1027-1038): events of
+    // the stream slot a batch runs on, with that slot's dispatch count.
+    unsigned long long generation[kMaxSlots] = {};  // dispatches per stream slot
+    unsigned long long batch_gen[kMaxSlots] = {};   // the generation of each buffer set's batch
+    auto emit = [&](int type, int slot, int count) {
+        if (!hk || !hk->sched) return;
+        const gb_scheduling_event ev{type, shared ? 0 : slot, batch_gen[slot], count};
+        hk->sched(hk->sched_user, &ev);
+    };
     auto launch_batch = [&](int slot) -> gb_status {
         gb_status st = GB_OK;
         std::vector<HostNode>& batch = batches[slot];
@@ -1344,6 +1420,8 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
         st = db.download();
         if (st != GB_OK) return st;
         relax_s += std::chrono::duration<double>(Clock::now() - r0).count();
+        batch_gen[slot] = ++generation[shared ? 0 : slot];
+        emit(GB_SCHED_DISPATCHED, slot, N);
         return GB_OK;
     };
 
@@ -1371,17 +1449,16 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             if (st != GB_OK) return st;
             if (!in_flight(head)) { final_bound = inc; finished = true; break; }
         }
-        // The other slots in launch order after head: idle ones take the next
-        // batches; their nodes count as open while head's are processed.
-        int queued_elsewhere = 0;
-        for (int d = 1; d < slots; ++d) {
-            const int k = (head + d) % slots;
-            if (!in_flight(k)) {
-                st = launch_batch(k);
-                if (st != GB_OK) return st;
+        // Separate streams: the other slots in launch order after head, idle
+        // ones take the next batches (full or not) before head completes.
+        if (!shared)
+            for (int d = 1; d < slots; ++d) {
+                const int k = (head + d) % slots;
+                if (!in_flight(k)) {
+                    st = launch_batch(k);
+                    if (st != GB_OK) return st;
+                }
             }
-            if (in_flight(k)) queued_elsewhere += (int)batches[k].size();
-        }
         DeviceBatch& db = bufs[head];
         std::vector<HostNode>& batch = batches[head];
         const int N = (int)batch.size();
@@ -1390,8 +1467,25 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             GB_HIP(hipStreamSynchronize(db.stream));
             relax_s += std::chrono::duration<double>(Clock::now() - w0).count();
         }
+        emit(GB_SCHED_COMPLETION_DETACHED, head, N);
         for (int sl : gathered[head]) pool.release(sl);
         gathered[head].clear();
+        // One stream: its next batch, if a full one is waiting, is launched
+        // before this one is post-processed.
+        if (shared) {
+            const int k = (head + 1) % slots;
+            if (!in_flight(k) && (int)q.size() >= B) {
+                st = launch_batch(k);
+                if (st != GB_OK) return st;
+            }
+        }
+        // Nodes of the other slots count as open while head's are processed.
+        int queued_elsewhere = 0;
+        for (int d = 1; d < slots; ++d) {
+            const int k = (head + d) % slots;
+            if (in_flight(k)) queued_elsewhere += (int)batches[k].size();
+        }
+        emit(GB_SCHED_POSTPROCESS_STARTED, head, N);
 
         for (int i = 0; i < N; ++i) fw_iters += db.hit[i] - batch[i].riters;
         // Branching variables of the rules that depend only on x, on the host
@@ -1433,6 +1527,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             }
             const double node_eps = db.heps[i];
             const double g = db.hgap[i];
+            if (hk) {  // device_resident_solver.cuh:1931-1937
+                hk->last_iters = db.hit[i];
+                hk->last_gap = g;
+                hk->last_conv = std::isfinite(g) && g >= 0.0 && g < node_eps;
+            }
             nd.bound = certified_fw_lower_bound(nd.bound, db.hval[i], g);
             if (nd.bvar >= 0 && nd.riters == 0 && nd.bfrac > 0.0 && std::isfinite(nd.bound)
                     && std::isfinite(nd.pbound))
@@ -1472,25 +1571,27 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 continue;
             }
             if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED && s->branch_rule == GB_BRANCH_CUSTOM) {
-                // Host BranchStrategy: children as returned, ids from the
-                // same fetch_add(8) sequence (device_resident_solver.cuh:1989).
-                std::vector<double> clbs((size_t)GB_MAX_CUSTOM_CHILDREN * n);
-                std::vector<double> cubs((size_t)GB_MAX_CUSTOM_CHILDREN * n);
-                const int cnt = s->branch(s->branch_user, pool.lb(nd.slot), pool.ub(nd.slot), xi, n,
-                                          nd.depth, nd.id, clbs.data(), cubs.data(),
-                                          GB_MAX_CUSTOM_CHILDREN);
+                // Host BranchStrategy (device_resident_solver.cuh:1986-1995):
+                // the node as launched, with its new bound; 8 ids reserved.
+                const gb_node view{nd.id, nd.parent, nd.depth, pool.lb(nd.slot), pool.ub(nd.slot),
+                                   nd.has_rx ? pool.rx(nd.slot) : nullptr, nd.riters, nd.bound};
+                gb_children kids_out;
+                kids_out.n = n;
                 int nid = next_id;
-                next_id += cnt > 8 ? cnt : 8;
-                for (int cc = 0; cc < cnt && cc < GB_MAX_CUSTOM_CHILDREN; ++cc) {
+                next_id += 8;
+                if (s->branch(s->branch_user, &view, &nid, xi, n, iv.data(), (int)iv.size(), &kids_out) != 0)
+                    return fail(GB_ERR_RUNTIME, "the branch callback failed");
+                if (nid > next_id) next_id = nid;
+                for (const gb_children::Child& ch : kids_out.kids) {
                     HostNode c;
-                    c.id = nid++;
-                    c.parent = nd.id;
-                    c.depth = nd.depth + 1;
-                    c.bound = nd.bound;
+                    c.id = ch.id;
+                    c.parent = ch.parent;
+                    c.depth = ch.depth;
+                    c.bound = ch.bound;
                     if (!(c.bound < inc)) continue;
                     c.slot = pool.alloc();
-                    std::copy(clbs.data() + (size_t)cc * n, clbs.data() + (size_t)(cc + 1) * n, pool.lb(c.slot));
-                    std::copy(cubs.data() + (size_t)cc * n, cubs.data() + (size_t)(cc + 1) * n, pool.ub(c.slot));
+                    std::copy(ch.lb.begin(), ch.lb.end(), pool.lb(c.slot));
+                    std::copy(ch.ub.begin(), ch.ub.end(), pool.ub(c.slot));
                     q.push(c);
                 }
             } else if (db.hst[i] < GB_NODE_INTEGRAL_UNCONVERGED) {
@@ -1711,4 +1812,172 @@ extern "C" gb_status gb_solve(const gb_problem* problem, const gb_objective_desc
     } catch (const std::exception& e) {
         return gb::fail(GB_ERR_RUNTIME, e.what());
     }
+}
+
+extern "C" gb_status gb_children_push(gb_children* children, const gb_node* child) {
+    if (!children || !child || !child->lb || !child->ub)
+        return gb::fail(GB_ERR_INVALID_ARGUMENT, "null child node");
+    try {
+        gb_children::Child c;
+        c.id = child->id;
+        c.parent = child->parent_id;
+        c.depth = child->depth;
+        c.bound = child->dual_bound;
+        c.lb.assign(child->lb, child->lb + children->n);
+        c.ub.assign(child->ub, child->ub + children->n);
+        children->kids.push_back(std::move(c));
+        return GB_OK;
+    } catch (const std::exception& e) {
+        return gb::fail(GB_ERR_RUNTIME, e.what());
+    }
+}
+
+// The low-level solver object (DeviceResidentHybridSolver,
+// device_resident_solver.cuh:1184-1298) over solve_impl: the problem is
+// copied, nodes are pushed with the reference's checks, and each solve
+// consumes the pushed nodes.
+struct gb_solver {
+    std::vector<double> lb, ub;
+    std::vector<int> iv;
+    gb_problem problem{};
+    gb_objective_desc objective{};
+    gb_lmo_desc lmo{};
+    gb_settings settings{};
+    std::vector<gb::InitNode> pending;
+    gb_scheduling_fn sched = nullptr;
+    void* sched_user = nullptr;
+    gb_solver_stats stats{};
+    std::vector<double> inc_x;
+    bool has_inc = false;
+};
+
+extern "C" gb_status gb_solver_create(const gb_problem* problem, const gb_objective_desc* objective,
+                                      const gb_lmo_desc* lmo, const gb_settings* settings, gb_solver** out) {
+    if (!out) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null solver handle");
+    *out = nullptr;
+    try {
+        gb_status st = gb::validate_settings(problem, objective, lmo, settings);
+        if (st != GB_OK) return st;
+        st = gb::validate_descriptors(objective, &settings->algorithm, lmo);
+        if (st != GB_OK) return st;
+        auto* sv = new gb_solver();
+        const int n = problem->n_vars;
+        sv->lb.assign(problem->lb, problem->lb + n);
+        sv->ub.assign(problem->ub, problem->ub + n);
+        if (problem->n_int_vars > 0) sv->iv.assign(problem->int_vars, problem->int_vars + problem->n_int_vars);
+        sv->problem = gb_problem{n, sv->lb.data(), sv->ub.data(), sv->iv.data(), (int)sv->iv.size()};
+        sv->objective = *objective;
+        sv->lmo = *lmo;
+        sv->settings = *settings;
+        sv->settings.root_initial_point = nullptr;
+        sv->settings.root_initial_point_size = 0;
+        sv->stats.incumbent_value = std::numeric_limits<double>::infinity();
+        sv->stats.dual_bound = -std::numeric_limits<double>::infinity();
+        sv->stats.last_gpu_gap = std::numeric_limits<double>::infinity();
+        sv->stats.effective_gpu_batch_size = settings->gpu_batch_size;
+        sv->stats.effective_num_streams = settings->num_streams;
+        *out = sv;
+        return GB_OK;
+    } catch (const std::exception& e) {
+        return gb::fail(GB_ERR_RUNTIME, e.what());
+    }
+}
+
+extern "C" gb_status gb_solver_destroy(gb_solver* solver) {
+    delete solver;
+    return GB_OK;
+}
+
+extern "C" gb_status gb_solver_push(gb_solver* solver, const gb_node* node) {
+    if (!solver || !node) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null solver or node");
+    // device_resident_solver.cuh:1247-1260
+    if (!node->lb || !node->ub || node->relaxation_iterations < 0
+            || (node->relaxation_iterations > 0 && !node->relaxation_x))
+        return gb::fail(GB_ERR_INVALID_ARGUMENT, "invalid node buffer dimensions or resume state");
+    const int n = solver->problem.n_vars;
+    if (node->relaxation_x)
+        for (int j = 0; j < n; ++j)
+            if (!std::isfinite(node->relaxation_x[j]) || node->relaxation_x[j] < node->lb[j] - 1e-9
+                    || node->relaxation_x[j] > node->ub[j] + 1e-9)
+                return gb::fail(GB_ERR_INVALID_ARGUMENT, "initial/resumed node iterate violates finite bounds");
+    try {
+        gb::InitNode in;
+        in.id = node->id;
+        in.parent = node->parent_id;
+        in.depth = node->depth;
+        in.lb.assign(node->lb, node->lb + n);
+        in.ub.assign(node->ub, node->ub + n);
+        if (node->relaxation_x) in.rx.assign(node->relaxation_x, node->relaxation_x + n);
+        in.riters = node->relaxation_iterations;
+        in.bound = node->dual_bound;
+        solver->pending.push_back(std::move(in));
+        return GB_OK;
+    } catch (const std::exception& e) {
+        return gb::fail(GB_ERR_RUNTIME, e.what());
+    }
+}
+
+extern "C" gb_status gb_solver_set_scheduling_callback(gb_solver* solver, gb_scheduling_fn fn, void* user) {
+    if (!solver) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null solver");
+    solver->sched = fn;
+    solver->sched_user = user;
+    return GB_OK;
+}
+
+extern "C" gb_status gb_solver_set_progress_callback(gb_solver* solver, gb_progress_fn fn, void* user) {
+    if (!solver) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null solver");
+    solver->settings.progress = fn;
+    solver->settings.progress_user = user;
+    return GB_OK;
+}
+
+extern "C" gb_status gb_solver_solve(gb_solver* solver) {
+    if (!solver) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null solver");
+    try {
+        const int n = solver->problem.n_vars;
+        std::vector<double> x(n);
+        gb_result r{};
+        r.x = x.data();
+        gb::SolveHooks hk;
+        hk.nodes = &solver->pending;
+        hk.sched = solver->sched;
+        hk.sched_user = solver->sched_user;
+        const gb_status st = gb::solve_impl(&solver->problem, &solver->objective, &solver->lmo, &solver->settings,
+                                            &r, nullptr, &hk);
+        solver->pending.clear();
+        if (st != GB_OK) return st;
+        gb_solver_stats& S = solver->stats;
+        S.incumbent_value = r.objective;
+        S.dual_bound = r.dual_bound;
+        S.nodes_processed = r.nodes_processed;
+        S.last_gpu_iterations = hk.last_iters;
+        S.last_gpu_gap = hk.last_gap;
+        S.last_gpu_converged = hk.last_conv ? 1 : 0;
+        S.reached_tolerance = r.reached_tolerance;
+        S.reached_time_limit = r.reached_time_limit;
+        S.fw_iterations = r.fw_iterations;
+        S.elapsed_seconds = r.elapsed_seconds;
+        S.relax_seconds = r.relax_seconds;
+        solver->has_inc = std::isfinite(r.objective);
+        if (solver->has_inc) solver->inc_x = x;
+        else solver->inc_x.clear();
+        return GB_OK;
+    } catch (const std::bad_alloc&) {
+        return gb::fail(GB_ERR_RUNTIME, "host memory exhausted");
+    } catch (const std::exception& e) {
+        return gb::fail(GB_ERR_RUNTIME, e.what());
+    }
+}
+
+extern "C" gb_status gb_solver_get_stats(const gb_solver* solver, gb_solver_stats* out) {
+    if (!solver || !out) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null solver or output");
+    *out = solver->stats;
+    return GB_OK;
+}
+
+extern "C" gb_status gb_solver_incumbent_x(const gb_solver* solver, double* x, int* has) {
+    if (!solver || !has) return gb::fail(GB_ERR_INVALID_ARGUMENT, "null solver or output");
+    *has = solver->has_inc ? 1 : 0;
+    if (solver->has_inc && x) std::copy(solver->inc_x.begin(), solver->inc_x.end(), x);
+    return GB_OK;
 }

@@ -8,9 +8,14 @@
 //   LineSearchOptions, DeviceResidentConfig, BoxLMO, UnitSimplexLMO<MaxN,Eq>,
 //   EqualitySimplexLMO<MaxN>, BirkhoffLMO<Order>,
 //   GlobalCachedDenseQuadraticObjective<Vars,Threads>,
-//   ILSGlobalCachedObjective<Vars,Threads>, gpuboscia::Settings,
-//   gpuboscia::Result, gpuboscia::create_default_settings, gpuboscia::solve,
-//   gpuboscia::RoundingHeuristic.
+//   ILSGlobalCachedObjective<Vars,Threads>, the algorithm policies
+//   (FrankWolfe<LS>, DecompositionInvariantFrankWolfe<LS> and their aliases,
+//   algorithms.cuh:57-362), top_k_infeasible / most_infeasible / branch_pow2
+//   (branching.h:24-118), DeviceResidentHybridSolver with push, solve, the
+//   accessors and the scheduling observer (device_resident_solver.cuh:
+//   1027-1038, 1184-1298), gpuboscia::Settings, gpuboscia::Result,
+//   gpuboscia::create_default_settings, gpuboscia::solve_with_algorithm,
+//   gpuboscia::solve, gpuboscia::RoundingHeuristic.
 // Objectives hold caller-owned device pointers exactly as in the reference;
 // the device computations are the library's gfx950 kernels, so objective and
 // LMO kinds are the built-in ones (plus QuadraticAssignmentObjective).
@@ -23,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <exception>
 #include <functional>
 #include <limits>
 #include <stdexcept>
@@ -68,6 +74,143 @@ struct LineSearchOptions {
     double secant_tol = 1e-10;
     double secant_denom_eps = 1e-14;
 };
+
+// ---- branching.h:24-118 --------------------------------------------------------
+// The integer variable whose fractional part is closest to 1/2 (first on ties).
+inline int most_infeasible(const std::vector<double>& x, const std::vector<int>& int_vars) {
+    int best = -1;
+    double best_dist = -1.0;
+    for (int i : int_vars) {
+        const double frac = x[i] - std::floor(x[i]);
+        const double dist = std::min(frac, 1.0 - frac);
+        if (dist > best_dist) {
+            best_dist = dist;
+            best = i;
+        }
+    }
+    return best;
+}
+
+// Up to k fractional integer variables (distance to the nearest integer
+// above integer_tol), most fractional first.  The candidates are sorted by
+// the same std::sort call on the same sequence as the reference, so equal
+// distances come out in the same order.
+inline std::vector<int> top_k_infeasible(const std::vector<double>& x, const std::vector<int>& int_vars, int k,
+                                         double integer_tol = 1e-6) {
+    struct Cand {
+        int idx;
+        double dist;
+    };
+    std::vector<Cand> cands;
+    cands.reserve(int_vars.size());
+    for (int i : int_vars) {
+        const double frac = x[i] - std::floor(x[i]);
+        const double dist = std::min(frac, 1.0 - frac);
+        if (dist > integer_tol) cands.push_back({i, dist});
+    }
+    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.dist > b.dist; });
+    std::vector<int> out;
+    for (int i = 0; i < k && i < (int)cands.size(); ++i) out.push_back(cands[i].idx);
+    return out;
+}
+
+// 2^k children on the k most fractional variables: bit b of the child's
+// index set means lb = ceil(x), clear means ub = floor(x).  Children take
+// ids next_id, next_id + 1, ..., the parent's bound and depth + 1, and start
+// fresh.
+inline std::vector<Node> branch_pow2(const Node& parent, int& next_id, const std::vector<double>& x,
+                                     const std::vector<int>& int_vars, int k = 3, double integer_tol = 1e-6) {
+    const std::vector<int> vars = top_k_infeasible(x, int_vars, k, integer_tol);
+    std::vector<Node> kids;
+    if (vars.empty()) return kids;
+    const int count = 1 << (int)vars.size();
+    kids.reserve(count);
+    for (int mask = 0; mask < count; ++mask) {
+        Node c = parent;
+        c.id = next_id++;
+        c.parent_id = parent.id;
+        c.depth = parent.depth + 1;
+        c.dual_bound = parent.dual_bound;
+        c.status = NodeStatus::Active;
+        c.relaxation_x.clear();
+        c.relaxation_iterations = 0;
+        for (size_t b = 0; b < vars.size(); ++b) {
+            const int v = vars[b];
+            if (mask & (1 << b)) c.lb[v] = std::ceil(x[v]);
+            else c.ub[v] = std::floor(x[v]);
+        }
+        kids.push_back(std::move(c));
+    }
+    return kids;
+}
+
+// ---- algorithm policies (algorithms.cuh:57-362) ---------------------------------
+// Host-side tags: each maps to the library's algorithm descriptor (FW or
+// DICG, and its step rule); the step rules themselves run in the kernels.
+struct OpenLoopLS {
+    static constexpr int kind = GB_LS_OPEN_LOOP;
+};
+struct ScaledOpenLoopLS {
+    static constexpr int kind = GB_LS_SCALED_OPEN_LOOP;
+};
+struct SafeguardedScaledOpenLoopLS {
+    static constexpr int kind = GB_LS_SAFEGUARDED_SCALED_OPEN_LOOP;
+};
+struct SecantLS {
+    static constexpr int kind = GB_LS_SECANT;
+    int max_iters;
+    double tolerance;
+    double denom_eps;
+    SecantLS(int max_iters_ = 8, double tolerance_ = 1e-10, double denom_eps_ = 1e-14)
+        : max_iters(max_iters_), tolerance(tolerance_), denom_eps(denom_eps_) {}
+};
+struct QuadraticExactLS {
+    static constexpr int kind = GB_LS_QUADRATIC_EXACT;
+};
+
+namespace gpuboscia_detail {
+template <class LS>
+inline void fill_line_search(const LS&, gb_algorithm_desc& a) {
+    a.line_search = LS::kind;
+}
+inline void fill_line_search(const SecantLS& ls, gb_algorithm_desc& a) {
+    a.line_search = GB_LS_SECANT;
+    a.secant_iters = ls.max_iters;
+    a.secant_tol = ls.tolerance;
+    a.secant_denom_eps = ls.denom_eps;
+}
+}  // namespace gpuboscia_detail
+
+template <typename LS = OpenLoopLS>
+struct FrankWolfe {
+    static constexpr bool uses_dicg = false;
+    LS ls{};
+    void fill(gb_algorithm_desc& a) const {
+        a.algorithm = GB_ALG_FRANK_WOLFE;
+        gpuboscia_detail::fill_line_search(ls, a);
+    }
+};
+
+template <typename LS = OpenLoopLS>
+struct DecompositionInvariantFrankWolfe {
+    static constexpr bool uses_dicg = true;
+    LS ls{};
+    void fill(gb_algorithm_desc& a) const {
+        a.algorithm = GB_ALG_DICG;
+        gpuboscia_detail::fill_line_search(ls, a);
+    }
+};
+
+using FrankWolfeOpenLoop = FrankWolfe<OpenLoopLS>;
+using FrankWolfeSecant = FrankWolfe<SecantLS>;
+using FrankWolfePolicy = FrankWolfeOpenLoop;
+using DICGOpenLoop = DecompositionInvariantFrankWolfe<OpenLoopLS>;
+using DICGScaledOpenLoop = DecompositionInvariantFrankWolfe<ScaledOpenLoopLS>;
+using DICGSafeguardedScaledOpenLoop = DecompositionInvariantFrankWolfe<SafeguardedScaledOpenLoopLS>;
+using DICGSecant = DecompositionInvariantFrankWolfe<SecantLS>;
+using DICGPolicy = DICGOpenLoop;
+using FrankWolfeQuadraticExact = FrankWolfe<QuadraticExactLS>;
+using DICGQuadraticExact = DecompositionInvariantFrankWolfe<QuadraticExactLS>;
 
 enum class FWExecutionMode { SingleBlock, MultiBlock };
 
@@ -217,6 +360,253 @@ struct FloorHeuristic {
     double offset() const { return offset_; }
 };
 
+namespace detail {
+
+inline void throw_status(gb_status st) {
+    if (st == GB_OK) return;
+    const std::string msg = gb_last_error();
+    if (st == GB_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+// Host callbacks behind the C-ABI: the user's std::function values, the
+// problem's integer variables, and the first exception a callback threw
+// (rethrown once the solve returns, as the reference's solve() would let it
+// propagate).
+struct Callbacks {
+    BranchStrategy branch;
+    std::function<void(int, int, double, double, bool)> progress;
+    RootInitialPointValidator validator;
+    std::function<void(const gb_scheduling_event&)> scheduling;
+    const std::vector<int>* int_vars = nullptr;
+    std::exception_ptr error;
+    void rethrow() {
+        if (error) {
+            std::exception_ptr e = error;
+            error = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+};
+
+inline void progress_tramp(void* u, int iter, int open, double bound, double inc, int new_inc) {
+    auto* c = static_cast<Callbacks*>(u);
+    if (c->error) return;
+    try {
+        c->progress(iter, open, bound, inc, new_inc != 0);
+    } catch (...) {
+        c->error = std::current_exception();
+    }
+}
+
+inline void scheduling_tramp(void* u, const gb_scheduling_event* ev) {
+    auto* c = static_cast<Callbacks*>(u);
+    if (c->error) return;
+    try {
+        c->scheduling(*ev);
+    } catch (...) {
+        c->error = std::current_exception();
+    }
+}
+
+inline int validator_tramp(void* u, const double* x, const double* lb, const double* ub, int n, double tol) {
+    auto* c = static_cast<Callbacks*>(u);
+    try {
+        return c->validator(std::vector<double>(x, x + n), std::vector<double>(lb, lb + n),
+                            std::vector<double>(ub, ub + n), tol) ? 1 : 0;
+    } catch (...) {
+        c->error = std::current_exception();
+        return 0;
+    }
+}
+
+// BranchStrategy behind gb_branch_fn: the parent as a Node (with its
+// relaxation_x, relaxation_iterations and certified dual bound), next_id in
+// and out, and every returned child handed back with its id, parent id,
+// depth, bound and box.
+inline int branch_tramp(void* u, const gb_node* p, int* next_id, const double* x, int n, const int*, int,
+                        gb_children* out) {
+    auto* c = static_cast<Callbacks*>(u);
+    try {
+        Node parent(p->id, p->parent_id, p->depth, std::vector<double>(p->lb, p->lb + n),
+                    std::vector<double>(p->ub, p->ub + n));
+        if (p->relaxation_x) parent.relaxation_x.assign(p->relaxation_x, p->relaxation_x + n);
+        parent.relaxation_iterations = p->relaxation_iterations;
+        parent.dual_bound = p->dual_bound;
+        int nid = *next_id;
+        const std::vector<Node> kids = c->branch(parent, nid, std::vector<double>(x, x + n), *c->int_vars);
+        *next_id = nid;
+        for (const Node& k : kids) {
+            if ((int)k.lb.size() != n || (int)k.ub.size() != n)
+                throw std::invalid_argument("invalid node buffer dimensions or resume state");
+            const gb_node cn{k.id, k.parent_id, k.depth, k.lb.data(), k.ub.data(), nullptr, 0, k.dual_bound};
+            throw_status(gb_children_push(out, &cn));
+        }
+        return 0;
+    } catch (...) {
+        c->error = std::current_exception();
+        return 1;
+    }
+}
+
+// Simplex integer marks live in the LMO functor (lmo.cuh:352); the kernels
+// read them from device memory for as long as the descriptor is used.
+struct DeviceMarks {
+    uint8_t* ptr = nullptr;
+    explicit DeviceMarks(const std::vector<uint8_t>& marks) {
+        if (marks.empty()) return;
+        if (hipMalloc(reinterpret_cast<void**>(&ptr), marks.size()) != hipSuccess
+                || hipMemcpy(ptr, marks.data(), marks.size(), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("HIP allocation of LMO integer marks failed");
+    }
+    ~DeviceMarks() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    DeviceMarks(const DeviceMarks&) = delete;
+    DeviceMarks& operator=(const DeviceMarks&) = delete;
+};
+
+inline void fill_config(const DeviceResidentConfig& d, gb_settings& s) {
+    s.gpu_batch_size = d.gpu_batch_size;
+    s.num_streams = d.num_streams;
+    s.gpu_block_threads = d.gpu_block_threads;
+    s.gpu_max_iters = d.gpu_max_iters;
+    s.gpu_eps = d.gpu_eps;
+    s.gpu_eps_decay = d.gpu_eps_decay;
+    s.gpu_eps_min = d.gpu_eps_min;
+    s.integer_tol = d.integer_tol;
+    s.abs_gap_tol = d.abs_gap_tol;
+    s.rel_gap_tol = d.rel_gap_tol;
+}
+
+}  // namespace detail
+}  // namespace gpuboscia
+
+// ---- device_resident_solver.cuh:1027-1038 ----------------------------------------
+enum class DRSSchedulingEventType { Dispatched, CompletionDetached, PostprocessStarted };
+
+struct DRSSchedulingEvent {
+    DRSSchedulingEventType type;
+    int slot_id;
+    std::uint64_t generation;
+    int n_nodes;
+};
+
+// ---- DeviceResidentHybridSolver (device_resident_solver.cuh:1184-1298) --------------
+// The low-level solver: nodes are pushed by the caller and relaxed with the
+// given algorithm policy; a BranchStrategy decides the children.  Over the
+// library's gb_solver object.  The Problem is held by reference, as in the
+// reference, and must outlive the solver.
+template <typename DeviceObjective, typename Algorithm, typename LMO = BoxLMO,
+          typename Heuristic = gpuboscia::RoundingHeuristic>
+class DeviceResidentHybridSolver {
+  public:
+    DeviceResidentHybridSolver(const Problem& prob, DeviceObjective obj, Algorithm alg, BranchStrategy branch_sel,
+                               DeviceResidentConfig cfg = {}, LMO lmo = {},
+                               double time_limit_seconds = std::numeric_limits<double>::infinity(),
+                               Heuristic heuristic = {})
+        : prob_(prob), marks_(lmo.integer_marks(prob.n_vars)) {
+        if (cfg.execution_mode == FWExecutionMode::MultiBlock)
+            throw std::invalid_argument("MultiBlock execution is not available in this build");
+        if (!std::isfinite(cfg.dicg_paired_occupancy_fraction) || cfg.dicg_paired_occupancy_fraction <= 0.0
+                || cfg.dicg_paired_occupancy_fraction > 1.0)
+            throw std::invalid_argument("invalid tolerance configuration");
+        gb_default_settings(&s_);
+        gpuboscia::detail::fill_config(cfg, s_);
+        alg.fill(s_.algorithm);
+        s_.algorithm.heuristic = heuristic.kind();
+        s_.algorithm.heuristic_offset = heuristic.offset();
+        s_.time_limit_seconds = time_limit_seconds;
+        cb_.int_vars = &prob_.int_vars;
+        if (branch_sel) {
+            cb_.branch = std::move(branch_sel);
+            s_.branch_rule = GB_BRANCH_CUSTOM;
+            s_.branch = gpuboscia::detail::branch_tramp;
+            s_.branch_user = &cb_;
+        }
+        od_ = obj.desc();
+        ld_ = lmo.desc(marks_.ptr);
+        const gb_problem gp{prob_.n_vars, prob_.lb.data(), prob_.ub.data(), prob_.int_vars.data(),
+                            (int)prob_.int_vars.size()};
+        gpuboscia::detail::throw_status(gb_solver_create(&gp, &od_, &ld_, &s_, &h_));
+    }
+    ~DeviceResidentHybridSolver() { (void)gb_solver_destroy(h_); }
+    DeviceResidentHybridSolver(const DeviceResidentHybridSolver&) = delete;
+    DeviceResidentHybridSolver& operator=(const DeviceResidentHybridSolver&) = delete;
+
+    void push(const Node& nd) {
+        if (nd.lb.size() != (size_t)prob_.n_vars || nd.ub.size() != nd.lb.size()
+                || (!nd.relaxation_x.empty() && nd.relaxation_x.size() != nd.lb.size()))
+            throw std::invalid_argument("invalid node buffer dimensions or resume state");
+        const gb_node gn{nd.id, nd.parent_id, nd.depth, nd.lb.data(), nd.ub.data(),
+                         nd.relaxation_x.empty() ? nullptr : nd.relaxation_x.data(), nd.relaxation_iterations,
+                         nd.dual_bound};
+        gpuboscia::detail::throw_status(gb_solver_push(h_, &gn));
+    }
+    void set_progress_callback(std::function<void(int, int, double, double, bool)> cb) {
+        cb_.progress = std::move(cb);
+        if (cb_.progress)
+            gpuboscia::detail::throw_status(
+                gb_solver_set_progress_callback(h_, gpuboscia::detail::progress_tramp, &cb_));
+        else
+            gpuboscia::detail::throw_status(gb_solver_set_progress_callback(h_, nullptr, nullptr));
+    }
+    // Invoked on the solving thread; must not call back into the solver.
+    void set_scheduling_callback(std::function<void(const DRSSchedulingEvent&)> cb) {
+        if (cb) {
+            auto f = std::move(cb);
+            cb_.scheduling = [f](const gb_scheduling_event& e) {
+                f(DRSSchedulingEvent{static_cast<DRSSchedulingEventType>(e.type), e.slot_id,
+                                     (std::uint64_t)e.generation, e.n_nodes});
+            };
+            gpuboscia::detail::throw_status(
+                gb_solver_set_scheduling_callback(h_, gpuboscia::detail::scheduling_tramp, &cb_));
+        } else {
+            cb_.scheduling = nullptr;
+            gpuboscia::detail::throw_status(gb_solver_set_scheduling_callback(h_, nullptr, nullptr));
+        }
+    }
+    void solve() {
+        const gb_status st = gb_solver_solve(h_);
+        cb_.rethrow();
+        gpuboscia::detail::throw_status(st);
+        gpuboscia::detail::throw_status(gb_solver_get_stats(h_, &stats_));
+        std::vector<double> x((size_t)prob_.n_vars);
+        int has = 0;
+        gpuboscia::detail::throw_status(gb_solver_incumbent_x(h_, x.data(), &has));
+        inc_x_ = has ? x : std::vector<double>{};
+    }
+
+    double incumbent_value() const { return stats_.incumbent_value; }
+    std::vector<double> incumbent_x() const { return inc_x_; }
+    int nodes_processed() const { return stats_.nodes_processed; }
+    double dual_bound() const { return stats_.dual_bound; }
+    int last_gpu_iterations() const { return stats_.last_gpu_iterations; }
+    double last_gpu_gap() const { return stats_.last_gpu_gap; }
+    bool last_gpu_converged() const { return stats_.last_gpu_converged != 0; }
+    bool reached_tolerance() const { return stats_.reached_tolerance != 0; }
+    bool reached_time_limit() const { return stats_.reached_time_limit != 0; }
+    int effective_gpu_batch_size() const { return s_.gpu_batch_size; }
+    int effective_num_streams() const { return s_.num_streams; }
+    int paired_gpu_batches() const { return 0; }
+    // Additions (no reference counterpart).
+    long long fw_iterations() const { return stats_.fw_iterations; }
+    double elapsed_seconds() const { return stats_.elapsed_seconds; }
+
+  private:
+    const Problem& prob_;
+    gpuboscia::detail::DeviceMarks marks_;
+    gb_settings s_{};
+    gb_objective_desc od_{};
+    gb_lmo_desc ld_{};
+    gpuboscia::detail::Callbacks cb_;
+    gb_solver* h_ = nullptr;
+    gb_solver_stats stats_{std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity()};
+    std::vector<double> inc_x_;
+};
+
+namespace gpuboscia {
+
 struct Settings {
     DeviceResidentConfig device{};
     FWAlgorithmKind algorithm = FWAlgorithmKind::DICG;
@@ -247,79 +637,17 @@ struct Result {
 
 inline Settings create_default_settings() { return Settings{}; }
 
-namespace detail {
-
-inline void throw_status(gb_status st) {
-    if (st == GB_OK) return;
-    const std::string msg = gb_last_error();
-    if (st == GB_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
-    throw std::runtime_error(msg);
-}
-
-struct Callbacks {
-    const Settings* s;
-    const Problem* p;
-};
-
-inline void progress_tramp(void* u, int iter, int open, double bound, double inc, int new_inc) {
-    auto* c = static_cast<Callbacks*>(u);
-    c->s->progress(iter, open, bound, inc, new_inc != 0);
-}
-
-inline int validator_tramp(void* u, const double* x, const double* lb, const double* ub, int n,
-                           double tol) {
-    auto* c = static_cast<Callbacks*>(u);
-    return c->s->root_initial_point_feasible(std::vector<double>(x, x + n),
-                                             std::vector<double>(lb, lb + n),
-                                             std::vector<double>(ub, ub + n), tol) ? 1 : 0;
-}
-
-inline int branch_tramp(void* u, const double* lb, const double* ub, const double* x, int n,
-                        int depth, int node_id, double* clb, double* cub, int max_children) {
-    auto* c = static_cast<Callbacks*>(u);
-    Node parent(node_id, -1, depth, std::vector<double>(lb, lb + n), std::vector<double>(ub, ub + n));
-    int next_id = 0;
-    const auto children = c->s->branch(parent, next_id, std::vector<double>(x, x + n), c->p->int_vars);
-    const int cnt = std::min<int>((int)children.size(), max_children);
-    for (int i = 0; i < cnt; ++i) {
-        std::copy(children[i].lb.begin(), children[i].lb.end(), clb + (size_t)i * n);
-        std::copy(children[i].ub.begin(), children[i].ub.end(), cub + (size_t)i * n);
-    }
-    return cnt;
-}
-
-}  // namespace detail
-
-// gpuboscia.cuh:108-170.  Objective and LMO are the built-in kinds above.
-template <typename DeviceObjective, typename LMO, typename Heuristic = RoundingHeuristic>
-Result solve(const Problem& problem, DeviceObjective objective, LMO lmo,
-             const Settings& settings = create_default_settings(), Heuristic heuristic = {}) {
-    const bool dicg = settings.algorithm == FWAlgorithmKind::DICG;
-    if (settings.line_search.kind == LineSearchKind::QuadraticExact
-            && !DeviceObjective::supports_quadratic_exact)
-        throw std::invalid_argument("quadratic-exact requires an objective with quadratic curvature support");
-    if (dicg && !LMO::supports_dicg)
-        throw std::invalid_argument("the selected LMO does not support DICG; use Frank-Wolfe");
-    // Root-point rules (tree.h:115-147) are applied by gb_solve with the same
-    // messages; the LMO's relaxation_feasible_cpu has a C-ABI twin there.
+// gpuboscia.cuh:56-104: one solve with an explicit algorithm policy.  The
+// root node carries the natural lower bound and the root initial point
+// (validated by tree.h:115-147's rules inside gb_solve); without a
+// BranchStrategy the built-in branch_pow2(k = 3) rule runs in the library.
+template <typename DeviceObjective, typename Algorithm, typename LMO, typename Heuristic = RoundingHeuristic>
+Result solve_with_algorithm(const Problem& problem, DeviceObjective objective, LMO lmo, const Settings& settings,
+                            Algorithm algorithm, Heuristic heuristic = {}) {
     gb_settings s;
     gb_default_settings(&s);
-    const DeviceResidentConfig& d = settings.device;
-    s.gpu_batch_size = d.gpu_batch_size;
-    s.num_streams = d.num_streams;
-    s.gpu_block_threads = d.gpu_block_threads;
-    s.gpu_max_iters = d.gpu_max_iters;
-    s.gpu_eps = d.gpu_eps;
-    s.gpu_eps_decay = d.gpu_eps_decay;
-    s.gpu_eps_min = d.gpu_eps_min;
-    s.integer_tol = d.integer_tol;
-    s.abs_gap_tol = d.abs_gap_tol;
-    s.rel_gap_tol = d.rel_gap_tol;
-    s.algorithm.algorithm = dicg ? GB_ALG_DICG : GB_ALG_FRANK_WOLFE;
-    s.algorithm.line_search = static_cast<int>(settings.line_search.kind);
-    s.algorithm.secant_iters = settings.line_search.secant_iters;
-    s.algorithm.secant_tol = settings.line_search.secant_tol;
-    s.algorithm.secant_denom_eps = settings.line_search.secant_denom_eps;
+    detail::fill_config(settings.device, s);
+    algorithm.fill(s.algorithm);
     s.algorithm.heuristic = heuristic.kind();
     s.algorithm.heuristic_offset = heuristic.offset();
     s.time_limit_seconds = settings.time_limit_seconds;
@@ -328,44 +656,40 @@ Result solve(const Problem& problem, DeviceObjective objective, LMO lmo,
     s.precision = settings.precision;
     s.branch_rule = settings.branch_rule;
     s.autotune = settings.autotune ? 1 : 0;
-    detail::Callbacks cb{&settings, &problem};
+    detail::Callbacks cb;
+    cb.int_vars = &problem.int_vars;
     if (!settings.root_initial_point.empty()) {
         s.root_initial_point = settings.root_initial_point.data();
         s.root_initial_point_size = (int)settings.root_initial_point.size();
     }
     if (settings.root_initial_point_feasible) {
+        cb.validator = settings.root_initial_point_feasible;
         s.root_validator = detail::validator_tramp;
         s.root_validator_user = &cb;
     }
     if (settings.progress) {
+        cb.progress = settings.progress;
         s.progress = detail::progress_tramp;
         s.progress_user = &cb;
     }
     if (settings.branch) {
+        cb.branch = settings.branch;
         s.branch_rule = GB_BRANCH_CUSTOM;
         s.branch = detail::branch_tramp;
         s.branch_user = &cb;
     }
-    // Simplex integer marks live in the LMO functor (lmo.cuh:352); the
-    // kernels read them from device memory for the duration of the solve.
-    uint8_t* d_marks = nullptr;
-    const std::vector<uint8_t> marks = lmo.integer_marks(problem.n_vars);
-    if (!marks.empty()) {
-        if (hipMalloc(reinterpret_cast<void**>(&d_marks), marks.size()) != hipSuccess
-                || hipMemcpy(d_marks, marks.data(), marks.size(), hipMemcpyHostToDevice) != hipSuccess)
-            throw std::runtime_error("HIP allocation of LMO integer marks failed");
-    }
-    const gb_lmo_desc ld = lmo.desc(d_marks);
+    detail::DeviceMarks marks(lmo.integer_marks(problem.n_vars));
+    const gb_lmo_desc ld = lmo.desc(marks.ptr);
     const gb_objective_desc od = objective.desc();
-    gb_problem gp{problem.n_vars, problem.lb.data(), problem.ub.data(), problem.int_vars.data(),
-                  (int)problem.int_vars.size()};
-    Result r;
+    const gb_problem gp{problem.n_vars, problem.lb.data(), problem.ub.data(), problem.int_vars.data(),
+                        (int)problem.int_vars.size()};
     std::vector<double> x(problem.n_vars > 0 ? problem.n_vars : 0);
     gb_result gr{};
     gr.x = x.data();
     const gb_status st = gb_solve(&gp, &od, &ld, &s, &gr);
-    if (d_marks) (void)hipFree(d_marks);
+    cb.rethrow();
     detail::throw_status(st);
+    Result r;
     r.objective = gr.objective;
     if (std::isfinite(gr.objective)) r.x = x;
     r.dual_bound = gr.dual_bound;
@@ -374,6 +698,42 @@ Result solve(const Problem& problem, DeviceObjective objective, LMO lmo,
     r.reached_tolerance = gr.reached_tolerance != 0;
     r.reached_time_limit = gr.reached_time_limit != 0;
     return r;
+}
+
+// gpuboscia.cuh:108-170: the algorithm policy from the settings, with the
+// reference's checks and messages.
+template <typename DeviceObjective, typename LMO, typename Heuristic = RoundingHeuristic>
+Result solve(const Problem& problem, DeviceObjective objective, LMO lmo,
+             const Settings& settings = create_default_settings(), Heuristic heuristic = {}) {
+    const LineSearchKind k = settings.line_search.kind;
+    const SecantLS secant{settings.line_search.secant_iters, settings.line_search.secant_tol,
+                          settings.line_search.secant_denom_eps};
+    if (k == LineSearchKind::QuadraticExact) {
+        if (!DeviceObjective::supports_quadratic_exact)
+            throw std::invalid_argument("quadratic-exact requires an objective with quadratic curvature support");
+        if (settings.algorithm == FWAlgorithmKind::DICG) {
+            if (!LMO::supports_dicg) throw std::invalid_argument("the selected LMO does not support DICG");
+            return solve_with_algorithm(problem, objective, lmo, settings, DICGQuadraticExact{}, heuristic);
+        }
+        return solve_with_algorithm(problem, objective, lmo, settings, FrankWolfeQuadraticExact{}, heuristic);
+    }
+    if (settings.algorithm == FWAlgorithmKind::DICG) {
+        if (!LMO::supports_dicg)
+            throw std::invalid_argument("the selected LMO does not support DICG; use Frank-Wolfe");
+        if (k == LineSearchKind::Secant)
+            return solve_with_algorithm(problem, objective, lmo, settings, DICGSecant{secant}, heuristic);
+        if (k == LineSearchKind::ScaledOpenLoop)
+            return solve_with_algorithm(problem, objective, lmo, settings, DICGScaledOpenLoop{}, heuristic);
+        if (k == LineSearchKind::SafeguardedScaledOpenLoop)
+            return solve_with_algorithm(problem, objective, lmo, settings, DICGSafeguardedScaledOpenLoop{},
+                                        heuristic);
+        return solve_with_algorithm(problem, objective, lmo, settings, DICGOpenLoop{}, heuristic);
+    }
+    if (k == LineSearchKind::ScaledOpenLoop || k == LineSearchKind::SafeguardedScaledOpenLoop)
+        throw std::invalid_argument("scaled open-loop is available only for DICG");
+    if (k == LineSearchKind::Secant)
+        return solve_with_algorithm(problem, objective, lmo, settings, FrankWolfeSecant{secant}, heuristic);
+    return solve_with_algorithm(problem, objective, lmo, settings, FrankWolfeOpenLoop{}, heuristic);
 }
 
 }  // namespace gpuboscia

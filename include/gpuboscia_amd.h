@@ -476,13 +476,36 @@ typedef void (*gb_branch_score_fn)(void* user, const double* x, int n,
                                    const int* candidates, int count,
                                    int depth, double* score);
 
-/* Custom BranchStrategy (branching.h:13-18): called on the host with the
-   node's bounds and final iterate; writes up to max_children child bound
-   vectors (n each) into child_lb / child_ub and returns their count (0 = the
-   node is not branched). */
-typedef int (*gb_branch_fn)(void* user, const double* lb, const double* ub,
-                            const double* x, int n, int depth, int node_id,
-                            double* child_lb, double* child_ub, int max_children);
+/* A branch-and-bound node (node.h:9-33); host arrays of n_vars doubles. */
+typedef struct {
+    int id;
+    int parent_id;
+    int depth;
+    const double* lb;
+    const double* ub;
+    const double* relaxation_x;   /* NULL = empty: the node starts fresh */
+    int relaxation_iterations;    /* > 0: resume the open-loop schedule (needs relaxation_x) */
+    double dual_bound;
+} gb_node;
+
+/* The child list a BranchStrategy callback fills. */
+typedef struct gb_children gb_children;
+/* Appends a copy of *child (id, parent_id, depth, dual_bound, lb, ub) to the
+   list; relaxation_x and relaxation_iterations are ignored, since children
+   start fresh (device_resident_solver.cuh:1990-1994). */
+gb_status gb_children_push(gb_children* children, const gb_node* child);
+
+/* Custom BranchStrategy (branching.h:13-18), called on the host after a
+   node's relaxation, as device_resident_solver.cuh:1986-1995 calls it:
+   `parent` is the node with its certified dual bound and the
+   relaxation_x / relaxation_iterations it was launched with; *next_id is the
+   first id reserved for its children (8 are reserved, as the reference's
+   fetch_add(8); a callback that takes more moves later reservations past
+   them); x is the node's final iterate.  Children are appended with
+   gb_children_push; children whose bound reaches the incumbent are dropped.
+   Return 0, or nonzero to stop the solve with GB_ERR_RUNTIME. */
+typedef int (*gb_branch_fn)(void* user, const gb_node* parent, int* next_id, const double* x, int n,
+                            const int* int_vars, int n_int_vars, gb_children* children);
 
 typedef enum {
     GB_BRANCH_POW2 = 0,            /* branch_pow2(k) on top_k_infeasible (default k=3) */
@@ -492,8 +515,6 @@ typedef enum {
     GB_BRANCH_CALLBACK = 3,        /* host scorer callback, 2-way split */
     GB_BRANCH_CUSTOM = 4           /* host BranchStrategy callback (gb_branch_fn) */
 } gb_branch_rule;
-
-#define GB_MAX_CUSTOM_CHILDREN 64
 
 /* Settings + DeviceResidentConfig (gpuboscia.cuh:21-32,
    device_resident_solver.cuh:1004-1025). */
@@ -560,6 +581,71 @@ gb_status gb_solve(const gb_problem* problem,
                    const gb_lmo_desc* lmo,
                    const gb_settings* settings,
                    gb_result* result);
+
+/* ---------------------------------------------------------------------- */
+/* The low-level solver object (DeviceResidentHybridSolver,                 */
+/* device_resident_solver.cuh:1184-1298): nodes pushed by the caller, a      */
+/* scheduling observer, and the accessors of the last solve.                */
+/* ---------------------------------------------------------------------- */
+
+/* DRSSchedulingEventType / DRSSchedulingEvent (device_resident_solver.cuh:1027-1038). */
+typedef enum {
+    GB_SCHED_DISPATCHED = 0,
+    GB_SCHED_COMPLETION_DETACHED = 1,
+    GB_SCHED_POSTPROCESS_STARTED = 2
+} gb_scheduling_event_type;
+
+typedef struct {
+    int type;               /* gb_scheduling_event_type */
+    int slot_id;            /* the stream slot */
+    unsigned long long generation;  /* dispatches of that slot so far, from 1 */
+    int n_nodes;
+} gb_scheduling_event;
+
+/* Called on the solving thread; must not call back into the solver. */
+typedef void (*gb_scheduling_fn)(void* user, const gb_scheduling_event* event);
+
+typedef struct gb_solver gb_solver;
+
+/* Constructor checks of the reference (device_resident_solver.cuh:1198-1218)
+   and the descriptor checks of gb_solve.  The problem's arrays are copied;
+   the objective's and LMO's device pointers must stay valid while the
+   solver solves.  settings->time_limit_seconds is the constructor's time
+   limit; settings->root_initial_point and natural_lower_bound are not used
+   (the caller pushes its nodes). */
+gb_status gb_solver_create(const gb_problem* problem, const gb_objective_desc* objective,
+                           const gb_lmo_desc* lmo, const gb_settings* settings, gb_solver** out);
+gb_status gb_solver_destroy(gb_solver* solver);
+/* push (device_resident_solver.cuh:1247-1260): the same checks and messages. */
+gb_status gb_solver_push(gb_solver* solver, const gb_node* node);
+gb_status gb_solver_set_scheduling_callback(gb_solver* solver, gb_scheduling_fn fn, void* user);
+/* set_progress_callback (device_resident_solver.cuh:1262-1266). */
+gb_status gb_solver_set_progress_callback(gb_solver* solver, gb_progress_fn fn, void* user);
+/* solve: relaxes the pushed nodes and their descendants to the proven
+   optimum or a limit; the pushed nodes are consumed. */
+gb_status gb_solver_solve(gb_solver* solver);
+
+/* The accessors (device_resident_solver.cuh:1280-1298) after the last solve. */
+typedef struct {
+    double incumbent_value;
+    double dual_bound;
+    int nodes_processed;
+    int last_gpu_iterations;
+    double last_gpu_gap;
+    int last_gpu_converged;
+    int reached_tolerance;
+    int reached_time_limit;
+    int effective_gpu_batch_size;
+    int effective_num_streams;
+    int paired_gpu_batches;       /* always 0: no paired cooperative launches here */
+    long long fw_iterations;      /* no reference counterpart */
+    double elapsed_seconds;
+    double relax_seconds;
+} gb_solver_stats;
+gb_status gb_solver_get_stats(const gb_solver* solver, gb_solver_stats* out);
+/* incumbent_x: n_vars doubles into x; *has = 0 (x untouched) when there is no
+   incumbent (the reference returns an empty vector). */
+gb_status gb_solver_incumbent_x(const gb_solver* solver, double* x, int* has);
 
 /* Strong-branching labels generated by the solver itself: runs the batched
    B&B with branch_pow2 on the settings' branch_k most fractional variables

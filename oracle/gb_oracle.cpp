@@ -1256,9 +1256,13 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
     double final_bound = -kInf;
     bool tol_hit = false, time_hit = false, finished = false;
     const int B = s->batch_size;
-    // Two slots when the batched schedule keeps two batches in flight: the
-    // next batch is popped (against the incumbent of that moment) before the
-    // current one is post-processed, as gpuboscia_amd/csrc/bnb.cpp does.
+    // Two slots.  When the batched schedule keeps two batches in flight
+    // (separate streams) the next batch is popped, full or not, before the
+    // current one is post-processed.  Otherwise (the plain form, or one
+    // stream) the next batch is popped before the post-processing only when
+    // a full batch is waiting (the reference's refill before post-processing,
+    // device_resident_solver.cuh:1585-1596), else after it; both as
+    // gpuboscia_amd/csrc/bnb.cpp does.
     const bool pipelined = s->form != 0 && s->num_streams >= 2;
     struct Slot {
         std::vector<BNode> batch;
@@ -1344,7 +1348,7 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
             if (!slot[head].busy()) { final_bound = inc; finished = true; break; }
         }
         const int other = 1 - head;
-        if (pipelined && !slot[other].busy()) launch(slot[other]);
+        if (!slot[other].busy() && (pipelined || (int)q.size() >= B)) launch(slot[other]);
         Slot& sl = slot[head];
         const int N = (int)sl.batch.size();
         const int elsewhere = slot[other].busy() ? (int)slot[other].batch.size() : 0;
@@ -1401,7 +1405,7 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
             finish_check(remaining);
         }
         if (!finished && q.empty() && !slot[other].busy()) { final_bound = inc; finished = true; }
-        if (pipelined && slot[other].busy()) head = other;
+        if (slot[other].busy()) head = other;
     }
     r->objective = inc;
     r->dual_bound = final_bound;

@@ -147,8 +147,10 @@ typedef struct {
     long long max_nodes;      /* safety cap; <=0 = none */
     int form;                 /* 0 = plain schedule, 1 = batched form (fast relax, warm starts, cutoff) */
     int refresh;              /* batched form: direct Q x recompute interval */
-    int num_streams;          /* batched form: >= 2 keeps two batches in flight (the next one
-                                 is popped before the current one is post-processed) */
+    int num_streams;          /* form 1 with >= 2: two batches in flight, the next one popped
+                                 before the current one is post-processed; otherwise the next
+                                 one is popped first only when a full batch is waiting (the
+                                 reference's refill before post-processing) */
 } orc_bnb_settings;
 
 typedef struct {

@@ -1,6 +1,8 @@
-"""The reference's examples/minimal.cu, written against the C++ drop-in
-header (include/gpuboscia/gpuboscia.hpp) and built by __graft_entry__.build():
-x = [0, 1], objective 0.18, plus the std::function hooks and exceptions."""
+"""C++ programs written against the drop-in header (include/gpuboscia/gpuboscia.hpp)
+and built by __graft_entry__.build(): the reference's examples/minimal.cu
+(x = [0, 1], objective 0.18, plus the std::function hooks and exceptions)
+and its solver-level known-answer tests through DeviceResidentHybridSolver
+(tests/cpp/reference_kats.cpp)."""
 import os
 import subprocess
 
@@ -19,3 +21,13 @@ def test_cpp_dropin_minimal_example():
     assert out.returncode == 0, out.stdout + out.stderr
     assert "x=[0, 1], objective=0.18000000" in out.stdout
     assert "FAIL" not in out.stdout
+
+
+def test_cpp_reference_solver_kats():
+    exe = os.path.join(ROOT, "tests", "cpp", "reference_kats")
+    assert os.path.exists(exe), "tests/cpp/reference_kats not built: run __graft_entry__.build()"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "FAIL" not in out.stdout
+    assert "0 failure(s)" in out.stdout
