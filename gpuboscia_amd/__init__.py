@@ -152,7 +152,7 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_relax_batch_noalloc", "gb_lmo_compute_v", "gb_mixed_warm_products",
             "gb_children_push", "gb_solver_create", "gb_solver_destroy", "gb_solver_push",
             "gb_solver_set_scheduling_callback", "gb_solver_set_progress_callback", "gb_solver_solve",
-            "gb_solver_get_stats", "gb_solver_incumbent_x"]
+            "gb_solver_get_stats", "gb_solver_incumbent_x", "gb_candidate_features"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -206,6 +206,9 @@ def load():
                  "gb_solver_set_scheduling_callback", "gb_solver_set_progress_callback", "gb_solver_solve",
                  "gb_solver_get_stats", "gb_solver_incumbent_x"):
         getattr(lib, name).restype = ctypes.c_int
+    lib.gb_candidate_features.restype = ctypes.c_int
+    lib.gb_candidate_features.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 10 \
+        + [ctypes.c_double, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
     lib.gb_shutdown.restype = ctypes.c_int
     lib.gb_shutdown.argtypes = []
     lib.gb_relax_batch_fast.restype = ctypes.c_int
@@ -333,6 +336,14 @@ def mixed_warm_products(obj: ObjectiveDesc, N, x, status, iterations, out, strea
     """gb_mixed_warm_products on torch device tensors."""
     check(load().gb_mixed_warm_products(ctypes.byref(obj), int(N), _ptr(x), _ptr(status), _ptr(iterations),
                                         _ptr(out), ctypes.c_void_p(stream or 0)))
+
+
+def candidate_features(n, N, X, G, LB, UB, depth, gap, value, pc_up, pc_down, Qt, gs, flags, tol, F,
+                       stream=None):
+    """gb_candidate_features on torch device tensors (F: float32 [N*n, 8])."""
+    check(load().gb_candidate_features(int(n), int(N), _ptr(X), _ptr(G), _ptr(LB), _ptr(UB), _ptr(depth), _ptr(gap),
+                                       _ptr(value), _ptr(pc_up), _ptr(pc_down), _ptr(Qt), float(gs), _ptr(flags),
+                                       float(tol), _ptr(F), ctypes.c_void_p(stream or 0)))
 
 
 def shutdown():

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "gb_fast.h"
 #include "gb_host.h"
 #include "gb_mixed.h"
 
@@ -496,4 +497,19 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
     } catch (const std::exception& ex) {
         return fail(GB_ERR_RUNTIME, ex.what());
     }
+}
+
+extern "C" gb_status gb_candidate_features(int n, int N, const double* X, const double* G, const double* LB,
+                                           const double* UB, const int* depth, const double* gap,
+                                           const double* value, const double* pc_up, const double* pc_down,
+                                           const double* Qt, double gs, const uint8_t* flags, double tol, float* F,
+                                           void* stream) {
+    if (n <= 0 || N < 0) return fail(GB_ERR_INVALID_ARGUMENT, "n must be positive and N non-negative");
+    if (N == 0) return GB_OK;
+    if (!X || !G || !LB || !UB || !depth || !gap || !value || !pc_up || !pc_down || !Qt || !flags || !F)
+        return fail(GB_ERR_INVALID_ARGUMENT, "null feature buffer");
+    const int e = launch_candidate_features(n, N, X, G, LB, UB, depth, gap, value, pc_up, pc_down, Qt,
+                                            (size_t)n + 1, gs, flags, tol, F, stream);
+    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    return GB_OK;
 }

@@ -79,11 +79,12 @@ def fit(X, T, epochs=30, batch=1024, lr=2e-3, seed=0, progress=False):
                "sigma": sigma.tolist()}
 
 
-def solve_with(Q, c, rule, brancher=None, time_limit=120.0, form=2, max_nodes=0):
+def solve_with(Q, c, rule, brancher=None, time_limit=120.0, form=2, max_nodes=0, branch_k=3):
     n = Q.shape[0]
     obj, keep = _objective(Q, c)
     s = _settings(time_limit, max_nodes=max_nodes, form=form)
     s.branch_rule = rule
+    s.branch_k = branch_k
     if brancher is not None:
         s.learned_model = brancher.handle
     t0 = time.perf_counter()
@@ -92,58 +93,75 @@ def solve_with(Q, c, rule, brancher=None, time_limit=120.0, form=2, max_nodes=0)
     return r
 
 
-def evaluate(brancher, n, seeds, time_limit=120.0, form=2):
-    """Node count and time of the learned rule vs the rule without it
-    (branch_pow2 on the most fractional variables, the default) on the same
-    instances and the same tree shape (k = 3 variables per split), both to the
-    proven optimum (gap 1e-6).  Keys keep the name "mf" for the
-    fractionality-based rule."""
+BASELINES = (("mf", gb.BRANCH_MOST_FRACTIONAL), ("pow2", gb.BRANCH_POW2))
+
+
+def evaluate(brancher, n, seeds, time_limit=120.0, form=2, learned_ks=(3,)):
+    """Node count and time of the learned rule against the rules without it,
+    on the same instances, all to the proven optimum (gap 1e-6): "mf" is
+    most-fractional branching (one variable, two children; the comparison
+    the learned-branching configuration names) and "pow2" the reference
+    default (branch_pow2 on the 3 most fractional variables, the learned
+    rule's own tree shape).  The learned rule splits on its branch_k
+    best-scored candidates: key "learned" for k = 3, "learned_k<k>" for the
+    other k in learned_ks."""
     rows = []
     for seed in seeds:
         Q, c, sha = instances.binary_qp(n, seed=seed)
-        mf = solve_with(Q, c, gb.BRANCH_POW2, None, time_limit, form)
-        ln = solve_with(Q, c, gb.BRANCH_LEARNED, brancher, time_limit, form)
-        rows.append({"seed": seed, "sha256": sha[:16], "mf_nodes": mf["nodes_processed"],
-                     "learned_nodes": ln["nodes_processed"], "mf_seconds": mf["wall"],
-                     "learned_seconds": ln["wall"], "mf_objective": mf["objective"],
-                     "learned_objective": ln["objective"],
-                     "same_optimum": abs(mf["objective"] - ln["objective"]) <= 1e-9 * max(1.0, abs(mf["objective"])),
-                     "proven": not (mf["reached_time_limit"] or ln["reached_time_limit"])})
+        row = {"seed": seed, "sha256": sha[:16]}
+        runs = {key: solve_with(Q, c, rule, None, time_limit, form) for key, rule in BASELINES}
+        for k in learned_ks:
+            runs["learned" if k == 3 else f"learned_k{k}"] = solve_with(Q, c, gb.BRANCH_LEARNED, brancher,
+                                                                        time_limit, form, branch_k=k)
+        if "learned" not in runs:
+            runs["learned"] = solve_with(Q, c, gb.BRANCH_LEARNED, brancher, time_limit, form)
+        for key, r in runs.items():
+            row[f"{key}_nodes"] = r["nodes_processed"]
+            row[f"{key}_seconds"] = r["wall"]
+            row[f"{key}_objective"] = r["objective"]
+        ref = runs["learned"]["objective"]
+        row["same_optimum"] = all(abs(r["objective"] - ref) <= 1e-9 * max(1.0, abs(ref)) for r in runs.values())
+        row["proven"] = not any(r["reached_time_limit"] for r in runs.values())
+        rows.append(row)
     return rows
 
 
 def evaluate_budget(brancher, n, seeds, max_nodes, time_limit=120.0, form=2):
-    """For sizes whose searches do not finish: the learned rule and the rule
-    without it (branch_pow2 on the most fractional variables, same tree
-    shape) each get the same node budget on the same instances; the better
-    rule ends with the higher global dual bound (both bounds are FP64-
-    certified, so each is a valid lower bound on the optimum)."""
+    """For sizes whose searches do not finish: the learned rule and the rules
+    without it (most-fractional "mf" and branch_pow2 "pow2") each get the same
+    node budget on the same instances; the better rule ends with the higher
+    global dual bound (all bounds are FP64-certified, so each is a valid
+    lower bound on the optimum)."""
     rows = []
     for k, seed in enumerate(seeds):
         if k % 128 == 0:
             print(f"[learned] budget evaluation, instance {k}", flush=True)
         Q, c, sha = instances.binary_qp(n, seed=seed)
-        mf = solve_with(Q, c, gb.BRANCH_POW2, None, time_limit, form, max_nodes)
-        ln = solve_with(Q, c, gb.BRANCH_LEARNED, brancher, time_limit, form, max_nodes)
-        rows.append({"seed": seed, "sha256": sha[:16], "mf_nodes": mf["nodes_processed"],
-                     "learned_nodes": ln["nodes_processed"], "mf_seconds": mf["wall"],
-                     "learned_seconds": ln["wall"], "mf_bound": mf["dual_bound"], "learned_bound": ln["dual_bound"],
-                     "mf_objective": mf["objective"], "learned_objective": ln["objective"]})
+        row = {"seed": seed, "sha256": sha[:16]}
+        runs = {key: solve_with(Q, c, rule, None, time_limit, form, max_nodes) for key, rule in BASELINES}
+        runs["learned"] = solve_with(Q, c, gb.BRANCH_LEARNED, brancher, time_limit, form, max_nodes)
+        for key, r in runs.items():
+            row[f"{key}_nodes"] = r["nodes_processed"]
+            row[f"{key}_seconds"] = r["wall"]
+            row[f"{key}_bound"] = r["dual_bound"]
+            row[f"{key}_objective"] = r["objective"]
+        rows.append(row)
     return rows
 
 
-def budget_summary(rows):
-    """Bound comparison at equal node budgets: the gap of each rule is measured
-    against the better incumbent of the two runs."""
-    mb = np.array([r["mf_bound"] for r in rows])
+def budget_summary(rows, baseline="mf"):
+    """Bound comparison at equal node budgets against one baseline rule: the
+    gap of each rule is measured against the better incumbent of the two
+    runs."""
+    mb = np.array([r[f"{baseline}_bound"] for r in rows])
     lb = np.array([r["learned_bound"] for r in rows])
-    inc = np.minimum([r["mf_objective"] for r in rows], [r["learned_objective"] for r in rows])
+    inc = np.minimum([r[f"{baseline}_objective"] for r in rows], [r["learned_objective"] for r in rows])
     gm, gl = inc - mb, inc - lb
     ok = (gm > 0) & (gl > 0)
-    return {"instances": len(rows), "learned_bound_higher": int((lb > mb).sum()),
+    return {"baseline": baseline, "instances": len(rows), "learned_bound_higher": int((lb > mb).sum()),
             "learned_bound_lower": int((lb < mb).sum()), "mean_bound_gain": float(np.mean(lb - mb)),
             "gap_ratio_geomean": float(np.exp(np.mean(np.log(gl[ok] / gm[ok])))) if ok.any() else None,
-            "mf_seconds": float(sum(r["mf_seconds"] for r in rows)),
+            "baseline_seconds": float(sum(r[f"{baseline}_seconds"] for r in rows)),
             "learned_seconds": float(sum(r["learned_seconds"] for r in rows))}
 
 
@@ -179,7 +197,8 @@ def train_main(argv=None):
         b = gb.Brancher.from_file(a.scorer)
         if a.node_budget > 0:
             rows = evaluate_budget(b, a.n, range(50000, 50000 + a.test), a.node_budget)
-            summary = {"n": a.n, "node_budget": a.node_budget, **budget_summary(rows)}
+            summary = {"n": a.n, "node_budget": a.node_budget, "vs_mf": budget_summary(rows, "mf"),
+                       "vs_pow2": budget_summary(rows, "pow2")}
             print(json.dumps(summary), flush=True)
             return summary
         rows = evaluate(b, a.n, seeds=range(50000, 50000 + a.test))
@@ -196,23 +215,25 @@ def train_main(argv=None):
             rows = evaluate_budget(b, a.n, range(50000, 50000 + a.test), a.node_budget)
             summary = {"n": a.n, "train_instances": a.train, "rows": info["rows"],
                        "label_seconds": round(t1 - t0, 2), "fit_seconds": round(t2 - t1, 2),
-                       "final_loss": info["final_loss"], "node_budget": a.node_budget, **budget_summary(rows)}
+                       "final_loss": info["final_loss"], "node_budget": a.node_budget,
+                       "vs_mf": budget_summary(rows, "mf"), "vs_pow2": budget_summary(rows, "pow2")}
             print(json.dumps(summary), flush=True)
             return summary
         rows = evaluate(b, a.n, seeds=range(50000, 50000 + a.test)) if a.test > 0 else []
-    mf = np.array([r["mf_nodes"] for r in rows], dtype=float)
-    ln = np.array([r["learned_nodes"] for r in rows], dtype=float)
     summary = {"n": a.n, "train_instances": a.train, "rows": info["rows"], "label_seconds": round(t1 - t0, 2),
                "fit_seconds": round(t2 - t1, 2), "initial_loss": info["initial_loss"],
                "final_loss": info["final_loss"], "test_instances": len(rows)}
     if rows:
-        summary.update({"mf_nodes": int(mf.sum()), "learned_nodes": int(ln.sum()),
-                        "node_ratio": float(ln.sum() / mf.sum()),
-                        "node_ratio_geomean": float(np.exp(np.mean(np.log(ln / mf)))),
-                        "mf_seconds": float(sum(r["mf_seconds"] for r in rows)),
+        ln = np.array([r["learned_nodes"] for r in rows], dtype=float)
+        summary.update({"learned_nodes": int(ln.sum()),
                         "learned_seconds": float(sum(r["learned_seconds"] for r in rows)),
                         "same_optimum": all(r["same_optimum"] for r in rows),
                         "proven": all(r["proven"] for r in rows)})
+        for key, _ in BASELINES:
+            bn = np.array([r[f"{key}_nodes"] for r in rows], dtype=float)
+            summary[f"vs_{key}"] = {"nodes": int(bn.sum()), "node_ratio": float(ln.sum() / bn.sum()),
+                                    "node_ratio_geomean": float(np.exp(np.mean(np.log(ln / bn)))),
+                                    "seconds": float(sum(r[f"{key}_seconds"] for r in rows))}
     print(json.dumps(summary), flush=True)
     return summary
 

@@ -429,6 +429,19 @@ gb_status gb_grid_barrier_selftest(int groups, int group_blocks, int threads,
    by its mean over the node's candidates; 5 depth / n,
    6 FW gap / (1 + |f(x)|), 7 ub_i - lb_i. */
 
+/* The feature rows the learned rule computes on the device for a relaxed
+   batch (diagnostic; no reference counterpart): row i * n + j of F [N*n][8]
+   holds the features of variable j of node i when j is a fractional
+   candidate (integer flag set, distance to the nearest integer above tol),
+   zeros otherwise.  Device arrays: X, G [N][n] iterate and gradient, LB, UB
+   [N][n], depth [N] int, gap, value [N], pc_up, pc_down [n] pseudocost means
+   (< 0: no history), Qt (the dense objective's, diagonal read in place) with
+   grad_scale gs, flags [n]. */
+gb_status gb_candidate_features(int n, int N, const double* X, const double* G, const double* LB,
+                                const double* UB, const int* depth, const double* gap, const double* value,
+                                const double* pc_up, const double* pc_down, const double* Qt, double gs,
+                                const uint8_t* flags, double tol, float* F, void* stream);
+
 typedef struct gb_brancher gb_brancher;
 
 gb_status gb_brancher_create(unsigned long long seed, gb_brancher** out);

@@ -79,3 +79,44 @@ def loss_grad(p, X, T, mu, sigma, rounding=True):
     g[OFF_W4:OFF_B4] = h3.T @ dy
     g[OFF_B4] = dy.sum()
     return loss, g
+
+
+def candidate_features(X, G, LB, UB, depth, gap, value, pc_up, pc_down, qdiag, gs, flags, tol):
+    """Feature rows of the learned rule, restated from their definitions (the
+    order documented next to GB_MLP_FEATURES): for node i and fractional
+    candidate j (flag set, min(frac, 1 - frac) > tol), in FP64 then rounded
+    to float32:
+      0 x_j, 1 min(frac, 1 - frac), 2 g_j / (max_k |g_k| + 1e-12),
+      3 / 4 up / down gain over its mean across the node's candidates (+1e-12):
+        gain = (pseudocost mean if >= 0, else 1/2 gs Q_jj d) * d, d the
+        distance to ceil (up) or floor (down),
+      5 depth / n (float32 division), 6 gap / (1 + |value|), 7 ub_j - lb_j;
+    other rows are zero.  Returns float32 [N * n, 8]."""
+    N, n = X.shape
+    F = np.zeros((N * n, 8), dtype=np.float32)
+    for i in range(N):
+        x, g = X[i], G[i]
+        frac = x - np.floor(x)
+        dist = np.minimum(frac, 1.0 - frac)
+        cand = (flags != 0) & (dist > tol)
+        gmax = float(np.max(np.abs(g)))
+        du = np.ceil(x) - x
+        dd = x - np.floor(x)
+        curv = 0.5 * gs * qdiag
+        gu = np.where(pc_up >= 0.0, pc_up, curv * du) * du
+        gd = np.where(pc_down >= 0.0, pc_down, curv * dd) * dd
+        cnt = int(cand.sum())
+        mu = float(gu[cand].sum()) / (cnt if cnt else 1.0)
+        md = float(gd[cand].sum()) / (cnt if cnt else 1.0)
+        rgap = gap[i] / (1.0 + abs(value[i]))
+        for j in np.nonzero(cand)[0]:
+            r = i * n + j
+            F[r, 0] = x[j]
+            F[r, 1] = dist[j]
+            F[r, 2] = g[j] / (gmax + 1e-12)
+            F[r, 3] = gu[j] / (mu + 1e-12)
+            F[r, 4] = gd[j] / (md + 1e-12)
+            F[r, 5] = np.float32(depth[i]) / np.float32(n)
+            F[r, 6] = rgap
+            F[r, 7] = UB[i, j] - LB[i, j]
+    return F

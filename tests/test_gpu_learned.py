@@ -149,3 +149,76 @@ def test_device_iterates_give_the_same_search(n, seed, form, monkeypatch):
     assert not dev["reached_time_limit"]
     for key in ("nodes_processed", "fw_iterations", "objective", "dual_bound"):
         assert dev[key] == host[key], key
+
+
+def test_candidate_features_kernel_matches_oracle():
+    """The device feature rows (candidate_features_kernel through
+    gb_candidate_features) against the oracle's restatement of their
+    definitions: features without reductions bit for bit, the two gain
+    ratios (a mean over the candidates, summed in another order) within a
+    few float32 ulps."""
+    import torch
+    from oracle import mlp_oracle as mo
+    from tests.gpu_util import dev
+    rng = np.random.default_rng(12)
+    N, n = 37, 150
+    X = rng.random((N, n))
+    X[:, ::7] = np.round(X[:, ::7])                 # some integral coordinates
+    X[5] = np.round(X[5])                            # a node without candidates
+    LB = np.where(rng.random((N, n)) < 0.1, np.floor(X), 0.0)
+    UB = np.where(rng.random((N, n)) < 0.1, np.ceil(X), 1.0)
+    G = rng.standard_normal((N, n))
+    M = rng.standard_normal((n, n))
+    Q = M.T @ M / n + 0.1 * np.eye(n)
+    depth = rng.integers(0, 30, N).astype(np.int32)
+    gap = rng.random(N) * 1e-2
+    val = rng.standard_normal(N)
+    pcu = np.where(rng.random(n) < 0.5, rng.random(n), -1.0)
+    pcd = np.where(rng.random(n) < 0.5, rng.random(n), -1.0)
+    flags = (rng.random(n) < 0.8).astype(np.uint8)
+    tol, gs = 1e-6, 2.0
+    F = torch.zeros((N * n, 8), dtype=torch.float32, device="cuda")
+    say("[learned] feature kernel vs oracle")
+    gb.candidate_features(n, N, dev(X), dev(G), dev(LB), dev(UB), dev(depth), dev(gap), dev(val), dev(pcu), dev(pcd),
+                          dev(np.ascontiguousarray(Q.T)), gs, dev(flags), tol, F)
+    torch.cuda.synchronize()
+    got = F.cpu().numpy()
+    ref = mo.candidate_features(X, G, LB, UB, depth, gap, val, pcu, pcd, np.diag(Q).copy(), gs, flags, tol)
+    exact = [0, 1, 2, 5, 6, 7]
+    assert np.array_equal(got[:, exact], ref[:, exact])
+    assert np.allclose(got[:, 3:5], ref[:, 3:5], rtol=5e-7, atol=0)
+    assert np.all(got[5 * n:6 * n] == 0.0)
+    assert (ref[:, 1] > 0).sum() > 0.5 * N * n
+
+
+def test_learned_rule_against_most_fractional_and_pow2():
+    """The committed scorer (trained on n = 40) on six held-out instances of
+    n = 60 to 80 (seeds 50000 on; 50004 is skipped because its
+    most-fractional search alone takes about a minute), every rule to the
+    same proven optimum:
+      * with 3 variables per split (the default; the tree shape of
+        branch_pow2, the rule it replaces) it needs fewer nodes than
+        branch_pow2;
+      * with 2 variables per split it needs fewer nodes than most-fractional
+        branching (one variable per split);
+      * it reaches the optimum in less time than most-fractional branching.
+    Node counts do not depend on timing: every search here is deterministic."""
+    import os
+    from gpuboscia_amd import learned
+    path = os.path.join(os.path.dirname(gb.__file__), "scorer_binary_qp_n40.npz")
+    b = gb.Brancher.from_file(path)
+    keys = ("mf", "pow2", "learned", "learned_k2")
+    nodes = {k: 0 for k in keys}
+    secs = {k: 0.0 for k in keys}
+    for n, seeds in ((60, (50000, 50001)), (70, (50002, 50003)), (80, (50005, 50006))):
+        for r in learned.evaluate(b, n, seeds, time_limit=60.0, form=2, learned_ks=(3, 2)):
+            say(f"[learned] n={n} seed {r['seed']}: nodes " + " ".join(f"{k} {r[k + '_nodes']}" for k in keys)
+                + "; seconds " + " ".join(f"{k} {r[k + '_seconds']:.3f}" for k in keys))
+            assert r["proven"] and r["same_optimum"], r
+            for k in keys:
+                nodes[k] += r[f"{k}_nodes"]
+                secs[k] += r[f"{k}_seconds"]
+    say(f"[learned] total nodes {nodes}, seconds {secs}")
+    assert nodes["learned"] < nodes["pow2"]
+    assert nodes["learned_k2"] < nodes["mf"]
+    assert secs["learned"] < secs["mf"]
