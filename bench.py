@@ -7,22 +7,30 @@ tolerance 1e-3 * 0.8^depth, 200 iterations per node per step.  Inputs are
 resident in HBM; each step resets the per-node iteration counters and
 statuses (fill kernels) and relaxes the batch again.
 
-  value      FW iterations/s of the mixed-precision form (BF16 MFMA pass,
-             one bf16 plane of Q, + FP64 certification pass), whole batch,
-             HIP events on the launch stream bracket the timed steps
-  mixed_fp32 the same with three bf16 planes of Q (FP32-class products), the
-             precision the searches in time_to_optimum use
+  value      FW iterations/s of the mixed-precision form as the searches run
+             it (three bf16 planes of Q on MFMA, FP32-class products, + the
+             FP64 certification pass), whole batch, HIP events on the launch
+             stream bracket the timed steps
+  mixed      the same with one bf16 plane of Q
   plain      the reference algorithm (FP64, DICG + secant), same batch
   batched    the FP64 batched form, same batch
-  scale      the mixed form on 16384 nodes (16 batches in flight)
-  roofline   the BF16 MFMA kernel (relax_mixed_kernel): algorithmic flops
-             2 n^2 per FW iteration / its launch time, against 2.5 PFLOP/s
+  scale      the mixed forms on 16384 nodes
+  roofline   the BF16 MFMA kernel of the headline form (relax_mixed_kernel,
+             three planes): algorithmic flops 2 n^2 per FW iteration / its
+             launch time, against 2.5 PFLOP/s
+  heterogeneous_batch  256 binary QPs, n log-uniform in [32, 512], box or
+             bounded-simplex LMO by seed, 1024 open nodes each, one
+             gb_relax_instances call (and its captured replay)
   cpu_baseline  the CPU oracle (FP64 restatement of the plain form) on the
-             same 1024 nodes and budget (about 8 s), 16 host threads
+             same 1024 nodes and budget, on the host CPUs this process may
+             use (capped by OMP_NUM_THREADS), with nproc and the CPU model
   time_to_optimum  gb_solve to the proven optimum (abs = rel gap 1e-6) on
              binary QPs of n = 20, 40, 60, 80, 100 (three seeds each): plain
-             vs batched vs mixed vs learned + batched vs learned + mixed, and the geometric-mean
-             speedups over plain per n
+             vs batched vs mixed vs learned + batched vs learned + mixed, and
+             the geometric-mean speedups over plain per n
+  time_to_target_n200  n = 200 held-out binary QPs: the time each form
+             needs to reach the certified relative gap plain reaches in
+             --tt200-seconds (python bench.py --only-tt200 runs only this)
 
 Prints ONE JSON line.
 
@@ -117,12 +125,12 @@ class Batch:
                                 self.eps, self.it, self.st, self.gap, self.val, self.bv, self.bvv, self.flags,
                                 1e-6, None, 64, 0, 0, s, self.stats)
         else:
-            planes = 3 if form == "mixed_fp32" else 1
+            planes = 3 if form.startswith("mixed_fp32") else 1
             if self.mq.get(planes) is None:
                 self.mq[planes] = gb.MixedQ(self.obj, planes, s)
             gb.relax_batch_mixed(self.mq[planes], self.obj, self.alg, self.lmo, self.N, budget, self.x, self.lb,
                                  self.ub, self.eps, self.it, self.st, self.gap, self.val, self.bv, self.bvv,
-                                 self.flags, 1e-6, None, fp64_max_iters=-1 if form == "mixed_pass1" else 0,
+                                 self.flags, 1e-6, None, fp64_max_iters=-1 if form.endswith("_pass1") else 0,
                                  mixed_iterations=self.mit, stream=s)
 
     def measure(self, form, budget, steps, warmup):
@@ -159,11 +167,14 @@ class Batch:
         return out
 
 
-def traffic_bytes():
-    """HBM bytes per launch of relax_mixed_kernel on this workload, from the
-    committed rocprofv3 counter passes (FETCH_SIZE + WRITE_SIZE, scaled by
-    the same counters on a copy of known size); None if not recorded."""
-    path = os.path.join(ROOT, "profiles", "relax_mixed_kernel_n200_1024nodes_traffic.json")
+def traffic_bytes(planes):
+    """HBM bytes per launch of relax_mixed_kernel with `planes` bf16 planes
+    on this workload, from the committed rocprofv3 counter passes
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy of known
+    size); None if not recorded."""
+    name = "relax_mixed_kernel_n200_1024nodes_traffic.json" if planes == 1 else \
+        "relax_mixed_kernel_3planes_n200_1024nodes_traffic.json"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -174,7 +185,7 @@ def cpu_baseline(args, Q, c, LB, UB, EPS):
     """The CPU oracle (FP64 restatement of the plain form) on the node batch
     (or its first --cpu-nodes nodes), node-parallel on host threads."""
     from oracle import pyoracle as orc
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     sample = min(len(LB), max(threads, args.cpu_nodes))
     obj = orc.dense_objective(Q, c, 2.0, 0.5, 0.0)
     a = orc.alg()
@@ -188,7 +199,31 @@ def cpu_baseline(args, Q, c, LB, UB, EPS):
     iters = int(r["iterations"].sum())
     return {"value": iters / dt, "unit": "FW iterations/s", "cores": threads, "kind": "port",
             "sample": f"{sample} of the {len(LB)} nodes, budget {args.budget}, {iters} iterations "
-                      f"in {dt:.2f} s (plain-form algorithm, FP64)"}
+                      f"in {dt:.2f} s (plain-form algorithm, FP64)",
+            "nproc": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0)), "cpu_model": cpu_model(),
+            "threads_basis": "the CPUs this process may use, capped by OMP_NUM_THREADS (the host share "
+                             "given to one GPU)"}
+
+
+def host_threads():
+    """Worker threads of the CPU baseline: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when it is set (the host's share per GPU)."""
+    allowed = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        allowed = min(allowed, int(cap))
+    return max(1, allowed)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def hetero_batches(args):
@@ -295,15 +330,15 @@ def hetero_batches(args):
             "value": iters / t, "unit": "FW iterations/s", "nodes_per_s": nodes / t, "converged_nodes": converged}
 
 
-def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0):
+def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, rel_gap=1e-6):
     import torch
     dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
     dc = torch.from_numpy(c).cuda()
     obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
     s = gb.default_settings()
     s.abs_gap_tol = 1e-6
-    s.rel_gap_tol = 1e-6
-    s.time_limit_seconds = 120.0
+    s.rel_gap_tol = rel_gap
+    s.time_limit_seconds = time_limit
     s.max_nodes = max_nodes
     s.form = form
     if form == 2:
@@ -317,6 +352,14 @@ def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0):
     return {"seconds": round(r["elapsed_seconds"], 5), "relax_seconds": round(r["relax_seconds"], 5),
             "nodes": r["nodes_processed"], "fw_iterations": r["fw_iterations"], "objective": r["objective"],
             "dual_bound": r["dual_bound"], "proven": not r["reached_time_limit"]}
+
+
+def gap_relative(bound, inc):
+    """gap_utils.h:34-42: (incumbent - bound) / min(|bound|, |incumbent|)."""
+    if not (np.isfinite(bound) and np.isfinite(inc)):
+        return float("inf")
+    scale = min(abs(bound), abs(inc))
+    return float("inf") if scale == 0.0 else (inc - bound) / scale
 
 
 def time_to_optimum(args):
@@ -337,23 +380,17 @@ def time_to_optimum(args):
     out = []
     for n in sizes:
         rows, seeds = 3 * n // 2, (1, 2, 3)
-        # an untimed, node-capped solve per form and size first: the first
-        # launches of the kernels of this size (full batches on every
-        # pipeline slot, so each kernel variant is loaded)
-        Qw, cw, _ = instances.binary_qp(n, seed=9, rows=rows)
-        for key, form, learned in forms:
-            solve_row(gb, Qw, cw, n, form, learned, max_nodes=4096)
         for seed in seeds:
             Q, c, sha = instances.binary_qp(n, seed=seed, rows=rows)
+            # Every form, plain included, first runs the same untimed,
+            # node-capped search on another instance of this size: the first
+            # launches of a kernel variant (some are reached only by some
+            # batch sizes) and of each pipeline slot are one-time costs of
+            # the process (about 6 ms), not of the search.
+            Qw, cw, _ = instances.binary_qp(n, seed=100 + seed, rows=rows)
             row = {"n": n, "seed": seed, "sha256": sha[:16]}
             for key, form, learned in forms:
-                if form != 0:
-                    # an untimed, node-capped run of the same search first:
-                    # the first launches of kernel variants that only some
-                    # batch sizes reach, and of every pipeline slot, are
-                    # one-time costs of the process (about 6 ms), not of the
-                    # search
-                    solve_row(gb, Q, c, n, form, learned, max_nodes=16384)
+                solve_row(gb, Qw, cw, n, form, learned, max_nodes=16384)
                 row[key] = solve_row(gb, Q, c, n, form, learned)
             if "plain" in row:
                 fp = row["plain"]["objective"]
@@ -365,6 +402,51 @@ def time_to_optimum(args):
             say(f"time-to-optimum n={n} seed={seed}: " +
                 " ".join(f"{k} {row[k]['seconds']:.4f}s/{row[k]['nodes']}n" for k, _, _ in forms))
     return out
+
+
+def time_to_target(args):
+    """n = 200 binary QPs of the learned-branching configuration's held-out
+    set (A 300 x 200, seeds 50000 + k): the searches cannot be proven within
+    minutes, so each form is timed to the certified relative gap the plain
+    form reaches in args.tt200_seconds.  Plain runs once with that time
+    limit (its final gap G); then every form, plain included, runs with
+    rel_gap_tol = G and stops at the first moment its FP64-certified gap is
+    within G (the library's own termination rule), so each time is a
+    time-to-G.  Every form first runs an untimed 2048-node search on another
+    instance."""
+    import gpuboscia_amd as gb
+    from gpuboscia_amd import instances
+    scorer = gb.Brancher.from_file(SCORER) if os.path.exists(SCORER) else None
+    forms = [("plain", 0, None), ("batched", 1, None), ("mixed", 2, None)]
+    if scorer is not None:
+        forms += [("learned_batched", 1, scorer), ("learned_mixed", 2, scorer)]
+    n, T = 200, args.tt200_seconds
+    Qw, cw, _ = instances.binary_qp(n, seed=49999)
+    for key, form, learned in forms:
+        solve_row(gb, Qw, cw, n, form, learned, max_nodes=2048)
+    out = []
+    for k in range(args.tt200_instances):
+        seed = args.tt200_first_seed + k
+        Q, c, sha = instances.binary_qp(n, seed=seed)
+        first = solve_row(gb, Q, c, n, 0, None, time_limit=T)
+        G = gap_relative(first["dual_bound"], first["objective"])
+        row = {"n": n, "seed": seed, "sha256": sha[:16], "plain_time_limit_s": T, "target_rel_gap": G,
+               "plain_first_run": first}
+        for key, form, learned in forms:
+            r = solve_row(gb, Q, c, n, form, learned, time_limit=max(2 * T, 10.0), rel_gap=G)
+            r["reached_target"] = gap_relative(r["dual_bound"], r["objective"]) <= G
+            row[key] = r
+        for key, _, _ in forms[1:]:
+            row[f"speedup_{key}"] = round(row["plain"]["seconds"] / max(row[key]["seconds"], 1e-9), 2)
+        out.append(row)
+        say(f"time-to-target n=200 seed={seed} gap {G:.4g}: " +
+            " ".join(f"{kk} {row[kk]['seconds']:.3f}s/{row[kk]['nodes']}n" for kk, _, _ in forms))
+    gm = {}
+    for key, _, _ in forms[1:]:
+        sp = [r[f"speedup_{key}"] for r in out if r[key]["reached_target"] and r["plain"]["reached_target"]]
+        gm[key] = round(float(np.exp(np.mean(np.log(sp)))), 2) if sp else None
+    return {"instances": out, "speedup_geomean": gm,
+            "all_reached_target": all(r[kk]["reached_target"] for r in out for kk, _, _ in forms)}
 
 
 def main():
@@ -382,13 +464,19 @@ def main():
     ap.add_argument("--tto-sizes", default="20,40,60,80,100", help="binary-QP sizes of time_to_optimum")
     ap.add_argument("--tto-forms", default="plain,batched,mixed,learned_batched,learned_mixed",
                     help="forms of time_to_optimum (speedups need plain)")
+    ap.add_argument("--no-tt200", action="store_true", help="skip the n=200 time-to-target leg")
+    ap.add_argument("--only-tt200", action="store_true", help="run only the n=200 time-to-target leg")
+    ap.add_argument("--tt200-instances", type=int, default=2)
+    ap.add_argument("--tt200-first-seed", type=int, default=50000)
+    ap.add_argument("--tt200-seconds", type=float, default=8.0,
+                    help="time limit of plain's first run; its final gap is every form's target")
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-instances", type=int, default=256)
     ap.add_argument("--c3-nodes", type=int, default=1024)
-    ap.add_argument("--c3-steps", type=int, default=3)
+    ap.add_argument("--c3-steps", type=int, default=2)
     ap.add_argument("--c3-planes", type=int, default=3, choices=(1, 3))
-    ap.add_argument("--forms", default="plain,batched,mixed,mixed_fp32,mixed_pass1",
+    ap.add_argument("--forms", default="plain,batched,mixed,mixed_fp32,mixed_pass1,mixed_fp32_pass1",
                     help="forms to time (the profiled runs time one)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
@@ -401,6 +489,9 @@ def main():
     import gpuboscia_amd as gb
     from gpuboscia_amd import instances
     gb.load()
+    if args.only_tt200:
+        print(json.dumps({"time_to_target_n200": time_to_target(args)}), flush=True)
+        return
     n = args.n
     Q, c, sha = instances.binary_qp(n, seed=1)
     LB, UB, EPS = node_batch(n, args.nodes, seed=2)
@@ -420,12 +511,14 @@ def main():
         torch.cuda.synchronize()
         say("copy probe: 268435456 bytes copied device to device")
         del src, dst
-    main_form = "mixed" if "mixed" in res else forms[0]
+    main_form = "mixed_fp32" if "mixed_fp32" in res else ("mixed" if "mixed" in res else forms[0])
     m = res[main_form]
+    planes = 3 if main_form.startswith("mixed_fp32") else 1
     result = {
-        "metric": "FW iterations/s of the node relaxation on n=200 binary-QP nodes (mixed-precision form: BF16 "
-                  "MFMA pass + FP64 certification); time-to-proven-optimum @gap 1e-6 plain vs batched / mixed / "
-                  "learned+batched / learned+mixed in time_to_optimum",
+        "metric": "FW iterations/s of the node relaxation on n=200 binary-QP nodes (mixed-precision form as the "
+                  "searches run it: three bf16 planes of Q on MFMA + FP64 certification); time-to-proven-optimum "
+                  "@gap 1e-6 plain vs batched / mixed / learned+batched / learned+mixed in time_to_optimum "
+                  "(n <= 100) and time to plain's certified gap at n = 200 in time_to_target_n200",
         "value": m["value"], "unit": "FW iterations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": m["wall_s"] / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
@@ -433,36 +526,39 @@ def main():
         "config": {"workload": f"binary QP n={n} (A {int(1.5 * n)}x{n}), {args.nodes} open B&B nodes, DICG, "
                                f"budget {args.budget} iterations/node/step",
                    "instance_sha256": sha[:16], "form": main_form,
-                   "precision": "BF16 MFMA products (FP32 accumulate), FP32 iterates, FP64 steps; FP64 "
-                                "certificate of gap, value and incumbent"},
+                   "precision": f"BF16 MFMA products over {planes} bf16 plane(s) of Q (FP32 accumulate), FP32 "
+                                "iterates, FP64 steps; FP64 certificate of gap, value and incumbent"},
         "forms": res,
     }
     if "plain" in res:
         result["speedup_vs_plain"] = m["value"] / res["plain"]["value"]
-    if "mixed_pass1" in res:
-        p1 = res["mixed_pass1"]
+    pass1 = f"{main_form}_pass1" if f"{main_form}_pass1" in res else ("mixed_pass1" if "mixed_pass1" in res else None)
+    if pass1 is not None:
+        p1 = res[pass1]
+        kplanes = 3 if pass1.startswith("mixed_fp32") else 1
         kernel_s = p1["launch_ms"] / 1e3
         alg_flops = 2.0 * n * n * p1["mixed_iters_per_step"]
         n_pad = 32 * ((n + 31) // 32)
-        mfma_flops = 2 * 2.0 * n_pad * n_pad * p1["mixed_iters_per_step"]  # Q v and Q a per iteration
+        # Q v and Q a per iteration, each over kplanes bf16 planes of Q
+        mfma_flops = 2 * kplanes * 2.0 * n_pad * n_pad * p1["mixed_iters_per_step"]
         achieved = alg_flops / kernel_s / 1e12
         result["roofline"] = {
             "bound": "mfma", "achieved": achieved, "peak": MI355X_BF16_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / MI355X_BF16_TFLOPS, "traffic": traffic_bytes(), "kernel": "relax_mixed_kernel",
-            "kernel_ms": p1["launch_ms"],
+            "frac": achieved / MI355X_BF16_TFLOPS, "traffic": traffic_bytes(kplanes), "kernel": "relax_mixed_kernel",
+            "planes": kplanes, "kernel_ms": p1["launch_ms"],
             "basis": "algorithmic 2 n^2 flops per FW iteration x iterations of one launch / launch time "
                      "(HIP events on the launch stream); bf16 dense MFMA peak",
-            "useful_mfma_tflops": mfma_flops / kernel_s / 1e12,
-            "useful_mfma_basis": "2 products (Q v, Q a) x 2 n_pad^2 per column-iteration, n_pad = 32 ceil(n/32)"}
+            "issued_mfma_tflops": mfma_flops / kernel_s / 1e12,
+            "issued_mfma_basis": "2 products (Q v, Q a) x planes x 2 n_pad^2 per column-iteration, "
+                                 "n_pad = 32 ceil(n/32)"}
     if not args.no_scale and "mixed" in forms:
         LBs, UBs, EPSs = node_batch(n, args.scale_nodes, seed=3)
         sb = Batch(Q, c, LBs, UBs, EPSs)
         steps = max(2, args.steps // 2)
-        sc = {f: sb.measure(f, args.budget, steps, 1) for f in ("mixed", "batched")}
+        sc = {f: sb.measure(f, args.budget, steps, 1) for f in ("mixed_fp32", "mixed", "batched")}
         result["scale"] = {"nodes": args.scale_nodes, "steps": steps,
-                           "mixed": {"value": sc["mixed"]["value"], "launch_ms": sc["mixed"]["launch_ms"]},
-                           "batched": {"value": sc["batched"]["value"], "launch_ms": sc["batched"]["launch_ms"]}}
-        say(f"scale {args.scale_nodes}: mixed {sc['mixed']['value']:.4g} batched {sc['batched']['value']:.4g}")
+                           **{f: {"value": v["value"], "launch_ms": v["launch_ms"]} for f, v in sc.items()}}
+        say(f"scale {args.scale_nodes}: " + " ".join(f"{f} {v['value']:.4g}" for f, v in sc.items()))
         del sb
     if not args.no_c3:
         result["heterogeneous_batch"] = hetero_batches(args)
@@ -487,6 +583,8 @@ def main():
             gm[key] = per_n
         result["time_to_optimum_speedup_geomean"] = gm
         result["time_to_optimum_all_same_optimum"] = all(r.get("same_optimum", True) for r in tto)
+    if not args.no_tt200:
+        result["time_to_target_n200"] = time_to_target(args)
     print(json.dumps(result), flush=True)
 
 
