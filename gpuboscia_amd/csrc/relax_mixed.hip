@@ -15,8 +15,9 @@
 //    serves up to 4, 8, 16 columns, so a tile with few columns still keeps
 //    every lane busy in the elementwise work, which bounds the pass.
 // One pass: the vertices v, a are chosen in the element layout and written
-// to LDS as bf16 0/1 B operands in MFMA k-slot order; Q v and Q a come from
-// the MFMA with the A operand Q (pre-split into 1..3 bf16 planes q = q0 +
+// to LDS as bf16 0/1 B operands in MFMA k-slot order (a tile of up to 8
+// columns puts v in B columns 0..7 and a in 8..15 of one operand, 16 columns
+// use one operand each); Q v and Q a come from the MFMA with the A operand Q (pre-split into 1..3 bf16 planes q = q0 +
 // q1 + q2, FP32 accumulation over the planes, fragments held in registers
 // where they fit); the accumulators return to the element layout through a
 // per-wave LDS exchange; partial sums are reduced over the column's lanes
@@ -170,20 +171,33 @@ __global__ void mixed_prepare_kernel(const double* Qt, const double* lin, double
 constexpr int kPhaseDone = 0, kPhaseInit = 1, kPhaseActive = 2, kPhaseFetch = 3;
 
 // The wave's Q fragments are the same in every pass: where they fit beside
-// the rest of the state (32..96 VGPRs for n <= 256) they are loaded once and
-// kept in registers for the kernel's lifetime.
-constexpr bool mixed_caches_q(int rw, int pl) { return (pl == 1 && rw <= 2) || (pl == 3 && rw == 1); }
+// the rest of the state they are loaded once and kept in registers for the
+// kernel's lifetime.  CS: the k-steps held (0: none): 8 (n <= 256) for one
+// plane and 1, 2 or 4 row tiles per wave and for three planes and one row
+// tile (32..128 registers); 7 (n <= 224) for three planes and 2 row tiles
+// per wave with one row per lane (168 VGPRs at two waves per SIMD) or 4 row
+// tiles per wave with up to 2 rows per lane (336 registers at one wave per
+// SIMD, partly in the AGPR file).  The other three-plane shapes would spill.
+constexpr int mixed_cached_steps(int rw, int pl, int rpt) {
+    if (pl == 1) return rw == 3 ? 0 : 8;
+    if (rw == 1) return 8;
+    return ((rw == 2 && rpt == 1) || (rw == 4 && rpt <= 2)) ? 7 : 0;
+}
 
 // QL: otherwise the tile copies all fragments into LDS once (when they fit)
 // and reads them from there each pass instead of from L2.  BIG: 17..32 row
 // tiles (n up to 512), 4 row tiles per wave, up to 16 k-steps.
-template <int RW, int PL, bool DICG, int RPT, bool QL, bool BIG = false>
+template <int RW, int PL, bool DICG, int RPT, bool QL, int CS, bool BIG = false>
 __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)))
     relax_mixed_kernel(MixedParams P) {
-    static_assert(!BIG || (RW == 4 && !QL), "wide tiles run 4 row tiles per wave from L2");
+    static_assert(!BIG || (RW == 4 && !QL && CS == 0), "wide tiles run 4 row tiles per wave from L2");
     constexpr int G = 16 / RPT;  // lanes of a column in the element layout
     constexpr int E = RW * RPT;  // elements per lane: rows 16 t_j + RPT m + r, e = RPT j + r
     constexpr unsigned long long kGroupMask = (1ull << G) - 1ull;
+    // Up to 8 columns (RPT <= 2): the away vertex a of column k is B column
+    // k + 8 of the one operand, so a single MFMA chain gives Q v and Q a.
+    constexpr bool kJoint = DICG && RPT <= 2;
+    constexpr bool kTwo = DICG && !kJoint;  // 16 columns: separate chains for v and a
     // Starting at a supplied point needs registers that the 3-tile and the
     // wide (streamed) shapes do not have; their nodes start at a vertex.
     constexpr bool kWarmStart = mixed_warm_start(RW, BIG);
@@ -229,9 +243,9 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
         x[e] = 0.0f;
         qx[e] = 0.0f;
     }
-    constexpr bool kCache = mixed_caches_q(RW, PL);
+    constexpr bool kCache = CS > 0;
     static_assert(!(kCache && QL), "register-cached fragments need no LDS copy");
-    constexpr int kSMax = 8;
+    constexpr int kSMax = kCache ? CS : 8;
     bf16x8 fc[kCache ? PL : 1][kCache ? RW : 1][kCache ? kSMax : 1];
     if (kCache) {
 #pragma unroll
@@ -371,7 +385,10 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                 const int off = (t & 1) * 4 + r0;
                 constexpr uint32_t kRows = (1u << RPT) - 1u;
                 put_rows<RPT>(reinterpret_cast<unsigned short*>(vops + slot) + off, (vm >> (RPT * j)) & kRows);
-                if (DICG) put_rows<RPT>(reinterpret_cast<unsigned short*>(aops + slot) + off, (am >> (RPT * j)) & kRows);
+                if (kJoint)
+                    put_rows<RPT>(reinterpret_cast<unsigned short*>(vops + slot + 32) + off, (am >> (RPT * j)) & kRows);
+                else if (DICG)
+                    put_rows<RPT>(reinterpret_cast<unsigned short*>(aops + slot) + off, (am >> (RPT * j)) & kRows);
             }
         }
         __syncthreads();
@@ -386,18 +403,19 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             acca[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
         auto products = [&](auto steps) {
-            constexpr int SS = decltype(steps)::value;
+            // (the launcher runs a register-cached kernel only for S <= CS)
+            constexpr int SS = decltype(steps)::value < kSMax ? decltype(steps)::value : kSMax;
             if constexpr (kCache) {
 #pragma unroll
                 for (int s = 0; s < SS; ++s) {
                     const bf16x8 bv = __builtin_bit_cast(bf16x8, vops[(s * 16 + c) * 4 + q]);
-                    const bf16x8 ba = DICG ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
+                    const bf16x8 ba = kTwo ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
 #pragma unroll
                     for (int p = 0; p < PL; ++p)
 #pragma unroll
                         for (int j = 0; j < RW; ++j) {
                             accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fc[p][j][s], bv, accv[j], 0, 0, 0);
-                            if (DICG)
+                            if (kTwo)
                                 acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fc[p][j][s], ba, acca[j], 0, 0, 0);
                         }
                 }
@@ -430,13 +448,13 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                             }
                     }
                     const bf16x8 bv = __builtin_bit_cast(bf16x8, vops[(s * 16 + c) * 4 + q]);
-                    const bf16x8 ba = DICG ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
+                    const bf16x8 ba = kTwo ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
 #pragma unroll
                     for (int p = 0; p < PL; ++p)
 #pragma unroll
                         for (int j = 0; j < RW; ++j) {
                             accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], bv, accv[j], 0, 0, 0);
-                            if (DICG) acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], ba, acca[j], 0, 0, 0);
+                            if (kTwo) acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], ba, acca[j], 0, 0, 0);
                         }
                     if (s + 1 < SS) {
 #pragma unroll
@@ -475,13 +493,13 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                         nxt[p][j] = (more && w + W * j < RT) ? load_frag(fp[p][j] + static_cast<size_t>(s + 1) * kWave)
                                                              : bf16x8{};
                 const bf16x8 bv = __builtin_bit_cast(bf16x8, vops[(s * 16 + c) * 4 + q]);
-                const bf16x8 ba = DICG ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
+                const bf16x8 ba = kTwo ? __builtin_bit_cast(bf16x8, aops[(s * 16 + c) * 4 + q]) : bv;
 #pragma unroll
                 for (int p = 0; p < PL; ++p)
 #pragma unroll
                     for (int j = 0; j < RW; ++j) {
                         accv[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], bv, accv[j], 0, 0, 0);
-                        if (DICG) acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], ba, acca[j], 0, 0, 0);
+                        if (kTwo) acca[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[p][j], ba, acca[j], 0, 0, 0);
                     }
 #pragma unroll
                 for (int p = 0; p < PL; ++p)
@@ -502,14 +520,16 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
         }
 
         // Accumulators to the element layout: the wave's own LDS region,
-        // [array][j][column][row] floats (no other wave touches it).
+        // [array][j][column][row] floats (no other wave touches it); with the
+        // joint operand, B column c >= 8 holds Q a of column c - 8.
         float qv[E], qa[E];
 #pragma unroll
         for (int j = 0; j < RW; ++j) {
-            float* pv = xch + ((0 * RW + j) * 16 + c) * 16 + 4 * q;
+            float* pv = kJoint ? xch + (((c >> 3) * RW + j) * 16 + (c & 7)) * 16 + 4 * q
+                               : xch + ((0 * RW + j) * 16 + c) * 16 + 4 * q;
 #pragma unroll
             for (int r = 0; r < 4; ++r) pv[r] = accv[j][r];
-            if (DICG) {
+            if (kTwo) {
                 float* pa = xch + ((1 * RW + j) * 16 + c) * 16 + 4 * q;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) pa[r] = acca[j][r];
@@ -753,14 +773,21 @@ int launch_mixed_shape(const MixedParams& p, dim3 grid, dim3 block, int waves, h
     const size_t base = mixed_lds_layout(p.rt, waves, RW, 0).total;
     if constexpr (RW == 4) {
         if (p.rt > kMixedSmallRowTiles)
-            return start_mixed(relax_mixed_kernel<4, PL, DI, RPT, false, true>, grid, block, base, st, p);
+            return start_mixed(relax_mixed_kernel<4, PL, DI, RPT, false, 0, true>, grid, block, base, st, p);
     }
-    if constexpr (!mixed_caches_q(RW, PL)) {
+    constexpr int kCS = mixed_cached_steps(RW, PL, RPT);
+    if constexpr (kCS > 0) {
+        if (p.rt / 2 <= kCS)
+            return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false, kCS>, grid, block, base, st, p);
+    }
+    if constexpr (kCS < kMixedSmallRowTiles / 2) {
         const size_t with_q = mixed_lds_layout(p.rt, waves, RW, mixed_q_bytes(p.rt, PL)).total;
         if (with_q <= kMixedLdsBudget)
-            return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, true>, grid, block, with_q, st, p);
+            return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, true, 0>, grid, block, with_q, st, p);
+        return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false, 0>, grid, block, base, st, p);
+    } else {
+        return static_cast<int>(hipErrorInvalidValue);  // unreachable: every k-step is held
     }
-    return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false>, grid, block, base, st, p);
 }
 
 int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void* stream) {
