@@ -670,6 +670,50 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
         __syncthreads();
         columns += count;
     };
+    // The updates of Q v and Q a in one gather: the union of their changed
+    // columns, k ascending, each column read once and applied to the
+    // accumulators whose coefficient is nonzero, so each keeps the FMA chain
+    // of its own update (the second coefficient list reuses zs2).
+    auto update2 = [&](double vnew, double vold, double& yv, double anew, double aold, double& ya) {
+        const bool cv = in && vnew != vold;
+        const bool ca = in && anew != aold;
+        const unsigned long long mask = __ballot(cv || ca);
+        if (lane == 0) misc[w] = __popcll(mask);
+        __syncthreads();
+        int off = 0, count = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int c = misc[t];
+            if (t < w) off += c;
+            count += c;
+        }
+        if (cv || ca) {
+            const int pos = off + __popcll(mask & ((1ull << lane) - 1ull));
+            klist[pos] = j;
+            dlist[pos] = cv ? vnew - vold : 0.0;
+            zs2[pos] = ca ? anew - aold : 0.0;
+        }
+        __syncthreads();
+        for (int g0 = 0; g0 < count; g0 += kB) {
+            double c[kB], dv[kB], da[kB];
+#pragma unroll
+            for (int g = 0; g < kB; ++g) {
+                const bool live = g0 + g < count;
+                const int k = live ? klist[g0 + g] : 0;
+                dv[g] = live ? dlist[g0 + g] : 0.0;
+                da[g] = live ? zs2[g0 + g] : 0.0;
+                c[g] = (live && in) ? Q[(size_t)k * n + j] : 0.0;
+            }
+#pragma unroll
+            for (int g = 0; g < kB; ++g)
+                if (g0 + g < count && in) {
+                    if (dv[g] != 0.0) yv = fma(c[g], dv[g], yv);
+                    if (da[g] != 0.0) ya = fma(c[g], da[g], ya);
+                }
+        }
+        __syncthreads();
+        columns += count;
+    };
 
     while (true) {
         if (threadIdx.x == 0) misc[NT] = atomicAdd(P.counter, 1);
@@ -761,10 +805,20 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
         int it = 0;
         for (; it < P.max_iters; ++it) {
             iterations = initial + it + 1;
+            bool a_done = false;
             if (it > 0) {
                 grad();
                 const double vn = fwd();
-                update(vn, v, qv);
+                if (P.dicg && have_a) {
+                    // the away vertex depends on x and g only: its update
+                    // joins v's (unused if the iteration stops below)
+                    const double an = away();
+                    update2(vn, v, qv, an, a, qa);
+                    a = an;
+                    a_done = true;
+                } else {
+                    update(vn, v, qv);
+                }
                 v = vn;
             }
             // score(v), gap and f(x) in one exchange
@@ -785,7 +839,7 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
             double d, qd, gmax;
             Ex r{};
             if (P.dicg) {
-                if (have_a) {
+                if (have_a && !a_done) {
                     const double an = away();
                     update(an, a, qa);
                     a = an;

@@ -207,8 +207,10 @@ typedef struct {
                                  0: one node per workgroup with its rows split over
                                  ceil(n/64) waves (n > 64; n <= 64 uses 4) */
     int blocks;               /* persistent workgroups (0 = automatic) */
-    long long* stats;         /* optional device [N][2]: Q-column updates and direct
-                                 n x n products per node (work accounting), or NULL */
+    long long* stats;         /* optional device [N][2]: Q columns read by the incremental
+                                 updates (a column serving Q v and Q a in one gather counts
+                                 once) and direct n x n products per node (work
+                                 accounting), or NULL */
     double* gradient;         /* optional device [N][n]: gradient at the returned x, or NULL */
 } gb_fast_options;
 
