@@ -17,7 +17,9 @@ statuses (fill kernels) and relaxes the batch again.
   scale      the mixed forms on 16384 nodes
   roofline   the BF16 MFMA kernel of the headline form (relax_mixed_kernel,
              three planes): algorithmic flops 2 n^2 per FW iteration / its
-             launch time, against 2.5 PFLOP/s
+             launch time, against 2.5 PFLOP/s; HBM bytes per launch and the
+             MFMA-busy and VALU-issue fractions of its SIMD-cycles from the
+             committed rocprofv3 counter passes under profiles/
   heterogeneous_batch  256 binary QPs, n log-uniform in [32, 512], box or
              bounded-simplex LMO by seed, 1024 open nodes each, one
              gb_relax_instances call (and its captured replay)
@@ -179,6 +181,22 @@ def traffic_bytes(planes):
         return None
     with open(path) as f:
         return json.load(f).get("bytes_per_launch")
+
+
+def kernel_counters(planes):
+    """Issue fractions of relax_mixed_kernel with `planes` bf16 planes on
+    this workload, from the committed rocprofv3 counter passes (MFMA-busy and
+    VALU-issue SIMD-cycles over the launch's SIMD-cycles); None if not
+    recorded."""
+    path = os.path.join(ROOT, "profiles", "relax_mixed_kernel_n200_1024nodes_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(f"{planes}_planes")
+    if k is None:
+        return None
+    return {key: k[key] for key in ("kernel", "trace_avg_ms", "mfma_busy_frac", "valu_issue_frac",
+                                    "wait_frac_of_wave_cycles", "per_wave_pass")}
 
 
 def cpu_baseline(args, Q, c, LB, UB, EPS):
@@ -540,7 +558,7 @@ def main():
         alg_flops = 2.0 * n * n * p1["mixed_iters_per_step"]
         n_pad = 32 * ((n + 31) // 32)
         # Q v and Q a per iteration, each over kplanes bf16 planes of Q
-        mfma_flops = 2 * kplanes * 2.0 * n_pad * n_pad * p1["mixed_iters_per_step"]
+        column_flops = 2 * kplanes * 2.0 * n_pad * n_pad * p1["mixed_iters_per_step"]
         achieved = alg_flops / kernel_s / 1e12
         result["roofline"] = {
             "bound": "mfma", "achieved": achieved, "peak": MI355X_BF16_TFLOPS, "unit": "TFLOP/s",
@@ -548,9 +566,11 @@ def main():
             "planes": kplanes, "kernel_ms": p1["launch_ms"],
             "basis": "algorithmic 2 n^2 flops per FW iteration x iterations of one launch / launch time "
                      "(HIP events on the launch stream); bf16 dense MFMA peak",
-            "issued_mfma_tflops": mfma_flops / kernel_s / 1e12,
-            "issued_mfma_basis": "2 products (Q v, Q a) x planes x 2 n_pad^2 per column-iteration, "
-                                 "n_pad = 32 ceil(n/32)"}
+            "column_product_tflops": column_flops / kernel_s / 1e12,
+            "column_product_basis": "2 products (Q v, Q a) x planes x 2 n_pad^2 per column-iteration, "
+                                    "n_pad = 32 ceil(n/32) (a 16-column MFMA tile carries 4 node columns "
+                                    "of v and 4 of a at 1024 nodes)",
+            "counters": kernel_counters(kplanes)}
     if not args.no_scale and "mixed" in forms:
         LBs, UBs, EPSs = node_batch(n, args.scale_nodes, seed=3)
         sb = Batch(Q, c, LBs, UBs, EPSs)

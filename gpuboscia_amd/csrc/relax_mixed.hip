@@ -717,6 +717,9 @@ __global__ void __launch_bounds__(256) mixed_warm_products_kernel(const double* 
         warm[threadIdx.x] = (i < N && (status[i] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0 && iters[i] == 0) ? 1 : 0;
     }
     __syncthreads();
+    bool any = false;
+    for (int v = 0; v < kWarmNodes; ++v) any = any || warm[v] != 0;
+    if (!any) return;  // block-uniform: no node of this block starts at a supplied point
     for (int v = 0; v < kWarmNodes; ++v)
         for (int k = threadIdx.x; k < n; k += blockDim.x)
             xs[v][k] = warm[v] ? X[static_cast<size_t>(node0 + v) * n + k] : 0.0;
