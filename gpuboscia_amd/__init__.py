@@ -128,7 +128,8 @@ class Settings(ctypes.Structure):
                 ("branch_score", BRANCH_SCORE_FN), ("branch_score_user", ctypes.c_void_p),
                 ("branch", BRANCH_FN), ("branch_user", ctypes.c_void_p),
                 ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_void_p),
-                ("max_nodes", ctypes.c_longlong), ("autotune", ctypes.c_int)]
+                ("max_nodes", ctypes.c_longlong), ("autotune", ctypes.c_int),
+                ("device_frontier", ctypes.c_int), ("device_nodes", ctypes.c_longlong)]
 
 
 class Result(ctypes.Structure):

@@ -481,11 +481,6 @@ bool gap_within_tolerance(double bound, double inc, double at, double rt) {
     if (!std::isfinite(bound) || !std::isfinite(inc)) return false;
     return gap_absolute(bound, inc) <= at || gap_relative(bound, inc) <= rt;
 }
-double depth_tolerance(double root, double decay, double minimum, int depth) {
-    if (depth <= 0) return std::max(root, minimum);
-    return std::max(minimum, root * std::pow(decay, depth));
-}
-
 // top_k_infeasible (branching.h:43-65).  Same candidate vector and
 // comparator as the reference so std::sort orders ties identically.
 std::vector<int> top_k_infeasible(const double* x, const std::vector<int>& iv, int k, double tol) {
@@ -1043,6 +1038,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     if (!(hk && hk->nodes)) {
         st = validate_root(pr, l, s);
         if (st != GB_OK) return st;
+    }
+    if (s->device_frontier) {
+        if (lab || hk)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "device_frontier runs plain gb_solve searches (no labeller or solver object)");
+        return device_search(pr, o, l, s, res);
     }
 
     using Clock = std::chrono::steady_clock;
@@ -1738,6 +1739,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
 }
 
 }  // namespace
+
+double depth_tolerance(double root, double decay, double minimum, int depth) {
+    if (depth <= 0) return std::max(root, minimum);
+    return std::max(minimum, root * std::pow(decay, depth));
+}
 
 void release_buffer_sets() { BufferPool::release_idle(); }
 

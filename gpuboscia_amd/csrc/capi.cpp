@@ -222,7 +222,8 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
                            const double* ub, const double* eps, int* iterations, uint8_t* status,
                            double* gap, double* value, double* best_vertex, double* best_vertex_val,
                            const uint8_t* flags, double ctol, const double* cutoff,
-                           const gb_fast_options* opt, void* stream, int* counter, bool rescore) {
+                           const gb_fast_options* opt, void* stream, int* counter, bool rescore,
+                           const int* count) {
     gb_status st = validate_descriptors(o, a, l);
     if (st != GB_OK) return st;
     if (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX)
@@ -272,6 +273,7 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     P.stats = op.stats;
     P.gout = op.gradient;
     P.rescore = rescore ? 1 : 0;
+    P.count = count;
     const int e = launch_relax_fast(P, op.waves_per_block, op.blocks, stream);
     if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
     return GB_OK;

@@ -32,6 +32,7 @@ struct FastParams {
     long long* stats;       // optional [N][2]: column updates, direct GEMVs (may be NULL)
     double* gout;           // optional [N][n]: gradient at the returned iterate (may be NULL)
     int rescore;            // re-score best_v[i] in FP64 first when best_val[i] < 1e300
+    const int* count;       // optional device batch size: nodes [0, *count) of the N rows are relaxed
 };
 
 constexpr int kMaxFastBlocks = 2048;  // persistent workgroups of the batched form (blocks = 0: automatic)

@@ -36,7 +36,15 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
                            double* gap, double* value, double* best_vertex, double* best_vertex_val,
                            const uint8_t* flags, double ctol, const double* cutoff,
                            const gb_fast_options* opt, void* stream, int* counter = nullptr,
-                           bool rescore = false);
+                           bool rescore = false, const int* count = nullptr);
+// The mixed-precision form (mixed_api.cpp); count as relax_batch_fast's.
+gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objective,
+                           const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo, int N, int max_iters,
+                           double* x, const double* lb, const double* ub, const double* eps, int* iterations,
+                           uint8_t* status, double* gap, double* value, double* best_vertex,
+                           double* best_vertex_val, const uint8_t* integer_flags, double candidate_tol,
+                           const double* cutoff, const gb_mixed_options* options, void* stream,
+                           int* counter, const int* count = nullptr);
 // A node counter of the persistent kernels lent to one call (resources.cpp):
 // no launch that may run at the same time uses it, and a counter lent while
 // the stream is being captured stays with the captured graph.
@@ -67,6 +75,11 @@ class ScopedCounter {
     void* stream_;
     CounterLease lease_;
 };
+// Node FW tolerance at a depth (the host driver's, from DeviceResidentConfig).
+double depth_tolerance(double root, double decay, double minimum, int depth);
+// The device-resident search of gb_solve (dsearch.cpp), settings validated.
+gb_status device_search(const gb_problem* pr, const gb_objective_desc* o, const gb_lmo_desc* l,
+                        const gb_settings* s, gb_result* res);
 // Teardown hooks of gb_shutdown: the idle batch buffer sets of the
 // branch-and-bound driver (bnb.cpp) and the fork streams of
 // gb_relax_instances (mixed_api.cpp).

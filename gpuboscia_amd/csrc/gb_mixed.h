@@ -51,6 +51,7 @@ struct MixedParams {
     // optional [N][n]: Q x of the nodes with a supplied point and no
     // iterations (launch_mixed_warm_products); such nodes start at x itself
     const double* qx0;
+    const int* count;       // optional device batch size: nodes [0, *count) of the N rows are relaxed
 };
 
 int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double ls, int n, int rt,
@@ -117,7 +118,7 @@ constexpr bool mixed_warm_start(int rw, bool wide) { return !wide && rw != 3; }
 // with status GB_NODE_INITIAL_POINT_SUPPLIED and iters[i] == 0; other rows
 // of out are left as they are.  n <= kMixedMaxN.
 int launch_mixed_warm_products(const double* Qt, int n, int N, const double* X, const uint8_t* status,
-                               const int* iters, double* out, void* stream);
+                               const int* iters, double* out, void* stream, const int* count = nullptr);
 int launch_mixed_products(const MixedQ& q, const double* V, int N, double* Y, void* stream);
 
 }  // namespace gb

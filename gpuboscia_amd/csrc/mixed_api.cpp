@@ -119,7 +119,7 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
                            uint8_t* status, double* gap, double* value, double* best_vertex,
                            double* best_vertex_val, const uint8_t* integer_flags, double candidate_tol,
                            const double* cutoff, const gb_mixed_options* options, void* stream,
-                           int* counter) {
+                           int* counter, const int* count) {
     {
         gb_status st = validate_descriptors(objective, algorithm, lmo);
         if (st != GB_OK) return st;
@@ -166,6 +166,7 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
             P.counter = counter;
             P.rel_floor = op.rel_floor;
             P.stats = op.mixed_iterations;
+            P.count = count;
             MixedShape sh = mixed_default_shape(q->rt, N);
             if (op.row_tiles_per_wave) sh.rw = op.row_tiles_per_wave;
             if (op.columns) sh.cols = op.columns;
@@ -181,7 +182,7 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
                 // into their best-vertex rows, which pass 1 reads when it takes
                 // the node and overwrites when it retires it.
                 const int we = launch_mixed_warm_products(objective->Qt, q->n, N, x, status, iterations,
-                                                          best_vertex, stream);
+                                                          best_vertex, stream, count);
                 if (we != 0) return hip_fail(static_cast<hipError_t>(we), __FILE__, __LINE__);
                 P.qx0 = best_vertex;
             }
@@ -197,7 +198,7 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
         const gb_fast_options fo{64, 0, 0, op.fp64_stats, op.gradient};
         return relax_batch_fast(objective, algorithm, lmo, N, fp64_iters, x, lb, ub, eps, iterations,
                                 status, gap, value, best_vertex, best_vertex_val, integer_flags,
-                                candidate_tol, cutoff, &fo, stream, counter, true);
+                                candidate_tol, cutoff, &fo, stream, counter, true, count);
     }
 }
 

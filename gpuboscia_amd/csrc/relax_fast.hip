@@ -207,6 +207,7 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
     extern __shared__ __align__(16) double lds_fast[];
     const int lane = lane_id();
     const int n = P.n;
+    const int N = P.count ? *P.count : P.N;
     const size_t qsize = QL ? (size_t)n * n : 0;
     if (QL) {
         for (size_t i = threadIdx.x; i < qsize; i += blockDim.x) lds_fast[i] = P.Qt[i];
@@ -225,7 +226,7 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
         int node = 0;
         if (lane == 0) node = atomicAdd(P.counter, 1);
         node = __shfl(node, 0, kWave);
-        if (node >= P.N) break;
+        if (node >= N) break;
         W.columns = 0;
         W.gemvs = 0;
 
@@ -523,6 +524,7 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
     const int lane = lane_id();
     const int w = __builtin_amdgcn_readfirstlane(wave_id());
     const int n = P.n;
+    const int N = P.count ? *P.count : P.N;
     const int np = P.n_pad;
     const int j = lane + kWave * w;  // this lane's row
     const bool in = j < n;
@@ -719,7 +721,7 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
         if (threadIdx.x == 0) misc[NT] = atomicAdd(P.counter, 1);
         __syncthreads();
         const int node = misc[NT];
-        if (node >= P.N) break;
+        if (node >= N) break;
         columns = 0;
         gemvs = 0;
 

@@ -213,6 +213,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
     int* okw = reinterpret_cast<int*>(mixed_smem + L.ok);
     const int lane = lane_id();
     const int w = __builtin_amdgcn_readfirstlane(wave_id());  // wave-uniform: tile tests branch on scalars
+    const int N = P.count ? *P.count : P.N;
     float* xch = reinterpret_cast<float*>(mixed_smem + L.xch) + static_cast<size_t>(w) * (2 * RW * 256);
     // Columns that never hold a node multiply zeros.
     for (int i = threadIdx.x; i < S * 64; i += blockDim.x) {
@@ -277,7 +278,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             __syncthreads();
             if (phase == kPhaseFetch) {
                 node = fetch[col];
-                if (node >= P.N) phase = kPhaseDone;
+                if (node >= N) phase = kPhaseDone;
             }
             const bool loading = phase == kPhaseFetch;
             bool ok = true;
@@ -708,7 +709,8 @@ __global__ void mixed_reset_kernel(int* counter) { *counter = 0; }
 constexpr int kWarmNodes = 8;
 __global__ void __launch_bounds__(256) mixed_warm_products_kernel(const double* Qt, int n, int N, const double* X,
                                                                   const uint8_t* status, const int* iters,
-                                                                  double* out) {
+                                                                  double* out, const int* count) {
+    if (count && *count < N) N = *count;
     __shared__ double xs[kWarmNodes][kMixedMaxN];
     __shared__ int warm[kWarmNodes];
     const int node0 = blockIdx.x * kWarmNodes;
@@ -742,12 +744,12 @@ __global__ void __launch_bounds__(256) mixed_warm_products_kernel(const double* 
 }  // namespace
 
 int launch_mixed_warm_products(const double* Qt, int n, int N, const double* X, const uint8_t* status,
-                               const int* iters, double* out, void* stream) {
+                               const int* iters, double* out, void* stream, const int* count) {
     if (N <= 0 || n <= 0) return 0;
     if (n > kMixedMaxN) return static_cast<int>(hipErrorInvalidValue);
     const unsigned blocks = static_cast<unsigned>((N + kWarmNodes - 1) / kWarmNodes);
     hipLaunchKernelGGL(mixed_warm_products_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
-                       Qt, n, N, X, status, iters, out);
+                       Qt, n, N, X, status, iters, out, count);
     return static_cast<int>(hipGetLastError());
 }
 

@@ -623,6 +623,7 @@ struct Settings {
     int precision = GB_PREC_FP64;       // form 2: GB_PREC_BF16 (1 bf16 plane) or FP32 (3 planes)
     int branch_rule = GB_BRANCH_POW2;   // used when `branch` is empty
     bool autotune = false;              // form 2: tune tile shape and FP64-pass budget
+    bool device_frontier = false;       // forms 1, 2: frontier, pruning and branching on the GPU
 };
 
 struct Result {
@@ -656,6 +657,7 @@ Result solve_with_algorithm(const Problem& problem, DeviceObjective objective, L
     s.precision = settings.precision;
     s.branch_rule = settings.branch_rule;
     s.autotune = settings.autotune ? 1 : 0;
+    s.device_frontier = settings.device_frontier ? 1 : 0;
     detail::Callbacks cb;
     cb.int_vars = &problem.int_vars;
     if (!settings.root_initial_point.empty()) {

@@ -570,6 +570,15 @@ typedef struct {
     long long max_nodes;      /* safety cap; <= 0 = none (no reference counterpart) */
     int autotune;             /* form 2: tune tile shape and FP64-pass budget on the first
                                  batch of at least 64 nodes (gb_mixed_autotune); 0 = defaults */
+    int device_frontier;      /* 1: device-resident search (no reference counterpart): the
+                                 frontier, incumbent, pruning and branching run on the GPU and
+                                 each round is one captured HIP graph; forms 1 and 2, dense
+                                 quadratic over a 0/1 box, pow2 (branch_k <= 3) or most-
+                                 fractional branching, gpu_batch_size <= 1024, no callbacks.
+                                 Same tree, incumbent, bound and counts as num_streams = 1;
+                                 the time limit is checked between groups of rounds */
+    long long device_nodes;   /* node slots of the device-resident search (0 = automatic: a
+                                 quarter of free device memory, at most 32 GiB) */
 } gb_settings;
 
 /* Result (gpuboscia.cuh:34-42). x is caller-provided, n_vars doubles. */
