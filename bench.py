@@ -28,8 +28,9 @@ statuses (fill kernels) and relaxes the batch again.
              use (capped by OMP_NUM_THREADS), with nproc and the CPU model
   time_to_optimum  gb_solve to the proven optimum (abs = rel gap 1e-6) on
              binary QPs of n = 20, 40, 60, 80, 100 (three seeds each): plain
-             vs batched vs mixed vs learned + batched vs learned + mixed, and
-             the geometric-mean speedups over plain per n
+             vs batched vs mixed vs mixed with the device-resident search
+             (mixed_device) vs learned + batched vs learned + mixed, and the
+             geometric-mean speedups over plain per n
   time_to_target_n200  n = 200 held-out binary QPs: the time each form
              needs to reach the certified relative gap plain reaches in
              --tt200-seconds (python bench.py --only-tt200 runs only this)
@@ -348,7 +349,7 @@ def hetero_batches(args):
             "value": iters / t, "unit": "FW iterations/s", "nodes_per_s": nodes / t, "converged_nodes": converged}
 
 
-def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, rel_gap=1e-6):
+def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, rel_gap=1e-6, device=False):
     import torch
     dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
     dc = torch.from_numpy(c).cuda()
@@ -366,6 +367,9 @@ def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, re
     if learned is not None:
         s.branch_rule = gb.BRANCH_LEARNED
         s.learned_model = learned.handle
+    # the device-resident search: frontier, pruning and branching on the GPU
+    # (the host driver's one-stream schedule)
+    s.device_frontier = 1 if device else 0
     r = gb.solve(n, [0] * n, [1] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), s)
     return {"seconds": round(r["elapsed_seconds"], 5), "relax_seconds": round(r["relax_seconds"], 5),
             "nodes": r["nodes_processed"], "fw_iterations": r["fw_iterations"], "objective": r["objective"],
@@ -388,10 +392,11 @@ def time_to_optimum(args):
     import gpuboscia_amd as gb
     from gpuboscia_amd import instances
     scorer = gb.Brancher.from_file(SCORER) if os.path.exists(SCORER) else None
-    forms = [("plain", 0, None), ("batched", 1, None), ("mixed", 2, None)]
+    forms = [("plain", 0, None, False), ("batched", 1, None, False), ("mixed", 2, None, False),
+             ("mixed_device", 2, None, True)]
     if scorer is not None:
-        forms.append(("learned_batched", 1, scorer))
-        forms.append(("learned_mixed", 2, scorer))
+        forms.append(("learned_batched", 1, scorer, False))
+        forms.append(("learned_mixed", 2, scorer, False))
     keep = set(args.tto_forms.split(","))
     forms = [f for f in forms if f[0] in keep]
     sizes = [int(v) for v in args.tto_sizes.split(",")]
@@ -407,18 +412,18 @@ def time_to_optimum(args):
             # the process (about 6 ms), not of the search.
             Qw, cw, _ = instances.binary_qp(n, seed=100 + seed, rows=rows)
             row = {"n": n, "seed": seed, "sha256": sha[:16]}
-            for key, form, learned in forms:
-                solve_row(gb, Qw, cw, n, form, learned, max_nodes=16384)
-                row[key] = solve_row(gb, Q, c, n, form, learned)
+            for key, form, learned, device in forms:
+                solve_row(gb, Qw, cw, n, form, learned, max_nodes=16384, device=device)
+                row[key] = solve_row(gb, Q, c, n, form, learned, device=device)
             if "plain" in row:
                 fp = row["plain"]["objective"]
                 row["same_optimum"] = all(abs(row[k]["objective"] - fp) <= 1e-9 * max(1.0, abs(fp))
-                                          for k, _, _ in forms)
-                for key, _, _ in forms[1:]:
+                                          for k, _, _, _ in forms)
+                for key, _, _, _ in forms[1:]:
                     row[f"speedup_{key}"] = round(row["plain"]["seconds"] / max(row[key]["seconds"], 1e-9), 2)
             out.append(row)
             say(f"time-to-optimum n={n} seed={seed}: " +
-                " ".join(f"{k} {row[k]['seconds']:.4f}s/{row[k]['nodes']}n" for k, _, _ in forms))
+                " ".join(f"{k} {row[k]['seconds']:.4f}s/{row[k]['nodes']}n" for k, _, _, _ in forms))
     return out
 
 
@@ -435,13 +440,14 @@ def time_to_target(args):
     import gpuboscia_amd as gb
     from gpuboscia_amd import instances
     scorer = gb.Brancher.from_file(SCORER) if os.path.exists(SCORER) else None
-    forms = [("plain", 0, None), ("batched", 1, None), ("mixed", 2, None)]
+    forms = [("plain", 0, None, False), ("batched", 1, None, False), ("mixed", 2, None, False),
+             ("mixed_device", 2, None, True)]
     if scorer is not None:
-        forms += [("learned_batched", 1, scorer), ("learned_mixed", 2, scorer)]
+        forms += [("learned_batched", 1, scorer, False), ("learned_mixed", 2, scorer, False)]
     n, T = 200, args.tt200_seconds
     Qw, cw, _ = instances.binary_qp(n, seed=49999)
-    for key, form, learned in forms:
-        solve_row(gb, Qw, cw, n, form, learned, max_nodes=2048)
+    for key, form, learned, device in forms:
+        solve_row(gb, Qw, cw, n, form, learned, max_nodes=2048, device=device)
     out = []
     for k in range(args.tt200_instances):
         seed = args.tt200_first_seed + k
@@ -450,21 +456,21 @@ def time_to_target(args):
         G = gap_relative(first["dual_bound"], first["objective"])
         row = {"n": n, "seed": seed, "sha256": sha[:16], "plain_time_limit_s": T, "target_rel_gap": G,
                "plain_first_run": first}
-        for key, form, learned in forms:
-            r = solve_row(gb, Q, c, n, form, learned, time_limit=max(2 * T, 10.0), rel_gap=G)
+        for key, form, learned, device in forms:
+            r = solve_row(gb, Q, c, n, form, learned, time_limit=max(2 * T, 10.0), rel_gap=G, device=device)
             r["reached_target"] = gap_relative(r["dual_bound"], r["objective"]) <= G
             row[key] = r
-        for key, _, _ in forms[1:]:
+        for key, _, _, _ in forms[1:]:
             row[f"speedup_{key}"] = round(row["plain"]["seconds"] / max(row[key]["seconds"], 1e-9), 2)
         out.append(row)
         say(f"time-to-target n=200 seed={seed} gap {G:.4g}: " +
-            " ".join(f"{kk} {row[kk]['seconds']:.3f}s/{row[kk]['nodes']}n" for kk, _, _ in forms))
+            " ".join(f"{kk} {row[kk]['seconds']:.3f}s/{row[kk]['nodes']}n" for kk, _, _, _ in forms))
     gm = {}
-    for key, _, _ in forms[1:]:
+    for key, _, _, _ in forms[1:]:
         sp = [r[f"speedup_{key}"] for r in out if r[key]["reached_target"] and r["plain"]["reached_target"]]
         gm[key] = round(float(np.exp(np.mean(np.log(sp)))), 2) if sp else None
     return {"instances": out, "speedup_geomean": gm,
-            "all_reached_target": all(r[kk]["reached_target"] for r in out for kk, _, _ in forms)}
+            "all_reached_target": all(r[kk]["reached_target"] for r in out for kk, _, _, _ in forms)}
 
 
 def main():
@@ -480,7 +486,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tto", action="store_true")
     ap.add_argument("--tto-sizes", default="20,40,60,80,100", help="binary-QP sizes of time_to_optimum")
-    ap.add_argument("--tto-forms", default="plain,batched,mixed,learned_batched,learned_mixed",
+    ap.add_argument("--tto-forms", default="plain,batched,mixed,mixed_device,learned_batched,learned_mixed",
                     help="forms of time_to_optimum (speedups need plain)")
     ap.add_argument("--no-tt200", action="store_true", help="skip the n=200 time-to-target leg")
     ap.add_argument("--only-tt200", action="store_true", help="run only the n=200 time-to-target leg")
@@ -591,7 +597,7 @@ def main():
         tto = time_to_optimum(args)
         result["time_to_optimum"] = tto
         gm = {}
-        for key in ("batched", "mixed", "learned_batched", "learned_mixed"):
+        for key in ("batched", "mixed", "mixed_device", "learned_batched", "learned_mixed"):
             per_n = {}
             for nn in sorted({r["n"] for r in tto}):
                 sp = [r[f"speedup_{key}"] for r in tto if f"speedup_{key}" in r and r["n"] == nn]

@@ -137,11 +137,11 @@ gb_status device_search(const gb_problem* pr, const gb_objective_desc* o, const 
 
     // Node slots: the frontier of the largest searches holds millions of
     // nodes (about 10 n + 53 bytes each); a quarter of free memory, at most
-    // 32 GiB, or what a node cap can need.
+    // 32 GiB and 16 M slots, or what a node cap can need.
     size_t free_b = 0, total_b = 0;
     GB_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t per_slot = (size_t)10 * n + 53;
-    size_t cap = std::min(free_b / 4, (size_t)32 << 30) / per_slot;
+    size_t cap = std::min(std::min(free_b / 4, (size_t)32 << 30) / per_slot, (size_t)16 << 20);
     if (s->max_nodes > 0) cap = std::min(cap, (size_t)(9 * (s->max_nodes + B) + 16));
     if (s->device_nodes > 0) cap = (size_t)s->device_nodes;
     cap = std::min(cap, (size_t)1 << 30);
@@ -288,12 +288,17 @@ gb_status device_search(const gb_problem* pr, const gb_objective_desc* o, const 
         GB_HIP(hipStreamSynchronize(stream));
         return GB_OK;
     };
-    // First round eagerly (it also settles every launch's one-time set-up),
-    // then the round as one graph.
-    st = round();
-    if (st != GB_OK) return st;
-    st = read_state();
-    if (st != GB_OK) return st;
+    // The first rounds eagerly (they also settle every launch's one-time
+    // set-up, and small searches end within them), then the round as one
+    // graph.
+    constexpr int kEagerRounds = 2;
+    for (int r = 0; r < kEagerRounds; ++r) {
+        st = round();
+        if (st != GB_OK) return st;
+        st = read_state();
+        if (st != GB_OK) return st;
+        if (hs.p->finished) break;
+    }
     GraphHolder gh;
     if (!hs.p->finished) {
         GB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
