@@ -129,7 +129,8 @@ class Settings(ctypes.Structure):
                 ("branch", BRANCH_FN), ("branch_user", ctypes.c_void_p),
                 ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_void_p),
                 ("max_nodes", ctypes.c_longlong), ("autotune", ctypes.c_int),
-                ("device_frontier", ctypes.c_int), ("device_nodes", ctypes.c_longlong)]
+                ("device_frontier", ctypes.c_int), ("device_nodes", ctypes.c_longlong),
+                ("blocks_per_node", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -150,7 +151,7 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_grid_barrier_selftest", "gb_relax_graph_create", "gb_relax_graph_launch",
             "gb_relax_graph_destroy", "gb_mixed_prepare", "gb_mixed_destroy", "gb_mixed_products",
             "gb_relax_batch_mixed", "gb_mixed_autotune", "gb_relax_instances", "gb_shutdown",
-            "gb_relax_batch_noalloc", "gb_lmo_compute_v", "gb_mixed_warm_products",
+            "gb_relax_batch_noalloc", "gb_relax_batch_multiblock", "gb_lmo_compute_v", "gb_mixed_warm_products",
             "gb_children_push", "gb_solver_create", "gb_solver_destroy", "gb_solver_push",
             "gb_solver_set_scheduling_callback", "gb_solver_set_progress_callback", "gb_solver_solve",
             "gb_solver_get_stats", "gb_solver_incumbent_x", "gb_candidate_features"]
@@ -189,6 +190,8 @@ def load():
         + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
     lib.gb_relax_batch_noalloc.restype = ctypes.c_int
     lib.gb_relax_batch_noalloc.argtypes = lib.gb_relax_batch.argtypes
+    lib.gb_relax_batch_multiblock.restype = ctypes.c_int
+    lib.gb_relax_batch_multiblock.argtypes = lib.gb_relax_batch.argtypes[:-1] + [ctypes.c_int, ctypes.c_void_p]
     lib.gb_lmo_compute_v.restype = ctypes.c_int
     lib.gb_lmo_compute_v.argtypes = [ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
     lib.gb_mixed_warm_products.restype = ctypes.c_int
@@ -324,6 +327,21 @@ def relax_batch_noalloc(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N,
                                       _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
                                       float(candidate_tol), int(block_threads),
                                       ctypes.c_void_p(stream or 0))
+    check(code)
+
+
+def relax_batch_multiblock(obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x, lb, ub,
+                           eps, iterations, status, gap, value, best_vertex, best_vertex_val, flags=None,
+                           candidate_tol=1e-6, block_threads=0, blocks_per_node=4, stream=None):
+    """gb_relax_batch_multiblock: gb_relax_batch with blocks_per_node
+    workgroups per node (MultiBlock), bit-identical outputs."""
+    lib = load()
+    code = lib.gb_relax_batch_multiblock(ctypes.byref(obj), ctypes.byref(alg), ctypes.byref(lmo), int(N),
+                                         int(max_iters), _ptr(x), _ptr(lb), _ptr(ub), _ptr(eps),
+                                         _ptr(iterations), _ptr(status), _ptr(gap), _ptr(value),
+                                         _ptr(best_vertex), _ptr(best_vertex_val), _ptr(flags),
+                                         float(candidate_tol), int(block_threads), int(blocks_per_node),
+                                         ctypes.c_void_p(stream or 0))
     check(code)
 
 

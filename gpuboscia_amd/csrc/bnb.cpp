@@ -1033,7 +1033,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                      const gb_settings* s, gb_result* res, Labeler* lab = nullptr, SolveHooks* hk = nullptr) {
     gb_status st = validate_settings(pr, o, l, s);
     if (st != GB_OK) return st;
-    st = validate_descriptors(o, &s->algorithm, l);
+    st = validate_descriptors(o, &s->algorithm, l, !(s->form == 0 && s->blocks_per_node > 1));
     if (st != GB_OK) return st;
     if (!(hk && hk->nodes)) {
         st = validate_root(pr, l, s);
@@ -1389,6 +1389,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             st = gb_relax_batch_mixed(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb,
                                       db.ub, db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv,
                                       db.flags, s->integer_tol, db.cut, &mo, db.stream);
+        } else if (s->blocks_per_node > 1) {
+            // MultiBlock (FWExecutionMode::MultiBlock): the same relaxation,
+            // bit for bit, with blocks_per_node workgroups per node
+            st = relax_batch_multiblock(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
+                                        db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
+                                        s->integer_tol, s->gpu_block_threads, s->blocks_per_node, db.stream);
         } else {
             st = relax_batch_fp64(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "gb_barrier.h"
 #include "gb_fast.h"
 #include "gb_internal.h"
 #include "gb_host.h"
@@ -52,13 +53,14 @@ static gb_status validate_lmo(const gb_lmo_desc* l, int n, int obj_kind, const g
 }
 
 gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
-                               const gb_lmo_desc* l) {
+                               const gb_lmo_desc* l, bool staged) {
     if (!o || !a || !l) return fail(GB_ERR_INVALID_ARGUMENT, "null descriptor");
     if (o->n <= 0) return fail(GB_ERR_INVALID_ARGUMENT, "objective n must be positive");
     if (o->kind == GB_OBJ_DENSE_QUADRATIC) {
         if (!o->Qt) return fail(GB_ERR_INVALID_ARGUMENT, "dense objective requires Qt");
-        if (o->n > kMaxDenseN)
-            return fail(GB_ERR_INVALID_ARGUMENT, "dense objective n exceeds the LDS-staged limit");
+        if (staged && o->n > kMaxDenseN)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "dense objective n exceeds the LDS-staged limit (MultiBlock takes any n)");
     } else if (o->kind == GB_OBJ_QAP) {
         if (o->order <= 0 || o->n != o->order * o->order || !o->A || !o->B)
             return fail(GB_ERR_INVALID_ARGUMENT, "QAP objective requires order, A, B and n == order^2");
@@ -101,8 +103,9 @@ static gb_status relax_fp64_setup(const gb_objective_desc* o, const gb_algorithm
                                   const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
                                   const double* ub, const double* eps, int* iterations, uint8_t* status,
                                   double* gap, double* value, double* best_vertex, double* best_vertex_val,
-                                  const uint8_t* flags, double ctol, int block_threads, Fp64Launch* out) {
-    gb_status st = validate_descriptors(o, a, l);
+                                  const uint8_t* flags, double ctol, int block_threads, Fp64Launch* out,
+                                  bool lds = true) {
+    gb_status st = validate_descriptors(o, a, l, lds);
     if (st != GB_OK) return st;
     if (N < 0 || max_iters < 0) return fail(GB_ERR_INVALID_ARGUMENT, "N and max_iters must be non-negative");
     if (N == 0) return GB_OK;
@@ -115,7 +118,7 @@ static gb_status relax_fp64_setup(const gb_objective_desc* o, const gb_algorithm
                     "gpu_block_threads must be zero or a positive multiple of 64 up to 1024");
     const int n = o->n;
     out->L = lds_layout(o->kind, l->kind, n, l->order);
-    if (out->L.total > kMaxLdsBytes) return fail(GB_ERR_INVALID_ARGUMENT, "problem exceeds the LDS budget");
+    if (lds && out->L.total > kMaxLdsBytes) return fail(GB_ERR_INVALID_ARGUMENT, "problem exceeds the LDS budget");
     out->threads = block_threads;
     out->work_bytes = (size_t)N * kWorkVectors * n * sizeof(double);
     RelaxParams& P = out->P;
@@ -197,6 +200,57 @@ static gb_status relax_batch_fp64_noalloc(const gb_objective_desc* o, const gb_a
     if (se != 0) return hip_fail(static_cast<hipError_t>(se), __FILE__, __LINE__);
     const int e = launch_relax_fp64(F.P, F.threads, F.L.total, F.L, stream);
     if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    return GB_OK;
+}
+
+// MultiBlock: K co-resident workgroups per node (relax_multiblock.hip); the
+// nodes run in launches of as many complete groups as are resident at once.
+gb_status relax_batch_multiblock(const gb_objective_desc* o, const gb_algorithm_desc* a, const gb_lmo_desc* l,
+                                 int N, int max_iters, double* x, const double* lb, const double* ub,
+                                 const double* eps, int* iterations, uint8_t* status, double* gap, double* value,
+                                 double* best_vertex, double* best_vertex_val, const uint8_t* flags, double ctol,
+                                 int block_threads, int blocks_per_node, void* stream) {
+    gb_status st = validate_descriptors(o, a, l, false);
+    if (st != GB_OK) return st;
+    if (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX)
+        return fail(GB_ERR_INVALID_ARGUMENT, "MultiBlock relaxes dense quadratics over a box");
+    const int K = blocks_per_node;
+    if (K < 1 || K > kMbMaxK) return fail(GB_ERR_INVALID_ARGUMENT, "blocks_per_node must be 1..256");
+    Fp64Launch F;
+    st = relax_fp64_setup(o, a, l, N, max_iters, x, lb, ub, eps, iterations, status, gap, value, best_vertex,
+                          best_vertex_val, flags, ctol, block_threads, &F, false);
+    if (st != GB_OK || N == 0) return st;
+    const int Tv = F.threads;  // the one-workgroup launch's block size: the order of every sum
+    if (K * kMbThreads < Tv)
+        return fail(GB_ERR_INVALID_ARGUMENT, "blocks_per_node * 256 must cover gpu_block_threads");
+    const int cap = multiblock_capacity();
+    const int per_launch = std::min(N, cap / K);
+    if (per_launch < 1) return fail(GB_ERR_INVALID_ARGUMENT, "blocks_per_node exceeds the resident workgroups");
+    const int n = o->n;
+    const size_t work_b = (size_t)per_launch * 6 * n * sizeof(double);
+    const size_t red_b = (size_t)per_launch * 3 * kMbSlots * sizeof(double);
+    const size_t bar_b = (size_t)per_launch * sizeof(GroupBarrier);
+    const size_t total = work_b + red_b + bar_b + sizeof(BarrierStatus) + 512;
+    double* ws = nullptr;
+    st = workspace_get(total, stream, &ws);
+    if (st != GB_OK) return st;
+    char* p = reinterpret_cast<char*>(ws);
+    MbParams M{};
+    M.K = K;
+    M.Tv = Tv;
+    M.work = reinterpret_cast<double*>(p);
+    M.red = reinterpret_cast<double*>(p + work_b);
+    char* bars = p + ((work_b + red_b + 255) & ~size_t(255));
+    M.bars = reinterpret_cast<GroupBarrier*>(bars);
+    M.bst = reinterpret_cast<BarrierStatus*>(bars + ((bar_b + 255) & ~size_t(255)));
+    M.spin = 1ull << 22;  // seconds of polling: a wait that long means the group is not resident
+    for (int node0 = 0; node0 < N; node0 += per_launch) {
+        const int nodes = std::min(per_launch, N - node0);
+        M.node0 = node0;
+        int e = launch_multiblock_clear(M.bars, nodes, M.bst, stream);
+        if (e == 0) e = launch_relax_multiblock(F.P, M, nodes, stream);
+        if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    }
     return GB_OK;
 }
 
@@ -387,6 +441,23 @@ extern "C" gb_status gb_relax_batch(const gb_objective_desc* objective,
         return relax_batch_fp64(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps,
                                 iterations, status, gap, value, best_vertex, best_vertex_val,
                                 integer_flags, candidate_tol, block_threads, stream);
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
+}
+
+extern "C" gb_status gb_relax_batch_multiblock(const gb_objective_desc* objective,
+                                               const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,
+                                               int N, int max_iters, double* x, const double* lb,
+                                               const double* ub, const double* eps, int* iterations,
+                                               uint8_t* status, double* gap, double* value,
+                                               double* best_vertex, double* best_vertex_val,
+                                               const uint8_t* integer_flags, double candidate_tol,
+                                               int block_threads, int blocks_per_node, void* stream) {
+    try {
+        return relax_batch_multiblock(objective, algorithm, lmo, N, max_iters, x, lb, ub, eps, iterations, status,
+                                      gap, value, best_vertex, best_vertex_val, integer_flags, candidate_tol,
+                                      block_threads, blocks_per_node, stream);
     } catch (const std::exception& ex) {
         return fail(GB_ERR_RUNTIME, ex.what());
     }

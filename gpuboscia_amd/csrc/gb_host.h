@@ -21,8 +21,10 @@ extern thread_local std::string g_last_error;
 gb_status fail(gb_status code, const std::string& msg);
 gb_status hip_fail(hipError_t e, const char* file, int line);
 gb_status workspace_get(size_t bytes, void* stream, double** out);
+// staged: the relaxation stages a dense objective's vector in LDS (n <=
+// kMaxDenseN); MultiBlock reads it from global memory (any n).
 gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
-                               const gb_lmo_desc* l);
+                               const gb_lmo_desc* l, bool staged = true);
 int auto_block_threads(int n);
 gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* a,
                            const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
@@ -75,6 +77,13 @@ class ScopedCounter {
     void* stream_;
     CounterLease lease_;
 };
+// MultiBlock relaxation (capi.cpp): the plain form with blocks_per_node
+// co-resident workgroups per node; dense quadratic over a box.
+gb_status relax_batch_multiblock(const gb_objective_desc* o, const gb_algorithm_desc* a, const gb_lmo_desc* l,
+                                 int N, int max_iters, double* x, const double* lb, const double* ub,
+                                 const double* eps, int* iterations, uint8_t* status, double* gap, double* value,
+                                 double* best_vertex, double* best_vertex_val, const uint8_t* flags, double ctol,
+                                 int block_threads, int blocks_per_node, void* stream);
 // Node FW tolerance at a depth (the host driver's, from DeviceResidentConfig).
 double depth_tolerance(double root, double decay, double minimum, int depth);
 // The device-resident search of gb_solve (dsearch.cpp), settings validated.

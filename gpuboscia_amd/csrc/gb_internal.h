@@ -116,6 +116,27 @@ int static_work_address(double** out);
 int launch_lmo_probe(const RelaxParams& p, int block_threads, size_t lds_bytes,
                      const LdsLayout& L, const double* G, void* stream);
 
+// Multi-workgroup relaxation (relax_multiblock.hip): K workgroups of
+// kMbThreads threads per node, nodes [node0, node0 + nodes) per launch.
+struct GroupBarrier;
+struct BarrierStatus;
+struct MbParams {
+    int K;                   // workgroups per node
+    int Tv;                  // lanes of the canonical sums (the one-workgroup block size)
+    int node0;               // first node of this launch
+    GroupBarrier* bars;      // [nodes per launch]
+    BarrierStatus* bst;
+    unsigned long long spin;
+    double* red;             // [nodes per launch][3][kMbSlots]
+    double* work;            // [nodes per launch][6][n]: g, v, a, trial x, trial g, product
+};
+constexpr int kMbThreads = 256;
+constexpr int kMbSlots = 16 + 256;   // wave totals (Tv <= 1024), then one slot per workgroup (K <= 256)
+constexpr int kMbMaxK = 256;
+int launch_multiblock_clear(GroupBarrier* bars, int count, BarrierStatus* bst, void* stream);
+int multiblock_capacity();
+int launch_relax_multiblock(const RelaxParams& p, const MbParams& m, int nodes, void* stream);
+
 // Launch the FP64 relaxation kernel (relax_fp64.hip).  Returns a hipError_t
 // value as int.
 int launch_relax_fp64(const RelaxParams& p, int block_threads, size_t lds_bytes,

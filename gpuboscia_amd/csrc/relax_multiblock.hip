@@ -1,0 +1,421 @@
+// Multi-workgroup relaxation of one node (MultiBlock mode, K > 1) for dense
+// quadratics over a box, gfx950: K co-resident workgroups share a node and
+// meet at its group barrier (gb_barrier.h), so one node's products stream
+// the whole of Q at the bandwidth of many CUs instead of one, and n is not
+// limited by LDS (multiblock.cuh, docs/multiblock.md).
+//
+// The relaxation is the plain form's, step for step (relax_fp64.hip), and
+// its arithmetic is the one-workgroup kernel's: every product coordinate is
+// the same k-ordered FMA chain (a thread computes whole rows), elementwise
+// work is the same per coordinate, and every sum is taken in the canonical
+// order of a workgroup of Tv threads (Tv = the one-workgroup launch's block
+// size): virtual lane l accumulates coordinates l, l + Tv, ... in order, each
+// virtual wave (64 consecutive global threads, one real wave) reduces with
+// the same shuffle tree, and every workgroup adds the wave totals in wave
+// order.  Minima and the logical ands are order-free.  The outputs therefore
+// equal the one-workgroup kernel's bit for bit, whatever K.
+//
+// Collectives: a workgroup writes its partial (wave totals, or one value
+// per workgroup) into the node's reduction slots, the group syncs, and every
+// workgroup combines the slots itself, so all workgroups hold the same
+// scalars and take the same branches.  The slots rotate over three buffers:
+// a buffer is rewritten only two syncs after its last read.  A failed sync
+// (timeout of the bounded wait, which needs a broken co-residency) turns the
+// remaining collectives into no-ops so the kernel ends, and the node's
+// status becomes 0xFF.
+#include "gb_barrier.h"
+#include "gb_device.h"
+#include "gb_internal.h"
+
+namespace gb {
+
+namespace {
+
+struct Mb {
+    const RelaxParams& P;
+    const MbParams& M;
+    int node, rank, gtid, G, n;
+    int* flag;       // LDS word of group_sync
+    double* sred;    // LDS block scratch
+    int call;
+    bool ok;
+
+    __device__ bool sync() {
+        if (!ok) return false;
+        ok = group_sync(M.bars + node, (unsigned)M.K, M.spin, M.bst, flag);
+        return ok;
+    }
+    __device__ double* slots() {
+        double* r = M.red + ((size_t)node * 3 + (call % 3)) * kMbSlots;
+        ++call;
+        return r;
+    }
+    // Canonical sum of the virtual lanes' values (meaningful for gtid < Tv).
+    __device__ double sum(double v) {
+        double* r = slots();
+        if (gtid < M.Tv) {
+            const double w = wave_tree_sum(v);
+            if ((gtid & (kWave - 1)) == 0) r[gtid / kWave] = w;
+        }
+        sync();
+        double t = r[0];
+        const int W = M.Tv / kWave;
+        for (int w = 1; w < W; ++w) t = t + r[w];
+        return t;
+    }
+    __device__ double minimum(double v) {
+        double* r = slots();
+        const double b = block_min(v, sred);
+        if (threadIdx.x == 0) r[16 + rank] = b;
+        sync();
+        double t = r[16];
+        for (int k = 1; k < M.K; ++k) t = fmin(t, r[16 + k]);
+        return t;
+    }
+    __device__ bool all(bool p) {
+        double* r = slots();
+        int* ired = reinterpret_cast<int*>(sred);
+        const bool b = block_and(p, ired);
+        if (threadIdx.x == 0) r[16 + rank] = b ? 1.0 : 0.0;
+        sync();
+        bool t = true;
+        for (int k = 0; k < M.K; ++k) t = t && r[16 + k] != 0.0;
+        return t;
+    }
+
+    // Row j of Q z: the k-ordered FMA chain of the one-workgroup kernel
+    // (z read from global memory instead of LDS: the same values).
+    __device__ double row(const double* z, int j) const {
+        const double* q = P.Qt + j;
+        double acc = 0.0;
+        int k = 0;
+        for (; k + 8 <= n; k += 8) {
+            double qq[8], zz[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                qq[u] = q[(size_t)(k + u) * n];
+                zz[u] = z[k + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = fma(qq[u], zz[u], acc);
+        }
+        for (; k < n; ++k) acc = fma(q[(size_t)k * n], z[k], acc);
+        return acc;
+    }
+    // out = Q z on this group (z complete on entry); synced on return.
+    __device__ void product(const double* z, double* out) {
+        for (int j = gtid; j < n; j += G) out[j] = row(z, j);
+        sync();
+    }
+    // Obj<GB_OBJ_DENSE_QUADRATIC>::grad
+    __device__ void grad(const double* x, double* g) {
+        for (int j = gtid; j < n; j += G) {
+            const double acc = row(x, j);
+            const double lin = P.lin ? P.ls * P.lin[j] : 0.0;
+            g[j] = P.gs * (acc + lin);
+        }
+        sync();
+    }
+    // Obj<GB_OBJ_DENSE_QUADRATIC>::value (x complete on entry)
+    __device__ double value(const double* x, double* prod) {
+        product(x, prod);
+        const double hs = 0.5 * P.gs, ls = P.gs * P.ls;
+        double val = 0.0;
+        if (gtid < M.Tv) {
+            double quad = 0.0, lin = 0.0;
+            for (int j = gtid; j < n; j += M.Tv) {
+                quad = fma(x[j], prod[j], quad);
+                if (P.lin) lin = fma(P.lin[j], x[j], lin);
+            }
+            val = fma(hs, quad, ls * lin);
+            if (gtid == 0) val = val + P.kappa;
+        }
+        return sum(val);
+    }
+    // Obj<GB_OBJ_DENSE_QUADRATIC>::curvature (d complete on entry)
+    __device__ double curvature(const double* d, double* prod) {
+        product(d, prod);
+        double c = 0.0;
+        if (gtid < M.Tv)
+            for (int j = gtid; j < n; j += M.Tv) c = fma(P.gs * d[j], prod[j], c);
+        return sum(c);
+    }
+    // sum over j of a[j] * b[j] in the canonical order (a, b complete)
+    __device__ double dot(const double* a, const double* b) {
+        double acc = 0.0;
+        if (gtid < M.Tv)
+            for (int j = gtid; j < n; j += M.Tv) acc = fma(a[j], b[j], acc);
+        return sum(acc);
+    }
+};
+
+__device__ inline double mb_quad_exact(double gap, double curvature, double maximum) {
+    if (maximum <= 0.0) return 0.0;
+    if (curvature > 0.0) {
+        if (gap <= 0.0) return 0.0;
+        const double step = gap / curvature;
+        return step < maximum ? step : maximum;
+    }
+    return -gap + 0.5 * curvature * maximum < 0.0 ? maximum : 0.0;
+}
+
+__global__ void __launch_bounds__(kMbThreads) relax_multiblock_kernel(RelaxParams P, MbParams M) {
+    __shared__ int flag;
+    __shared__ double sred[kMaxWaves];
+    const int local = blockIdx.x / M.K;
+    const int node = M.node0 + local;
+    const int rank = blockIdx.x % M.K;
+    const int n = P.n;
+    Mb c{P, M, local, rank, rank * (int)blockDim.x + (int)threadIdx.x, M.K * (int)blockDim.x, n, &flag, sred, 0, true};
+    if (node >= P.N) return;  // whole groups only (the launch covers complete groups)
+    const int gtid = c.gtid, G = c.G;
+    const double eps = P.eps[node];
+    const int initial = P.iters[node];
+    const bool supplied = (P.status[node] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0;
+    double* x = P.x + (size_t)node * n;
+    const double* lb = P.lb + (size_t)node * n;
+    const double* ub = P.ub + (size_t)node * n;
+    double* best_v = P.best_v + (size_t)node * n;
+    double* g = M.work + (size_t)local * 6 * n;
+    double* v = g + n;
+    double* aw = v + n;
+    double* tx = aw + n;
+    double* tg = tx + n;
+    double* prod = tg + n;
+    const double atol = 1e-6, rtol = 1e-4;
+
+    // node feasibility before any objective access
+    bool valid = true;
+    for (int j = gtid; j < n; j += G)
+        if (isnan(lb[j]) || isnan(ub[j]) || lb[j] > ub[j] || (P.flags && P.flags[j] && ceil(lb[j]) > floor(ub[j])))
+            valid = false;
+    if (!c.all(valid)) {
+        if (rank == 0 && threadIdx.x == 0) {
+            P.status[node] = c.ok ? GB_NODE_INFEASIBLE : 0xFF;
+            P.value[node] = __builtin_inf();
+            P.gap[node] = 0.0;
+            P.best_val[node] = kNone;
+            P.iters[node] = initial;
+        }
+        return;
+    }
+
+    // drs_integer_value
+    auto integer_value = [&](const double* vv) {
+        bool ok = true;
+        for (int j = gtid; j < n; j += G) {
+            const double vj = vv[j];
+            if (!isfinite(vj) || vj < lb[j] - P.ctol || vj > ub[j] + P.ctol
+                    || (P.flags && P.flags[j] && vj != round(vj)))
+                ok = false;
+        }
+        if (!c.all(ok)) return kNone;
+        const double val = c.value(vv, prod);
+        return isfinite(val) ? val : kNone;
+    };
+    double best_s = kNone;
+    auto track = [&](const double* vert) {
+        const double fv = integer_value(vert);
+        if (fv < best_s) best_s = fv;
+        if (fv < 1e299 && fv == best_s)
+            for (int j = gtid; j < n; j += G) best_v[j] = vert[j];
+    };
+    auto compute_v = [&](const double* gg, double* out) {
+        for (int j = gtid; j < n; j += G) out[j] = (gg[j] > 0.0) ? lb[j] : ub[j];
+        c.sync();
+    };
+
+    // SecantLS::search (algorithms.cuh:121-258), as the one-workgroup kernel
+    auto secant = [&](double gap, double max_gamma, const double* dir) {
+        const double tol = P.secant_tol;
+        const double f_base = c.value(x, prod);
+        double lo = 0.0, hi = max_gamma, best_f = f_base, best_gamma = 0.0;
+        double phi_lo = -gap, phi_hi = 0.0, gamma = 0.0;
+        bool done = phi_lo >= -tol;
+        const double f_abs = f_base >= 0.0 ? f_base : -f_base;
+        const double f_allow = tol * (1.0 + f_abs);
+        auto trial = [&](double gm, double& phi, double& f) {
+            for (int j = gtid; j < n; j += G) tx[j] = fma(gm, dir[j], x[j]);
+            c.sync();
+            c.grad(tx, tg);
+            phi = c.dot(tg, dir);
+            f = c.value(tx, prod);
+        };
+        double f_hi;
+        trial(max_gamma, phi_hi, f_hi);
+        if (f_hi < best_f || (best_gamma == 0.0 && f_hi <= f_base + f_allow)) {
+            best_f = f_hi;
+            best_gamma = max_gamma;
+        }
+        if (!done && phi_hi <= tol && f_hi <= f_base + f_allow) { gamma = max_gamma; done = true; }
+        for (int it = 0; it < P.secant_iters; ++it) {
+            if (done) break;
+            const double denom = phi_hi - phi_lo;
+            double gm = (denom > P.secant_denom || denom < -P.secant_denom)
+                ? hi - phi_hi * (hi - lo) / denom : 0.5 * (lo + hi);
+            if (!(gm > lo && gm < hi)) gm = 0.5 * (lo + hi);
+            gamma = gm;
+            double phi, ft;
+            trial(gamma, phi, ft);
+            if (ft < best_f || (best_gamma == 0.0 && ft <= f_base + f_allow)) { best_f = ft; best_gamma = gamma; }
+            if (phi < 0.0) { lo = gamma; phi_lo = phi; } else { hi = gamma; phi_hi = phi; }
+            if ((phi <= tol && phi >= -tol) || (hi - lo) <= tol) { gamma = 0.5 * (lo + hi); done = true; }
+        }
+        if (best_gamma > 0.0 && best_f <= f_base + f_allow) gamma = best_gamma;
+        else if (!done) gamma = 0.0;
+        return gamma;
+    };
+
+    if (initial == 0 && !supplied) {
+        for (int j = gtid; j < n; j += G) g[j] = -1.0 - (double)j / (n + 1.0);
+        compute_v(g, v);
+        for (int j = gtid; j < n; j += G) x[j] = v[j];
+    }
+    c.sync();
+
+    const bool secant_like = P.ls_kind == GB_LS_SECANT || P.ls_kind == GB_LS_QUADRATIC_EXACT;
+    int iterations = initial;
+    for (int iter = 0; iter < P.max_iters; ++iter) {
+        iterations = initial + iter + 1;
+        c.grad(x, g);
+        compute_v(g, v);
+        track(v);
+        for (int j = gtid; j < n; j += G) tx[j] = x[j] - v[j];
+        c.sync();
+        const double gap = c.dot(g, tx);
+        if (gap >= 0.0 && gap < eps) break;
+        if (!P.dicg) {
+            if (secant_like) {
+                for (int j = gtid; j < n; j += G) v[j] = v[j] - x[j];
+                c.sync();
+                double gamma;
+                if (P.ls_kind == GB_LS_SECANT) gamma = secant(gap, 1.0, v);
+                else gamma = mb_quad_exact(gap, c.curvature(v, prod), 1.0);
+                for (int j = gtid; j < n; j += G) x[j] = fma(gamma, v[j], x[j]);
+            } else {
+                const double gamma = 2.0 / (double)(initial + iter + 2);
+                for (int j = gtid; j < n; j += G) x[j] = fma(gamma, v[j] - x[j], x[j]);
+            }
+            c.sync();
+            continue;
+        }
+        // the in-face (away) vertex of -g (Lmo<GB_LMO_BOX>::inface)
+        for (int j = gtid; j < n; j += G) {
+            const double gj = -g[j];
+            const double ut = atol + rtol * fabs(ub[j]);
+            const double lt = atol + rtol * fabs(lb[j]);
+            if (fabs(x[j] - ub[j]) <= ut) aw[j] = ub[j];
+            else if (fabs(x[j] - lb[j]) <= lt) aw[j] = lb[j];
+            else aw[j] = (gj > 0.0) ? lb[j] : ub[j];
+        }
+        c.sync();
+        track(aw);
+        for (int j = gtid; j < n; j += G) aw[j] = aw[j] - v[j];
+        c.sync();
+        // Lmo<GB_LMO_BOX>::max_step
+        double r = 1.0;
+        for (int j = gtid; j < n; j += G) {
+            const double dj = aw[j];
+            if (dj < 0.0) r = fmin(r, (ub[j] - x[j]) / -dj);
+            else if (dj > 0.0) r = fmin(r, (x[j] - lb[j]) / dj);
+        }
+        const double gmax = fmax(0.0, fmin(c.minimum(r), 1.0));
+        if (secant_like) {
+            for (int j = gtid; j < n; j += G) {
+                aw[j] = -aw[j];
+                tg[j] = -g[j];
+            }
+            c.sync();
+            const double pgap = c.dot(tg, aw);
+            double gamma;
+            if (P.ls_kind == GB_LS_SECANT) gamma = secant(pgap, gmax, aw);
+            else gamma = mb_quad_exact(pgap, c.curvature(aw, prod), gmax);
+            for (int j = gtid; j < n; j += G) x[j] = fma(gamma, aw[j], x[j]);
+        } else {
+            const double ol = 2.0 / (double)(initial + iter + 2);
+            double gamma;
+            if (P.ls_kind == GB_LS_SAFEGUARDED_SCALED_OPEN_LOOP) gamma = gmax <= ol ? gmax : gmax * ol;
+            else if (P.ls_kind == GB_LS_SCALED_OPEN_LOOP) gamma = gmax * ol;
+            else gamma = fmin(gmax, ol);
+            for (int j = gtid; j < n; j += G) x[j] = fma(-gamma, aw[j], x[j]);
+        }
+        c.sync();
+    }
+
+    // final certificate at the returned iterate
+    c.grad(x, g);
+    compute_v(g, v);
+    track(v);
+    for (int j = gtid; j < n; j += G) tx[j] = x[j] - v[j];
+    c.sync();
+    const double fgap = c.dot(g, tx);
+    const double fval = c.value(x, prod);
+    uint8_t st = (fgap >= 0.0 && fgap < eps) ? 1 : 0;
+    if (P.flags) {
+        for (int j = gtid; j < n; j += G) {
+            double cc = x[j];
+            if (P.flags[j]) cc = P.heuristic == GB_HEUR_FLOOR_OFFSET ? floor(x[j]) + P.heuristic_offset : round(x[j]);
+            v[j] = cc;
+        }
+        c.sync();
+        const double cv = integer_value(v);  // a box accepts every rounded point
+        if (cv < best_s) {
+            for (int j = gtid; j < n; j += G) best_v[j] = v[j];
+            best_s = cv;
+        }
+    }
+    bool integral = true;
+    for (int j = gtid; j < n; j += G)
+        if (!isfinite(x[j]) || (P.flags && P.flags[j] && fabs(x[j] - round(x[j])) > P.ctol)) integral = false;
+    const bool all_int = c.all(integral);
+    if (all_int) st = st ? GB_NODE_INTEGRAL_CONVERGED : GB_NODE_INTEGRAL_UNCONVERGED;
+    if (rank == 0 && threadIdx.x == 0) {
+        P.value[node] = fval;
+        P.gap[node] = fgap;
+        P.status[node] = c.ok ? st : 0xFF;
+        P.iters[node] = iterations;
+        P.best_val[node] = best_s;
+    }
+}
+
+__global__ void mb_clear_kernel(GroupBarrier* bars, int count, BarrierStatus* bst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        bars[i].arrived = 0u;
+        bars[i].generation = 0u;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        bst->abort = 0u;
+        bst->timeouts = 0ull;
+    }
+}
+
+}  // namespace
+
+int launch_multiblock_clear(GroupBarrier* bars, int count, BarrierStatus* bst, void* stream) {
+    hipLaunchKernelGGL(mb_clear_kernel, dim3(64), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), bars, count,
+                       bst);
+    return (int)hipGetLastError();
+}
+
+// Workgroups of the multi-workgroup kernel that are resident on the current
+// device at once.
+int multiblock_capacity() {
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_multiblock_kernel, kMbThreads, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return per_cu * cus;
+}
+
+int launch_relax_multiblock(const RelaxParams& p, const MbParams& m, int nodes, void* stream) {
+    RelaxParams pp = p;
+    MbParams mm = m;
+    void* args[] = {&pp, &mm};
+    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(relax_multiblock_kernel),
+                                                    dim3(nodes * m.K), dim3(kMbThreads), args, 0,
+                                                    reinterpret_cast<hipStream_t>(stream));
+    return (int)e;
+}
+
+}  // namespace gb

@@ -175,6 +175,28 @@ gb_status gb_relax_batch(const gb_objective_desc* objective,
  * call's contents; the kernel writes every work entry before reading it.
  */
 #define GB_NOALLOC_WORK_BYTES (16u * 1024u * 1024u)
+/*
+ * MultiBlock relaxation (FWExecutionMode::MultiBlock with blocks_per_node,
+ * device_resident_solver.cuh:1011-1013, multiblock.cuh, docs/multiblock.md):
+ * gb_relax_batch with blocks_per_node co-resident workgroups of 256 threads
+ * per node meeting at a device-atomic group barrier, for dense quadratics
+ * over a box (n not limited by LDS).  Every product coordinate and every
+ * sum follow the one-workgroup kernel's order for block_threads (0 =
+ * automatic, as gb_relax_batch), so the outputs equal gb_relax_batch's bit
+ * for bit.  blocks_per_node * 256 >= block_threads; nodes run in launches of
+ * as many complete groups as are resident.  A node whose group barrier timed
+ * out (not resident together) gets status 0xFF.
+ */
+gb_status gb_relax_batch_multiblock(const gb_objective_desc* objective,
+                                    const gb_algorithm_desc* algorithm,
+                                    const gb_lmo_desc* lmo,
+                                    int N, int max_iters,
+                                    double* x, const double* lb, const double* ub,
+                                    const double* eps, int* iterations, uint8_t* status,
+                                    double* gap, double* value,
+                                    double* best_vertex, double* best_vertex_val,
+                                    const uint8_t* integer_flags, double candidate_tol,
+                                    int block_threads, int blocks_per_node, void* stream);
 gb_status gb_relax_batch_noalloc(const gb_objective_desc* objective,
                                  const gb_algorithm_desc* algorithm,
                                  const gb_lmo_desc* lmo,
@@ -579,6 +601,9 @@ typedef struct {
                                  the time limit is checked between groups of rounds */
     long long device_nodes;   /* node slots of the device-resident search (0 = automatic: a
                                  quarter of free device memory, at most 32 GiB) */
+    int blocks_per_node;      /* plain form: > 1 relaxes every node with that many workgroups
+                                 (MultiBlock, gb_relax_batch_multiblock; dense quadratic over a
+                                 box); 0 or 1: one workgroup per node (SingleBlock) */
 } gb_settings;
 
 /* Result (gpuboscia.cuh:34-42). x is caller-provided, n_vars doubles. */
