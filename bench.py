@@ -31,6 +31,10 @@ statuses (fill kernels) and relaxes the batch again.
              vs batched vs mixed vs mixed with the device-resident search
              (mixed_device) vs learned + batched vs learned + mixed, and the
              geometric-mean speedups over plain per n
+  multiblock  one node of a dense quadratic at n = 8192 .. 65536 (Q up to
+             32 GiB) relaxed by 256 workgroups (MultiBlock): FW iterations/s and
+             the bytes of Q streamed per second against 8 TB/s, with the
+             one-workgroup kernel at n = 8192 for comparison
   time_to_target_n200  n = 200 held-out binary QPs: the time each form
              needs to reach the certified relative gap plain reaches in
              --tt200-seconds (python bench.py --only-tt200 runs only this)
@@ -349,6 +353,78 @@ def hetero_batches(args):
             "value": iters / t, "unit": "FW iterations/s", "nodes_per_s": nodes / t, "converged_nodes": converged}
 
 
+def multiblock_large_n(args):
+    """MultiBlock leg: one node of a dense quadratic over the 0/1 box at n
+    beyond the one-workgroup kernel's LDS limit, relaxed by K = 256
+    workgroups (gb_relax_batch_multiblock), FW open loop for a fixed count of
+    iterations (tolerance 0), so the products are counted exactly: per node
+    2 iterations + 4 passes over Q (gradient and best-vertex value per
+    iteration; final gradient, vertex value, value and rounded value), each
+    reading 8 n^2 bytes of Q.  HIP events on the launch stream time each
+    call.  At n = 8192 the one-workgroup kernel (gb_relax_batch) runs the
+    same node for comparison."""
+    import torch
+    import gpuboscia_amd as gb
+    f64 = dict(dtype=torch.float64, device="cuda")
+    rows = []
+    stream = torch.cuda.Stream()
+    for n, K, iters, nodes in [(8192, 0, 4, 1), (8192, 64, 20, 1), (16384, 256, 20, 1), (16384, 64, 20, 4),
+                               (32768, 256, 10, 1), (65536, 256, 4, 1)]:
+        if n > args.mb_max_n:
+            continue
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(n)
+        with torch.cuda.stream(stream):
+            A = torch.randn(n, n, generator=gen, **f64) / np.sqrt(n)
+            Qt = (A + A.T) * 0.5
+            del A
+            Qt.diagonal().add_(2.0)
+            cvec = torch.rand(n, generator=gen, **f64) * 2 - 1
+        stream.synchronize()
+        obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, Qt.data_ptr(), cvec.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
+        lmo = gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0)
+        alg = gb.algorithm(dicg=False, line_search=gb.LS_OPEN_LOOP)
+        fl = torch.ones(n, dtype=torch.uint8, device="cuda")
+        times = []
+        for rep in range(3):
+            with torch.cuda.stream(stream):
+                x = torch.zeros((nodes, n), **f64)
+                lb, ub = torch.zeros((nodes, n), **f64), torch.ones((nodes, n), **f64)
+                e = torch.zeros(nodes, **f64)
+                it = torch.zeros(nodes, dtype=torch.int32, device="cuda")
+                st = torch.zeros(nodes, dtype=torch.uint8, device="cuda")
+                gap, val = torch.zeros(nodes, **f64), torch.zeros(nodes, **f64)
+                bv, bvv = torch.zeros((nodes, n), **f64), torch.full((nodes,), 1e300, **f64)
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                if K == 0:
+                    gb.relax_batch(obj, alg, lmo, nodes, iters, x, lb, ub, e, it, st, gap, val, bv, bvv, fl, 1e-6,
+                                   256, stream=stream.cuda_stream)
+                else:
+                    gb.relax_batch_multiblock(obj, alg, lmo, nodes, iters, x, lb, ub, e, it, st, gap, val, bv, bvv,
+                                              fl, 1e-6, 256, K, stream=stream.cuda_stream)
+                b.record(stream)
+            stream.synchronize()
+            if rep > 0 or K == 0:
+                times.append(a.elapsed_time(b) / 1e3)
+        assert int(it.sum().item()) == nodes * iters and int((st.cpu() == 0xFF).sum()) == 0
+        t = float(np.mean(times))
+        passes = nodes * (2 * iters + 4)
+        gbs = passes * 8.0 * n * n / t / 1e9
+        row = {"n": n, "nodes": nodes, "blocks_per_node": K, "fw_iterations": nodes * iters, "call_ms": t * 1e3,
+               "fw_iters_per_s": nodes * iters / t, "q_passes": passes, "q_bytes_per_s": gbs * 1e9,
+               "hbm_frac": gbs / 8000.0}
+        rows.append(row)
+        say(f"multiblock n={n} nodes={nodes} K={K}: {t * 1e3:.2f} ms per call, {gbs:.0f} GB/s of Q")
+        del Qt
+        torch.cuda.empty_cache()
+    return {"workload": "dense quadratic (Q = (A + A')/2 + 2 I, A ~ N(0, 1/n)) over the 0/1 box, one node at "
+                        "the root, FW open loop, tolerance 0; K = 0: one workgroup (gb_relax_batch)",
+            "basis": "q_bytes_per_s = passes over Q x 8 n^2 bytes / call time (HIP events on the launch stream); "
+                     "hbm_frac against 8 TB/s; several nodes of one Q share its reads in L2",
+            "rows": rows}
+
+
 def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, rel_gap=1e-6, device=False):
     import torch
     dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
@@ -502,6 +578,9 @@ def main():
     ap.add_argument("--c3-planes", type=int, default=3, choices=(1, 3))
     ap.add_argument("--forms", default="plain,batched,mixed,mixed_fp32,mixed_pass1,mixed_fp32_pass1",
                     help="forms to time (the profiled runs time one)")
+    ap.add_argument("--no-multiblock", action="store_true", help="skip the MultiBlock large-n leg")
+    ap.add_argument("--only-multiblock", action="store_true", help="run only the MultiBlock large-n leg")
+    ap.add_argument("--mb-max-n", type=int, default=65536, help="largest n of the MultiBlock leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -515,6 +594,9 @@ def main():
     gb.load()
     if args.only_tt200:
         print(json.dumps({"time_to_target_n200": time_to_target(args)}), flush=True)
+        return
+    if args.only_multiblock:
+        print(json.dumps({"multiblock": multiblock_large_n(args)}), flush=True)
         return
     n = args.n
     Q, c, sha = instances.binary_qp(n, seed=1)
@@ -591,6 +673,8 @@ def main():
         h = result["heterogeneous_batch"]
         say(f"heterogeneous batch: {h['instances']} instances x {h['nodes_per_instance']} nodes, n {h['n_min']}.."
             f"{h['n_max']}: {h['call_ms']:.1f} ms per call, {h['value']:.4g} FW it/s, {h['nodes_per_s']:.4g} nodes/s")
+    if not args.no_multiblock:
+        result["multiblock"] = multiblock_large_n(args)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, Q, c, LB, UB, EPS)
     if not args.no_tto:

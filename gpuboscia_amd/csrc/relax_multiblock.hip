@@ -31,6 +31,12 @@ namespace gb {
 
 namespace {
 
+// Loads of Q per batch of a product row (two batches in flight).
+constexpr int kMbRowLoads = 16;
+
+// Global-memory view of a pointer, so loads retire in order.
+typedef const __attribute__((address_space(1))) double gdouble;
+
 struct Mb {
     const RelaxParams& P;
     const MbParams& M;
@@ -84,32 +90,51 @@ struct Mb {
     }
 
     // Row j of Q z: the k-ordered FMA chain of the one-workgroup kernel
-    // (z read from global memory instead of LDS: the same values).
-    __device__ double row(const double* z, int j) const {
-        const double* q = P.Qt + j;
-        double acc = 0.0;
-        int k = 0;
-        for (; k + 8 <= n; k += 8) {
-            double qq[8], zz[8];
+    // (z read from global memory instead of LDS: the same values).  A node
+    // has only n such chains, so each thread streams its row in batches of
+    // kMbRowLoads loads of Q and issues the next batch before it consumes
+    // the current one (global loads retire in order, so the FMAs wait only
+    // for their own batch).  The chain's order is unchanged.
+    __device__ static void fetch(gdouble* q, int k, int n, double* out) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                qq[u] = q[(size_t)(k + u) * n];
-                zz[u] = z[k + u];
-            }
+        for (int u = 0; u < kMbRowLoads; ++u) out[u] = q[(size_t)(k + u) * n];
+    }
+    __device__ static double chain(const double* qq, gdouble* z, int k, double acc) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc = fma(qq[u], zz[u], acc);
-        }
-        for (; k < n; ++k) acc = fma(q[(size_t)k * n], z[k], acc);
+        for (int u = 0; u < kMbRowLoads; ++u) acc = fma(qq[u], z[k + u], acc);
         return acc;
+    }
+    __device__ double row(const double* zp, int j) const {
+        gdouble* q = (gdouble*)(P.Qt + j);
+        gdouble* z = (gdouble*)zp;
+        const int full = n / kMbRowLoads;
+        double acc = 0.0, a[kMbRowLoads], b[kMbRowLoads];
+        if (full > 0) fetch(q, 0, n, a);
+        int t = 0;
+        for (; t + 2 <= full; t += 2) {
+            fetch(q, (t + 1) * kMbRowLoads, n, b);
+            acc = chain(a, z, t * kMbRowLoads, acc);
+            if (t + 2 < full) fetch(q, (t + 2) * kMbRowLoads, n, a);
+            acc = chain(b, z, (t + 1) * kMbRowLoads, acc);
+        }
+        if (t < full) acc = chain(a, z, t * kMbRowLoads, acc);
+        for (int k = full * kMbRowLoads; k < n; ++k) acc = fma(q[(size_t)k * n], z[k], acc);
+        return acc;
+    }
+    // Rows go to threads wave by wave across the workgroups (row wave w to
+    // workgroup w mod K), so a product of fewer than G rows still spreads
+    // over every workgroup's CU.  Any row-to-thread map gives the same rows.
+    __device__ int row0() const {
+        return ((int)(threadIdx.x / kWave) * M.K + rank) * kWave + (int)(threadIdx.x & (kWave - 1));
     }
     // out = Q z on this group (z complete on entry); synced on return.
     __device__ void product(const double* z, double* out) {
-        for (int j = gtid; j < n; j += G) out[j] = row(z, j);
+        for (int j = row0(); j < n; j += G) out[j] = row(z, j);
         sync();
     }
     // Obj<GB_OBJ_DENSE_QUADRATIC>::grad
     __device__ void grad(const double* x, double* g) {
-        for (int j = gtid; j < n; j += G) {
+        for (int j = row0(); j < n; j += G) {
             const double acc = row(x, j);
             const double lin = P.lin ? P.ls * P.lin[j] : 0.0;
             g[j] = P.gs * (acc + lin);
