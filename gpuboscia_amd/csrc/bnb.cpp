@@ -926,6 +926,18 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     if (s->form == 2 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 512))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "the mixed-precision form supports dense objectives over a box with n <= 512");
+    // device_resident_solver.cuh:1395-1440 (MultiBlock configuration)
+    if (s->blocks_per_node < 0) return fail(GB_ERR_INVALID_ARGUMENT, "MultiBlock blocks_per_node must be positive");
+    if (s->blocks_per_node > 1) {
+        if (s->form != 0)
+            return fail(GB_ERR_INVALID_ARGUMENT, "MultiBlock relaxes nodes in the plain form (form = 0)");
+        if (o->kind != GB_OBJ_DENSE_QUADRATIC || (l->kind != GB_LMO_BOX && l->kind != GB_LMO_SIMPLEX))
+            return fail(GB_ERR_INVALID_ARGUMENT, "MultiBlock relaxes dense quadratics over a box or a simplex");
+        const int cap = multiblock_capacity(l->kind);
+        if (s->blocks_per_node > kMbMaxK || (cap > 0 && s->blocks_per_node > cap))
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "MultiBlock blocks_per_node exceeds a complete node group of resident workgroups");
+    }
     return GB_OK;
 }
 

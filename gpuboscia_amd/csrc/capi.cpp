@@ -29,11 +29,14 @@ gb_status hip_fail(hipError_t e, const char* file, int line) {
 }
 
 // LMO descriptor checks for n variables (obj_kind: the objective it is
-// paired with; a: the algorithm, may be NULL).
-static gb_status validate_lmo(const gb_lmo_desc* l, int n, int obj_kind, const gb_algorithm_desc* a) {
+// paired with; a: the algorithm, may be NULL; staged: the one-workgroup
+// kernels hold the LMO's vectors in LDS).
+static gb_status validate_lmo(const gb_lmo_desc* l, int n, int obj_kind, const gb_algorithm_desc* a,
+                              bool staged = true) {
     if (l->kind == GB_LMO_SIMPLEX) {
         if (!l->is_int) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO requires is_int");
-        if (n > kMaxSimplexN) return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO n exceeds the LDS limit");
+        if (staged && n > kMaxSimplexN)
+            return fail(GB_ERR_INVALID_ARGUMENT, "simplex LMO n exceeds the LDS limit (MultiBlock takes any n)");
     } else if (l->kind == GB_LMO_BIRKHOFF) {
         if (l->order <= 0 || l->order > kMaxBirkhoffOrder)
             return fail(GB_ERR_INVALID_ARGUMENT, "Birkhoff order must be in [1, 256]");
@@ -67,7 +70,7 @@ gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_de
     } else {
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown objective kind");
     }
-    gb_status st = validate_lmo(l, o->n, o->kind, a);
+    gb_status st = validate_lmo(l, o->n, o->kind, a, staged);
     if (st != GB_OK) return st;
     if (a->algorithm != GB_ALG_FRANK_WOLFE && a->algorithm != GB_ALG_DICG)
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown FW algorithm");
@@ -212,8 +215,8 @@ gb_status relax_batch_multiblock(const gb_objective_desc* o, const gb_algorithm_
                                  int block_threads, int blocks_per_node, void* stream) {
     gb_status st = validate_descriptors(o, a, l, false);
     if (st != GB_OK) return st;
-    if (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX)
-        return fail(GB_ERR_INVALID_ARGUMENT, "MultiBlock relaxes dense quadratics over a box");
+    if (o->kind != GB_OBJ_DENSE_QUADRATIC || (l->kind != GB_LMO_BOX && l->kind != GB_LMO_SIMPLEX))
+        return fail(GB_ERR_INVALID_ARGUMENT, "MultiBlock relaxes dense quadratics over a box or a simplex");
     const int K = blocks_per_node;
     if (K < 1 || K > kMbMaxK) return fail(GB_ERR_INVALID_ARGUMENT, "blocks_per_node must be 1..256");
     Fp64Launch F;
@@ -223,14 +226,17 @@ gb_status relax_batch_multiblock(const gb_objective_desc* o, const gb_algorithm_
     const int Tv = F.threads;  // the one-workgroup launch's block size: the order of every sum
     if (K * kMbThreads < Tv)
         return fail(GB_ERR_INVALID_ARGUMENT, "blocks_per_node * 256 must cover gpu_block_threads");
-    const int cap = multiblock_capacity();
+    const int cap = multiblock_capacity(l->kind);
     const int per_launch = std::min(N, cap / K);
     if (per_launch < 1) return fail(GB_ERR_INVALID_ARGUMENT, "blocks_per_node exceeds the resident workgroups");
     const int n = o->n;
-    const size_t work_b = (size_t)per_launch * 6 * n * sizeof(double);
-    const size_t red_b = (size_t)per_launch * 3 * kMbSlots * sizeof(double);
-    const size_t bar_b = (size_t)per_launch * sizeof(GroupBarrier);
-    const size_t total = work_b + red_b + bar_b + sizeof(BarrierStatus) + 512;
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t work_b = up((size_t)per_launch * 6 * n * sizeof(double));
+    const size_t red_b = up((size_t)per_launch * 3 * kMbSlots * sizeof(double));
+    const size_t lstride = multiblock_lmo_bytes(l->kind, n);
+    const size_t lwork_b = up((size_t)per_launch * lstride);
+    const size_t bar_b = up((size_t)per_launch * sizeof(GroupBarrier));
+    const size_t total = work_b + red_b + lwork_b + bar_b + sizeof(BarrierStatus) + 512;
     double* ws = nullptr;
     st = workspace_get(total, stream, &ws);
     if (st != GB_OK) return st;
@@ -240,9 +246,11 @@ gb_status relax_batch_multiblock(const gb_objective_desc* o, const gb_algorithm_
     M.Tv = Tv;
     M.work = reinterpret_cast<double*>(p);
     M.red = reinterpret_cast<double*>(p + work_b);
-    char* bars = p + ((work_b + red_b + 255) & ~size_t(255));
+    M.lwork = reinterpret_cast<unsigned char*>(p + work_b + red_b);
+    M.lwork_stride = lstride;
+    char* bars = p + work_b + red_b + lwork_b;
     M.bars = reinterpret_cast<GroupBarrier*>(bars);
-    M.bst = reinterpret_cast<BarrierStatus*>(bars + ((bar_b + 255) & ~size_t(255)));
+    M.bst = reinterpret_cast<BarrierStatus*>(bars + bar_b);
     M.spin = 1ull << 22;  // seconds of polling: a wait that long means the group is not resident
     for (int node0 = 0; node0 < N; node0 += per_launch) {
         const int nodes = std::min(per_launch, N - node0);

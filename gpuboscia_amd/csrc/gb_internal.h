@@ -129,12 +129,16 @@ struct MbParams {
     unsigned long long spin;
     double* red;             // [nodes per launch][3][kMbSlots]
     double* work;            // [nodes per launch][6][n]: g, v, a, trial x, trial g, product
+    unsigned char* lwork;    // [nodes per launch][lwork_stride]: the LMO's own vectors
+    size_t lwork_stride;
 };
 constexpr int kMbThreads = 256;
 constexpr int kMbSlots = 16 + 256;   // wave totals (Tv <= 1024), then one slot per workgroup (K <= 256)
 constexpr int kMbMaxK = 256;
 int launch_multiblock_clear(GroupBarrier* bars, int count, BarrierStatus* bst, void* stream);
-int multiblock_capacity();
+// Bytes per node of the LMO's vectors (multiblock_lmo_layout), 0 for a box.
+size_t multiblock_lmo_bytes(int lmo_kind, int n);
+int multiblock_capacity(int lmo_kind);
 int launch_relax_multiblock(const RelaxParams& p, const MbParams& m, int nodes, void* stream);
 
 // Launch the FP64 relaxation kernel (relax_fp64.hip).  Returns a hipError_t

@@ -1,5 +1,5 @@
 // Multi-workgroup relaxation of one node (MultiBlock mode, K > 1) for dense
-// quadratics over a box, gfx950: K co-resident workgroups share a node and
+// quadratics over a box or a simplex, gfx950: K co-resident workgroups share a node and
 // meet at its group barrier (gb_barrier.h), so one node's products stream
 // the whole of Q at the bandwidth of many CUs instead of one, and n is not
 // limited by LDS (multiblock.cuh, docs/multiblock.md).
@@ -174,6 +174,291 @@ struct Mb {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// LMOs on the group.  Each mirrors Lmo<...> of the one-workgroup kernel: the
+// same per-coordinate arithmetic, sums in the canonical order, and the same
+// ranks and scan, so the vertices are the same bits.
+// ---------------------------------------------------------------------------
+template <int LMO>
+struct MbLmo;
+
+// BoxLMO (lmo.cuh:212-321), as Lmo<GB_LMO_BOX>.
+template <>
+struct MbLmo<GB_LMO_BOX> {
+    Mb& c;
+    __device__ MbLmo(Mb& cc, unsigned char*) : c(cc) {}
+    __device__ bool feasible(const double*, const double*) { return true; }
+    __device__ void compute_v(const double* g, const double* lb, const double* ub, double* v) {
+        for (int j = c.gtid; j < c.n; j += c.G) v[j] = (g[j] > 0.0) ? lb[j] : ub[j];
+        c.sync();
+    }
+    // in-face vertex for the direction -g (DICG away vertex)
+    __device__ void inface_neg(const double* g, const double* x, const double* lb, const double* ub, double* a) {
+        const double atol = 1e-6, rtol = 1e-4;
+        for (int j = c.gtid; j < c.n; j += c.G) {
+            const double gj = -g[j];
+            const double ut = atol + rtol * fabs(ub[j]);
+            const double lt = atol + rtol * fabs(lb[j]);
+            if (fabs(x[j] - ub[j]) <= ut) a[j] = ub[j];
+            else if (fabs(x[j] - lb[j]) <= lt) a[j] = lb[j];
+            else a[j] = (gj > 0.0) ? lb[j] : ub[j];
+        }
+        c.sync();
+    }
+    __device__ double max_step(const double* d, const double* x, const double* lb, const double* ub) {
+        double r = 1.0;
+        for (int j = c.gtid; j < c.n; j += c.G) {
+            const double dj = d[j];
+            if (dj < 0.0) r = fmin(r, (ub[j] - x[j]) / -dj);
+            else if (dj > 0.0) r = fmin(r, (x[j] - lb[j]) / dj);
+        }
+        return fmax(0.0, fmin(c.minimum(r), 1.0));
+    }
+    __device__ bool rounded_feasible(const double*) { return true; }
+};
+
+// Per-node vectors of the simplex LMO (S = next_pow2(n) ranks).
+struct MbSimplexWork {
+    int* rank;        // [n] rank of an eligible coordinate by (key, index)
+    int* sidx;        // [S] coordinate at each rank (-1: none)
+    double* scap;     // [S] capacity at each rank
+    double* spre;     // [S] exclusive prefix of the capacities
+    uint8_t* fixed;   // [n] coordinate fixed on the current face
+};
+
+__host__ __device__ inline size_t mb_up256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Byte offsets of the vectors in a node's region: rank, sidx, scap, spre,
+// fixed, and the total.
+__host__ __device__ inline void mb_simplex_offsets(int n, int S, size_t off[6]) {
+    off[0] = 0;
+    off[1] = off[0] + mb_up256((size_t)n * sizeof(int));
+    off[2] = off[1] + mb_up256((size_t)S * sizeof(int));
+    off[3] = off[2] + mb_up256((size_t)S * sizeof(double));
+    off[4] = off[3] + mb_up256((size_t)S * sizeof(double));
+    off[5] = off[4] + mb_up256((size_t)n);
+}
+
+__device__ inline MbSimplexWork mb_simplex_layout(unsigned char* base, int n, int S) {
+    size_t off[6];
+    mb_simplex_offsets(n, S, off);
+    MbSimplexWork w;
+    w.rank = reinterpret_cast<int*>(base + off[0]);
+    w.sidx = reinterpret_cast<int*>(base + off[1]);
+    w.scap = reinterpret_cast<double*>(base + off[2]);
+    w.spre = reinterpret_cast<double*>(base + off[3]);
+    w.fixed = base + off[4];
+    return w;
+}
+
+// Coordinates of one rank tile held in LDS.
+constexpr int kMbRankTile = 1024;
+
+// UnitSimplexLMO / EqualitySimplexLMO (lmo.cuh:335-912), as
+// Lmo<GB_LMO_SIMPLEX>: the greedy allocation ranks the eligible coordinates
+// by (key, index) -- each rank is an integer count of smaller keys, taken
+// over LDS tiles of the keys by every workgroup of the group and added with
+// integer atomics (exact in any order) -- scatters the capacities by rank,
+// scans them in the one-workgroup kernel's chunked order on one wave, and
+// fills every coordinate from its prefix mass.
+template <>
+struct MbLmo<GB_LMO_SIMPLEX> {
+    Mb& c;
+    MbSimplexWork w;
+    int S;
+    __device__ MbLmo(Mb& cc, unsigned char* base) : c(cc) {
+        S = 1;
+        while (S < c.n) S <<= 1;
+        w = mb_simplex_layout(base, c.n, S);
+    }
+    __device__ bool elig(const double* g, double sgn, bool use_fixed, bool force_full, int j) const {
+        return (!use_fixed || !w.fixed[j]) && (force_full || sgn * g[j] <= 0.0);
+    }
+    // out holds its base values on entry (complete); synced on return.
+    __device__ void greedy(const double* g, double sgn, const double* ub, double* out, bool use_fixed,
+                           double remaining, bool force_full, bool floor_int) {
+        const RelaxParams& P = c.P;
+        const int n = c.n, G = c.G, gtid = c.gtid;
+        const uint8_t* is_int = P.is_int;
+        __shared__ double tkey[kMbRankTile];
+        __shared__ uint8_t tel[kMbRankTile];
+        for (int p = gtid; p < S; p += G) { w.sidx[p] = -1; w.scap[p] = 0.0; }
+        for (int j = gtid; j < n; j += G) w.rank[j] = 0;
+        c.sync();
+        // ranks: job (j block of kMbThreads, i tile of kMbRankTile) round-robin over the workgroups
+        const int jb = (n + kMbThreads - 1) / kMbThreads;
+        const int ib = (n + kMbRankTile - 1) / kMbRankTile;
+        for (int job = c.rank; job < jb * ib; job += c.M.K) {
+            const int j0 = (job / ib) * kMbThreads, i0 = (job % ib) * kMbRankTile;
+            const int len = min(kMbRankTile, n - i0);
+            for (int t = threadIdx.x; t < len; t += blockDim.x) {
+                tkey[t] = sgn * g[i0 + t];
+                tel[t] = elig(g, sgn, use_fixed, force_full, i0 + t) ? 1 : 0;
+            }
+            __syncthreads();
+            const int j = j0 + (int)threadIdx.x;
+            if (j < n && elig(g, sgn, use_fixed, force_full, j)) {
+                const double kj = sgn * g[j];
+                int cnt = 0;
+                for (int t = 0; t < len; ++t) {
+                    const double ki = tkey[t];
+                    cnt += (tel[t] && (ki < kj || (ki == kj && i0 + t < j))) ? 1 : 0;
+                }
+                if (cnt) atomicAdd(&w.rank[j], cnt);
+            }
+            __syncthreads();
+        }
+        c.sync();
+        for (int j = gtid; j < n; j += G) {
+            if (!elig(g, sgn, use_fixed, force_full, j)) continue;
+            const int r = w.rank[j];
+            w.sidx[r] = j;
+            w.scap[r] = is_int[j] ? fmax(ub[j] - out[j], 0.0) : remaining;
+        }
+        c.sync();
+        if (gtid < kWave) {
+            const int l = gtid;
+            const int C = (S + kWave - 1) / kWave;
+            double t = 0.0;
+            for (int i = 0; i < C; ++i) {
+                const int p = l * C + i;
+                if (p < S) t = t + w.scap[p];
+            }
+            double inc = t;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const double o = __shfl_up(inc, off, kWave);
+                if (l >= off) inc = inc + o;
+            }
+            double run = __shfl_up(inc, 1, kWave);
+            if (l == 0) run = 0.0;
+            for (int i = 0; i < C; ++i) {
+                const int p = l * C + i;
+                if (p >= S) break;
+                w.spre[p] = run;
+                run = run + w.scap[p];
+            }
+        }
+        c.sync();
+        const bool eq = P.equality != 0;
+        for (int p = gtid; p < S; p += G) {
+            const int j = w.sidx[p];
+            if (j < 0) continue;
+            double add = fmax(remaining - w.spre[p], 0.0);
+            const double cp = is_int[j] ? fmax(ub[j] - out[j], 0.0) : remaining;
+            add = fmin(add, cp);
+            if (floor_int && is_int[j]) add = floor(add + (eq ? 1e-9 : 0.0));
+            out[j] = out[j] + add;
+        }
+        c.sync();
+    }
+    __device__ bool feasible(const double* lb, const double* ub) {
+        const RelaxParams& P = c.P;
+        bool valid = true, cont = false;
+        double mand = 0.0, capsum = 0.0;
+        for (int j = c.gtid; j < c.n; j += c.G) {
+            const bool integer = P.flags ? P.flags[j] != 0 : P.is_int[j] != 0;
+            const double lo = integer ? ceil(fmax(0.0, lb[j])) : fmax(0.0, lb[j]);
+            const double hi = integer ? floor(ub[j]) : ub[j];
+            valid = valid && lo <= hi;
+            cont = cont || !integer;
+        }
+        if (c.gtid < c.M.Tv)
+            for (int j = c.gtid; j < c.n; j += c.M.Tv) {
+                const bool integer = P.flags ? P.flags[j] != 0 : P.is_int[j] != 0;
+                mand = mand + (integer ? ceil(fmax(0.0, lb[j])) : fmax(0.0, lb[j]));
+                capsum = capsum + (integer ? floor(ub[j]) : ub[j]);
+            }
+        const bool iv = c.all(valid);
+        const double ms = c.sum(mand);
+        bool feas = iv && ms <= P.rhs + 1e-9;
+        if (P.equality) {
+            const double cs = c.sum(capsum);
+            const bool hc = !c.all(!cont);
+            feas = feas && cs >= P.rhs - 1e-9 && (hc || fabs(P.rhs - round(P.rhs)) <= 1e-9);
+        }
+        return feas;
+    }
+    __device__ void compute_v(const double* g, const double* lb, const double* ub, double* v) {
+        const RelaxParams& P = c.P;
+        for (int j = c.gtid; j < c.n; j += c.G) v[j] = P.is_int[j] ? lb[j] : 0.0;
+        double mass = 0.0;
+        if (c.gtid < c.M.Tv)
+            for (int j = c.gtid; j < c.n; j += c.M.Tv) mass = mass + (P.is_int[j] ? lb[j] : 0.0);
+        const double total = c.sum(mass);  // its sync also publishes v
+        const bool eq = P.equality != 0;
+        greedy(g, 1.0, ub, v, false, P.rhs - total, eq, eq);
+    }
+    __device__ void inface_neg(const double* g, const double* x, const double* lb, const double* ub, double* a) {
+        const RelaxParams& P = c.P;
+        const double atol = 1e-6, rtol = 1e-4;
+        auto face = [&](int j, uint8_t& fx) {
+            fx = 0;
+            const bool ii = P.is_int[j] != 0;
+            double aj = ii ? lb[j] : 0.0;
+            const double lower = ii ? lb[j] : 0.0;
+            const double ltol = atol + rtol * fmax(fabs(x[j]), fabs(lower));
+            if (fabs(x[j] - lower) <= ltol) { fx = 1; aj = lower; }
+            else if (ii) {
+                const double utol = atol + rtol * fmax(fabs(x[j]), fabs(ub[j]));
+                if (fabs(x[j] - ub[j]) <= utol) { fx = 1; aj = ub[j]; }
+            }
+            return aj;
+        };
+        for (int j = c.gtid; j < c.n; j += c.G) {
+            uint8_t fx;
+            a[j] = face(j, fx);
+            w.fixed[j] = fx;
+        }
+        double mass = 0.0, xmass = 0.0;
+        if (c.gtid < c.M.Tv)
+            for (int j = c.gtid; j < c.n; j += c.M.Tv) {
+                uint8_t fx;
+                mass = mass + face(j, fx);
+                xmass = xmass + x[j];
+            }
+        const double m = c.sum(mass);  // its sync also publishes a and fixed
+        const double xm = c.sum(xmass);
+        const bool force = P.equality || fabs(xm - P.rhs) <= atol + rtol * fmax(fabs(xm), fabs(P.rhs));
+        greedy(g, -1.0, ub, a, true, P.rhs - m, force, true);
+    }
+    __device__ double max_step(const double* d, const double* x, const double* lb, const double* ub) {
+        const RelaxParams& P = c.P;
+        const double tol = 1e-6;
+        double r = 1.0;
+        for (int j = c.gtid; j < c.n; j += c.G) {
+            const double dj = d[j];
+            const bool ii = P.is_int[j] != 0;
+            if (dj > tol) r = fmin(r, (x[j] - (ii ? lb[j] : 0.0)) / dj);
+            else if (dj < -tol) r = fmin(r, ((ii ? ub[j] : P.rhs) - x[j]) / -dj);
+        }
+        const double coord = c.minimum(r);
+        double sx = 0.0, sd = 0.0;
+        if (c.gtid < c.M.Tv)
+            for (int j = c.gtid; j < c.n; j += c.M.Tv) sx = sx + x[j];
+        const double tx = c.sum(sx);
+        if (c.gtid < c.M.Tv)
+            for (int j = c.gtid; j < c.n; j += c.M.Tv) sd = sd + d[j];
+        const double td = c.sum(sd);
+        double agg;
+        if (P.equality && fabs(td) > 1e-9) agg = 0.0;
+        else agg = td < 0.0 ? fmin(1.0, (tx - P.rhs) / td) : 1.0;
+        return fmax(0.0, fmin(coord, agg));
+    }
+    __device__ bool rounded_feasible(const double* x) {
+        const RelaxParams& P = c.P;
+        bool valid = true;
+        for (int j = c.gtid; j < c.n; j += c.G) valid = valid && x[j] >= -P.ctol;
+        double sum = 0.0;
+        if (c.gtid < c.M.Tv)
+            for (int j = c.gtid; j < c.n; j += c.M.Tv) sum = sum + x[j];
+        const bool nn = c.all(valid);
+        const double t = c.sum(sum);
+        return nn && (P.equality ? fabs(t - P.rhs) <= P.ctol : t <= P.rhs + P.ctol);
+    }
+};
+
 __device__ inline double mb_quad_exact(double gap, double curvature, double maximum) {
     if (maximum <= 0.0) return 0.0;
     if (curvature > 0.0) {
@@ -184,6 +469,7 @@ __device__ inline double mb_quad_exact(double gap, double curvature, double maxi
     return -gap + 0.5 * curvature * maximum < 0.0 ? maximum : 0.0;
 }
 
+template <int LMO>
 __global__ void __launch_bounds__(kMbThreads) relax_multiblock_kernel(RelaxParams P, MbParams M) {
     __shared__ int flag;
     __shared__ double sred[kMaxWaves];
@@ -207,14 +493,16 @@ __global__ void __launch_bounds__(kMbThreads) relax_multiblock_kernel(RelaxParam
     double* tx = aw + n;
     double* tg = tx + n;
     double* prod = tg + n;
-    const double atol = 1e-6, rtol = 1e-4;
+    MbLmo<LMO> L(c, M.lwork + (size_t)local * M.lwork_stride);
 
     // node feasibility before any objective access
     bool valid = true;
     for (int j = gtid; j < n; j += G)
         if (isnan(lb[j]) || isnan(ub[j]) || lb[j] > ub[j] || (P.flags && P.flags[j] && ceil(lb[j]) > floor(ub[j])))
             valid = false;
-    if (!c.all(valid)) {
+    bool feasible = c.all(valid);
+    if (feasible) feasible = L.feasible(lb, ub);
+    if (!feasible) {
         if (rank == 0 && threadIdx.x == 0) {
             P.status[node] = c.ok ? GB_NODE_INFEASIBLE : 0xFF;
             P.value[node] = __builtin_inf();
@@ -245,10 +533,7 @@ __global__ void __launch_bounds__(kMbThreads) relax_multiblock_kernel(RelaxParam
         if (fv < 1e299 && fv == best_s)
             for (int j = gtid; j < n; j += G) best_v[j] = vert[j];
     };
-    auto compute_v = [&](const double* gg, double* out) {
-        for (int j = gtid; j < n; j += G) out[j] = (gg[j] > 0.0) ? lb[j] : ub[j];
-        c.sync();
-    };
+    auto compute_v = [&](const double* gg, double* out) { L.compute_v(gg, lb, ub, out); };
 
     // SecantLS::search (algorithms.cuh:121-258), as the one-workgroup kernel
     auto secant = [&](double gap, double max_gamma, const double* dir) {
@@ -324,27 +609,12 @@ __global__ void __launch_bounds__(kMbThreads) relax_multiblock_kernel(RelaxParam
             c.sync();
             continue;
         }
-        // the in-face (away) vertex of -g (Lmo<GB_LMO_BOX>::inface)
-        for (int j = gtid; j < n; j += G) {
-            const double gj = -g[j];
-            const double ut = atol + rtol * fabs(ub[j]);
-            const double lt = atol + rtol * fabs(lb[j]);
-            if (fabs(x[j] - ub[j]) <= ut) aw[j] = ub[j];
-            else if (fabs(x[j] - lb[j]) <= lt) aw[j] = lb[j];
-            else aw[j] = (gj > 0.0) ? lb[j] : ub[j];
-        }
-        c.sync();
+        // the in-face (away) vertex of -g
+        L.inface_neg(g, x, lb, ub, aw);
         track(aw);
         for (int j = gtid; j < n; j += G) aw[j] = aw[j] - v[j];
         c.sync();
-        // Lmo<GB_LMO_BOX>::max_step
-        double r = 1.0;
-        for (int j = gtid; j < n; j += G) {
-            const double dj = aw[j];
-            if (dj < 0.0) r = fmin(r, (ub[j] - x[j]) / -dj);
-            else if (dj > 0.0) r = fmin(r, (x[j] - lb[j]) / dj);
-        }
-        const double gmax = fmax(0.0, fmin(c.minimum(r), 1.0));
+        const double gmax = L.max_step(aw, x, lb, ub);
         if (secant_like) {
             for (int j = gtid; j < n; j += G) {
                 aw[j] = -aw[j];
@@ -383,10 +653,12 @@ __global__ void __launch_bounds__(kMbThreads) relax_multiblock_kernel(RelaxParam
             v[j] = cc;
         }
         c.sync();
-        const double cv = integer_value(v);  // a box accepts every rounded point
-        if (cv < best_s) {
-            for (int j = gtid; j < n; j += G) best_v[j] = v[j];
-            best_s = cv;
+        if (L.rounded_feasible(v)) {
+            const double cv = integer_value(v);
+            if (cv < best_s) {
+                for (int j = gtid; j < n; j += G) best_v[j] = v[j];
+                best_s = cv;
+            }
         }
     }
     bool integral = true;
@@ -424,10 +696,19 @@ int launch_multiblock_clear(GroupBarrier* bars, int count, BarrierStatus* bst, v
 
 // Workgroups of the multi-workgroup kernel that are resident on the current
 // device at once.
-int multiblock_capacity() {
+size_t multiblock_lmo_bytes(int lmo_kind, int n) {
+    if (lmo_kind != GB_LMO_SIMPLEX) return 0;
+    size_t off[6];
+    mb_simplex_offsets(n, next_pow2_host(n), off);
+    return off[5];
+}
+
+int multiblock_capacity(int lmo_kind) {
     int dev = 0, per_cu = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_multiblock_kernel, kMbThreads, 0) != hipSuccess ||
+    const void* k = lmo_kind == GB_LMO_SIMPLEX ? reinterpret_cast<const void*>(relax_multiblock_kernel<GB_LMO_SIMPLEX>)
+                                               : reinterpret_cast<const void*>(relax_multiblock_kernel<GB_LMO_BOX>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kMbThreads, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
     return per_cu * cus;
@@ -437,7 +718,9 @@ int launch_relax_multiblock(const RelaxParams& p, const MbParams& m, int nodes, 
     RelaxParams pp = p;
     MbParams mm = m;
     void* args[] = {&pp, &mm};
-    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(relax_multiblock_kernel),
+    const void* k = p.lmo_kind == GB_LMO_SIMPLEX ? reinterpret_cast<const void*>(relax_multiblock_kernel<GB_LMO_SIMPLEX>)
+                                                 : reinterpret_cast<const void*>(relax_multiblock_kernel<GB_LMO_BOX>);
+    const hipError_t e = hipLaunchCooperativeKernel(k,
                                                     dim3(nodes * m.K), dim3(kMbThreads), args, 0,
                                                     reinterpret_cast<hipStream_t>(stream));
     return (int)e;

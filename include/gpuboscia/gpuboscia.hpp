@@ -214,9 +214,11 @@ using DICGQuadraticExact = DecompositionInvariantFrankWolfe<QuadraticExactLS>;
 
 enum class FWExecutionMode { SingleBlock, MultiBlock };
 
-// device_resident_solver.cuh:1004-1025 (same fields and defaults).  Fields
-// describing CUDA launch policies (paired DICG, MultiBlock) are accepted and
-// validated; the gfx950 kernels choose their own launch shapes.
+// device_resident_solver.cuh:1004-1025 (same fields and defaults).
+// execution_mode = MultiBlock relaxes every node with blocks_per_node
+// workgroups (dense quadratics over a box or a simplex); the paired-DICG
+// fields are accepted and validated, and the gfx950 kernels choose their
+// own launch shapes.
 struct DeviceResidentConfig {
     enum class DICGOracleParallelism { Auto, Serial, Paired };
     int gpu_batch_size = 1024;
@@ -466,7 +468,17 @@ struct DeviceMarks {
     DeviceMarks& operator=(const DeviceMarks&) = delete;
 };
 
+// MultiBlock (device_resident_solver.cuh:1395-1440): blocks_per_node
+// workgroups per node on one stream; the library launches a batch in as many
+// rounds of resident node groups as it needs, so gpu_batch_size is kept.
 inline void fill_config(const DeviceResidentConfig& d, gb_settings& s) {
+    if (d.execution_mode != FWExecutionMode::SingleBlock && d.execution_mode != FWExecutionMode::MultiBlock)
+        throw std::invalid_argument("unknown FW execution mode");
+    if (d.execution_mode == FWExecutionMode::MultiBlock) {
+        if (d.blocks_per_node < 1) throw std::invalid_argument("MultiBlock blocks_per_node must be positive");
+        if (d.dicg_oracle_parallelism == DeviceResidentConfig::DICGOracleParallelism::Paired)
+            throw std::invalid_argument("MultiBlock cannot combine with paired DICG");
+    }
     s.gpu_batch_size = d.gpu_batch_size;
     s.num_streams = d.num_streams;
     s.gpu_block_threads = d.gpu_block_threads;
@@ -477,6 +489,10 @@ inline void fill_config(const DeviceResidentConfig& d, gb_settings& s) {
     s.integer_tol = d.integer_tol;
     s.abs_gap_tol = d.abs_gap_tol;
     s.rel_gap_tol = d.rel_gap_tol;
+    if (d.execution_mode == FWExecutionMode::MultiBlock) {
+        s.blocks_per_node = d.blocks_per_node;
+        s.num_streams = 1;
+    }
 }
 
 }  // namespace detail
@@ -506,8 +522,6 @@ class DeviceResidentHybridSolver {
                                double time_limit_seconds = std::numeric_limits<double>::infinity(),
                                Heuristic heuristic = {})
         : prob_(prob), marks_(lmo.integer_marks(prob.n_vars)) {
-        if (cfg.execution_mode == FWExecutionMode::MultiBlock)
-            throw std::invalid_argument("MultiBlock execution is not available in this build");
         if (!std::isfinite(cfg.dicg_paired_occupancy_fraction) || cfg.dicg_paired_occupancy_fraction <= 0.0
                 || cfg.dicg_paired_occupancy_fraction > 1.0)
             throw std::invalid_argument("invalid tolerance configuration");
@@ -653,6 +667,8 @@ Result solve_with_algorithm(const Problem& problem, DeviceObjective objective, L
     s.algorithm.heuristic_offset = heuristic.offset();
     s.time_limit_seconds = settings.time_limit_seconds;
     s.natural_lower_bound = settings.natural_lower_bound;
+    if (settings.form != 0 && settings.device.execution_mode == FWExecutionMode::MultiBlock)
+        throw std::invalid_argument("MultiBlock relaxes nodes in the plain form (form = 0)");
     s.form = settings.form;
     s.precision = settings.precision;
     s.branch_rule = settings.branch_rule;

@@ -180,10 +180,11 @@ gb_status gb_relax_batch(const gb_objective_desc* objective,
  * device_resident_solver.cuh:1011-1013, multiblock.cuh, docs/multiblock.md):
  * gb_relax_batch with blocks_per_node co-resident workgroups of 256 threads
  * per node meeting at a device-atomic group barrier, for dense quadratics
- * over a box (n not limited by LDS).  Every product coordinate and every
- * sum follow the one-workgroup kernel's order for block_threads (0 =
- * automatic, as gb_relax_batch), so the outputs equal gb_relax_batch's bit
- * for bit.  blocks_per_node * 256 >= block_threads; nodes run in launches of
+ * over a box or a unit / equality simplex (n not limited by LDS; the
+ * simplex ranks are taken over the whole group, lmo.cuh:620-784).  Every
+ * product coordinate, sum, rank and scan follows the one-workgroup kernel's
+ * order for block_threads (0 = automatic, as gb_relax_batch), so the outputs
+ * equal gb_relax_batch's bit for bit where that kernel runs.  blocks_per_node * 256 >= block_threads; nodes run in launches of
  * as many complete groups as are resident.  A node whose group barrier timed
  * out (not resident together) gets status 0xFF.
  */
@@ -603,7 +604,7 @@ typedef struct {
                                  quarter of free device memory, at most 32 GiB) */
     int blocks_per_node;      /* plain form: > 1 relaxes every node with that many workgroups
                                  (MultiBlock, gb_relax_batch_multiblock; dense quadratic over a
-                                 box); 0 or 1: one workgroup per node (SingleBlock) */
+                                 box or a simplex); 0 or 1: one workgroup per node (SingleBlock) */
 } gb_settings;
 
 /* Result (gpuboscia.cuh:34-42). x is caller-provided, n_vars doubles. */

@@ -9,12 +9,16 @@
 //   * Birkhoff DICG finds the target permutation (:968-998);
 //   * secant tolerance stall regression through the solver's last-GPU
 //     accessors (:1133-1205);
-//   * equality-simplex convergence matrix (:1062-1131).
+//   * equality-simplex convergence matrix (:1062-1131);
+//   * MultiBlock configuration errors and the linear-objective solves that
+//     end at the LMO's own minimum, K = 1, 2, 4, 8 (test_multiblock.cu:287-406),
+//     and the same search with K = 3 as with one workgroup per node.
 // Exit code 0 = every check passed.
 #include <gpuboscia/gpuboscia.hpp>
 
 #include <cmath>
 #include <cstdio>
+#include <limits>
 #include <numeric>
 #include <random>
 #include <stdexcept>
@@ -350,6 +354,128 @@ void solve_with_algorithm_and_push_errors() {
     check(threw, "a BranchStrategy exception propagates out of solve()");
 }
 
+
+template <class F>
+void must_throw(F fn, const char* fragment, const std::string& what) {
+    bool ok = false;
+    std::string msg;
+    try {
+        fn();
+    } catch (const std::invalid_argument& e) {
+        msg = e.what();
+        ok = msg.find(fragment) != std::string::npos;
+    }
+    check(ok, what + " (" + (msg.empty() ? std::string("no exception") : msg) + ")");
+}
+
+BranchStrategy no_branch() {
+    return [](const Node&, int&, const std::vector<double>&, const std::vector<int>&) { return std::vector<Node>{}; };
+}
+
+// oracle_solver_integration (test_multiblock.cu:369-406): f(x) = g'x with
+// g_j = j mod 7 - 3 over [0, 1]^N, no integer variables, so the incumbent is
+// the LMO's minimum of g'v.
+template <int N, class LMO>
+void multiblock_linear(int k, const LMO& lmo, const char* name, double expected) {
+    std::vector<double> g(N);
+    for (int j = 0; j < N; ++j) g[j] = double(j % 7) - 3.0;
+    DeviceVec dQ(std::vector<double>((size_t)N * N, 0.0)), dg(g);
+    using Obj = GlobalCachedDenseQuadraticObjective<N>;
+    Obj obj(dQ.p, dg.p, 1.0, 1.0, 0.0);
+    Problem prob{};
+    prob.n_vars = N;
+    prob.lb.assign(N, 0.0);
+    prob.ub.assign(N, 1.0);
+    DeviceResidentConfig cfg;
+    cfg.execution_mode = FWExecutionMode::MultiBlock;
+    cfg.blocks_per_node = k;
+    cfg.gpu_batch_size = 4;
+    cfg.gpu_max_iters = 10;
+    cfg.gpu_eps = cfg.gpu_eps_min = 1e-8;
+    DeviceResidentHybridSolver<Obj, FrankWolfeOpenLoop, LMO> solver(prob, obj, {}, no_branch(), cfg, lmo);
+    solver.push(Node(0, -1, 0, prob.lb, prob.ub));
+    solver.solve();
+    check(std::abs(solver.incumbent_value() - expected) <= 1e-8 && solver.effective_num_streams() == 1,
+          std::string("MultiBlock K=") + std::to_string(k) + " " + name + " n=" + std::to_string(N)
+              + ": optimum " + std::to_string(solver.incumbent_value()));
+}
+
+void multiblock_solver() {
+    std::printf("\n[MultiBlock solver (test_multiblock.cu:287-406)]\n");
+    constexpr int n = 16;
+    DeviceVec dQ(identity(n)), dc(std::vector<double>(n, -0.5));
+    using Obj = GlobalCachedDenseQuadraticObjective<n>;
+    Obj obj(dQ.p, dc.p);
+    Problem prob = make_problem(n, std::vector<double>(n, 0.0), std::vector<double>(n, 1.0));
+    DeviceResidentConfig cfg;
+    cfg.execution_mode = FWExecutionMode::MultiBlock;
+    cfg.blocks_per_node = 2;
+    auto construct = [&](DeviceResidentConfig c) {
+        DeviceResidentHybridSolver<Obj, FrankWolfeSecant> solver(prob, obj, {}, no_branch(), c);
+    };
+    auto bad = cfg;
+    bad.blocks_per_node = 0;
+    must_throw([&] { construct(bad); }, "positive", "blocks_per_node = 0 is rejected");
+    bad = cfg;
+    bad.blocks_per_node = -2;
+    must_throw([&] { construct(bad); }, "positive", "blocks_per_node = -2 is rejected");
+    bad = cfg;
+    bad.execution_mode = static_cast<FWExecutionMode>(99);
+    must_throw([&] { construct(bad); }, "unknown FW execution mode", "an unknown execution mode is rejected");
+    bad = cfg;
+    bad.dicg_oracle_parallelism = DeviceResidentConfig::DICGOracleParallelism::Paired;
+    must_throw([&] { construct(bad); }, "cannot combine", "paired DICG with MultiBlock is rejected");
+    bad = cfg;
+    bad.blocks_per_node = std::numeric_limits<int>::max();
+    must_throw([&] { construct(bad); }, "complete node group", "a group larger than the resident workgroups is rejected");
+    bad = cfg;
+    bad.gpu_block_threads = 33;
+    must_throw([&] { construct(bad); }, "multiple of 64", "gpu_block_threads = 33 is rejected (wave64)");
+    for (int k : {1, 2, 4, 8}) {
+        multiblock_linear<16>(k, UnitSimplexLMO<16>{}, "unit simplex", -3.0);
+        multiblock_linear<16>(k, EqualitySimplexLMO<16>{}, "equality simplex", -3.0);
+        multiblock_linear<257>(k, UnitSimplexLMO<257>{}, "unit simplex", -3.0);
+        multiblock_linear<257>(k, EqualitySimplexLMO<257>{}, "equality simplex", -3.0);
+        multiblock_linear<257>(k, BoxLMO{}, "box", -3.0 * 37 - 2.0 * 37 - 1.0 * 37);
+    }
+    // the same search, node for node, with K = 3 as with one workgroup per node
+    constexpr int m = 30;
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<double> A((size_t)m * m), Q((size_t)m * m, 0.0), c(m);
+    for (double& a : A) a = nd(rng);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            double acc = 0.0;
+            for (int t = 0; t < m; ++t) acc += A[(size_t)t * m + i] * A[(size_t)t * m + j];
+            Q[(size_t)i * m + j] = acc / m + (i == j ? 0.1 : 0.0);
+        }
+    for (int j = 0; j < m; ++j) c[j] = nd(rng);
+    DeviceVec dQm(Q), dcm(c);
+    using ObjM = GlobalCachedDenseQuadraticObjective<m>;
+    ObjM objm(dQm.p, dcm.p, 2.0, 0.5, 0.0);
+    Problem pm = make_problem(m, std::vector<double>(m, 0.0), std::vector<double>(m, 1.0));
+    double val[2], bound[2];
+    int nodes[2];
+    std::vector<double> xs[2];
+    for (int r = 0; r < 2; ++r) {
+        DeviceResidentConfig dc = device_cfg(64, 1);
+        dc.abs_gap_tol = dc.rel_gap_tol = 1e-6;
+        if (r == 1) {
+            dc.execution_mode = FWExecutionMode::MultiBlock;
+            dc.blocks_per_node = 3;
+        }
+        DeviceResidentHybridSolver<ObjM, DICGPolicy> solver(pm, objm, {}, pow2_branch(), dc);
+        solver.push(Node(0, -1, 0, pm.lb, pm.ub));
+        solver.solve();
+        val[r] = solver.incumbent_value();
+        bound[r] = solver.dual_bound();
+        nodes[r] = solver.nodes_processed();
+        xs[r] = solver.incumbent_x();
+    }
+    check(val[0] == val[1] && bound[0] == bound[1] && nodes[0] == nodes[1] && xs[0] == xs[1],
+          "MultiBlock K=3 search equals the one-workgroup search (" + std::to_string(nodes[0]) + " nodes)");
+}
 }  // namespace
 
 int main() {
@@ -361,6 +487,7 @@ int main() {
         secant_stall();
         convergence_matrix();
         solve_with_algorithm_and_push_errors();
+        multiblock_solver();
     } catch (const std::exception& e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 1;

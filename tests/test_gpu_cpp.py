@@ -31,3 +31,4 @@ def test_cpp_reference_solver_kats():
     assert out.returncode == 0, out.stdout + out.stderr
     assert "FAIL" not in out.stdout
     assert "0 failure(s)" in out.stdout
+    assert "PASS: MultiBlock K=3 search equals the one-workgroup search" in out.stdout

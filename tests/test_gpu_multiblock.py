@@ -2,9 +2,11 @@
 meeting at the device-atomic group barrier): the same relaxation as the
 one-workgroup kernel, so every output must equal gb_relax_batch's bit for
 bit for any K, over every step rule of FW and DICG, with branched nodes, a
-warm start and a continuation.  Beyond the one-workgroup kernel's LDS limit
-(n > 8192) the oracle is the reference; and gb_solve with blocks_per_node
-must find the same tree as without it."""
+warm start and a continuation, over a box and over unit and equality
+simplices (integer and continuous coordinates, fixed faces, an infeasible
+node).  Beyond the one-workgroup kernel's LDS limit (n > 8192 over a box,
+n > about 4000 over a simplex) the oracle is the reference; and gb_solve
+with blocks_per_node must find the same tree as without it."""
 import sys
 
 import numpy as np
@@ -44,7 +46,7 @@ def nodes(n, N, seed):
     return LB, UB, X0, ST, IT
 
 
-def run(prob, alg, LB, UB, X0, ST, IT, eps, max_iters, T, K):
+def run(prob, alg, LB, UB, X0, ST, IT, eps, max_iters, T, K, flags=None):
     import torch
     N, n = LB.shape
     x, lb, ub, e = dev(X0), dev(LB), dev(UB), dev(np.full(N, eps))
@@ -52,7 +54,7 @@ def run(prob, alg, LB, UB, X0, ST, IT, eps, max_iters, T, K):
     f64 = dict(dtype=torch.float64, device="cuda")
     gap, val = torch.zeros(N, **f64), torch.zeros(N, **f64)
     bv, bvv = torch.zeros((N, n), **f64), torch.full((N,), 1e300, **f64)
-    fl = dev(np.ones(n, np.uint8))
+    fl = dev(np.ones(n, np.uint8) if flags is None else flags)
     if K == 0:
         gb.relax_batch(prob.gobj, alg_gpu(alg), prob.glmo, N, max_iters, x, lb, ub, e, it, st, gap, val, bv, bvv,
                        fl, 1e-6, T)
@@ -130,3 +132,61 @@ def test_solve_with_blocks_per_node_is_the_same_search():
     for key in ("objective", "dual_bound", "nodes_processed", "fw_iterations"):
         assert a[key] == b[key], (key, a[key], b[key])
     assert a["x"] == b["x"]
+
+
+def simplex_problem(n, equality, seed):
+    """Mixed integer/continuous simplex nodes after the simplex parity test:
+    integer coordinates up to 3, continuous up to 5, rhs 4, branched nodes
+    (upper bounds 0/1 on integers, a lower bound 1 on one integer), and one
+    node whose fixed mass exceeds rhs (infeasible)."""
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n // 2, n))
+    Q = M.T @ M / n + 0.05 * np.eye(n)
+    c = rng.uniform(-1, 1, n)
+    is_int = (rng.random(n) < 0.5).astype(np.uint8)
+    N = 7
+    LB = np.zeros((N, n))
+    UB = np.where(is_int, 3.0, 5.0) * np.ones((N, n))
+    ints = np.flatnonzero(is_int)
+    for i in range(1, N - 1):
+        j = rng.integers(0, n, 3)
+        UB[i, j] = np.where(is_int[j], rng.integers(0, 2, 3), UB[i, j])
+    LB[N - 2, ints[0]] = 1.0
+    LB[N - 1, ints[:5]] = 1.0  # fixed mass 5 > rhs 4
+    prob = Problem("dense", Q, c, lmo="simplex", rhs=4.0, equality=equality, is_int=is_int)
+    return prob, LB, UB, is_int
+
+
+@pytest.mark.parametrize("n", [100, 700, 2000])
+@pytest.mark.parametrize("equality", [False, True])
+@pytest.mark.parametrize("dicg,ls", [(True, gb.LS_SECANT), (True, gb.LS_OPEN_LOOP), (False, gb.LS_QUADRATIC_EXACT)])
+def test_multiblock_simplex_equals_one_workgroup(n, equality, dicg, ls):
+    prob, LB, UB, is_int = simplex_problem(n, equality, seed=n + equality)
+    N = LB.shape[0]
+    X0 = LB.copy()
+    ST, IT = np.zeros(N, np.uint8), np.zeros(N, np.int32)
+    alg = {"dicg": dicg, "ls": ls}
+    T = auto_T(n)
+    say(f"[multiblock] simplex n={n} eq={equality} dicg={dicg} ls={ls} one workgroup")
+    ref = run(prob, alg, LB, UB, X0, ST, IT, 1e-6, 30, T, 0, flags=is_int)
+    assert ref["status"][-1] == gb.NODE_INFEASIBLE
+    for K in (1, 3, 8):
+        say(f"[multiblock] simplex n={n} eq={equality} dicg={dicg} ls={ls} K={K}")
+        got = run(prob, alg, LB, UB, X0, ST, IT, 1e-6, 30, T, K, flags=is_int)
+        assert_bit_equal(got, ref, f"simplex n={n} K={K}")
+
+
+@pytest.mark.parametrize("equality", [False, True])
+def test_multiblock_simplex_beyond_lds_matches_oracle(equality):
+    """n = 9000 over a simplex (past the one-workgroup kernel's LDS-staged
+    ranks): the oracle is the reference."""
+    n = 9000
+    prob, LB, UB, is_int = simplex_problem(n, equality, seed=17 + equality)
+    N = LB.shape[0]
+    X0 = LB.copy()
+    ST, IT = np.zeros(N, np.uint8), np.zeros(N, np.int32)
+    alg = {"dicg": True, "ls": gb.LS_QUADRATIC_EXACT}
+    say(f"[multiblock] simplex n=9000 eq={equality} K=32 against the oracle")
+    got = run(prob, alg, LB, UB, X0, ST, IT, 1e-6, 6, 256, 32, flags=is_int)
+    o = prob.run_orc(alg, X0, LB, UB, np.full(N, 1e-6), IT, ST, is_int, 6, T=256, threads=8)
+    assert_bit_equal(got, o, f"simplex n=9000 eq={equality}")
