@@ -1486,6 +1486,13 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             GB_HIP(hipStreamSynchronize(db.stream));
             relax_s += std::chrono::duration<double>(Clock::now() - w0).count();
         }
+        // status 0xFF: a MultiBlock group whose workgroups were not resident
+        // together gave up at its barrier, so the node was not relaxed
+        for (int i = 0; i < N && s->form == 0 && s->blocks_per_node > 1; ++i)
+            if (db.hst[i] > GB_NODE_INTEGRAL_CONVERGED)
+                return fail(GB_ERR_DEVICE, "a node's relaxation did not complete (status "
+                                               + std::to_string(db.hst[i])
+                                               + "): its MultiBlock workgroups were not resident together");
         emit(GB_SCHED_COMPLETION_DETACHED, head, N);
         for (int sl : gathered[head]) pool.release(sl);
         gathered[head].clear();

@@ -252,11 +252,20 @@ gb_status relax_batch_multiblock(const gb_objective_desc* o, const gb_algorithm_
     M.bars = reinterpret_cast<GroupBarrier*>(bars);
     M.bst = reinterpret_cast<BarrierStatus*>(bars + bar_b);
     M.spin = 1ull << 22;  // seconds of polling: a wait that long means the group is not resident
+    {
+        // the clear kernel's status is read with hipGetLastError, so an error
+        // left by an earlier call would be reported as that launch's
+        const hipError_t pending = hipGetLastError();
+        if (pending != hipSuccess)
+            return fail(GB_ERR_DEVICE, std::string("HIP error pending before the MultiBlock launch: ")
+                                           + hipGetErrorString(pending));
+    }
     for (int node0 = 0; node0 < N; node0 += per_launch) {
         const int nodes = std::min(per_launch, N - node0);
         M.node0 = node0;
         int e = launch_multiblock_clear(M.bars, nodes, M.bst, stream);
-        if (e == 0) e = launch_relax_multiblock(F.P, M, nodes, stream);
+        if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+        e = launch_relax_multiblock(F.P, M, nodes, stream);
         if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
     }
     return GB_OK;

@@ -720,9 +720,13 @@ int launch_relax_multiblock(const RelaxParams& p, const MbParams& m, int nodes, 
     void* args[] = {&pp, &mm};
     const void* k = p.lmo_kind == GB_LMO_SIMPLEX ? reinterpret_cast<const void*>(relax_multiblock_kernel<GB_LMO_SIMPLEX>)
                                                  : reinterpret_cast<const void*>(relax_multiblock_kernel<GB_LMO_BOX>);
-    const hipError_t e = hipLaunchCooperativeKernel(k,
-                                                    dim3(nodes * m.K), dim3(kMbThreads), args, 0,
-                                                    reinterpret_cast<hipStream_t>(stream));
+    // A plain launch: the caller sizes it to the resident capacity, and the
+    // group barrier's bounded waits end a group that is not resident
+    // together (status 0xFF) instead of hanging.  hipLaunchCooperativeKernel
+    // was refused (hipErrorIllegalState) in a process that had run many
+    // searches on streams with hardware queues of their own.
+    const hipError_t e = hipLaunchKernel(k, dim3(nodes * m.K), dim3(kMbThreads), args, 0,
+                                         reinterpret_cast<hipStream_t>(stream));
     return (int)e;
 }
 
