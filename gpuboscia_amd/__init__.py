@@ -130,7 +130,7 @@ class Settings(ctypes.Structure):
                 ("precision", ctypes.c_int), ("form", ctypes.c_int), ("learned_model", ctypes.c_void_p),
                 ("max_nodes", ctypes.c_longlong), ("autotune", ctypes.c_int),
                 ("device_frontier", ctypes.c_int), ("device_nodes", ctypes.c_longlong),
-                ("blocks_per_node", ctypes.c_int)]
+                ("blocks_per_node", ctypes.c_int), ("plugin", ctypes.c_void_p)]
 
 
 class Result(ctypes.Structure):
@@ -154,7 +154,8 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_relax_batch_noalloc", "gb_relax_batch_multiblock", "gb_lmo_compute_v", "gb_mixed_warm_products",
             "gb_children_push", "gb_solver_create", "gb_solver_destroy", "gb_solver_push",
             "gb_solver_set_scheduling_callback", "gb_solver_set_progress_callback", "gb_solver_solve",
-            "gb_solver_get_stats", "gb_solver_incumbent_x", "gb_candidate_features"]
+            "gb_solver_get_stats", "gb_solver_incumbent_x", "gb_candidate_features",
+            "gb_relax_batch_plugin"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 

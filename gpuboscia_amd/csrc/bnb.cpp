@@ -926,6 +926,11 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     if (s->form == 2 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 512))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "the mixed-precision form supports dense objectives over a box with n <= 512");
+    if (s->plugin && (s->form != 0 || s->blocks_per_node > 1 || s->device_frontier))
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "device plug-ins run in the plain form with one workgroup per node (form = 0, SingleBlock)");
+    if (s->plugin && s->branch_rule == GB_BRANCH_LEARNED && s->learned_model)
+        return fail(GB_ERR_INVALID_ARGUMENT, "the learned scorer needs a dense quadratic objective");
     // device_resident_solver.cuh:1395-1440 (MultiBlock configuration)
     if (s->blocks_per_node < 0) return fail(GB_ERR_INVALID_ARGUMENT, "MultiBlock blocks_per_node must be positive");
     if (s->blocks_per_node > 1) {
@@ -956,6 +961,12 @@ gb_status validate_root(const gb_problem* pr, const gb_lmo_desc* l, const gb_set
     bool feasible;
     if (s->root_validator) {
         feasible = s->root_validator(s->root_validator_user, x, pr->lb, pr->ub, n, tol) != 0;
+    } else if (l->kind == GB_LMO_PLUGIN) {
+        if (!s->plugin || !s->plugin->relaxation_feasible)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "root initial point requires LMO::relaxation_feasible_cpu or "
+                        "Settings::root_initial_point_feasible for custom constraints");
+        feasible = s->plugin->relaxation_feasible(s->plugin->ctx, x, pr->lb, pr->ub, n, tol) != 0;
     } else {
         feasible = gb_lmo_relaxation_feasible_cpu(l, nullptr, x, pr->lb, pr->ub, n, tol) != 0;
     }
@@ -1045,7 +1056,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                      const gb_settings* s, gb_result* res, Labeler* lab = nullptr, SolveHooks* hk = nullptr) {
     gb_status st = validate_settings(pr, o, l, s);
     if (st != GB_OK) return st;
-    st = validate_descriptors(o, &s->algorithm, l, !(s->form == 0 && s->blocks_per_node > 1));
+    st = validate_descriptors(o, &s->algorithm, l, !(s->form == 0 && s->blocks_per_node > 1), s->plugin);
     if (st != GB_OK) return st;
     if (!(hk && hk->nodes)) {
         st = validate_root(pr, l, s);
@@ -1401,6 +1412,11 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             st = gb_relax_batch_mixed(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb,
                                       db.ub, db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv,
                                       db.flags, s->integer_tol, db.cut, &mo, db.stream);
+        } else if (s->plugin) {
+            // the caller's own objective / LMO / heuristic types in the same kernel
+            st = relax_batch_plugin(o, &s->algorithm, l, s->plugin, N, s->gpu_max_iters, db.x, db.lb, db.ub,
+                                    db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
+                                    s->integer_tol, s->gpu_block_threads, db.stream);
         } else if (s->blocks_per_node > 1) {
             // MultiBlock (FWExecutionMode::MultiBlock): the same relaxation,
             // bit for bit, with blocks_per_node workgroups per node
@@ -1889,7 +1905,7 @@ extern "C" gb_status gb_solver_create(const gb_problem* problem, const gb_object
     try {
         gb_status st = gb::validate_settings(problem, objective, lmo, settings);
         if (st != GB_OK) return st;
-        st = gb::validate_descriptors(objective, &settings->algorithm, lmo);
+        st = gb::validate_descriptors(objective, &settings->algorithm, lmo, true, settings->plugin);
         if (st != GB_OK) return st;
         auto* sv = new gb_solver();
         const int n = problem->n_vars;

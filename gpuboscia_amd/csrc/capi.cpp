@@ -49,6 +49,8 @@ static gb_status validate_lmo(const gb_lmo_desc* l, int n, int obj_kind, const g
         if (a && a->algorithm == GB_ALG_DICG)
             return fail(GB_ERR_INVALID_ARGUMENT,
                         "the portfolio LMO has no in-face oracle: use the Frank-Wolfe algorithm");
+    } else if (l->kind == GB_LMO_PLUGIN) {
+        // checked against the plug-in in validate_descriptors
     } else if (l->kind != GB_LMO_BOX) {
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown LMO kind");
     }
@@ -56,10 +58,29 @@ static gb_status validate_lmo(const gb_lmo_desc* l, int n, int obj_kind, const g
 }
 
 gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
-                               const gb_lmo_desc* l, bool staged) {
+                               const gb_lmo_desc* l, bool staged, const gb_plugin_desc* plugin) {
     if (!o || !a || !l) return fail(GB_ERR_INVALID_ARGUMENT, "null descriptor");
     if (o->n <= 0) return fail(GB_ERR_INVALID_ARGUMENT, "objective n must be positive");
-    if (o->kind == GB_OBJ_DENSE_QUADRATIC) {
+    // Device plug-ins (the reference's compile-time checks, gpuboscia.cuh:135-149)
+    const bool pobj = o->kind == GB_OBJ_PLUGIN, plmo = l->kind == GB_LMO_PLUGIN,
+               pheur = a->heuristic == GB_HEUR_PLUGIN;
+    if ((pobj || plmo || pheur) && !plugin)
+        return fail(GB_ERR_INVALID_ARGUMENT, "a plug-in objective, LMO or heuristic needs a gb_plugin_desc");
+    if (plugin) {
+        if (!plugin->launch) return fail(GB_ERR_INVALID_ARGUMENT, "plug-in without a launcher");
+        if (pobj != (plugin->objective != 0) || plmo != (plugin->lmo != 0) || pheur != (plugin->heuristic != 0))
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "plug-in flags must match the GB_OBJ_PLUGIN / GB_LMO_PLUGIN / GB_HEUR_PLUGIN kinds");
+        if (a->algorithm == GB_ALG_DICG && plmo && !plugin->supports_dicg)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "DICG requires compute_inface_v, dicg_maximum_step_local and dicg_sum_maximum_step");
+        if (a->line_search == GB_LS_QUADRATIC_EXACT && pobj && !plugin->supports_quadratic_exact)
+            return fail(GB_ERR_INVALID_ARGUMENT, "QuadraticExactLS requires a quadratic objective");
+        if (plugin->block_threads < 0) return fail(GB_ERR_INVALID_ARGUMENT, "plug-in block_threads must be >= 0");
+    }
+    if (pobj) {
+        // n only; the device computations are the plug-in's
+    } else if (o->kind == GB_OBJ_DENSE_QUADRATIC) {
         if (!o->Qt) return fail(GB_ERR_INVALID_ARGUMENT, "dense objective requires Qt");
         if (staged && o->n > kMaxDenseN)
             return fail(GB_ERR_INVALID_ARGUMENT,
@@ -83,7 +104,7 @@ gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_de
         return fail(GB_ERR_INVALID_ARGUMENT, "scaled open-loop is available only for DICG");
     if (a->line_search == GB_LS_SECANT && a->secant_iters < 0)
         return fail(GB_ERR_INVALID_ARGUMENT, "secant iterations must be non-negative");
-    if (a->heuristic != GB_HEUR_ROUND && a->heuristic != GB_HEUR_FLOOR_OFFSET)
+    if (a->heuristic != GB_HEUR_ROUND && a->heuristic != GB_HEUR_FLOOR_OFFSET && a->heuristic != GB_HEUR_PLUGIN)
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown heuristic");
     return GB_OK;
 }
@@ -107,15 +128,20 @@ static gb_status relax_fp64_setup(const gb_objective_desc* o, const gb_algorithm
                                   const double* ub, const double* eps, int* iterations, uint8_t* status,
                                   double* gap, double* value, double* best_vertex, double* best_vertex_val,
                                   const uint8_t* flags, double ctol, int block_threads, Fp64Launch* out,
-                                  bool lds = true) {
-    gb_status st = validate_descriptors(o, a, l, lds);
+                                  bool lds = true, const gb_plugin_desc* plugin = nullptr) {
+    gb_status st = validate_descriptors(o, a, l, lds, plugin);
     if (st != GB_OK) return st;
     if (N < 0 || max_iters < 0) return fail(GB_ERR_INVALID_ARGUMENT, "N and max_iters must be non-negative");
     if (N == 0) return GB_OK;
     if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
             || !best_vertex_val)
         return fail(GB_ERR_INVALID_ARGUMENT, "null node buffer");
-    if (block_threads == 0) block_threads = auto_block_threads(o->n);
+    if (block_threads == 0) {
+        block_threads = auto_block_threads(o->n);
+        // a plug-in objective's block_threads bound (device_resident_solver.cuh:1366-1393)
+        if (plugin && plugin->objective && plugin->block_threads > 0)
+            block_threads = std::min(block_threads, std::max(64, plugin->block_threads / 64 * 64));
+    }
     if (block_threads < 64 || block_threads > 1024 || block_threads % 64 != 0)
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "gpu_block_threads must be zero or a positive multiple of 64 up to 1024");
@@ -125,6 +151,11 @@ static gb_status relax_fp64_setup(const gb_objective_desc* o, const gb_algorithm
     out->threads = block_threads;
     out->work_bytes = (size_t)N * kWorkVectors * n * sizeof(double);
     RelaxParams& P = out->P;
+    if (plugin && plugin->objective && plugin->objective_bytes > 0) {
+        // the plug-in objective's workspace follows the work vectors
+        P.obj_stride = (plugin->objective_bytes + 255) & ~size_t(255);
+        out->work_bytes = ((out->work_bytes + 255) & ~size_t(255)) + (size_t)N * P.obj_stride;
+    }
     P.obj_kind = o->kind;
     P.n = n;
     P.Qt = o->Qt;
@@ -178,6 +209,30 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
     st = workspace_get(F.work_bytes, stream, &F.P.work);
     if (st != GB_OK) return st;
     const int e = launch_relax_fp64(F.P, F.threads, F.L.total, F.L, stream);
+    if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
+    return GB_OK;
+}
+
+// Device plug-ins: the batch is validated and laid out as for the built-in
+// kernel, and the caller's launcher runs its own instantiation of it.
+gb_status relax_batch_plugin(const gb_objective_desc* o, const gb_algorithm_desc* a, const gb_lmo_desc* l,
+                             const gb_plugin_desc* plugin, int N, int max_iters, double* x, const double* lb,
+                             const double* ub, const double* eps, int* iterations, uint8_t* status, double* gap,
+                             double* value, double* best_vertex, double* best_vertex_val, const uint8_t* flags,
+                             double ctol, int block_threads, void* stream) {
+    if (!plugin) return fail(GB_ERR_INVALID_ARGUMENT, "null plug-in descriptor");
+    Fp64Launch F;
+    gb_status st = relax_fp64_setup(o, a, l, N, max_iters, x, lb, ub, eps, iterations, status, gap, value,
+                                    best_vertex, best_vertex_val, flags, ctol, block_threads, &F, true, plugin);
+    if (st != GB_OK || N == 0) return st;
+    st = workspace_get(F.work_bytes, stream, &F.P.work);
+    if (st != GB_OK) return st;
+    if (F.P.obj_stride) {
+        const size_t wb = ((size_t)N * kWorkVectors * o->n * sizeof(double) + 255) & ~size_t(255);
+        F.P.obj_work = reinterpret_cast<unsigned char*>(F.P.work) + wb;
+    }
+    PluginLaunch pl{F.P, F.L, F.threads, F.L.total};
+    const int e = plugin->launch(plugin->ctx, &pl, stream);
     if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
     return GB_OK;
 }
@@ -444,6 +499,23 @@ extern "C" gb_status gb_relax_graph_launch(gb_relax_graph* g, void* stream) {
     if (!g || !g->exec) return fail(GB_ERR_INVALID_ARGUMENT, "null graph handle");
     GB_HIP(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)));
     return GB_OK;
+}
+
+extern "C" gb_status gb_relax_batch_plugin(const gb_objective_desc* objective,
+                                           const gb_algorithm_desc* algorithm, const gb_lmo_desc* lmo,
+                                           const gb_plugin_desc* plugin, int N, int max_iters, double* x,
+                                           const double* lb, const double* ub, const double* eps,
+                                           int* iterations, uint8_t* status, double* gap, double* value,
+                                           double* best_vertex, double* best_vertex_val,
+                                           const uint8_t* integer_flags, double candidate_tol,
+                                           int block_threads, void* stream) {
+    try {
+        return relax_batch_plugin(objective, algorithm, lmo, plugin, N, max_iters, x, lb, ub, eps, iterations,
+                                  status, gap, value, best_vertex, best_vertex_val, integer_flags,
+                                  candidate_tol, block_threads, stream);
+    } catch (const std::exception& ex) {
+        return fail(GB_ERR_RUNTIME, ex.what());
+    }
 }
 
 extern "C" gb_status gb_relax_batch(const gb_objective_desc* objective,

@@ -24,7 +24,15 @@ gb_status workspace_get(size_t bytes, void* stream, double** out);
 // staged: the relaxation stages a dense objective's vector in LDS (n <=
 // kMaxDenseN); MultiBlock reads it from global memory (any n).
 gb_status validate_descriptors(const gb_objective_desc* o, const gb_algorithm_desc* a,
-                               const gb_lmo_desc* l, bool staged = true);
+                               const gb_lmo_desc* l, bool staged = true,
+                               const gb_plugin_desc* plugin = nullptr);
+// gb_relax_batch_plugin: the caller's launcher runs its instantiation of the
+// FP64 kernel on a batch validated and laid out here.
+gb_status relax_batch_plugin(const gb_objective_desc* o, const gb_algorithm_desc* a, const gb_lmo_desc* l,
+                             const gb_plugin_desc* plugin, int N, int max_iters, double* x, const double* lb,
+                             const double* ub, const double* eps, int* iterations, uint8_t* status, double* gap,
+                             double* value, double* best_vertex, double* best_vertex_val, const uint8_t* flags,
+                             double ctol, int block_threads, void* stream);
 int auto_block_threads(int n);
 gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* a,
                            const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,

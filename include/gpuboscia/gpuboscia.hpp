@@ -33,9 +33,14 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../gpuboscia_amd.h"
+#if defined(__HIPCC__)
+#include "device/relax_fp64.hpp"
+#endif
 
 // ---- problem.h / node.h / branching.h / line_search_options.h ---------------
 struct Problem {
@@ -240,6 +245,7 @@ struct DeviceResidentConfig {
 // ---- LMOs (lmo.cuh) ----------------------------------------------------------
 struct BoxLMO {
     static constexpr bool supports_dicg = true;
+    static constexpr int device_kind = GB_LMO_BOX;
     bool relaxation_feasible_cpu(const std::vector<double>&, const std::vector<double>&,
                                  const std::vector<double>&, double) const {
         return true;
@@ -251,6 +257,7 @@ struct BoxLMO {
 template <int MaxN, bool RequireEquality = false>
 struct UnitSimplexLMO {
     static constexpr bool supports_dicg = true;
+    static constexpr int device_kind = GB_LMO_SIMPLEX;
     double right_side = 1.0;
     unsigned char is_int[MaxN]{};
     UnitSimplexLMO() = default;
@@ -290,6 +297,7 @@ template <int Order>
 struct BirkhoffLMO {
     static_assert(Order > 0, "Birkhoff order must be positive");
     static constexpr bool supports_dicg = true;
+    static constexpr int device_kind = GB_LMO_BIRKHOFF;
     static constexpr int variable_count = Order * Order;
     bool relaxation_feasible_cpu(const std::vector<double>& x, const std::vector<double>&,
                                  const std::vector<double>&, double tol) const {
@@ -310,6 +318,7 @@ struct BirkhoffLMO {
 template <int Vars, int Threads = 64>
 struct GlobalCachedDenseQuadraticObjective {
     static constexpr bool supports_quadratic_exact = true;
+    static constexpr int device_kind = GB_OBJ_DENSE_QUADRATIC;
     static constexpr int n_vars = Vars;
     static constexpr int block_threads = Threads;
     const double* d_Qt;
@@ -338,6 +347,7 @@ struct ILSGlobalCachedObjective : GlobalCachedDenseQuadraticObjective<Vars, Thre
 template <int Order>
 struct QuadraticAssignmentObjective {
     static constexpr bool supports_quadratic_exact = true;
+    static constexpr int device_kind = GB_OBJ_QAP;
     static constexpr int n_vars = Order * Order;
     const double* d_A;
     const double* d_B;
@@ -347,6 +357,25 @@ struct QuadraticAssignmentObjective {
 };
 
 namespace gpuboscia {
+
+// tree.h:104-113, gpuboscia.cuh:49-54, algorithms.cuh:265-269
+template <class LMO, class = void>
+struct HasRelaxationPointCheck : std::false_type {};
+template <class LMO>
+struct HasRelaxationPointCheck<LMO, std::void_t<decltype(std::declval<const LMO&>().relaxation_feasible_cpu(
+                                        std::declval<const std::vector<double>&>(),
+                                        std::declval<const std::vector<double>&>(),
+                                        std::declval<const std::vector<double>&>(), 0.0))>> : std::true_type {};
+template <class LMO, class = void>
+struct lmo_supports_dicg : std::false_type {};
+template <class LMO>
+struct lmo_supports_dicg<LMO, std::void_t<decltype(LMO::supports_dicg)>>
+    : std::integral_constant<bool, LMO::supports_dicg> {};
+template <class Objective, class = void>
+struct SupportsQuadraticExact : std::false_type {};
+template <class Objective>
+struct SupportsQuadraticExact<Objective, std::void_t<decltype(Objective::supports_quadratic_exact)>>
+    : std::integral_constant<bool, Objective::supports_quadratic_exact> {};
 
 using RootInitialPointValidator = std::function<bool(
     const std::vector<double>&, const std::vector<double>&, const std::vector<double>&, double)>;
@@ -468,6 +497,151 @@ struct DeviceMarks {
     DeviceMarks& operator=(const DeviceMarks&) = delete;
 };
 
+// ---- device plug-ins (lmo.cuh:11-66, heuristics.h:11-35, README.md:224-255) --
+// The built-in types describe themselves to the library (desc(), kind());
+// any other objective, LMO or heuristic type is a device plug-in: its
+// methods are compiled into the library's relaxation kernel template in the
+// caller's translation unit (hipcc, gfx950) and launched through
+// gb_plugin_desc.  Plug-in types are copied into the kernel by value, as in
+// the reference, so they hold scalars, fixed arrays and device pointers.
+template <class T, class = void>
+struct has_objective_desc : std::false_type {};
+template <class T>
+struct has_objective_desc<T, std::void_t<decltype(std::declval<const T&>().desc())>> : std::true_type {};
+template <class T, class = void>
+struct has_lmo_desc : std::false_type {};
+template <class T>
+struct has_lmo_desc<T, std::void_t<decltype(std::declval<const T&>().desc((const uint8_t*)nullptr))>>
+    : std::true_type {};
+template <class T, class = void>
+struct has_heuristic_kind : std::false_type {};
+template <class T>
+struct has_heuristic_kind<T, std::void_t<decltype(std::declval<const T&>().kind())>> : std::true_type {};
+template <class T, class = void>
+struct shared_bytes_of : std::integral_constant<size_t, 0> {};
+template <class T>
+struct shared_bytes_of<T, std::void_t<decltype(T::shared_bytes)>> : std::integral_constant<size_t, (size_t)T::shared_bytes> {};
+template <class T, class = void>
+struct block_threads_of : std::integral_constant<int, 0> {};
+template <class T>
+struct block_threads_of<T, std::void_t<decltype(T::block_threads)>> : std::integral_constant<int, (int)T::block_threads> {};
+template <class T, class = void>
+struct n_vars_of : std::integral_constant<int, 0> {};
+template <class T>
+struct n_vars_of<T, std::void_t<decltype(T::n_vars)>> : std::integral_constant<int, (int)T::n_vars> {};
+
+#if defined(__HIPCC__)
+template <class O>
+auto device_objective(const O& o) {
+    if constexpr (has_objective_desc<O>::value) {
+        if constexpr (O::device_kind == GB_OBJ_DENSE_QUADRATIC) return gb::Obj<GB_OBJ_DENSE_QUADRATIC>{};
+        else return gb::Obj<GB_OBJ_QAP>{};
+    } else {
+        return gb::PluginObjective<O>{o};
+    }
+}
+template <class L>
+auto device_lmo(const L& l) {
+    if constexpr (has_lmo_desc<L>::value) {
+        if constexpr (L::device_kind == GB_LMO_BOX) return gb::Lmo<GB_LMO_BOX>{};
+        else if constexpr (L::device_kind == GB_LMO_SIMPLEX) return gb::Lmo<GB_LMO_SIMPLEX>{};
+        else return gb::Lmo<GB_LMO_BIRKHOFF>{};
+    } else {
+        return gb::PluginLmo<L>{l};
+    }
+}
+template <class H>
+auto device_heuristic(const H& h) {
+    if constexpr (has_heuristic_kind<H>::value) return gb::BuiltinHeuristic{};
+    else return gb::PluginHeuristic<H>{h};
+}
+#endif
+
+// Descriptors of one (objective, LMO, heuristic) triple for the C-ABI, with
+// the plug-in launcher when any of them is not built in.  Holds copies of
+// the objects for the length of a solve.
+template <class O, class L, class H>
+class Binding {
+  public:
+    static constexpr bool builtin =
+        has_objective_desc<O>::value && has_lmo_desc<L>::value && has_heuristic_kind<H>::value;
+#if !defined(__HIPCC__)
+    static_assert(builtin, "custom device objective, LMO or heuristic types are compiled into the relaxation "
+                           "kernel: build this translation unit with hipcc --offload-arch=gfx950");
+#endif
+    Binding(const O& o, const L& l, const H& h, int n)
+        : o_(o), l_(l), h_(h), marks_(marks_of(l, n)) {
+        if constexpr (has_objective_desc<O>::value) {
+            od_ = o_.desc();
+        } else {
+            od_ = gb_objective_desc{GB_OBJ_PLUGIN, n_vars_of<O>::value > 0 ? n_vars_of<O>::value : n,
+                                    nullptr, nullptr, 1.0, 1.0, 0.0, 0, nullptr, nullptr};
+        }
+        if constexpr (has_lmo_desc<L>::value) ld_ = l_.desc(marks_.ptr);
+        else ld_ = gb_lmo_desc{GB_LMO_PLUGIN, 1.0, 0, nullptr, 0, 0.0, 0.0};
+#if defined(__HIPCC__)
+        if constexpr (!builtin) {
+            pd_.launch = &Binding::launch;
+            pd_.ctx = this;
+            pd_.objective = has_objective_desc<O>::value ? 0 : 1;
+            pd_.lmo = has_lmo_desc<L>::value ? 0 : 1;
+            pd_.heuristic = has_heuristic_kind<H>::value ? 0 : 1;
+            pd_.supports_dicg = gb::plugin_traits::has_inface<L>::value ? 1 : 0;
+            pd_.supports_quadratic_exact = gb::plugin_traits::has_curvature<O>::value ? 1 : 0;
+            pd_.block_threads = has_objective_desc<O>::value ? 0 : block_threads_of<O>::value;
+            pd_.objective_bytes = has_objective_desc<O>::value ? 0 : shared_bytes_of<O>::value;
+            pd_.relaxation_feasible = HasRelaxationPointCheck<L>::value ? &Binding::feasible : nullptr;
+        }
+#endif
+    }
+    Binding(const Binding&) = delete;
+    Binding& operator=(const Binding&) = delete;
+    const gb_objective_desc* objective() const { return &od_; }
+    const gb_lmo_desc* lmo() const { return &ld_; }
+    const gb_plugin_desc* plugin() const { return builtin ? nullptr : &pd_; }
+    void fill_heuristic(gb_algorithm_desc& a) const {
+        if constexpr (has_heuristic_kind<H>::value) {
+            a.heuristic = h_.kind();
+            a.heuristic_offset = h_.offset();
+        } else {
+            a.heuristic = GB_HEUR_PLUGIN;
+            a.heuristic_offset = 0.0;
+        }
+    }
+
+  private:
+    static std::vector<uint8_t> marks_of(const L& l, int n) {
+        if constexpr (has_lmo_desc<L>::value) return l.integer_marks(n);
+        else return {};
+    }
+#if defined(__HIPCC__)
+    static int launch(void* ctx, const void* pl, void* stream) {
+        const Binding* b = static_cast<const Binding*>(ctx);
+        return gb::launch_relax_plugin(*static_cast<const gb::PluginLaunch*>(pl), device_objective(b->o_),
+                                       device_lmo(b->l_), device_heuristic(b->h_), stream);
+    }
+    // tree.h:106-113: the LMO's relaxation_feasible_cpu, when it has one
+    static int feasible(void* ctx, const double* x, const double* lb, const double* ub, int n, double tol) {
+        const Binding* b = static_cast<const Binding*>(ctx);
+        if constexpr (HasRelaxationPointCheck<L>::value) {
+            return b->l_.relaxation_feasible_cpu(std::vector<double>(x, x + n), std::vector<double>(lb, lb + n),
+                                                 std::vector<double>(ub, ub + n), tol)
+                       ? 1
+                       : 0;
+        } else {
+            return 0;
+        }
+    }
+#endif
+    O o_;
+    L l_;
+    H h_;
+    DeviceMarks marks_;
+    gb_objective_desc od_{};
+    gb_lmo_desc ld_{};
+    gb_plugin_desc pd_{};
+};
+
 // MultiBlock (device_resident_solver.cuh:1395-1440): blocks_per_node
 // workgroups per node on one stream; the library launches a batch in as many
 // rounds of resident node groups as it needs, so gpu_batch_size is kept.
@@ -521,15 +695,15 @@ class DeviceResidentHybridSolver {
                                DeviceResidentConfig cfg = {}, LMO lmo = {},
                                double time_limit_seconds = std::numeric_limits<double>::infinity(),
                                Heuristic heuristic = {})
-        : prob_(prob), marks_(lmo.integer_marks(prob.n_vars)) {
+        : prob_(prob), bind_(obj, lmo, heuristic, prob.n_vars) {
         if (!std::isfinite(cfg.dicg_paired_occupancy_fraction) || cfg.dicg_paired_occupancy_fraction <= 0.0
                 || cfg.dicg_paired_occupancy_fraction > 1.0)
             throw std::invalid_argument("invalid tolerance configuration");
         gb_default_settings(&s_);
         gpuboscia::detail::fill_config(cfg, s_);
         alg.fill(s_.algorithm);
-        s_.algorithm.heuristic = heuristic.kind();
-        s_.algorithm.heuristic_offset = heuristic.offset();
+        bind_.fill_heuristic(s_.algorithm);
+        s_.plugin = bind_.plugin();
         s_.time_limit_seconds = time_limit_seconds;
         cb_.int_vars = &prob_.int_vars;
         if (branch_sel) {
@@ -538,11 +712,9 @@ class DeviceResidentHybridSolver {
             s_.branch = gpuboscia::detail::branch_tramp;
             s_.branch_user = &cb_;
         }
-        od_ = obj.desc();
-        ld_ = lmo.desc(marks_.ptr);
         const gb_problem gp{prob_.n_vars, prob_.lb.data(), prob_.ub.data(), prob_.int_vars.data(),
                             (int)prob_.int_vars.size()};
-        gpuboscia::detail::throw_status(gb_solver_create(&gp, &od_, &ld_, &s_, &h_));
+        gpuboscia::detail::throw_status(gb_solver_create(&gp, bind_.objective(), bind_.lmo(), &s_, &h_));
     }
     ~DeviceResidentHybridSolver() { (void)gb_solver_destroy(h_); }
     DeviceResidentHybridSolver(const DeviceResidentHybridSolver&) = delete;
@@ -609,10 +781,8 @@ class DeviceResidentHybridSolver {
 
   private:
     const Problem& prob_;
-    gpuboscia::detail::DeviceMarks marks_;
+    gpuboscia::detail::Binding<DeviceObjective, LMO, Heuristic> bind_;
     gb_settings s_{};
-    gb_objective_desc od_{};
-    gb_lmo_desc ld_{};
     gpuboscia::detail::Callbacks cb_;
     gb_solver* h_ = nullptr;
     gb_solver_stats stats_{std::numeric_limits<double>::infinity(), -std::numeric_limits<double>::infinity()};
@@ -663,8 +833,9 @@ Result solve_with_algorithm(const Problem& problem, DeviceObjective objective, L
     gb_default_settings(&s);
     detail::fill_config(settings.device, s);
     algorithm.fill(s.algorithm);
-    s.algorithm.heuristic = heuristic.kind();
-    s.algorithm.heuristic_offset = heuristic.offset();
+    detail::Binding<DeviceObjective, LMO, Heuristic> bind(objective, lmo, heuristic, problem.n_vars);
+    bind.fill_heuristic(s.algorithm);
+    s.plugin = bind.plugin();
     s.time_limit_seconds = settings.time_limit_seconds;
     s.natural_lower_bound = settings.natural_lower_bound;
     if (settings.form != 0 && settings.device.execution_mode == FWExecutionMode::MultiBlock)
@@ -696,15 +867,12 @@ Result solve_with_algorithm(const Problem& problem, DeviceObjective objective, L
         s.branch = detail::branch_tramp;
         s.branch_user = &cb;
     }
-    detail::DeviceMarks marks(lmo.integer_marks(problem.n_vars));
-    const gb_lmo_desc ld = lmo.desc(marks.ptr);
-    const gb_objective_desc od = objective.desc();
     const gb_problem gp{problem.n_vars, problem.lb.data(), problem.ub.data(), problem.int_vars.data(),
                         (int)problem.int_vars.size()};
     std::vector<double> x(problem.n_vars > 0 ? problem.n_vars : 0);
     gb_result gr{};
     gr.x = x.data();
-    const gb_status st = gb_solve(&gp, &od, &ld, &s, &gr);
+    const gb_status st = gb_solve(&gp, bind.objective(), bind.lmo(), &s, &gr);
     cb.rethrow();
     detail::throw_status(st);
     Result r;
@@ -727,16 +895,16 @@ Result solve(const Problem& problem, DeviceObjective objective, LMO lmo,
     const SecantLS secant{settings.line_search.secant_iters, settings.line_search.secant_tol,
                           settings.line_search.secant_denom_eps};
     if (k == LineSearchKind::QuadraticExact) {
-        if (!DeviceObjective::supports_quadratic_exact)
+        if (!SupportsQuadraticExact<DeviceObjective>::value)
             throw std::invalid_argument("quadratic-exact requires an objective with quadratic curvature support");
         if (settings.algorithm == FWAlgorithmKind::DICG) {
-            if (!LMO::supports_dicg) throw std::invalid_argument("the selected LMO does not support DICG");
+            if (!lmo_supports_dicg<LMO>::value) throw std::invalid_argument("the selected LMO does not support DICG");
             return solve_with_algorithm(problem, objective, lmo, settings, DICGQuadraticExact{}, heuristic);
         }
         return solve_with_algorithm(problem, objective, lmo, settings, FrankWolfeQuadraticExact{}, heuristic);
     }
     if (settings.algorithm == FWAlgorithmKind::DICG) {
-        if (!LMO::supports_dicg)
+        if (!lmo_supports_dicg<LMO>::value)
             throw std::invalid_argument("the selected LMO does not support DICG; use Frank-Wolfe");
         if (k == LineSearchKind::Secant)
             return solve_with_algorithm(problem, objective, lmo, settings, DICGSecant{secant}, heuristic);

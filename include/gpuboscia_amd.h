@@ -60,7 +60,11 @@ typedef enum {
     /* Quadratic assignment over a Birkhoff polytope of order m (n = m*m):
        f(P) = ||A P - P B||_F^2, P row-major. No reference counterpart
        (analogue: examples/main.cu:131-181). */
-    GB_OBJ_QAP = 1
+    GB_OBJ_QAP = 1,
+    /* A caller-compiled device objective (gb_plugin_desc): value_local, grad,
+       optional prepare_block and quadratic_curvature_local, as the
+       reference's objective contract (device_resident_solver.cuh:8-24). */
+    GB_OBJ_PLUGIN = 2
 } gb_objective_kind;
 
 /* LMO plugins. */
@@ -73,13 +77,19 @@ typedef enum {
        reference counterpart; analogue LinkedCardinalityLMO,
        examples/sparse_sharpe.cu:266-546).  Exact for gradients with gz >= 0;
        Frank-Wolfe only (no in-face oracle). */
-    GB_LMO_PORTFOLIO = 3
+    GB_LMO_PORTFOLIO = 3,
+    /* A caller-compiled device LMO (gb_plugin_desc): compute_v,
+       check_feasible_device, optional DICG trio and candidate hooks, as the
+       reference's LMO contract (lmo.cuh:11-66, README.md:224-255). */
+    GB_LMO_PLUGIN = 4
 } gb_lmo_kind;
 
 /* Candidate heuristics (heuristics.h:23-35). */
 typedef enum {
     GB_HEUR_ROUND = 0,        /* RoundingHeuristic: round integer coordinates */
-    GB_HEUR_FLOOR_OFFSET = 1  /* floor(x) + offset on integer coordinates    */
+    GB_HEUR_FLOOR_OFFSET = 1, /* floor(x) + offset on integer coordinates    */
+    GB_HEUR_PLUGIN = 2        /* a caller-compiled Heuristic::generate (gb_plugin_desc;
+                                 heuristics.h:11-35, README.md:240-255) */
 } gb_heuristic_kind;
 
 /* Arithmetic of the relaxation. */
@@ -159,6 +169,63 @@ gb_status gb_relax_batch(const gb_objective_desc* objective,
                          double* best_vertex, double* best_vertex_val,
                          const uint8_t* integer_flags, double candidate_tol,
                          int block_threads, void* stream);
+
+/*
+ * Device plug-ins: a caller-compiled objective, LMO and/or candidate
+ * heuristic (the reference's template extension points: lmo.cuh:11-66,
+ * heuristics.h:11-35, device_resident_solver.cuh:8-24 and :229-261,
+ * README.md:224-255).  The caller builds the FP64 relaxation kernel with its
+ * own types from include/gpuboscia/device/relax_fp64.hpp (hipcc, gfx950) and
+ * registers a launcher; the library validates the batch, fills the launch
+ * parameters and work vectors, and calls the launcher in place of its
+ * built-in kernel.  include/gpuboscia/gpuboscia.hpp does all of this for
+ * C++ types passed to gpuboscia::solve and DeviceResidentHybridSolver.
+ *   launch     (ctx, launch, stream): launch = const gb::PluginLaunch*
+ *              (relax_params.h); returns a hipError_t value (0 = success).
+ *   ctx        passed back unchanged.
+ *   objective  1: the objective descriptor has kind GB_OBJ_PLUGIN and gives
+ *              only n; objective_bytes of global scratch per node.
+ *   lmo        1: the LMO descriptor has kind GB_LMO_PLUGIN.
+ *   heuristic  1: the algorithm's heuristic is GB_HEUR_PLUGIN.
+ *   supports_dicg / supports_quadratic_exact: what the plug-in objective and
+ *              LMO implement (DICG or the exact step are rejected otherwise,
+ *              gpuboscia.cuh:135-149).
+ *   block_threads  the objective's block_threads bound (0 = none).
+ *   relaxation_feasible  host check of a root point (relaxation_feasible_cpu,
+ *              lmo.cuh:30-35; tree.h:106-113), or NULL.
+ */
+typedef int (*gb_plugin_launch_fn)(void* ctx, const void* launch, void* stream);
+typedef int (*gb_plugin_feasible_fn)(void* ctx, const double* x, const double* lb, const double* ub,
+                                     int n, double tol);
+typedef struct {
+    gb_plugin_launch_fn launch;
+    void* ctx;
+    int objective;
+    int lmo;
+    int heuristic;
+    int supports_dicg;
+    int supports_quadratic_exact;
+    int block_threads;
+    size_t objective_bytes;
+    gb_plugin_feasible_fn relaxation_feasible;
+} gb_plugin_desc;
+
+/*
+ * gb_relax_batch with device plug-ins (the plain FP64 form; same arguments
+ * and results).  Replaces drs_fw_relax_batch_kernel instantiated with user
+ * Objective / LMO / Heuristic types (device_resident_solver.cuh:631-660).
+ */
+gb_status gb_relax_batch_plugin(const gb_objective_desc* objective,
+                                const gb_algorithm_desc* algorithm,
+                                const gb_lmo_desc* lmo,
+                                const gb_plugin_desc* plugin,
+                                int N, int max_iters,
+                                double* x, const double* lb, const double* ub,
+                                const double* eps, int* iterations, uint8_t* status,
+                                double* gap, double* value,
+                                double* best_vertex, double* best_vertex_val,
+                                const uint8_t* integer_flags, double candidate_tol,
+                                int block_threads, void* stream);
 
 /*
  * Allocation-free variant of gb_relax_batch.  No counterpart in the
@@ -605,6 +672,8 @@ typedef struct {
     int blocks_per_node;      /* plain form: > 1 relaxes every node with that many workgroups
                                  (MultiBlock, gb_relax_batch_multiblock; dense quadratic over a
                                  box or a simplex); 0 or 1: one workgroup per node (SingleBlock) */
+    const gb_plugin_desc* plugin;  /* device plug-ins (gb_relax_batch_plugin), plain form only;
+                                      NULL = built-in kinds */
 } gb_settings;
 
 /* Result (gpuboscia.cuh:34-42). x is caller-provided, n_vars doubles. */
