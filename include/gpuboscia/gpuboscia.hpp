@@ -17,8 +17,11 @@
 //   gpuboscia::create_default_settings, gpuboscia::solve_with_algorithm,
 //   gpuboscia::solve, gpuboscia::RoundingHeuristic.
 // Objectives hold caller-owned device pointers exactly as in the reference;
-// the device computations are the library's gfx950 kernels, so objective and
-// LMO kinds are the built-in ones (plus QuadraticAssignmentObjective).
+// the built-in kinds (and QuadraticAssignmentObjective) run the library's
+// gfx950 kernels.  Any other objective, LMO or heuristic type is a device
+// plug-in, as in the reference: its methods are compiled into the library's
+// relaxation kernel template (gpuboscia/device/relax_fp64.hpp) in the
+// caller's translation unit, which must then be built with hipcc for gfx950.
 // Host-side callbacks (progress, root validator, BranchStrategy) are plain
 // std::function values as in the reference and run on the calling thread.
 #pragma once

@@ -8,7 +8,9 @@
  * plain descriptors: every entry point takes plain pointers and sizes.  The
  * C++ drop-in header <gpuboscia/gpuboscia.hpp> maps the reference's types
  * (Settings, Result, BoxLMO, UnitSimplexLMO<N>, BirkhoffLMO<N>,
- * GlobalCachedDenseQuadraticObjective<N,T>, ...) onto these descriptors.
+ * GlobalCachedDenseQuadraticObjective<N,T>, ...) onto these descriptors, and
+ * compiles any other objective, LMO or heuristic type into the relaxation
+ * kernel as a device plug-in (gb_plugin_desc, gb_relax_batch_plugin).
  *
  * Errors: every function returns a gb_status; on failure gb_last_error()
  * returns a thread-local message.  Device errors read "HIP <file>:<line> -
