@@ -155,7 +155,8 @@ EXPORTED = ["gb_relax_batch", "gb_relax_batch_fast", "gb_default_settings", "gb_
             "gb_children_push", "gb_solver_create", "gb_solver_destroy", "gb_solver_push",
             "gb_solver_set_scheduling_callback", "gb_solver_set_progress_callback", "gb_solver_solve",
             "gb_solver_get_stats", "gb_solver_incumbent_x", "gb_candidate_features",
-            "gb_relax_batch_plugin"]
+            "gb_relax_batch_plugin", "gb_qap32_prepare", "gb_qap32_destroy", "gb_relax_batch_qap32",
+            "gb_qap32_assignments"]
 MLP_FEATURES = 8
 MLP_PARAMS = 8961
 
@@ -197,6 +198,15 @@ def load():
     lib.gb_lmo_compute_v.argtypes = [ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
     lib.gb_mixed_warm_products.restype = ctypes.c_int
     lib.gb_mixed_warm_products.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.c_int] + [ctypes.c_void_p] * 5
+    lib.gb_qap32_prepare.restype = ctypes.c_int
+    lib.gb_qap32_prepare.argtypes = [ctypes.POINTER(ObjectiveDesc), ctypes.c_void_p, ctypes.c_void_p]
+    lib.gb_qap32_destroy.restype = ctypes.c_int
+    lib.gb_qap32_destroy.argtypes = [ctypes.c_void_p]
+    lib.gb_relax_batch_qap32.restype = ctypes.c_int
+    lib.gb_relax_batch_qap32.argtypes = [ctypes.c_void_p, ctypes.POINTER(LmoDesc), ctypes.c_int, ctypes.c_int] \
+        + [ctypes.c_void_p] * 11 + [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    lib.gb_qap32_assignments.restype = ctypes.c_int
+    lib.gb_qap32_assignments.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4
     lib.gb_children_push.argtypes = [ctypes.c_void_p, ctypes.POINTER(Node)]
     lib.gb_solver_create.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(ObjectiveDesc), ctypes.POINTER(LmoDesc),
                                      ctypes.POINTER(Settings), ctypes.POINTER(ctypes.c_void_p)]
@@ -415,6 +425,44 @@ class MixedQ:
             self.close()
         except Exception:
             pass
+
+
+class Qap32:
+    """gb_qap32_*: a QAP instance prepared for the FP32 form (FP32 copies of
+    A and B, A'A and B B'); A and B stay owned by the caller."""
+
+    def __init__(self, obj: ObjectiveDesc, stream=None):
+        h = ctypes.c_void_p()
+        check(load().gb_qap32_prepare(ctypes.byref(obj), ctypes.c_void_p(stream or 0), ctypes.byref(h)))
+        self.handle = h
+
+    def relax(self, lmo: LmoDesc, N, max_iters, x, lb, ub, eps, iterations, status, gap, value, best_vertex,
+              best_vertex_val, flags=None, candidate_tol=1e-6, stop_value=float("-inf"), refresh=0, stream=None):
+        """gb_relax_batch_qap32 on torch device tensors (gb_relax_batch's layout)."""
+        check(load().gb_relax_batch_qap32(self.handle, ctypes.byref(lmo), int(N), int(max_iters), _ptr(x),
+                                          _ptr(lb), _ptr(ub), _ptr(eps), _ptr(iterations), _ptr(status),
+                                          _ptr(gap), _ptr(value), _ptr(best_vertex), _ptr(best_vertex_val),
+                                          _ptr(flags), float(candidate_tol), float(stop_value), int(refresh),
+                                          ctypes.c_void_p(stream or 0)))
+
+    def close(self):
+        if self.handle:
+            load().gb_qap32_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def qap32_assignments(costs, perms, bounds, stream=None):
+    """gb_qap32_assignments: costs float32 [T, m, m] (+inf forbidden), perms
+    int32 [T, m], bounds float64 [T] (torch device tensors); synchronous."""
+    T, m = int(costs.shape[0]), int(costs.shape[1])
+    check(load().gb_qap32_assignments(m, T, _ptr(costs), _ptr(perms), _ptr(bounds),
+                                      ctypes.c_void_p(stream or 0)))
 
 
 def relax_batch_mixed(mq: MixedQ, obj: ObjectiveDesc, alg: AlgorithmDesc, lmo: LmoDesc, N, max_iters, x,

@@ -30,6 +30,7 @@
 #include "gb_fast.h"
 #include "gb_host.h"
 #include "gb_mlp.h"
+#include "gb_qap32.h"
 
 // Children a BranchStrategy callback returns (gb_children_push).
 struct gb_children {
@@ -923,9 +924,15 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     if (s->form == 1 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > kMaxFastN))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "the batched form supports dense objectives over a box with n <= 512");
-    if (s->form == 2 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 512))
+    const bool qap32 = s->form == 2 && o->kind == GB_OBJ_QAP;
+    if (qap32 && (l->kind != GB_LMO_BIRKHOFF || l->order != o->order || o->order > kQapMaxOrder
+                  || s->device_frontier || s->branch_rule == GB_BRANCH_LEARNED))
         return fail(GB_ERR_INVALID_ARGUMENT,
-                    "the mixed-precision form supports dense objectives over a box with n <= 512");
+                    "the QAP's FP32 form relaxes over its Birkhoff polytope (order <= 256), on the host search");
+    if (s->form == 2 && !qap32 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 512))
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "the mixed-precision form supports dense objectives over a box with n <= 512, "
+                    "or a QAP over its Birkhoff polytope");
     if (s->plugin && (s->form != 0 || s->blocks_per_node > 1 || s->device_frontier))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "device plug-ins run in the plain form with one workgroup per node (form = 0, SingleBlock)");
@@ -1152,7 +1159,18 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
     // Mixed-precision form: Q split into bf16 planes once per solve (one
     // plane for GB_PREC_BF16, three otherwise).
     std::unique_ptr<gb_mixed_q, gb_status (*)(gb_mixed_q*)> mq(nullptr, gb_mixed_destroy);
-    if (s->form == 2) {
+    // The QAP's FP32 form: FP32 copies of A, B and A'A, B B' once per solve.
+    std::unique_ptr<gb_qap32, gb_status (*)(gb_qap32*)> qq(nullptr, gb_qap32_destroy);
+    const bool qap32 = s->form == 2 && o->kind == GB_OBJ_QAP;
+    const double qstop = std::isfinite(s->natural_lower_bound) ? s->natural_lower_bound + s->abs_gap_tol
+                                                               : -std::numeric_limits<double>::infinity();
+    if (qap32) {
+        gb_qap32* h = nullptr;
+        st = gb_qap32_prepare(o, bufs[0].stream, &h);
+        if (st != GB_OK) return st;
+        qq.reset(h);
+        GB_HIP(hipStreamSynchronize(bufs[0].stream));
+    } else if (s->form == 2) {
         gb_mixed_q* h = nullptr;
         st = gb_mixed_prepare(o, s->precision == GB_PREC_BF16 ? 1 : 3, bufs[0].stream, &h);
         if (st != GB_OK) return st;
@@ -1399,6 +1417,10 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
             st = relax_batch_fast(o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,
                                   db.eps, db.it, db.st, db.gap, db.val, db.bv, db.bvv, db.flags,
                                   s->integer_tol, db.cut, &fo, db.stream);
+        } else if (qap32) {
+            st = gb_relax_batch_qap32(qq.get(), l, N, s->gpu_max_iters, db.x, db.lb, db.ub, db.eps, db.it, db.st,
+                                      db.gap, db.val, db.bv, db.bvv, db.flags, s->integer_tol, qstop, 0,
+                                      db.stream);
         } else if (s->form == 2) {
             if (s->autotune && !tuned && N >= 64) {
                 st = gb_mixed_autotune(mq.get(), o, &s->algorithm, l, N, s->gpu_max_iters, db.x, db.lb, db.ub,

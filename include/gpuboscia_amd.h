@@ -444,6 +444,46 @@ gb_status gb_relax_instances(int count, const gb_instance_batch* batches,
                              double candidate_tol, int concurrency, void* stream);
 
 /*
+ * FP32 form of the QAP relaxation (configuration "Birkhoff QAP: FP32
+ * iterates, Hungarian on FP32 costs, FP64 bound"; no reference counterpart,
+ * the reference runs BirkhoffLMO<Order> with an FP64 objective,
+ * lmo.cuh:918-1419).  One workgroup per node, DICG with the exact quadratic
+ * step: iterate and gradient in FP32 (pairwise steps update the gradient in
+ * O(K m^2) for K rows moved), both linear minimisations exact assignments on
+ * FP32 costs by a warm-started Hungarian with FP64 potentials, and every
+ * returned value, gap and vertex value in FP64: gap = <G, P> - LB(v), LB the
+ * Lagrangian bound of the solver's column potentials on the FP64 gradient,
+ * so value - gap is a valid lower bound.  Orders up to 256; the m x m costs
+ * sit in LDS up to m = 200.
+ *   gb_qap32_prepare    FP32 copies of A, B and A'A, B B' (the objective's
+ *                       A and B stay caller-owned and must outlive it).
+ *   gb_relax_batch_qap32  gb_relax_batch's node buffers and meaning (n = m*m,
+ *                       Birkhoff LMO of order m, node bounds forbid (ub < 1/2)
+ *                       or force (lb > 1/2) edges); a node stops when a
+ *                       vertex reaches stop_value (-inf: never); the FP32
+ *                       gradient is recomputed from P every `refresh` steps
+ *                       (0 = 64).  A supplied iterate that is not doubly
+ *                       stochastic on the allowed edges is replaced by a
+ *                       vertex.
+ *   gb_qap32_assignments  T assignment problems solved in sequence by one
+ *                       workgroup, each warm-started from the previous one:
+ *                       costs [T][m][m] FP32 device (+inf = forbidden), perms
+ *                       [T][m] row -> column (-1 if infeasible), bounds [T]
+ *                       the Lagrangian bound of each solve.  Synchronous.
+ */
+typedef struct gb_qap32 gb_qap32;
+gb_status gb_qap32_prepare(const gb_objective_desc* objective, void* stream, gb_qap32** out);
+gb_status gb_qap32_destroy(gb_qap32* q);
+gb_status gb_relax_batch_qap32(gb_qap32* q, const gb_lmo_desc* lmo, int N, int max_iters,
+                               double* x, const double* lb, const double* ub,
+                               const double* eps, int* iterations, uint8_t* status,
+                               double* gap, double* value,
+                               double* best_vertex, double* best_vertex_val,
+                               const uint8_t* integer_flags, double candidate_tol,
+                               double stop_value, int refresh, void* stream);
+gb_status gb_qap32_assignments(int m, int T, const float* costs, int* perms, double* bounds, void* stream);
+
+/*
  * Autotuner of the mixed-precision form (no reference counterpart): on a
  * scratch copy of a batch (the caller's buffers are read, never written) it
  * times the mixed pass for every valid tile shape (row tiles per wave 1..4,
