@@ -425,6 +425,53 @@ def multiblock_large_n(args):
             "rows": rows}
 
 
+def qap_root_proof(args):
+    """Birkhoff QAP (planted: B = Pi A Pi', optimum 0 at P = Pi'): wall time of
+    gb_solve to the proven optimum with the natural lower bound 0, FP32 form
+    (form 2: FP32 iterates and assignment costs, FP64 certificate) at every
+    size in --qap-sizes, and the plain FP64 form (form 0, the reference's
+    BirkhoffLMO arithmetic) at the sizes in --qap-plain-sizes.  DICG with the
+    exact step in both; gap tolerances 1e-6."""
+    import torch
+    import gpuboscia_amd as gb
+    from gpuboscia_amd import instances
+    rows = []
+    plain = {int(v) for v in args.qap_plain_sizes.split(",") if v}
+    for m in [int(v) for v in args.qap_sizes.split(",") if v]:
+        A, B, perm, sha = instances.qap_instance(m, seed=m)
+        n = m * m
+        dA = torch.from_numpy(A).cuda()
+        dB = torch.from_numpy(B).cuda()
+        obj = gb.ObjectiveDesc(gb.OBJ_QAP, n, 0, 0, 1.0, 1.0, 0.0, m, dA.data_ptr(), dB.data_ptr())
+        lmo = gb.LmoDesc(gb.LMO_BIRKHOFF, 1.0, 0, 0, m)
+        row = {"m": m, "variables": n, "sha256": sha[:16]}
+        for form in ([2, 0] if m in plain else [2]):
+            s = gb.default_settings()
+            s.form = form
+            s.precision = gb.PREC_FP32 if form == 2 else gb.PREC_FP64
+            s.abs_gap_tol = s.rel_gap_tol = 1e-6
+            s.natural_lower_bound = 0.0
+            s.time_limit_seconds = args.qap_seconds
+            s.algorithm.line_search = gb.LS_QUADRATIC_EXACT
+            t0 = time.perf_counter()
+            r = gb.solve(n, [0.0] * n, [1.0] * n, list(range(n)), obj, lmo, s)
+            t = time.perf_counter() - t0
+            X = np.asarray(r["x"]).reshape(m, m)
+            key = "fp32" if form == 2 else "plain"
+            row[key] = {"seconds": t, "objective": r["objective"], "dual_bound": r["dual_bound"],
+                        "nodes": r["nodes_processed"], "fw_iterations": r["fw_iterations"],
+                        "proven": bool(r["objective"] - r["dual_bound"] <= 1e-6) and not r["reached_time_limit"],
+                        "planted_found": bool(r["objective"] == 0.0 and np.allclose(A @ X, X @ B))}
+            say(f"qap m={m} {key}: {t:.3f} s, objective {r['objective']}, {r['fw_iterations']} FW iterations")
+        if "plain" in row:
+            row["speedup_fp32_vs_plain"] = row["plain"]["seconds"] / row["fp32"]["seconds"]
+        rows.append(row)
+        del dA, dB
+    return {"workload": "planted Birkhoff QAP (symmetric ER(0.3) A, B = Pi A Pi'), gb_solve to the proven "
+                        "optimum 0 with natural_lower_bound 0, DICG + exact step, gap 1e-6",
+            "rows": rows}
+
+
 def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, rel_gap=1e-6, device=False):
     import torch
     dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
@@ -581,6 +628,10 @@ def main():
     ap.add_argument("--no-multiblock", action="store_true", help="skip the MultiBlock large-n leg")
     ap.add_argument("--only-multiblock", action="store_true", help="run only the MultiBlock large-n leg")
     ap.add_argument("--mb-max-n", type=int, default=65536, help="largest n of the MultiBlock leg")
+    ap.add_argument("--no-qap", action="store_true", help="skip the Birkhoff QAP root-proof leg")
+    ap.add_argument("--qap-sizes", default="64,128,200", help="QAP orders of the FP32 form")
+    ap.add_argument("--qap-plain-sizes", default="64", help="QAP orders also solved by the plain FP64 form")
+    ap.add_argument("--qap-seconds", type=float, default=120.0, help="time limit of each QAP solve")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -675,6 +726,8 @@ def main():
             f"{h['n_max']}: {h['call_ms']:.1f} ms per call, {h['value']:.4g} FW it/s, {h['nodes_per_s']:.4g} nodes/s")
     if not args.no_multiblock:
         result["multiblock"] = multiblock_large_n(args)
+    if not args.no_qap:
+        result["qap_root_proof"] = qap_root_proof(args)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, Q, c, LB, UB, EPS)
     if not args.no_tto:

@@ -54,7 +54,9 @@ struct QapLds {
 inline QapLds qap_lds(int m, size_t limit) {
     QapLds L{};
     const size_t mm4 = (size_t)m * m * 4;
-    const size_t chunk = (size_t)4 * kQapChunk * m * 4;
+    // gradient pass: four [kQapChunk][m rounded up to 4] float tiles, then
+    // the two vertices and their inverses (4 m ints)
+    const size_t chunk = (size_t)4 * kQapChunk * ((m + 3) & ~3) * 4 + (size_t)4 * m * 4;
     auto al = [](size_t v) { return (v + 15) & ~size_t(15); };
     const size_t aux = al((size_t)(m + 1) * 8) * 2 + al((size_t)m + 1) + al(16 * 8 + 16 * 4);
     L.staged = al(mm4) + aux <= limit ? 1 : 0;

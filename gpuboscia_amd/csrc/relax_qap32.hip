@@ -81,44 +81,77 @@ __device__ __forceinline__ void set4(T (&a)[4], int t, T v) {
     else a[3] = v;
 }
 
+// Wave-wide minima by DPP within rows of 16 lanes (quad permutes, half-row
+// and row mirrors) and lane reads across the four rows; wave-uniform.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    return __hiloint2double(dpp_i<CTRL>(__double2hiint(v)), dpp_i<CTRL>(__double2loint(v)));
+}
 __device__ __forceinline__ double wave_min_all(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
-    return v;
+    v = fmin(v, dpp_d<0xB1>(v));   // lane ^ 1
+    v = fmin(v, dpp_d<0x4E>(v));   // lane ^ 2
+    v = fmin(v, dpp_d<0x141>(v));  // mirror within 8
+    v = fmin(v, dpp_d<0x140>(v));  // mirror within 16
+    return fmin(fmin(rl_d(v, 0), rl_d(v, 16)), fmin(rl_d(v, 32), rl_d(v, 48)));
+}
+__device__ __forceinline__ int wave_min_int(int v) {
+    v = min(v, dpp_i<0xB1>(v));
+    v = min(v, dpp_i<0x4E>(v));
+    v = min(v, dpp_i<0x141>(v));
+    v = min(v, dpp_i<0x140>(v));
+    return min(min(rl_i(v, 0), rl_i(v, 16)), min(rl_i(v, 32), rl_i(v, 48)));
 }
 
 // (value, column) minimum with ties to the lower column: the order of the
-// reference's column scan.
+// reference's column scan.  Values are never NaN.
 __device__ __forceinline__ void wave_argmin_dj(double& v, int& j) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off, kWave);
-        const int oj = __shfl_xor(j, off, kWave);
-        if (ov < v || (ov == v && oj < j)) {
-            v = ov;
-            j = oj;
-        }
-    }
+    const double vm = wave_min_all(v);
+    j = wave_min_int(v == vm ? j : INT_MAX);
+    v = vm;
+}
+
+// The staged costs are read through an LDS pointer (ds_read), the global
+// copy through a global one.
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+template <bool kL>
+__device__ __forceinline__ float cost_at(const float* C, size_t e) {
+    if constexpr (kL) return ((const lds_cfloat*)C)[e];
+    else return C[e];
 }
 
 // Assignment of minimum cost on C (m x m row-major FP32, +inf forbidden),
 // warm-started from (vcol, pcol) and leaving the new state there; sigma
 // gets row -> column.  Every thread calls it; returns false (block-uniform)
 // when some row cannot be matched.
+template <bool kL>
 __device__ bool assign(int m, const float* C, double* vcol, int* pcol, int* sigma, const QView& V) {
     const int tid = threadIdx.x, lane = lane_id(), w = wave_id(), W = wave_count();
     for (int j = tid; j <= m; j += blockDim.x) V.vsh[j] = vcol[j];
     __syncthreads();
-    // row potentials: u_i = min_j (c_ij - v_j), dual feasible for every row
+    // row potentials: u_i = min_j (c_ij - v_j), dual feasible for every
+    // row, and the lowest column attaining it (in sigma until the end)
     for (int i = w; i < m; i += W) {
         double r = kDInf;
-        const float* cr = C + (size_t)i * m;
+        int rj = INT_MAX;
         for (int j = lane; j < m; j += kWave) {
-            const float c = cr[j];
-            if (c < kQInf) r = fmin(r, (double)c - V.vsh[j + 1]);
+            const float c = cost_at<kL>(C, (size_t)i * m + j);
+            if (c < kQInf) {
+                const double d = (double)c - V.vsh[j + 1];
+                if (d < r) {
+                    r = d;
+                    rj = j;
+                }
+            }
         }
-        r = wave_min_all(r);
-        if (lane == 0) V.urow[i + 1] = r;
+        wave_argmin_dj(r, rj);
+        if (lane == 0) {
+            V.urow[i + 1] = r;
+            sigma[i] = rj;
+        }
     }
     __syncthreads();
     if (w == 0) {
@@ -132,7 +165,7 @@ __device__ bool assign(int m, const float* C, double* vcol, int* pcol, int* sigm
             v[t] = valid ? V.vsh[j] : 0.0;
             int pi = valid ? pcol[j] : 0;
             if (pi != 0) {  // keep the edge only if it is still tight
-                const float c = C[(size_t)(pi - 1) * m + (j - 1)];
+                const float c = cost_at<kL>(C, (size_t)(pi - 1) * m + (j - 1));
                 if (!(c < kQInf) || ((double)c - v[t]) != V.urow[pi]) pi = 0;
             }
             p[t] = pi;
@@ -146,6 +179,22 @@ __device__ bool assign(int m, const float* C, double* vcol, int* pcol, int* sigm
 #pragma unroll
         for (int t = 0; t < 4; ++t)
             if (p[t]) V.rowm[p[t]] = 1;
+        wave_sync();
+        // Greedy start: a free row takes its row minimum's column when that
+        // column is free (reduced cost 0, so the duals stay feasible and the
+        // matched edges tight).
+        for (int i = 1; i <= m; ++i) {
+            if (V.rowm[i]) continue;
+            const int c = sigma[i - 1];
+            if (c == INT_MAX) continue;
+            const int o = c & 63, t0 = c >> 6;
+            if (rl_i(sel4(p, t0), o) != 0) continue;
+            if (lane == o) {
+                set4(p, t0, i);
+                set4(up, t0, V.urow[i]);
+            }
+            if (lane == 0) V.rowm[i] = 1;
+        }
         wave_sync();
         bool ok = true;
         for (int i = 1; i <= m && ok; ++i) {
@@ -173,14 +222,14 @@ __device__ bool assign(int m, const float* C, double* vcol, int* pcol, int* sigm
                     ui0 = rl_d(sel4(up, t0), o);
                     if (lane == o) set4(used, t0, true);
                 }
-                const float* cr = C + (size_t)(i0 - 1) * m;
+                const size_t row = (size_t)(i0 - 1) * m;
                 double best = kDInf;
                 int bj = INT_MAX;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const int j = 1 + lane + 64 * t;
                     if (j <= m && !used[t]) {
-                        const float c = cr[j - 1];
+                        const float c = cost_at<kL>(C, row + (j - 1));
                         if (c < kQInf) {
                             const double cur = ((double)c - v[t]) - ui0;
                             if (cur < mv[t]) {
@@ -342,58 +391,98 @@ __device__ void invert(int m, const int* s, int* inv) {
 
 // G += coef * G(D) for D = V - W on the rows S (K of them), V = sv, W = sa
 // (sa null: D = the permutation sv on every row, K = m).  iv / ia invert
-// sv / sa.
+// sv / sa.  Each thread owns 1 x 4 blocks of G; the rank-K terms read four
+// LDS tiles per chunk of 32 rows of S (row stride m rounded up to 4, so a
+// block's four columns are one 16-byte read).
+typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(3))) int lds_int;
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4f lds_v4f;
+
 __device__ void grad_update(const QapParams& Q, float* G, double coef, const int* S, int K, const int* sv,
                             const int* sa, const int* iv, const int* ia, const QView& V) {
-    const int m = Q.m, mm = m * m;
+    const int m = Q.m, mp = (m + 3) & ~3, nb = mp / 4;
     const float cf = (float)coef;
-    float* acol = V.cost;                      // [kQapChunk][m]: A[k][x]
-    float* arow = acol + kQapChunk * m;        //                 A[x][k]
-    float* db = arow + kQapChunk * m;          //                 B[sv(k)][x] - B[sa(k)][x]
-    float* dbt = db + kQapChunk * m;           //                 B[x][sv(k)] - B[x][sa(k)]
+    lds_float* acol = (lds_float*)V.cost;        // [kQapChunk][mp]: A[k][x]
+    lds_float* arow = acol + kQapChunk * mp;     //                  A[x][k]
+    lds_float* db = arow + kQapChunk * mp;       //                  B[sv(k)][x] - B[sa(k)][x]
+    lds_float* dbt = db + kQapChunk * mp;        //                  B[x][sv(k)] - B[x][sa(k)]
+    lds_int* isv = (lds_int*)(dbt + kQapChunk * mp);
+    lds_int* isa = isv + m;
+    lds_int* iiv = isa + m;
+    lds_int* iia = iiv + m;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        isv[i] = sv[i];
+        iiv[i] = iv[i];
+        isa[i] = sa ? sa[i] : -1;
+        iia[i] = sa ? ia[i] : -1;
+    }
     for (int c0 = 0; c0 < K; c0 += kQapChunk) {
         const int cnt = min(kQapChunk, K - c0);
-        for (int e = threadIdx.x; e < cnt * m; e += blockDim.x) {
-            const int r = e / m, x = e - r * m;
-            const int k = S[c0 + r];
-            const int a = sv[k];
-            acol[r * m + x] = Q.A[(size_t)k * m + x];
-            arow[r * m + x] = Q.A[(size_t)x * m + k];
-            float d1 = Q.B[(size_t)a * m + x], d2 = Q.B[(size_t)x * m + a];
-            if (sa) {
-                d1 -= Q.B[(size_t)sa[k] * m + x];
-                d2 -= Q.B[(size_t)x * m + sa[k]];
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt * mp; e += blockDim.x) {
+            const int r = e / mp, x = e - r * mp;
+            float a1 = 0.0f, a2 = 0.0f, d1 = 0.0f, d2 = 0.0f;
+            if (x < m) {
+                const int k = S[c0 + r];
+                const int a = sv[k];
+                a1 = Q.A[(size_t)k * m + x];
+                a2 = Q.A[(size_t)x * m + k];
+                d1 = Q.B[(size_t)a * m + x];
+                d2 = Q.B[(size_t)x * m + a];
+                if (sa) {
+                    d1 -= Q.B[(size_t)sa[k] * m + x];
+                    d2 -= Q.B[(size_t)x * m + sa[k]];
+                }
             }
-            db[r * m + x] = d1;
-            dbt[r * m + x] = d2;
+            acol[e] = a1;
+            arow[e] = a2;
+            db[e] = d1;
+            dbt[e] = d2;
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < mm; e += blockDim.x) {
-            const int i = e / m, j = e - i * m;
-            float t = 0.0f;
+        for (int b = threadIdx.x; b < m * nb; b += blockDim.x) {
+            const int i = b / nb, j0 = (b - i * nb) * 4;
+            float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             if (c0 == 0) {
                 // A'A D and D B B' (one entry each)
-                const int kv = iv[j];
-                if (!sa || sa[kv] != j) t += Q.AtA[(size_t)i * m + kv];
-                if (sa) {
-                    const int ka = ia[j];
-                    if (sv[ka] != j) t -= Q.AtA[(size_t)i * m + ka];
-                }
-                if (!sa || sv[i] != sa[i]) {
-                    t += Q.BBt[(size_t)sv[i] * m + j];
-                    if (sa) t -= Q.BBt[(size_t)sa[i] * m + j];
+                const bool rowS = !sa || isv[i] != isa[i];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int j = j0 + q;
+                    if (j >= m) continue;
+                    const int kv = iiv[j];
+                    if (!sa || isa[kv] != j) t[q] += Q.AtA[(size_t)i * m + kv];
+                    if (sa) {
+                        const int ka = iia[j];
+                        if (isv[ka] != j) t[q] -= Q.AtA[(size_t)i * m + ka];
+                    }
+                    if (rowS) {
+                        t[q] += Q.BBt[(size_t)isv[i] * m + j];
+                        if (sa) t[q] -= Q.BBt[(size_t)isa[i] * m + j];
+                    }
                 }
             }
             // - A'D B - A D B' over this chunk of S
-            float u = 0.0f;
+            float u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int r = 0; r < cnt; ++r) {
-                u = fmaf(acol[r * m + i], db[r * m + j], u);
-                u = fmaf(arow[r * m + i], dbt[r * m + j], u);
+                const float ac = acol[r * mp + i], ar = arow[r * mp + i];
+                const v4f d = *(const lds_v4f*)(db + r * mp + j0);
+                const v4f dt = *(const lds_v4f*)(dbt + r * mp + j0);
+                u[0] = fmaf(ar, dt.x, fmaf(ac, d.x, u[0]));
+                u[1] = fmaf(ar, dt.y, fmaf(ac, d.y, u[1]));
+                u[2] = fmaf(ar, dt.z, fmaf(ac, d.z, u[2]));
+                u[3] = fmaf(ar, dt.w, fmaf(ac, d.w, u[3]));
             }
-            G[e] = fmaf(cf, t - u, G[e]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (j0 + q < m) {
+                    float* g = G + (size_t)i * m + j0 + q;
+                    *g = fmaf(cf, t[q] - u[q], *g);
+                }
         }
-        __syncthreads();
     }
+    __syncthreads();
 }
 
 // P, G of a node; the kernel's view of its workspace.
@@ -528,7 +617,7 @@ __global__ void __launch_bounds__(kQapThreads) qap32_kernel(QapParams Q) {
     if (!from_x) {
         for (int e = tid; e < n; e += T) C[e] = q.mask[e] ? (float)(-1.0 - (double)e / (n + 1.0)) : kQInf;
         __syncthreads();
-        if (!assign(m, C, q.vF, q.pF, q.sv, V)) {
+        if (!assign<kStaged>(m, C, q.vF, q.pF, q.sv, V)) {
             infeasible();
             return;
         }
@@ -555,7 +644,7 @@ __global__ void __launch_bounds__(kQapThreads) qap32_kernel(QapParams Q) {
         // forward vertex: assignment on G
         for (int e = tid; e < n; e += T) C[e] = q.mask[e] ? q.G[e] : kQInf;
         __syncthreads();
-        if (!assign(m, C, q.vF, q.pF, q.sv, V)) break;
+        if (!assign<kStaged>(m, C, q.vF, q.pF, q.sv, V)) break;
         invert(m, q.sv, q.iv);
         track(q.sv, vertex_value(m, Q.A64, Q.B64, q.sv, q.iv, V));
         if (best_s <= Q.stop_value) break;
@@ -567,7 +656,7 @@ __global__ void __launch_bounds__(kQapThreads) qap32_kernel(QapParams Q) {
         // away vertex: assignment on -G over the support of P
         for (int e = tid; e < n; e += T) C[e] = (q.mask[e] && q.P[e] > 1e-7f) ? -q.G[e] : kQInf;
         __syncthreads();
-        if (!assign(m, C, q.vA, q.pA, q.sa, V)) break;
+        if (!assign<kStaged>(m, C, q.vA, q.pA, q.sa, V)) break;
         invert(m, q.sa, q.ia);
         track(q.sa, vertex_value(m, Q.A64, Q.B64, q.sa, q.ia, V));
         // rows where the vertices differ (wave ballots, prefix in LDS)
@@ -633,7 +722,7 @@ __global__ void __launch_bounds__(kQapThreads) qap32_kernel(QapParams Q) {
     for (int e = tid; e < n; e += T) C[e] = q.mask[e] ? (float)q.G64[e] : kQInf;
     __syncthreads();
     double lbnd = -kDInf;
-    if (assign(m, C, q.vF, q.pF, q.sv, V)) {
+    if (assign<kStaged>(m, C, q.vF, q.pF, q.sv, V)) {
         invert(m, q.sv, q.iv);
         track(q.sv, vertex_value(m, Q.A64, Q.B64, q.sv, q.iv, V));
         lbnd = lagrange_bound(m, q.G64, q.mask, q.vF, V);
@@ -719,7 +808,8 @@ __global__ void __launch_bounds__(kQapThreads) qap32_assign_kernel(int m, int T,
         for (int e = threadIdx.x; e < mm; e += blockDim.x) C[e] = src[e];
         __syncthreads();
         int* out = perms + (size_t)t * m;
-        const bool ok = assign(m, C, vstate, pstate, out, V);
+        const bool ok = L.staged ? assign<true>(m, C, vstate, pstate, out, V)
+                                 : assign<false>(m, C, vstate, pstate, out, V);
         double part = 0.0;
         {
             const int lane = lane_id(), w = wave_id(), W = wave_count();
