@@ -359,6 +359,38 @@ struct QuadraticAssignmentObjective {
     }
 };
 
+// The same objective and the Birkhoff LMO with the order chosen at run
+// time (instances read from files); no reference counterpart, whose types
+// fix it at compile time.
+struct RuntimeQuadraticAssignmentObjective {
+    static constexpr bool supports_quadratic_exact = true;
+    static constexpr int device_kind = GB_OBJ_QAP;
+    int order;
+    const double* d_A;
+    const double* d_B;
+    gb_objective_desc desc() const {
+        return gb_objective_desc{GB_OBJ_QAP, order * order, nullptr, nullptr, 1.0, 1.0, 0.0, order, d_A, d_B};
+    }
+};
+struct RuntimeBirkhoffLMO {
+    static constexpr bool supports_dicg = true;
+    static constexpr int device_kind = GB_LMO_BIRKHOFF;
+    int order;
+    bool relaxation_feasible_cpu(const std::vector<double>& x, const std::vector<double>&,
+                                 const std::vector<double>&, double tol) const {
+        if (x.size() != static_cast<size_t>(order) * order) return false;
+        for (double v : x) if (v < -tol || v > 1 + tol) return false;
+        for (int i = 0; i < order; ++i) {
+            double row = 0, col = 0;
+            for (int j = 0; j < order; ++j) { row += x[i * order + j]; col += x[j * order + i]; }
+            if (std::abs(row - 1) > tol || std::abs(col - 1) > tol) return false;
+        }
+        return true;
+    }
+    gb_lmo_desc desc(const uint8_t*) const { return gb_lmo_desc{GB_LMO_BIRKHOFF, 1.0, 0, nullptr, order}; }
+    std::vector<uint8_t> integer_marks(int) const { return {}; }
+};
+
 namespace gpuboscia {
 
 // tree.h:104-113, gpuboscia.cuh:49-54, algorithms.cuh:265-269
