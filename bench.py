@@ -257,7 +257,7 @@ def hetero_batches(args):
     (instances.open_nodes), relaxed by ONE gb_relax_instances call over 8
     streams: box instances in the mixed form (args.c3_planes bf16 planes of
     Q on MFMA with FP32 accumulation, then the FP64 certification pass),
-    bounded-simplex instances in the plain FP64 form; DICG with the exact
+    bounded-simplex instances in the batched FP64 form; DICG with the exact
     step, budget args.budget iterations per node.  HIP events on the calling
     stream bracket each call; the same call captured as one HIP graph is
     replayed on refilled inputs."""
@@ -345,7 +345,7 @@ def hetero_batches(args):
             "simplex_instances": kinds.count("simplex"), "nodes_per_instance": N, "n_min": int(sizes.min()),
             "n_max": int(sizes.max()),
             "precision": f"box: {args.c3_planes} bf16 plane(s) of Q + FP64 certification; "
-                         "bounded simplex: plain FP64 form",
+                         "bounded simplex: batched FP64 form (one wave per node)",
             "algorithm": "DICG, exact quadratic step", "budget": args.budget,
             "call_ms": t * 1e3, "captured_replay_ms": float(np.mean(replay)) * 1e3,
             "replay_iterations_equal": iters_replay == iters,
@@ -629,6 +629,7 @@ def main():
     ap.add_argument("--only-multiblock", action="store_true", help="run only the MultiBlock large-n leg")
     ap.add_argument("--mb-max-n", type=int, default=65536, help="largest n of the MultiBlock leg")
     ap.add_argument("--no-qap", action="store_true", help="skip the Birkhoff QAP root-proof leg")
+    ap.add_argument("--only-qap", action="store_true", help="run only the Birkhoff QAP root-proof leg")
     ap.add_argument("--qap-sizes", default="64,128,200", help="QAP orders of the FP32 form")
     ap.add_argument("--qap-plain-sizes", default="64", help="QAP orders also solved by the plain FP64 form")
     ap.add_argument("--qap-seconds", type=float, default=120.0, help="time limit of each QAP solve")
@@ -648,6 +649,9 @@ def main():
         return
     if args.only_multiblock:
         print(json.dumps({"multiblock": multiblock_large_n(args)}), flush=True)
+        return
+    if args.only_qap:
+        print(json.dumps({"qap_root_proof": qap_root_proof(args)}), flush=True)
         return
     n = args.n
     Q, c, sha = instances.binary_qp(n, seed=1)

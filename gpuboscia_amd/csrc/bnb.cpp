@@ -921,9 +921,22 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     if (s->form == 2 && s->precision != GB_PREC_FP64 && s->precision != GB_PREC_FP32
             && s->precision != GB_PREC_BF16)
         return fail(GB_ERR_INVALID_ARGUMENT, "unknown precision");
-    if (s->form == 1 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > kMaxFastN))
+    if (s->form == 1 && (o->kind != GB_OBJ_DENSE_QUADRATIC || o->n > kMaxFastN
+                         || (l->kind != GB_LMO_BOX && !fast_simplex_ok(l))))
         return fail(GB_ERR_INVALID_ARGUMENT,
-                    "the batched form supports dense objectives over a box with n <= 512");
+                    "the batched form supports dense objectives with n <= 512 over a box, or over a simplex "
+                    "with an integral right side");
+    if (s->form == 1 && l->kind == GB_LMO_SIMPLEX) {
+        // the batched form's simplex relaxes 0/1 nodes: every variable
+        // integer with root bounds 0 or 1 (branching keeps them so)
+        bool ok = pr->n_int_vars == pr->n_vars && s->branch_rule != GB_BRANCH_CUSTOM && !s->device_frontier;
+        for (int j = 0; j < pr->n_vars && ok; ++j)
+            ok = (pr->lb[j] == 0.0 || pr->lb[j] == 1.0) && (pr->ub[j] == 0.0 || pr->ub[j] == 1.0);
+        if (!ok)
+            return fail(GB_ERR_INVALID_ARGUMENT,
+                        "the batched form's simplex needs a 0/1 problem (every variable integer, bounds 0 or 1), "
+                        "no custom branching and the host search");
+    }
     const bool qap32 = s->form == 2 && o->kind == GB_OBJ_QAP;
     if (qap32 && (l->kind != GB_LMO_BIRKHOFF || l->order != o->order || o->order > kQapMaxOrder
                   || s->device_frontier || s->branch_rule == GB_BRANCH_LEARNED))
@@ -1531,6 +1544,10 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                 return fail(GB_ERR_DEVICE, "a node's relaxation did not complete (status "
                                                + std::to_string(db.hst[i])
                                                + "): its MultiBlock workgroups were not resident together");
+        // status 0xFF from the batched form's simplex: a node outside its 0/1 scope
+        for (int i = 0; i < N && s->form == 1 && l->kind == GB_LMO_SIMPLEX; ++i)
+            if (db.hst[i] > GB_NODE_INTEGRAL_CONVERGED)
+                return fail(GB_ERR_DEVICE, "the batched form's simplex met a node that is not 0/1");
         emit(GB_SCHED_COMPLETION_DETACHED, head, N);
         for (int sl : gathered[head]) pool.release(sl);
         gathered[head].clear();
@@ -1722,10 +1739,12 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                     if (!(c.bound < inc)) continue;
                     c.slot = pool.alloc();
                     child_tasks.push_back({c.slot, nd.slot, i, mask, k, var0});
-                    if (s->form != 0) {
+                    if (s->form != 0 && l->kind != GB_LMO_SIMPLEX) {
                         // Batched forms: the child starts from the parent's
                         // iterate, kept once in the parent's slot and clipped to
-                        // the child's box when the child is staged.
+                        // the child's box when the child is staged (a box point
+                        // is feasible; over a simplex the clipped point may not
+                        // be, so those children start at the LMO vertex).
                         if (!parent_x_kept) {
                             if (!db.xdev) x_tasks.push_back({nd.slot, i});
                             parent_x_kept = true;

@@ -343,6 +343,11 @@ hipError_t create_dedicated_stream(hipStream_t* out) {
     return hipStreamCreate(out);
 }
 
+bool fast_simplex_ok(const gb_lmo_desc* l) {
+    return l->kind == GB_LMO_SIMPLEX && std::isfinite(l->right_side) && l->right_side == std::floor(l->right_side)
+        && std::fabs(l->right_side) < 1e15;
+}
+
 gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* a,
                            const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
                            const double* ub, const double* eps, int* iterations, uint8_t* status,
@@ -352,9 +357,11 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
                            const int* count) {
     gb_status st = validate_descriptors(o, a, l);
     if (st != GB_OK) return st;
-    if (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX)
-        return fail(GB_ERR_INVALID_ARGUMENT, "the batched form supports dense objectives over a box");
+    if (o->kind != GB_OBJ_DENSE_QUADRATIC || (l->kind != GB_LMO_BOX && l->kind != GB_LMO_SIMPLEX))
+        return fail(GB_ERR_INVALID_ARGUMENT, "the batched form supports dense objectives over a box or a simplex");
     if (o->n > kMaxFastN) return fail(GB_ERR_INVALID_ARGUMENT, "the batched form supports n <= 512");
+    if (l->kind == GB_LMO_SIMPLEX && !fast_simplex_ok(l))
+        return fail(GB_ERR_INVALID_ARGUMENT, "the batched form's simplex needs a finite integral right_side");
     if (N < 0 || max_iters < 0) return fail(GB_ERR_INVALID_ARGUMENT, "N and max_iters must be non-negative");
     if (N == 0) return GB_OK;
     if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
@@ -400,6 +407,10 @@ gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* 
     P.gout = op.gradient;
     P.rescore = rescore ? 1 : 0;
     P.count = count;
+    P.simplex = l->kind == GB_LMO_SIMPLEX;
+    P.equality = l->equality != 0;
+    P.rhs = l->right_side;
+    P.is_int = l->is_int;
     const int e = launch_relax_fast(P, op.waves_per_block, op.blocks, stream);
     if (e != 0) return hip_fail(static_cast<hipError_t>(e), __FILE__, __LINE__);
     return GB_OK;

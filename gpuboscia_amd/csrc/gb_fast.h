@@ -33,6 +33,12 @@ struct FastParams {
     double* gout;           // optional [N][n]: gradient at the returned iterate (may be NULL)
     int rescore;            // re-score best_v[i] in FP64 first when best_val[i] < 1e300
     const int* count;       // optional device batch size: nodes [0, *count) of the N rows are relaxed
+    // Bounded-simplex LMO (simplex != 0): sum x <= rhs (== rhs when equality),
+    // integral rhs, 0/1 variables all marked in is_int (device, n bytes).
+    int simplex;
+    int equality;
+    double rhs;
+    const uint8_t* is_int;
 };
 
 constexpr int kMaxFastBlocks = 2048;  // persistent workgroups of the batched form (blocks = 0: automatic)

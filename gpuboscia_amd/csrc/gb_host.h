@@ -40,6 +40,9 @@ gb_status relax_batch_fp64(const gb_objective_desc* o, const gb_algorithm_desc* 
                            double* gap, double* value, double* best_vertex, double* best_vertex_val,
                            const uint8_t* flags, double ctol, int block_threads, void* stream);
 
+// The batched form takes a simplex LMO with a finite integral right side
+// (its nodes must be 0/1; others come back with status 0xFF).
+bool fast_simplex_ok(const gb_lmo_desc* l);
 gb_status relax_batch_fast(const gb_objective_desc* o, const gb_algorithm_desc* a,
                            const gb_lmo_desc* l, int N, int max_iters, double* x, const double* lb,
                            const double* ub, const double* eps, int* iterations, uint8_t* status,

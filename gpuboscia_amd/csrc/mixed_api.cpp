@@ -364,10 +364,12 @@ struct ForkStreams {
 ForkStreams g_fork[64];
 
 // Form of a gb_relax_instances entry: 2 mixed, 1 batched FP64 (dense over a
-// box, n <= 512), 0 plain FP64 (everything else).
+// box or a simplex with an integral right side, n <= 512), 0 plain FP64
+// (everything else).
 int instance_form(const gb_instance_batch& b) {
     if (b.mixed) return 2;
-    if (b.objective->kind == GB_OBJ_DENSE_QUADRATIC && b.lmo->kind == GB_LMO_BOX && b.objective->n <= kMaxFastN)
+    if (b.objective->kind == GB_OBJ_DENSE_QUADRATIC && b.objective->n <= kMaxFastN
+            && (b.lmo->kind == GB_LMO_BOX || fast_simplex_ok(b.lmo)))
         return 1;
     return 0;
 }

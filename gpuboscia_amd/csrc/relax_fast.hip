@@ -39,6 +39,92 @@ __device__ __forceinline__ double wave_min_all(double v) {
 }
 __device__ __forceinline__ bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 
+// Order-preserving unsigned image of a double, with -0 and +0 equal (the
+// greedy's comparisons are on doubles, lmo.cuh:517-618).
+__device__ __forceinline__ unsigned long long order_key(double g) {
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(g == 0.0 ? 0.0 : g));
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// Largest X with #{c && w < X} < want (1 <= want <= #c): MSB-first
+// bisection over the 32-bit words w; every count is a ballot popcount, so
+// X is wave-uniform.
+template <int NT>
+__device__ __forceinline__ unsigned int wave_cut(const unsigned int (&w)[NT], const unsigned long long (&cm)[NT],
+                                                 int want) {
+    unsigned int X = 0;
+    for (int b = 31; b >= 0; --b) {
+        const unsigned int Y = X | (1u << b);
+        int cnt = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) cnt += __popcll(__ballot(w[t] < Y) & cm[t]);
+        if (cnt < want) X = Y;
+    }
+    return X;
+}
+
+// sel = the `want` smallest members of the set c by (key, index), index
+// j = lane + 64 t (t-major order): the first `want` coordinates of the
+// greedy's rank order.  The cut is found on the high words, then, within the
+// high word it splits, on the low words; equal keys go by index.
+template <int NT>
+__device__ __forceinline__ void wave_select(const unsigned long long (&key)[NT], const bool (&c)[NT], double want_d,
+                            bool (&sel)[NT], int lane) {
+    unsigned long long cm[NT];
+    int total = 0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        cm[t] = __ballot(c[t]);
+        total += __popcll(cm[t]);
+    }
+    if (want_d >= static_cast<double>(total) || !(want_d >= 1.0)) {
+        const bool all = want_d >= static_cast<double>(total);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) sel[t] = all && c[t];
+        return;
+    }
+    const int want = static_cast<int>(want_d);
+    unsigned int hi[NT], lo[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        hi[t] = static_cast<unsigned int>(key[t] >> 32);
+        lo[t] = static_cast<unsigned int>(key[t]);
+    }
+    const unsigned int th = wave_cut<NT>(hi, cm, want);
+    int below = 0, group = 0;
+    unsigned long long gm[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        below += __popcll(__ballot(hi[t] < th) & cm[t]);
+        gm[t] = __ballot(hi[t] == th) & cm[t];
+        group += __popcll(gm[t]);
+    }
+    const int need = want - below;  // 1 <= need <= group
+#pragma unroll
+    for (int t = 0; t < NT; ++t) sel[t] = c[t] && hi[t] < th;
+    if (need >= group) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) sel[t] = sel[t] || (c[t] && hi[t] == th);
+        return;
+    }
+    const unsigned int tl = wave_cut<NT>(lo, gm, need);
+    int below2 = 0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) below2 += __popcll(__ballot(lo[t] < tl) & gm[t]);
+    const int need2 = need - below2;  // >= 1 members with key == (th, tl), taken by index
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int base = 0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const bool grp = c[t] && hi[t] == th;
+        const unsigned long long tm = __ballot(lo[t] == tl) & gm[t];
+        const bool tie = grp && lo[t] == tl;
+        const int r = base + __popcll(tm & lt);
+        sel[t] = sel[t] || (grp && lo[t] < tl) || (tie && r < need2);
+        base += __popcll(tm);
+    }
+}
+
 __device__ __forceinline__ double quad_exact(double gap, double curvature, double maximum) {
     if (maximum <= 0.0) return 0.0;
     if (curvature > 0.0) {
@@ -51,7 +137,7 @@ __device__ __forceinline__ double quad_exact(double gap, double curvature, doubl
 
 template <int NT, bool QL>
 struct WaveNode {
-    const FastParams& P;
+    int n_;             // problem size (a copy: the kernel argument stays in registers)
     const double* Q;    // Qt in LDS (QL) or global memory
     int lane;
     double* zs;  // this wave's LDS staging vector (n doubles)
@@ -64,8 +150,8 @@ struct WaveNode {
     // are in flight per round trip.  Every output is the
     // FMA chain over k ascending, whatever NV and kBatch.
     static constexpr int kBatch = NT >= 8 ? 2 : (NT >= 4 ? 4 : 8);
-    __device__ void load_batch(int k0, double (&qq)[kBatch][NT]) {
-        const int n = P.n;
+    __device__ __forceinline__ void load_batch(int k0, double (&qq)[kBatch][NT]) {
+        const int n = n_;
 #pragma unroll
         for (int u = 0; u < kBatch; ++u)
 #pragma unroll
@@ -76,8 +162,8 @@ struct WaveNode {
             }
     }
     template <int NV>
-    __device__ void sweep(const double* za, const double* zb, double (&ya)[NT], double (&yb)[NT]) {
-        const int n = P.n;
+    __device__ __forceinline__ void sweep(const double* za, const double* zb, double (&ya)[NT], double (&yb)[NT]) {
+        const int n = n_;
         double acc[NV][NT];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
@@ -110,16 +196,16 @@ struct WaveNode {
         }
     }
 
-    __device__ void stage(double* dst, const double (&z)[NT]) {
+    __device__ __forceinline__ void stage(double* dst, const double (&z)[NT]) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int j = lane + kWave * t;
-            if (j < P.n) dst[j] = z[t];
+            if (j < n_) dst[j] = z[t];
         }
     }
 
     // y = Q z for the lane's rows; z given in registers.
-    __device__ void gemv(const double (&z)[NT], double (&y)[NT]) {
+    __device__ __forceinline__ void gemv(const double (&z)[NT], double (&y)[NT]) {
         stage(zs, z);
         wave_sync();
         sweep<1>(zs, zs, y, y);
@@ -129,7 +215,7 @@ struct WaveNode {
 
     // y1 = Q z1 and y2 = Q z2 in one sweep over Q (the change-list buffer
     // stages z2; no update is in progress).
-    __device__ void gemv2(const double (&z1)[NT], double (&y1)[NT], const double (&z2)[NT], double (&y2)[NT]) {
+    __device__ __forceinline__ void gemv2(const double (&z1)[NT], double (&y1)[NT], const double (&z2)[NT], double (&y2)[NT]) {
         if constexpr (NT >= 8) {  // n > 256: two sweeps keep the state in registers
             gemv(z1, y1);
             gemv(z2, y2);
@@ -153,8 +239,8 @@ struct WaveNode {
 
     // y += sum over k with znew_k != zold_k of Q[:,k] (znew_k - zold_k), k
     // ascending (the FMA order per output coordinate is k order).
-    __device__ int update(const double (&znew)[NT], const double (&zold)[NT], double (&y)[NT]) {
-        const int n = P.n;
+    __device__ __forceinline__ int update(const double (&znew)[NT], const double (&zold)[NT], double (&y)[NT]) {
+        const int n = n_;
         int count = 0;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -202,8 +288,17 @@ struct WaveNode {
 
 // QL: Q is copied once into LDS and shared by the block's resident waves
 // (the block is persistent, so the copy is amortised over all its nodes).
-template <int NT, bool QL>
-__global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
+// SX: the bounded-simplex LMO (sum x <= rhs, or == rhs) over 0/1 integer
+// variables instead of the box.  With 0/1 bounds and an integral rhs every
+// capacity of the greedy (lmo.cuh:517-618) is 0 or 1 and every prefix mass an
+// exact integer, so its vertex is: v = lb, plus 1 on the first r = rhs - sum lb
+// eligible coordinates of capacity 1 in (gradient, index) order.  The kernel
+// selects them with wave_select (no sort); the in-face vertex
+// (lmo.cuh:814-863) and the maximum step (lmo.cuh:865-906) follow the same
+// formulas with the wave's sums.  A node whose bounds are not 0/1 or with a
+// variable not marked integer is not relaxed: status 0xFF.
+template <int NT, bool QL, bool SX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, NT >= 8 ? 1 : 8))) relax_fast_kernel(FastParams P) {
     extern __shared__ __align__(16) double lds_fast[];
     const int lane = lane_id();
     const int n = P.n;
@@ -216,7 +311,7 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
     // Per-wave LDS: staging vector (n_pad doubles), changed-column
     // coefficients (n_pad doubles), changed-column indices (n_pad ints).
     double* wbase = lds_fast + qsize + (size_t)wave_id() * (size_t)P.n_pad * 5 / 2;
-    WaveNode<NT, QL> W{P, QL ? lds_fast : P.Qt, lane, wbase, 0, 0,
+    WaveNode<NT, QL> W{P.n, QL ? lds_fast : P.Qt, lane, wbase, 0, 0,
                        reinterpret_cast<int*>(wbase + 2 * P.n_pad), wbase + P.n_pad};
     const double hs = 0.5 * P.gs;
     const double lsc_scale = P.gs * P.ls;
@@ -254,7 +349,47 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
                        || (fl[t] && ceil(lb[t]) > floor(ub[t]))))
                 valid = false;
         }
-        if (!wave_all(valid)) {
+        // Simplex: the 0/1 scope, then the node feasibility of
+        // Lmo<GB_LMO_SIMPLEX>::feasible (lmo.cuh:373-397) with wave sums.
+        double lbsum = 0.0;
+        bool node_ok = wave_all(valid);
+        if constexpr (SX) if (node_ok) {
+            bool binary = true;
+            double mand = 0.0, capsum = 0.0;
+            bool cont = false;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                if (j < n) {
+                    if (!P.is_int[j] || (lb[t] != 0.0 && lb[t] != 1.0) || (ub[t] != 0.0 && ub[t] != 1.0))
+                        binary = false;
+                    const bool integer = P.flags ? fl[t] : true;
+                    mand += fmax(0.0, lb[t]);
+                    capsum += ub[t];
+                    cont = cont || !integer;
+                }
+            }
+            if (!wave_all(binary)) {
+                if (lane == 0) {
+                    P.status[node] = 0xFF;
+                    P.value[node] = __builtin_nan("");
+                    P.gap[node] = __builtin_nan("");
+                    P.best_val[node] = kNone;
+                    P.iters[node] = initial;
+                }
+                continue;
+            }
+            const double ms = wave_sum_all(mand);
+            bool feas = ms <= P.rhs + 1e-9;
+            if (P.equality) {
+                const double cs = wave_sum_all(capsum);
+                const bool hc = !wave_all(!cont);
+                feas = feas && cs >= P.rhs - 1e-9 && (hc || fabs(P.rhs - round(P.rhs)) <= 1e-9);
+            }
+            node_ok = feas;
+            lbsum = ms;  // lb is 0/1: sum of lb == sum of max(0, lb)
+        }
+        if (!node_ok) {
             if (lane == 0) {
                 P.status[node] = GB_NODE_INFEASIBLE;
                 P.value[node] = __builtin_inf();
@@ -295,12 +430,36 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
             }
         };
 
-        // Starting point: LMO vertex of the artificial direction (ub for a
-        // box, since it is negative) or the supplied / warm-start iterate.
+        // Simplex vertex for the direction h: lb plus the first rhs - sum lb
+        // coordinates of capacity 1 among the eligible ones (h <= 0, or all
+        // of them for an equality) in (h, index) order.
+        auto sx_vertex = [&](const double (&h)[NT], double (&out)[NT]) {
+            unsigned long long key[NT];
+            bool c[NT], sel[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int j = lane + kWave * t;
-            x[t] = (j < n) ? ((initial == 0 && !supplied) ? ub[t] : xn[j]) : 0.0;
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                key[t] = order_key(h[t]);
+                c[t] = j < n && ub[t] > lb[t] && (P.equality || h[t] <= 0.0);
+            }
+            wave_select<NT>(key, c, P.rhs - lbsum, sel, lane);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) out[t] = sel[t] ? ub[t] : lb[t];
+        };
+        // Starting point: LMO vertex of the artificial direction
+        // g_j = -1 - j / (n + 1) (ub for a box, since it is negative) or the
+        // supplied / warm-start iterate.
+        if (SX && initial == 0 && !supplied) {
+            double h[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) h[t] = -1.0 - (double)(lane + kWave * t) / (n + 1.0);
+            sx_vertex(h, x);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int j = lane + kWave * t;
+                x[t] = (j < n) ? ((initial == 0 && !supplied) ? ub[t] : xn[j]) : 0.0;
+            }
         }
         // A candidate from an earlier phase (the mixed-precision pass) is
         // re-scored exactly before this node's own candidates; its product
@@ -322,18 +481,57 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
             for (int t = 0; t < NT; ++t) g[t] = P.gs * qx[t] + lsc[t];
         };
         auto fwd = [&](double (&out)[NT]) {
+            if constexpr (SX) {
+                sx_vertex(g, out);
+            } else {
 #pragma unroll
-            for (int t = 0; t < NT; ++t) out[t] = (g[t] > 0.0) ? lb[t] : ub[t];
+                for (int t = 0; t < NT; ++t) out[t] = (g[t] > 0.0) ? lb[t] : ub[t];
+            }
         };
-        // In-face away vertex at x (lmo.cuh:278-295 face tolerances).
+        // In-face away vertex at x: the LMO over the minimal face of x for
+        // the direction -g (box: lmo.cuh:278-295 face tolerances; simplex:
+        // lmo.cuh:814-863, coordinates at a bound stay fixed and the face
+        // keeps the full mass when sum x == rhs within tolerance).
         auto away = [&](double (&out)[NT]) {
+            if constexpr (SX) {
+                unsigned long long key[NT];
+                bool c[NT], sel[NT];
+                double m = 0.0, xm = 0.0;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const double ut = atol + rtol * fabs(ub[t]);
-                const double lt = atol + rtol * fabs(lb[t]);
-                if (fabs(x[t] - ub[t]) <= ut) out[t] = ub[t];
-                else if (fabs(x[t] - lb[t]) <= lt) out[t] = lb[t];
-                else out[t] = (-g[t] > 0.0) ? lb[t] : ub[t];
+                for (int t = 0; t < NT; ++t) {
+                    const int j = lane + kWave * t;
+                    bool fixed = false;
+                    out[t] = lb[t];
+                    if (j < n) {
+                        if (fabs(x[t] - lb[t]) <= atol + rtol * fmax(fabs(x[t]), fabs(lb[t]))) {
+                            fixed = true;
+                        } else if (fabs(x[t] - ub[t]) <= atol + rtol * fmax(fabs(x[t]), fabs(ub[t]))) {
+                            fixed = true;
+                            out[t] = ub[t];
+                        }
+                        m += out[t];
+                        xm += x[t];
+                    }
+                    key[t] = order_key(-g[t]);
+                    c[t] = j < n && !fixed && ub[t] > out[t];
+                }
+                const double mass = wave_sum_all(m);
+                const double xmass = wave_sum_all(xm);
+                const bool force = P.equality || fabs(xmass - P.rhs) <= atol + rtol * fmax(fabs(xmass), fabs(P.rhs));
+#pragma unroll
+                for (int t = 0; t < NT; ++t) c[t] = c[t] && (force || -g[t] <= 0.0);
+                wave_select<NT>(key, c, P.rhs - mass, sel, lane);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) out[t] = sel[t] ? ub[t] : out[t];
+            } else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const double ut = atol + rtol * fabs(ub[t]);
+                    const double lt = atol + rtol * fabs(lb[t]);
+                    if (fabs(x[t] - ub[t]) <= ut) out[t] = ub[t];
+                    else if (fabs(x[t] - lb[t]) <= lt) out[t] = lb[t];
+                    else out[t] = (-g[t] > 0.0) ? lb[t] : ub[t];
+                }
             }
         };
         grad();
@@ -389,19 +587,37 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
                 }
                 have_a = true;
                 score(a, qa);
-                double r = 1.0;
+                double r = 1.0, sx = 0.0, sd = 0.0;
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                     const int j = lane + kWave * t;
                     const double dv = a[t] - v[t];  // direction a - v as the reference
                     if (j < n) {
-                        if (dv < 0.0) r = fmin(r, (ub[t] - x[t]) / -dv);
-                        else if (dv > 0.0) r = fmin(r, (x[t] - lb[t]) / dv);
+                        if constexpr (SX) {
+                            if (dv > 1e-6) r = fmin(r, (x[t] - lb[t]) / dv);
+                            else if (dv < -1e-6) r = fmin(r, (ub[t] - x[t]) / -dv);
+                            sx += x[t];
+                            sd += dv;
+                        } else {
+                            if (dv < 0.0) r = fmin(r, (ub[t] - x[t]) / -dv);
+                            else if (dv > 0.0) r = fmin(r, (x[t] - lb[t]) / dv);
+                        }
                     }
                     d[t] = v[t] - a[t];
                     qd[t] = qv[t] - qa[t];
                 }
-                gmax = fmax(0.0, fmin(wave_min_all(r), 1.0));
+                if constexpr (SX) {
+                    // coordinate ratios, then the mass limit (lmo.cuh:865-906)
+                    const double coord = wave_min_all(r);
+                    const double tx = wave_sum_all(sx);
+                    const double td = wave_sum_all(sd);
+                    double agg;
+                    if (P.equality && fabs(td) > 1e-9) agg = 0.0;
+                    else agg = td < 0.0 ? fmin(1.0, (tx - P.rhs) / td) : 1.0;
+                    gmax = fmax(0.0, fmin(coord, agg));
+                } else {
+                    gmax = fmax(0.0, fmin(wave_min_all(r), 1.0));
+                }
             } else {
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
@@ -480,7 +696,21 @@ __global__ void __launch_bounds__(256) relax_fast_kernel(FastParams P) {
         if (P.flags) {
             double cand[NT];
             rounded(cand);
-            score(cand, qc);
+            bool take = true;
+            if constexpr (SX) {  // Lmo<GB_LMO_SIMPLEX>::rounded_feasible
+                bool nonneg = true;
+                double sum = 0.0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if (lane + kWave * t < n) {
+                        nonneg = nonneg && cand[t] >= -P.ctol;
+                        sum += cand[t];
+                    }
+                const double tot = wave_sum_all(sum);
+                take = wave_all(nonneg)
+                    && (P.equality ? fabs(tot - P.rhs) <= P.ctol : tot <= P.rhs + P.ctol);
+            }
+            if (take) score(cand, qc);
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -1197,9 +1427,11 @@ static int rows_capacity(size_t lds) {
 int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void* stream) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, p.counter);
-    if (waves_per_block == 0 && p.n > 64) {
+    const bool sx = p.simplex != 0;
+    if (waves_per_block == 0 && p.n > 64 && !sx) {
         // One node per workgroup when the whole batch is in flight at once;
-        // otherwise the one-wave-per-node layout keeps more nodes resident.
+        // otherwise the one-wave-per-node layout keeps more nodes resident
+        // (the simplex LMO runs in the one-wave-per-node layout only).
         const int nt = p.n <= 128 ? 2 : (p.n <= 256 ? 4 : 8);
         const size_t lds = rows_lds_bytes(p.n_pad, nt);
         const int cap = nt == 2 ? rows_capacity<2>(lds) : nt == 4 ? rows_capacity<4>(lds) : rows_capacity<8>(lds);
@@ -1222,9 +1454,9 @@ int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void
     const bool ql = q_lds + wave_lds <= kFastLdsBudget;
     const size_t lds = wave_lds + (ql ? q_lds : 0);
     const dim3 grid(blocks), block(waves_per_block * kWave);
-#define GB_FAST(NT, QL)                                                                   \
+#define GB_FAST_SX(NT, QL, SXK)                                                           \
     do {                                                                                  \
-        auto k = relax_fast_kernel<NT, QL>;                                               \
+        auto k = relax_fast_kernel<NT, QL, SXK>;                                          \
         if (lds > 64 * 1024) {                                                            \
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),          \
                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                    \
@@ -1232,12 +1464,18 @@ int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void
         }                                                                                 \
         hipLaunchKernelGGL(k, grid, block, lds, st, p);                                   \
     } while (0)
+#define GB_FAST(NT, QL)                                                                   \
+    do {                                                                                  \
+        if (sx) GB_FAST_SX(NT, QL, true);                                                 \
+        else GB_FAST_SX(NT, QL, false);                                                   \
+    } while (0)
     if (p.n <= 64) { if (ql) GB_FAST(1, true); else GB_FAST(1, false); }
     else if (p.n <= 128) { if (ql) GB_FAST(2, true); else GB_FAST(2, false); }
     else if (p.n <= 256) GB_FAST(4, false);
     else if (p.n <= 512) GB_FAST(8, false);
     else return (int)hipErrorInvalidValue;
 #undef GB_FAST
+#undef GB_FAST_SX
     return (int)hipGetLastError();
 }
 

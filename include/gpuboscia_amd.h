@@ -307,15 +307,21 @@ typedef struct {
 } gb_fast_options;
 
 /*
- * Batched-form relaxation of N nodes of a dense quadratic over a box (no
- * reference counterpart; it replaces the per-node block of
- * drs_fw_relax_batch_kernel, device_resident_solver.cuh:631-660, for the
- * box LMO).  One wave64 per node, persistent waves fed by a device node
+ * Batched-form relaxation of N nodes of a dense quadratic over a box or a
+ * bounded simplex (no reference counterpart; it replaces the per-node block
+ * of drs_fw_relax_batch_kernel, device_resident_solver.cuh:631-660, for
+ * these LMOs).  One wave64 per node, persistent waves fed by a device node
  * counter, incremental FP64 products Q x / Q v / Q a, exact quadratic step,
  * early stop once f(x) - gap >= cutoff[i] (cutoff may be NULL), and a final
  * direct FP64 certificate (gap, value) at the returned x.  Buffers and
  * status codes as gb_relax_batch; the algorithm descriptor selects FW or
  * DICG (its line search is ignored: the step is always exact).  n <= 512.
+ * Simplex (UnitSimplexLMO / EqualitySimplexLMO, lmo.cuh:335-912): the
+ * right side must be integral and every node 0/1 (is_int set for every
+ * variable, node bounds 0 or 1), where the greedy vertex is the first
+ * rhs - sum lb eligible coordinates in (gradient, index) order, as the
+ * reference's; a node outside that scope is not relaxed and gets status
+ * 0xFF (value and gap NaN).
  */
 gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
                               const gb_algorithm_desc* algorithm,
@@ -410,9 +416,10 @@ gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
  * compile time, device_resident_solver.cuh:362, :1217-1218): independent
  * node batches of different problems and sizes relaxed in one call.  Each
  * entry is relaxed like gb_relax_batch_mixed (when `mixed` is set: dense
- * objective, box LMO, n <= 512), like gb_relax_batch_fast (other box
- * entries of a dense objective, n <= 512), or like gb_relax_batch (every
- * other entry, e.g. a bounded simplex: FP64, the algorithm's own line
+ * objective, box LMO, n <= 512), like gb_relax_batch_fast (other entries of
+ * a dense objective with n <= 512 over a box, or over a simplex with an
+ * integral right side, whose nodes must then be 0/1), or like
+ * gb_relax_batch (every other entry: FP64, the algorithm's own line
  * search, cutoff unused), with the entry's own buffers, on one of
  * `concurrency` internal streams (1..8) forked from and joined back into
  * `stream`, so small batches of different instances share the GPU.  Every

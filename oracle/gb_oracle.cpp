@@ -1072,14 +1072,70 @@ struct FastNode {
 
 }  // namespace
 
-extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, int max_iters,
-                                     double* X, const double* LB, const double* UB,
-                                     const double* EPS, int* ITERS, uint8_t* STATUS, double* GAP,
-                                     double* VALUE, double* BV, double* BVV, const uint8_t* flags,
-                                     double ctol, const double* cutoff, int refresh, int rescore,
-                                     int threads) {
+// The batched form's simplex vertices (0/1 data: every capacity 0 or 1).
+// The greedy of lmo.cuh:517-618 restated as a sort of the eligible
+// coordinates by (direction, index); the first rem of them move to ub.
+namespace {
+void greedy01(int n, const double* h, const std::vector<char>& elig, double rem, const double* ub, double* out) {
+    std::vector<int> c;
+    for (int j = 0; j < n; ++j)
+        if (elig[j]) c.push_back(j);
+    std::sort(c.begin(), c.end(), [&](int a, int b) { return h[a] < h[b] || (h[a] == h[b] && a < b); });
+    const size_t take = rem >= (double)c.size() ? c.size() : (rem >= 1.0 ? (size_t)rem : 0);
+    for (size_t i = 0; i < take; ++i) out[c[i]] = ub[c[i]];
+}
+}  // namespace
+
+// Forward vertex for direction h: v = lb, eligible = capacity 1 and h <= 0
+// (every coordinate for an equality), rem = rhs - sum lb (block order).
+extern "C" void orc_fast_simplex_vertex(int n, const double* h, const double* lb, const double* ub, double rhs,
+                                        int equality, double* out) {
+    std::vector<char> elig(n);
+    for (int j = 0; j < n; ++j) {
+        out[j] = lb[j];
+        elig[j] = ub[j] > lb[j] && (equality || h[j] <= 0.0);
+    }
+    const double lbsum = block_sum(n, 64, [&](double acc, int j) { return acc + std::fmax(0.0, lb[j]); });
+    greedy01(n, h, elig, rhs - lbsum, ub, out);
+}
+
+// In-face vertex for direction h (the away vertex takes h = -g) at x
+// (lmo.cuh:814-863): coordinates at a bound (atol 1e-6, rtol 1e-4) stay,
+// the others start at lb; the face keeps the full mass when sum x == rhs.
+extern "C" void orc_fast_simplex_inface(int n, const double* h, const double* x, const double* lb,
+                                        const double* ub, double rhs, int equality, double* out) {
+    const double atol = 1e-6, rtol = 1e-4;
+    std::vector<char> fixed(n, 0), elig(n);
+    for (int j = 0; j < n; ++j) {
+        out[j] = lb[j];
+        if (std::fabs(x[j] - lb[j]) <= atol + rtol * std::fmax(std::fabs(x[j]), std::fabs(lb[j]))) {
+            fixed[j] = 1;
+        } else if (std::fabs(x[j] - ub[j]) <= atol + rtol * std::fmax(std::fabs(x[j]), std::fabs(ub[j]))) {
+            fixed[j] = 1;
+            out[j] = ub[j];
+        }
+    }
+    const double mass = block_sum(n, 64, [&](double acc, int j) { return acc + out[j]; });
+    const double xmass = block_sum(n, 64, [&](double acc, int j) { return acc + x[j]; });
+    const bool force = equality || std::fabs(xmass - rhs) <= atol + rtol * std::fmax(std::fabs(xmass), std::fabs(rhs));
+    for (int j = 0; j < n; ++j) elig[j] = !fixed[j] && ub[j] > out[j] && (force || h[j] <= 0.0);
+    greedy01(n, h, elig, rhs - mass, ub, out);
+}
+
+extern "C" void orc_relax_fast_batch_lmo(const orc_objective* op, const orc_lmo* lm, int dicg, int N,
+                                         int max_iters, double* X, const double* LB, const double* UB,
+                                         const double* EPS, int* ITERS, uint8_t* STATUS, double* GAP,
+                                         double* VALUE, double* BV, double* BVV, const uint8_t* flags,
+                                         double ctol, const double* cutoff, int refresh, int rescore,
+                                         int threads) {
     const orc_objective& o = *op;
     const int n = o.n;
+    // Bounded simplex over 0/1 variables (the batched form's scope): the
+    // greedy of lmo.cuh:517-618 with every capacity 0 or 1, restated as a
+    // sort of the eligible coordinates by (direction, index).
+    const bool sx = lm && lm->kind == ORC_LMO_SIMPLEX;
+    const double rhs = sx ? lm->right_side : 0.0;
+    const bool eq = sx && lm->equality;
     auto work = [&](int node) {
         FastNode F{o, n, std::vector<double>(n, 0.0), 0.5 * o.grad_scale};
         const double lsc_scale = o.grad_scale * o.linear_scale;
@@ -1102,6 +1158,39 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
                 ITERS[node] = initial;
                 return;
             }
+        if (sx) {
+            bool binary = true, cont = false;
+            for (int j = 0; j < n; ++j) {
+                if (!lm->is_int[j] || (lb[j] != 0.0 && lb[j] != 1.0) || (ub[j] != 0.0 && ub[j] != 1.0)) binary = false;
+                if (flags && !flags[j]) cont = true;
+            }
+            if (!binary) {
+                STATUS[node] = 0xFF;
+                VALUE[node] = std::nan("");
+                GAP[node] = std::nan("");
+                BVV[node] = kNone;
+                ITERS[node] = initial;
+                return;
+            }
+            // Lmo<SIMPLEX>::feasible (lmo.cuh:373-397)
+            const double ms = block_sum(n, 64, [&](double acc, int j) { return acc + std::fmax(0.0, lb[j]); });
+            bool feas = ms <= rhs + 1e-9;
+            if (eq) {
+                const double cs = block_sum(n, 64, [&](double acc, int j) { return acc + ub[j]; });
+                feas = feas && cs >= rhs - 1e-9 && (cont || std::fabs(rhs - std::round(rhs)) <= 1e-9);
+            }
+            if (!feas) {
+                STATUS[node] = 2;
+                VALUE[node] = kInf;
+                GAP[node] = 0.0;
+                BVV[node] = kNone;
+                ITERS[node] = initial;
+                return;
+            }
+        }
+        auto sx_vertex = [&](const std::vector<double>& h, std::vector<double>& out) {
+            orc_fast_simplex_vertex(n, h.data(), lb, ub, rhs, eq, out.data());
+        };
         double best = kNone;
         auto score = [&](const std::vector<double>& z, const std::vector<double>& qz) {
             bool ok = true;
@@ -1122,10 +1211,19 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
             F.gemv(z, qz);
             score(z, qz);
         }
-        for (int j = 0; j < n; ++j) x[j] = (initial == 0 && !supplied) ? ub[j] : xn[j];
+        if (sx && initial == 0 && !supplied) {
+            std::vector<double> h(n);
+            for (int j = 0; j < n; ++j) h[j] = -1.0 - (double)j / (n + 1.0);
+            sx_vertex(h, x);
+        } else {
+            for (int j = 0; j < n; ++j) x[j] = (initial == 0 && !supplied) ? ub[j] : xn[j];
+        }
         F.gemv(x, qx);
         auto grad = [&]() { for (int j = 0; j < n; ++j) g[j] = o.grad_scale * qx[j] + F.lsc[j]; };
-        auto fwd = [&](std::vector<double>& out) { for (int j = 0; j < n; ++j) out[j] = g[j] > 0.0 ? lb[j] : ub[j]; };
+        auto fwd = [&](std::vector<double>& out) {
+            if (sx) sx_vertex(g, out);
+            else for (int j = 0; j < n; ++j) out[j] = g[j] > 0.0 ? lb[j] : ub[j];
+        };
         grad();
         fwd(v);
         F.gemv(v, qv);
@@ -1147,12 +1245,18 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
             if (fx - gap >= cut) break;
             double gmax;
             if (dicg) {
-                for (int j = 0; j < n; ++j) {
-                    const double ut = atol + rtol * std::fabs(ub[j]);
-                    const double lt = atol + rtol * std::fabs(lb[j]);
-                    if (std::fabs(x[j] - ub[j]) <= ut) an[j] = ub[j];
-                    else if (std::fabs(x[j] - lb[j]) <= lt) an[j] = lb[j];
-                    else an[j] = (-g[j] > 0.0) ? lb[j] : ub[j];
+                if (sx) {  // simplex in-face vertex for -g (lmo.cuh:814-863) on 0/1 data
+                    std::vector<double> h(n);
+                    for (int j = 0; j < n; ++j) h[j] = -g[j];
+                    orc_fast_simplex_inface(n, h.data(), x.data(), lb, ub, rhs, eq, an.data());
+                } else {
+                    for (int j = 0; j < n; ++j) {
+                        const double ut = atol + rtol * std::fabs(ub[j]);
+                        const double lt = atol + rtol * std::fabs(lb[j]);
+                        if (std::fabs(x[j] - ub[j]) <= ut) an[j] = ub[j];
+                        else if (std::fabs(x[j] - lb[j]) <= lt) an[j] = lb[j];
+                        else an[j] = (-g[j] > 0.0) ? lb[j] : ub[j];
+                    }
                 }
                 if (have_a) F.update(an, a, qa);
                 else F.gemv(an, qa);
@@ -1164,12 +1268,27 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
                     double r = 1.0;
                     for (int j = l; j < n; j += 64) {
                         const double dv = a[j] - v[j];
-                        if (dv < 0.0) r = std::fmin(r, (ub[j] - x[j]) / -dv);
-                        else if (dv > 0.0) r = std::fmin(r, (x[j] - lb[j]) / dv);
+                        if (sx) {
+                            if (dv > 1e-6) r = std::fmin(r, (x[j] - lb[j]) / dv);
+                            else if (dv < -1e-6) r = std::fmin(r, (ub[j] - x[j]) / -dv);
+                        } else {
+                            if (dv < 0.0) r = std::fmin(r, (ub[j] - x[j]) / -dv);
+                            else if (dv > 0.0) r = std::fmin(r, (x[j] - lb[j]) / dv);
+                        }
                     }
                     lane[l] = r;
                 }
-                gmax = std::fmax(0.0, std::fmin(lanes_min(lane), 1.0));
+                if (sx) {  // dicg_maximum_step_local + the mass limit (lmo.cuh:865-906)
+                    const double coord = lanes_min(lane);
+                    const double tx = block_sum(n, 64, [&](double acc, int j) { return acc + x[j]; });
+                    const double td = block_sum(n, 64, [&](double acc, int j) { return acc + (a[j] - v[j]); });
+                    double agg;
+                    if (eq && std::fabs(td) > 1e-9) agg = 0.0;
+                    else agg = td < 0.0 ? std::fmin(1.0, (tx - rhs) / td) : 1.0;
+                    gmax = std::fmax(0.0, std::fmin(coord, agg));
+                } else {
+                    gmax = std::fmax(0.0, std::fmin(lanes_min(lane), 1.0));
+                }
                 for (int j = 0; j < n; ++j) { d[j] = v[j] - a[j]; qd[j] = qv[j] - qa[j]; }
             } else {
                 for (int j = 0; j < n; ++j) { d[j] = v[j] - x[j]; qd[j] = qv[j] - qx[j]; }
@@ -1196,7 +1315,14 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
             std::vector<double> cand(n), qc(n);
             for (int j = 0; j < n; ++j) cand[j] = flags[j] ? std::round(x[j]) : x[j];
             F.gemv(cand, qc);
-            score(cand, qc);
+            bool take = true;
+            if (sx) {  // Lmo<SIMPLEX>::rounded_feasible
+                bool nonneg = true;
+                for (int j = 0; j < n; ++j) nonneg = nonneg && cand[j] >= -ctol;
+                const double tot = block_sum(n, 64, [&](double acc, int j) { return acc + cand[j]; });
+                take = nonneg && (eq ? std::fabs(tot - rhs) <= ctol : tot <= rhs + ctol);
+            }
+            if (take) score(cand, qc);
         }
         bool integral = true;
         for (int j = 0; j < n; ++j) {
@@ -1219,6 +1345,16 @@ extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, i
     for (int t = 0; t < std::min(threads, N); ++t)
         pool.emplace_back([&] { for (int i = next.fetch_add(1); i < N; i = next.fetch_add(1)) work(i); });
     for (auto& th : pool) th.join();
+}
+
+extern "C" void orc_relax_fast_batch(const orc_objective* op, int dicg, int N, int max_iters,
+                                     double* X, const double* LB, const double* UB,
+                                     const double* EPS, int* ITERS, uint8_t* STATUS, double* GAP,
+                                     double* VALUE, double* BV, double* BVV, const uint8_t* flags,
+                                     double ctol, const double* cutoff, int refresh, int rescore,
+                                     int threads) {
+    orc_relax_fast_batch_lmo(op, nullptr, dicg, N, max_iters, X, LB, UB, EPS, ITERS, STATUS, GAP, VALUE, BV, BVV,
+                             flags, ctol, cutoff, refresh, rescore, threads);
 }
 
 // ---------------------------------------------------------------------------
@@ -1327,7 +1463,7 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
             sl.open_min[i] = i + 1 < N ? std::min(sl.batch[i].bound, sl.open_min[i + 1]) : sl.batch[i].bound;
         if (s->form == 1) {
             std::vector<double> CUT(N, inc);
-            orc_relax_fast_batch(o, a->dicg, N, s->max_iters, sl.X.data(), sl.L.data(), sl.U.data(), sl.E.data(),
+            orc_relax_fast_batch_lmo(o, l, a->dicg, N, s->max_iters, sl.X.data(), sl.L.data(), sl.U.data(), sl.E.data(),
                                  sl.IT.data(), sl.ST.data(), sl.G.data(), sl.V.data(), sl.BV.data(),
                                  sl.BVV.data(), flags.data(), s->integer_tol, CUT.data(), s->refresh, 0,
                                  s->threads);
@@ -1394,7 +1530,7 @@ extern "C" int orc_bnb_solve(const orc_objective* o, const orc_alg* a, const orc
                         if (mask & (1 << b)) c.lb[vi] = std::ceil(xi[vi]);
                         else c.ub[vi] = std::floor(xi[vi]);
                     }
-                    if (s->form == 1) {
+                    if (s->form == 1 && l->kind != ORC_LMO_SIMPLEX) {  // a clipped iterate may break sum x <= rhs
                         c.rx.resize(n);
                         for (int j = 0; j < n; ++j) c.rx[j] = std::min(std::max(xi[j], c.lb[j]), c.ub[j]);
                     }

@@ -134,6 +134,24 @@ void orc_relax_fast_batch(const orc_objective* o, int dicg, int N, int max_iters
                           double candidate_tol, const double* cutoff, int refresh, int rescore,
                           int threads);
 
+/* The batched form's simplex vertices on 0/1 data (lb, ub in {0,1},
+   integral rhs): forward vertex for direction h, and in-face vertex at x
+   for direction h (the away vertex passes h = -g). */
+void orc_fast_simplex_vertex(int n, const double* h, const double* lb, const double* ub, double rhs,
+                             int equality, double* out);
+void orc_fast_simplex_inface(int n, const double* h, const double* x, const double* lb,
+                             const double* ub, double rhs, int equality, double* out);
+
+/* The same with an LMO: NULL or a box as above; a simplex over 0/1
+   variables (every is_int set, 0/1 node bounds, integral rhs; other nodes
+   get status 0xFF) with the batched form's greedy vertex. */
+void orc_relax_fast_batch_lmo(const orc_objective* o, const orc_lmo* l, int dicg, int N, int max_iters,
+                              double* x, const double* lb, const double* ub, const double* eps,
+                              int* iterations, uint8_t* status, double* gap, double* value,
+                              double* best_vertex, double* best_vertex_val, const uint8_t* flags,
+                              double candidate_tol, const double* cutoff, int refresh, int rescore,
+                              int threads);
+
 /* Deterministic best-first B&B over orc_relax_batch. */
 typedef struct {
     int batch_size;

@@ -195,8 +195,9 @@ def enumerate_binary_qp(Q, c, threads=8):
 
 def relax_fast_batch(obj, dicg, x, lb, ub, eps, iterations, status, flags=None, max_iters=100,
                      candidate_tol=1e-6, cutoff=None, refresh=64, threads=1, best_vertex=None,
-                     best_vertex_val=None):
-    """orc_relax_fast_batch on numpy arrays (mirror of the batched-form kernel).
+                     best_vertex_val=None, lmo=None):
+    """orc_relax_fast_batch_lmo on numpy arrays (mirror of the batched-form
+    kernel; lmo None or a box: the box form, a simplex_lmo: the 0/1 simplex).
     best_vertex / best_vertex_val given: candidates of an earlier phase,
     re-scored first (the FP64 pass after the mixed-precision pass)."""
     lib = load()
@@ -208,9 +209,10 @@ def relax_fast_batch(obj, dicg, x, lb, ub, eps, iterations, status, flags=None, 
     bvv = np.full(N, 1e300) if best_vertex_val is None else np.ascontiguousarray(best_vertex_val, dtype=np.float64)
     fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
     cut = None if cutoff is None else f64(cutoff)
-    lib.orc_relax_fast_batch(ctypes.byref(obj), int(dicg), N, max_iters, p(x), p(lb), p(ub), p(eps),
-                             p(iterations), p(status), p(gap), p(value), p(bv), p(bvv), p(fl),
-                             ctypes.c_double(candidate_tol), p(cut), refresh, int(rescore), threads)
+    lib.orc_relax_fast_batch_lmo(ctypes.byref(obj), None if lmo is None else ctypes.byref(lmo), int(dicg), N,
+                                 max_iters, p(x), p(lb), p(ub), p(eps), p(iterations), p(status), p(gap),
+                                 p(value), p(bv), p(bvv), p(fl), ctypes.c_double(candidate_tol), p(cut),
+                                 refresh, int(rescore), threads)
     return {"x": x, "gap": gap, "value": value, "iterations": iterations, "status": status,
             "best_vertex": bv, "best_vertex_val": bvv}
 

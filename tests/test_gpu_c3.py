@@ -2,7 +2,7 @@
 log-uniform in [32, 512], each with a box or a bounded-simplex LMO
 (sum x <= ceil(n / 4)) chosen by seed and 1024 open nodes, relaxed by one
 gb_relax_instances call (box instances in the mixed form with 3 bf16 planes
-of Q and the FP64 certification pass, simplex instances in the plain FP64
+of Q and the FP64 certification pass, simplex instances in the batched FP64
 form).  Every node's outputs are checked for what must hold whatever the
 arithmetic (valid status, feasible iterate, FP64 bound f - gap no greater
 than the best vertex found), and four sampled instances are compared bit
@@ -15,7 +15,7 @@ import pytest
 
 import gpuboscia_amd as gb
 from gpuboscia_amd import instances
-from tests.gpu_util import assert_bit_equal, auto_T
+from tests.gpu_util import assert_bit_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -98,9 +98,10 @@ def check_properties(e):
 
 
 def compare_with_oracle(e, alg, rows):
-    """A slice of one instance's nodes against the oracle: the plain form
-    directly, the mixed form through its pass-1 output (a solo call of pass 1
-    only, which equals the in-call pass 1) and the oracle's FP64 pass."""
+    """A slice of one instance's nodes against the oracle: the batched FP64
+    form over the simplex directly, the mixed form through its pass-1 output
+    (a solo call of pass 1 only, which equals the in-call pass 1) and the
+    oracle's FP64 pass."""
     import torch
     from oracle import pyoracle as orc
     n = e["n"]
@@ -109,10 +110,9 @@ def compare_with_oracle(e, alg, rows):
     got = results(e, rows)
     ones = np.ones(n, np.uint8)
     if e["kind"] == "simplex":
-        o = orc.relax_batch(orc.dense_objective(e["Q"], e["c"], 2.0, 0.5, 0.0),
-                            orc.alg(dicg=True, ls=alg.line_search), orc.simplex_lmo(e["K"], ones),
-                            LB.copy(), LB.copy(), UB.copy(), EPS.copy(), np.zeros(N, np.int32),
-                            np.zeros(N, np.uint8), flags=ones, max_iters=BUDGET, lanes=auto_T(n), threads=8)
+        o = orc.relax_fast_batch(orc.dense_objective(e["Q"], e["c"], 2.0, 0.5, 0.0), True, LB.copy(), LB.copy(),
+                                 UB.copy(), EPS.copy(), np.zeros(N, np.int32), np.zeros(N, np.uint8), flags=ones,
+                                 max_iters=BUDGET, refresh=64, threads=8, lmo=orc.simplex_lmo(e["K"], ones))
         assert_bit_equal(got, o, f"simplex instance {e['k']} n={n}")
         return
     f64 = dict(dtype=torch.float64, device="cuda")

@@ -19,7 +19,7 @@ def say(*a):
 
 
 def make_entry(n, N, seed, mixed_planes, card=0):
-    """card > 0: a bounded-simplex entry (sum x <= card), relaxed in the plain FP64 form."""
+    """card > 0: a bounded-simplex entry (sum x <= card), relaxed in the batched FP64 form."""
     import torch
     Q, c, _ = instances.binary_qp(n, seed=seed)
     dQt, dc = dev(np.ascontiguousarray(Q.T)), dev(c)
@@ -64,12 +64,12 @@ def results(e):
 @pytest.mark.parametrize("concurrency", [1, 4])
 def test_instances_equal_solo_runs(concurrency):
     import torch
-    sizes = [(32, 40, 1), (45, 17, 3), (64, 100, 3), (100, 64, 1), (150, 33, 3), (200, 80, 1), (256, 20, 3),
-             (300, 50, 1), (420, 25, 3), (512, 16, 0)]
+    sizes = [(32, 40, 1, 0), (45, 17, 3, 0), (64, 100, 3, 0), (70, 30, 0, 18), (100, 64, 1, 0), (150, 33, 3, 0),
+             (200, 80, 1, 0), (256, 20, 3, 0), (260, 24, 0, 65), (300, 50, 1, 0), (420, 25, 3, 0), (512, 16, 0, 0)]
     alg = gb.algorithm(dicg=True)
     budget = 250
-    together = [make_entry(n, N, 100 + k, planes) for k, (n, N, planes) in enumerate(sizes)]
-    solo = [make_entry(n, N, 100 + k, planes) for k, (n, N, planes) in enumerate(sizes)]
+    together = [make_entry(n, N, 100 + k, planes, card) for k, (n, N, planes, card) in enumerate(sizes)]
+    solo = [make_entry(n, N, 100 + k, planes, card) for k, (n, N, planes, card) in enumerate(sizes)]
     say(f"[instances] {len(sizes)} instances, n from 32 to 512, concurrency {concurrency}")
     gb.relax_instances(together, alg, budget, concurrency=concurrency)
     for e in solo:
@@ -138,7 +138,7 @@ def test_instances_captured_call_replays_match_eager():
 
 def test_instances_captured_call_with_simplex_entries():
     """Box entries (mixed and batched forms) and bounded-simplex entries
-    (plain FP64 form) in one captured gb_relax_instances call: every replay,
+    (batched FP64 form) in one captured gb_relax_instances call: every replay,
     with refills in between, equals the eager call."""
     import torch
     specs = [(48, 40, 1, 0), (150, 33, 3, 38), (300, 20, 0, 0), (64, 25, 0, 16), (420, 16, 3, 0)]
