@@ -1,4 +1,5 @@
-// Batched-form node relaxation for dense quadratics over boxes, gfx950.
+// Batched-form node relaxation for dense quadratics over boxes and over
+// bounded simplices of 0/1 variables, gfx950.
 //
 // One wave64 owns one B&B node; waves are persistent and pull node indices
 // from a device counter, so converged nodes hand their wave to the next node
@@ -233,7 +234,7 @@ struct WaveNode {
     // and their loads issued kGroup at a time before the FMAs: the vertex of
     // an interior iterate flips many coordinates per iteration, and one L2
     // round trip per column would leave the wave latency-bound.
-    static constexpr int kGroup = NT >= 4 ? 4 : 8;
+    static constexpr int kGroup = NT >= 8 ? 8 : (NT >= 4 ? 4 : 8);
     int* klist;      // this wave's LDS list of changed columns (n ints)
     double* dlist;   // their coefficients (n doubles)
 
