@@ -256,8 +256,9 @@ def hetero_batches(args):
     by seed (instances.batched_lmos) and args.c3_nodes open nodes
     (instances.open_nodes), relaxed by ONE gb_relax_instances call over 8
     streams: box instances in the mixed form (args.c3_planes bf16 planes of
-    Q on MFMA with FP32 accumulation, then the FP64 certification pass),
-    bounded-simplex instances in the batched FP64 form; DICG with the exact
+    Q on MFMA with FP32 accumulation, then the FP64 certification pass) for
+    every instance, the simplex's greedy vertices picked by one wave per
+    node column; DICG with the exact
     step, budget args.budget iterations per node.  HIP events on the calling
     stream bracket each call; the same call captured as one HIP graph is
     replayed on refilled inputs."""
@@ -282,7 +283,7 @@ def hetero_batches(args):
         lmo = (gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0) if kind == "box"
                else gb.LmoDesc(gb.LMO_SIMPLEX, float(K), 0, flags.data_ptr(), 0))
         e = {"obj": obj, "lmo": lmo, "keep": (dQt, dc), "kind": kind,
-             "mixed": gb.MixedQ(obj, args.c3_planes, stream.cuda_stream) if kind == "box" else None,
+             "mixed": gb.MixedQ(obj, args.c3_planes, stream.cuda_stream),
              "x": torch.from_numpy(LB).cuda(), "lb": torch.from_numpy(LB).cuda(), "ub": torch.from_numpy(UB).cuda(),
              "eps": torch.from_numpy(EPS).cuda(), "iterations": torch.zeros(N, dtype=torch.int32, device="cuda"),
              "status": torch.zeros(N, dtype=torch.uint8, device="cuda"), "gap": torch.zeros(N, **f64),
@@ -344,8 +345,8 @@ def hetero_batches(args):
     return {"instances": len(entries), "box_instances": kinds.count("box"),
             "simplex_instances": kinds.count("simplex"), "nodes_per_instance": N, "n_min": int(sizes.min()),
             "n_max": int(sizes.max()),
-            "precision": f"box: {args.c3_planes} bf16 plane(s) of Q + FP64 certification; "
-                         "bounded simplex: batched FP64 form (one wave per node)",
+            "precision": f"box and bounded simplex: {args.c3_planes} bf16 plane(s) of Q on MFMA (FP32 "
+                         "accumulate, FP32 iterates) + FP64 certification pass",
             "algorithm": "DICG, exact quadratic step", "budget": args.budget,
             "call_ms": t * 1e3, "captured_replay_ms": float(np.mean(replay)) * 1e3,
             "replay_iterations_equal": iters_replay == iters,

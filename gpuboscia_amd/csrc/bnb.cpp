@@ -926,9 +926,9 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "the batched form supports dense objectives with n <= 512 over a box, or over a simplex "
                     "with an integral right side");
-    if (s->form == 1 && l->kind == GB_LMO_SIMPLEX) {
-        // the batched form's simplex relaxes 0/1 nodes: every variable
-        // integer with root bounds 0 or 1 (branching keeps them so)
+    if (s->form != 0 && l->kind == GB_LMO_SIMPLEX) {
+        // the batched and mixed forms' simplex relaxes 0/1 nodes: every
+        // variable integer with root bounds 0 or 1 (branching keeps them so)
         bool ok = pr->n_int_vars == pr->n_vars && s->branch_rule != GB_BRANCH_CUSTOM && !s->device_frontier;
         for (int j = 0; j < pr->n_vars && ok; ++j)
             ok = (pr->lb[j] == 0.0 || pr->lb[j] == 1.0) && (pr->ub[j] == 0.0 || pr->ub[j] == 1.0);
@@ -942,10 +942,13 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
                   || s->device_frontier || s->branch_rule == GB_BRANCH_LEARNED))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "the QAP's FP32 form relaxes over its Birkhoff polytope (order <= 256), on the host search");
-    if (s->form == 2 && !qap32 && (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || o->n > 512))
+    if (s->form == 2 && !qap32
+            && (o->kind != GB_OBJ_DENSE_QUADRATIC || o->n > 512 || (l->kind != GB_LMO_BOX && !fast_simplex_ok(l))))
         return fail(GB_ERR_INVALID_ARGUMENT,
-                    "the mixed-precision form supports dense objectives over a box with n <= 512, "
-                    "or a QAP over its Birkhoff polytope");
+                    "the mixed-precision form supports dense objectives with n <= 512 over a box or a simplex "
+                    "with an integral right side, or a QAP over its Birkhoff polytope");
+    if (s->form == 2 && l->kind == GB_LMO_SIMPLEX && s->algorithm.algorithm != GB_ALG_DICG)
+        return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision form runs DICG over a simplex");
     if (s->plugin && (s->form != 0 || s->blocks_per_node > 1 || s->device_frontier))
         return fail(GB_ERR_INVALID_ARGUMENT,
                     "device plug-ins run in the plain form with one workgroup per node (form = 0, SingleBlock)");
@@ -1545,7 +1548,7 @@ gb_status solve_impl(const gb_problem* pr, const gb_objective_desc* o, const gb_
                                                + std::to_string(db.hst[i])
                                                + "): its MultiBlock workgroups were not resident together");
         // status 0xFF from the batched form's simplex: a node outside its 0/1 scope
-        for (int i = 0; i < N && s->form == 1 && l->kind == GB_LMO_SIMPLEX; ++i)
+        for (int i = 0; i < N && s->form != 0 && l->kind == GB_LMO_SIMPLEX; ++i)
             if (db.hst[i] > GB_NODE_INTEGRAL_CONVERGED)
                 return fail(GB_ERR_DEVICE, "the batched form's simplex met a node that is not 0/1");
         emit(GB_SCHED_COMPLETION_DETACHED, head, N);

@@ -52,6 +52,12 @@ struct MixedParams {
     // iterations (launch_mixed_warm_products); such nodes start at x itself
     const double* qx0;
     const int* count;       // optional device batch size: nodes [0, *count) of the N rows are relaxed
+    // Bounded simplex (simplex != 0): sum x <= rhs (== rhs when equality)
+    // over 0/1 variables all marked in is_int (device, n bytes), DICG only.
+    int simplex;
+    int equality;
+    double rhs;
+    const uint8_t* is_int;
 };
 
 int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double ls, int n, int rt,
@@ -62,19 +68,23 @@ int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double 
 //              and 2s+1 as 8 bf16 (1 KB per k-step);
 //  xch         per wave, Q v and Q a of its tiles as [array][j][column][row]
 //              floats (2 rw KB per wave);
-//  red         per-column partial sums of each wave, 8 floats;
+//  red         per-column partial sums of each wave, mixed_red(sx) floats;
 //  fetch, ok   fetched node ids, per-wave scope flags;
+//  sel         (simplex) per column: the gradient of every row (float), the
+//              row flags (byte) and the column's feasibility (int);
 //  qf          (optional) the Q fragments, planes x rt x rt/2 KB.
 // At most 8 + 8 + 32 + 8 KB for n <= 256 and 16 + 16 + 64 + 4 KB for n <= 512,
 // plus the fragments when copied.
 struct MixedLds {
-    size_t vops, aops, xch, red, fetch, ok, qf, total;
+    size_t vops, aops, xch, red, fetch, ok, sg, sf, sok, qf, total;
 };
+// reduction slots per (wave, column): 8, or 12 with the simplex's mass sums
+__host__ __device__ constexpr int mixed_red(bool sx) { return sx ? 12 : 8; }
 constexpr size_t kMixedLdsBudget = 160 * 1024;  // LDS of one CU
 __host__ __device__ inline size_t mixed_q_bytes(int rt, int planes) {
     return static_cast<size_t>(planes) * rt * (rt / 2) * 1024;
 }
-__host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw, size_t q_bytes) {
+__host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw, size_t q_bytes, bool sx = false) {
     MixedLds L{};
     size_t o = 0;
     L.vops = o;
@@ -84,11 +94,18 @@ __host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw, 
     L.xch = o;
     o += sizeof(float) * 2 * 256 * static_cast<size_t>(rw) * waves;
     L.red = o;
-    o += sizeof(float) * 8 * 16 * static_cast<size_t>(waves);
+    o += sizeof(float) * mixed_red(sx) * 16 * static_cast<size_t>(waves);
     L.fetch = o;
     o += sizeof(int) * 16;
     L.ok = o;
     o += sizeof(int) * static_cast<size_t>(waves) * 16;
+    const size_t np = 16 * static_cast<size_t>(rt);
+    L.sg = o;
+    o += sx ? sizeof(float) * 16 * np : 0;
+    L.sf = o;
+    o += sx ? 16 * np : 0;
+    L.sok = o;
+    o += sx ? sizeof(int) * 16 : 0;
     o = (o + 15) & ~static_cast<size_t>(15);
     L.qf = o;
     o += q_bytes;

@@ -21,8 +21,9 @@ gb_status check_mixed_scope(const gb_mixed_q* q, const gb_objective_desc* o, con
     if (!q || !q->frag) return fail(GB_ERR_INVALID_ARGUMENT, "null mixed-precision Q");
     if (o && (o->kind != GB_OBJ_DENSE_QUADRATIC || o->n != q->n))
         return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision Q does not match the objective");
-    if (l && l->kind != GB_LMO_BOX)
-        return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision form supports the box LMO");
+    if (l && l->kind != GB_LMO_BOX && !fast_simplex_ok(l))
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "the mixed-precision form supports the box LMO, or a simplex with an integral right side");
     return GB_OK;
 }
 
@@ -125,6 +126,9 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
         if (st != GB_OK) return st;
         st = check_mixed_scope(q, objective, lmo);
         if (st != GB_OK) return st;
+        const bool simplex = lmo->kind == GB_LMO_SIMPLEX;
+        if (simplex && algorithm->algorithm != GB_ALG_DICG)
+            return fail(GB_ERR_INVALID_ARGUMENT, "the mixed-precision form runs DICG over a simplex");
         if (N < 0 || max_iters < 0) return fail(GB_ERR_INVALID_ARGUMENT, "N and max_iters must be non-negative");
         if (N == 0) return GB_OK;
         if (!x || !lb || !ub || !eps || !iterations || !status || !gap || !value || !best_vertex
@@ -167,6 +171,10 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
             P.rel_floor = op.rel_floor;
             P.stats = op.mixed_iterations;
             P.count = count;
+            P.simplex = simplex ? 1 : 0;
+            P.equality = simplex && lmo->equality ? 1 : 0;
+            P.rhs = simplex ? lmo->right_side : 0.0;
+            P.is_int = simplex ? lmo->is_int : nullptr;
             MixedShape sh = mixed_default_shape(q->rt, N);
             if (op.row_tiles_per_wave) sh.rw = op.row_tiles_per_wave;
             if (op.columns) sh.cols = op.columns;
@@ -177,7 +185,7 @@ gb_status relax_mixed_impl(const gb_mixed_q* q, const gb_objective_desc* objecti
             if (op.blocks) sh.blocks = op.blocks;
             else sh.blocks = (N + sh.cols - 1) / sh.cols;
             P.cols = sh.cols;
-            if (mixed_warm_start(sh.rw, q->rt > kMixedSmallRowTiles)) {
+            if (!simplex && mixed_warm_start(sh.rw, q->rt > kMixedSmallRowTiles)) {
                 // Nodes with a supplied point start there: Q x in FP64 goes
                 // into their best-vertex rows, which pass 1 reads when it takes
                 // the node and overwrites when it retires it.

@@ -46,6 +46,7 @@
 
 #include "gb_device.h"
 #include "gb_mixed.h"
+#include "gb_select.h"
 
 namespace gb {
 namespace {
@@ -170,6 +171,69 @@ __global__ void mixed_prepare_kernel(const double* Qt, const double* lin, double
 
 constexpr int kPhaseDone = 0, kPhaseInit = 1, kPhaseActive = 2, kPhaseFetch = 3;
 
+// Simplex vertices of one node column, computed by one wave from the
+// column's rows staged in LDS: g (float) and flags (bit 0: lb = 1, bit 1:
+// ub = 1, bit 2: x at a bound, bit 3: x at its upper bound).  The forward
+// vertex v is the greedy's for g (for the artificial direction
+// g_j = -1 - j / (n + 1) while the column initialises); the in-face vertex a
+// the greedy's over the free rows of the face for -g, the face keeping the
+// full mass when sum x (txm) equals rhs within tolerance (lmo.cuh:517-618,
+// :814-863; 0/1 capacities, so wave_select gives their vertices exactly on
+// the FP32 gradient).  Results go back into the flags as bits 4 (v) and 5
+// (a); *ok = 0 for a node the set cannot hold (sum lb > rhs, or for an
+// equality sum ub < rhs).
+template <int NTS>
+__device__ __forceinline__ void mixed_simplex_column(const float* sgc, uint8_t* sfc, int n, double rhs, bool eq,
+                                                     bool ini, float txm, int lane, int* ok) {
+    unsigned long long key[NTS];
+    bool c[NTS], sel[NTS], a[NTS];
+    uint32_t f[NTS];
+    int lbc = 0, ubc = 0;
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) {
+        const int j = lane + kWave * t;
+        const bool in = j < n;
+        f[t] = in ? sfc[j] : 0u;
+        const double h = !in ? 0.0 : (ini ? -1.0 - static_cast<double>(j) / (n + 1.0) : static_cast<double>(sgc[j]));
+        key[t] = order_key(h);
+        c[t] = in && (f[t] & 2u) && !(f[t] & 1u) && (eq || h <= 0.0);
+        lbc += __popcll(__ballot(in && (f[t] & 1u)));
+        ubc += __popcll(__ballot(in && (f[t] & 2u)));
+    }
+    const bool feasible = lbc <= rhs + 1e-9 && (!eq || ubc >= rhs - 1e-9);
+    wave_select<NTS>(key, c, rhs - lbc, sel, lane);
+    uint32_t vb[NTS];
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) vb[t] = ((f[t] & 1u) || sel[t]) ? 16u : 0u;
+    int mass = 0;
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) {
+        const bool in = lane + kWave * t < n;
+        a[t] = (f[t] & 1u) || (f[t] & 12u) == 12u;  // lb, or ub where x sits at its upper bound
+        mass += __popcll(__ballot(in && a[t]));
+    }
+    if (!ini) {
+        const bool force = eq || fabs(static_cast<double>(txm) - rhs) <=
+                                     1e-6 + 1e-4 * fmax(fabs(static_cast<double>(txm)), fabs(rhs));
+#pragma unroll
+        for (int t = 0; t < NTS; ++t) {
+            const int j = lane + kWave * t;
+            const bool in = j < n;
+            const double h = in ? -static_cast<double>(sgc[j]) : 0.0;
+            key[t] = order_key(h);
+            c[t] = in && !(f[t] & 4u) && (f[t] & 2u) && !a[t] && (force || h <= 0.0);
+        }
+        wave_select<NTS>(key, c, rhs - mass, sel, lane);
+    }
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) {
+        const int j = lane + kWave * t;
+        const bool ab = !ini && (a[t] || sel[t]);
+        if (j < n) sfc[j] = static_cast<uint8_t>(f[t] | vb[t] | (ab ? 32u : 0u));
+    }
+    if (lane == 0) *ok = feasible ? 1 : 0;
+}
+
 // The wave's Q fragments are the same in every pass: where they fit beside
 // the rest of the state they are loaded once and kept in registers for the
 // kernel's lifetime.  CS: the k-steps held (0: none): 8 (n <= 256) for one
@@ -187,10 +251,12 @@ constexpr int mixed_cached_steps(int rw, int pl, int rpt) {
 // QL: otherwise the tile copies all fragments into LDS once (when they fit)
 // and reads them from there each pass instead of from L2.  BIG: 17..32 row
 // tiles (n up to 512), 4 row tiles per wave, up to 16 k-steps.
-template <int RW, int PL, bool DICG, int RPT, bool QL, int CS, bool BIG = false>
+template <int RW, int PL, bool DICG, int RPT, bool QL, int CS, bool BIG = false, bool SX = false>
 __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 1) / RW)))
     relax_mixed_kernel(MixedParams P) {
     static_assert(!BIG || (RW == 4 && !QL && CS == 0), "wide tiles run 4 row tiles per wave from L2");
+    static_assert(!SX || (DICG && !QL), "the simplex runs DICG, without an LDS copy of Q");
+    constexpr int kRed = mixed_red(SX);
     constexpr int G = 16 / RPT;  // lanes of a column in the element layout
     constexpr int E = RW * RPT;  // elements per lane: rows 16 t_j + RPT m + r, e = RPT j + r
     constexpr unsigned long long kGroupMask = (1ull << G) - 1ull;
@@ -205,7 +271,10 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
     const int RT = P.rt;
     const int S = RT / 2;
     const int W = blockDim.x / kWave;
-    const MixedLds L = mixed_lds_layout(RT, W, RW, QL ? mixed_q_bytes(RT, PL) : 0);
+    const MixedLds L = mixed_lds_layout(RT, W, RW, QL ? mixed_q_bytes(RT, PL) : 0, SX);
+    float* sg = reinterpret_cast<float*>(mixed_smem + L.sg);
+    uint8_t* sf = reinterpret_cast<uint8_t*>(mixed_smem + L.sf);
+    int* sok = reinterpret_cast<int*>(mixed_smem + L.sok);
     uint4* vops = reinterpret_cast<uint4*>(mixed_smem + L.vops);
     uint4* aops = reinterpret_cast<uint4*>(mixed_smem + L.aops);
     float* red = reinterpret_cast<float*>(mixed_smem + L.red);
@@ -262,6 +331,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                 }
     }
     uint32_t lbm = 0, ubm = 0, x0m = 0, bestm = 0;
+    float txm = 0.0f;  // simplex: sum x of the column's iterate
     int phase = col < P.cols ? kPhaseFetch : kPhaseDone;
     int node = -1, it = 0, initial = 0;
     double eps = 0.0, cutoff = 0.0, best = kNone;
@@ -292,7 +362,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                 init_it = P.iters[node];
                 const bool from_x = init_it != 0 || (P.status[node] & GB_NODE_INITIAL_POINT_SUPPLIED) != 0;
                 // a supplied point with Q x at hand: the column starts at x itself
-                warm = kWarmStart && P.qx0 != nullptr && from_x && init_it == 0;
+                warm = kWarmStart && !SX && P.qx0 != nullptr && from_x && init_it == 0;
 #pragma unroll 1
                 for (int e = 0; e < E; ++e) {
                     const int t = w + W * (e / RPT);
@@ -300,6 +370,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                     if (t < RT && row < n) {
                         const double l = lbn[row], u = ubn[row];
                         if (!(l <= u) || !(l == 0.0 || l == 1.0) || !(u == 0.0 || u == 1.0)) ok = false;
+                        if (SX && !P.is_int[row]) ok = false;
                         const double xv = from_x ? xn[row] : u;
                         const bool xb = (xv >= 0.5 * (l + u)) ? (u == 1.0) : (l == 1.0);
                         lm |= static_cast<uint32_t>(l == 1.0) << e;
@@ -347,13 +418,76 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             __syncthreads();
         }
         if (__ballot(phase != kPhaseDone) == 0ull) break;
-        const bool act = phase == kPhaseActive;
-        const bool ini = phase == kPhaseInit;
+        bool act = phase == kPhaseActive;
+        bool ini = phase == kPhaseInit;
 
         // Vertices of the lane's rows, partial sums at x.
         uint32_t vm = 0, am = 0;
         float gp = 0.0f, fp = 0.0f;
         float g[E];
+        if constexpr (SX) {
+            // The column's rows go to LDS, one wave per column picks both
+            // vertices, and the rows read their bits back.
+            const int NP = 16 * RT;
+            float* sgc = sg + col * NP;
+            uint8_t* sfc = sf + col * NP;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int t = w + W * (e / RPT);
+                const int row = 16 * t + RPT * m + e % RPT;
+                const bool lb1 = (lbm >> e) & 1u;
+                const bool ub1 = (ubm >> e) & 1u;
+                g[e] = gs * qx[e] + ls[e];
+                if (t < RT && row < n && (act || ini)) {
+                    const float lf = lb1 ? 1.0f : 0.0f, uf = ub1 ? 1.0f : 0.0f;
+                    const bool at_l = fabsf(x[e] - lf) <= 1e-6f + 1e-4f * fmaxf(fabsf(x[e]), lf);
+                    const bool at_u = !at_l && fabsf(x[e] - uf) <= 1e-6f + 1e-4f * fmaxf(fabsf(x[e]), uf);
+                    sgc[row] = g[e];
+                    sfc[row] = static_cast<uint8_t>((lb1 ? 1u : 0u) | (ub1 ? 2u : 0u) | ((at_l || at_u) ? 4u : 0u) |
+                                                    (at_u ? 8u : 0u));
+                }
+            }
+            __syncthreads();
+            const int nts = (n + kWave - 1) / kWave;
+            for (int k = w; k < kWave / G; k += W) {  // the tile's 4 RPT columns
+                const int kp = __builtin_amdgcn_readlane(phase, k * G);
+                if (kp != kPhaseActive && kp != kPhaseInit) continue;
+                const float kt = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, txm), k * G));
+                const bool kini = kp == kPhaseInit;
+                const float* gk = sg + k * NP;
+                uint8_t* fk = sf + k * NP;
+                if constexpr (BIG) {
+                    mixed_simplex_column<8>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
+                } else {
+                    if (nts <= 1) mixed_simplex_column<1>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
+                    else if (nts <= 2) mixed_simplex_column<2>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
+                    else mixed_simplex_column<4>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int t = w + W * (e / RPT);
+                const int row = 16 * t + RPT * m + e % RPT;
+                if (t < RT && row < n) {
+                    const uint32_t fl = sfc[row];
+                    vm |= ((fl >> 4) & 1u) << e;
+                    am |= ((fl >> 5) & 1u) << e;
+                }
+            }
+            if ((act || ini) && sok[col] == 0) {
+                // a node the set cannot hold: left to the FP64 kernel
+                act = false;
+                ini = false;
+                phase = kPhaseFetch;
+            }
+            if (ini) x0m = vm;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                gp += g[e] * (x[e] - (((vm >> e) & 1u) ? 1.0f : 0.0f));
+                fp += x[e] * (hs * qx[e] + ls[e]);
+            }
+        } else
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const bool lb1 = (lbm >> e) & 1u;
@@ -550,11 +684,17 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
         // Vertex values and step partials; Q d replaces Q v (Q v for a column
         // that is being initialised).
         float fvp = 0.0f, fap = 0.0f, pgp = 0.0f, cvp = 0.0f, rr = 1.0f;
+        float sxp = 0.0f, sdp = 0.0f, svp = 0.0f;  // simplex: sum x, sum (v - a), sum v
         float qd[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const float vf = ((vm >> e) & 1u) ? 1.0f : 0.0f;
             fvp += vf * (hs * qv[e] + ls[e]);
+            if (SX) {
+                sxp += x[e];
+                sdp += vf - (((am >> e) & 1u) ? 1.0f : 0.0f);
+                svp += vf;
+            }
             float d, qde;
             if (DICG) {
                 const float af = ((am >> e) & 1u) ? 1.0f : 0.0f;
@@ -575,31 +715,55 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             qd[e] = ini ? qv[e] : qde;
         }
         // Column totals: the column's lanes (DPP), then the waves in order.
-        float part[8] = {group_sum<G>(gp), group_sum<G>(fp), group_sum<G>(fvp), DICG ? group_sum<G>(fap) : 0.0f,
-                         group_sum<G>(pgp), group_sum<G>(cvp), DICG ? group_min<G>(rr) : 1.0f, 0.0f};
+        float part[kRed];
+        part[0] = group_sum<G>(gp);
+        part[1] = group_sum<G>(fp);
+        part[2] = group_sum<G>(fvp);
+        part[3] = DICG ? group_sum<G>(fap) : 0.0f;
+        part[4] = group_sum<G>(pgp);
+        part[5] = group_sum<G>(cvp);
+        part[6] = DICG ? group_min<G>(rr) : 1.0f;
+        part[7] = SX ? group_sum<G>(sxp) : 0.0f;
+        if constexpr (SX) {
+            part[8] = group_sum<G>(sdp);
+            part[9] = group_sum<G>(svp);
+            part[10] = 0.0f;
+            part[11] = 0.0f;
+        }
         if (m == 0) {
-            float* o = red + (w * 16 + col) * 8;
+            float* o = red + (w * 16 + col) * kRed;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) o[k] = part[k];
+            for (int k = 0; k < kRed; ++k) o[k] = part[k];
         }
         __syncthreads();
-        float tot[7];
+        constexpr int kTot = SX ? 10 : 7;
+        float tot[kTot];
         {
-            const float* o = red + col * 8;
+            const float* o = red + col * kRed;
 #pragma unroll
-            for (int k = 0; k < 7; ++k) tot[k] = o[k];
+            for (int k = 0; k < kTot; ++k) tot[k] = o[k];
         }
         for (int v = 1; v < W; ++v) {
-            const float* o = red + (v * 16 + col) * 8;
+            const float* o = red + (v * 16 + col) * kRed;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) tot[k] = tot[k] + o[k];
-            tot[6] = fminf(tot[6], o[6]);
+            for (int k = 0; k < kTot; ++k) {
+                if (k == 6) tot[6] = fminf(tot[6], o[6]);
+                else tot[k] = tot[k] + o[k];
+            }
         }
         const float gap = tot[0];
         const double fx = static_cast<double>(tot[1]) + P.kappa;
         const double fv = static_cast<double>(tot[2]) + P.kappa;
         const double fa = DICG ? static_cast<double>(tot[3]) + P.kappa : kNone;
-        const float gmax = DICG ? fmaxf(0.0f, fminf(tot[6], 1.0f)) : 1.0f;
+        float gmax = DICG ? fmaxf(0.0f, fminf(tot[6], 1.0f)) : 1.0f;
+        if constexpr (SX) {
+            // the mass limit of the maximum step (lmo.cuh:865-906), d = a - v
+            const double tx = tot[7], td = -static_cast<double>(tot[8]);
+            double agg;
+            if (P.equality && fabs(td) > 1e-9) agg = 0.0;
+            else agg = td < 0.0 ? fmin(1.0, (tx - P.rhs) / td) : 1.0;
+            gmax = fmaxf(0.0f, fminf(gmax, static_cast<float>(agg)));
+        }
 
         bool retire = false;
         float gm = 0.0f;
@@ -636,6 +800,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             x[e] = ini ? x0 : (step ? nx : x[e]);
             qx[e] = ini ? qd[e] : (step ? nq : qx[e]);
         }
+        if constexpr (SX) txm = ini ? tot[9] : (step ? fmaf(gm, tot[8], tot[7]) : tot[7]);
         if (ini) phase = kPhaseActive;
 
         if (retire) {
@@ -773,23 +938,27 @@ int start_mixed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, con
     return 0;
 }
 
-template <int RW, int PL, bool DI, int RPT>
+template <int RW, int PL, bool DI, int RPT, bool SX = false>
 int launch_mixed_shape(const MixedParams& p, dim3 grid, dim3 block, int waves, hipStream_t st) {
-    const size_t base = mixed_lds_layout(p.rt, waves, RW, 0).total;
+    const size_t base = mixed_lds_layout(p.rt, waves, RW, 0, SX).total;
     if constexpr (RW == 4) {
         if (p.rt > kMixedSmallRowTiles)
-            return start_mixed(relax_mixed_kernel<4, PL, DI, RPT, false, 0, true>, grid, block, base, st, p);
+            return start_mixed(relax_mixed_kernel<4, PL, DI, RPT, false, 0, true, SX>, grid, block, base, st, p);
     }
-    constexpr int kCS = mixed_cached_steps(RW, PL, RPT);
+    // (the simplex's selection stage leaves no room for three planes of
+    // fragments in registers beside more than one row tile per wave)
+    constexpr int kCS = (SX && PL == 3 && RW >= 2) ? 0 : mixed_cached_steps(RW, PL, RPT);
     if constexpr (kCS > 0) {
         if (p.rt / 2 <= kCS)
-            return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false, kCS>, grid, block, base, st, p);
+            return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false, kCS, false, SX>, grid, block, base, st, p);
     }
     if constexpr (kCS < kMixedSmallRowTiles / 2) {
-        const size_t with_q = mixed_lds_layout(p.rt, waves, RW, mixed_q_bytes(p.rt, PL)).total;
-        if (with_q <= kMixedLdsBudget)
-            return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, true, 0>, grid, block, with_q, st, p);
-        return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false, 0>, grid, block, base, st, p);
+        if constexpr (!SX) {  // the simplex keeps its rows in LDS instead of a copy of Q
+            const size_t with_q = mixed_lds_layout(p.rt, waves, RW, mixed_q_bytes(p.rt, PL)).total;
+            if (with_q <= kMixedLdsBudget)
+                return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, true, 0>, grid, block, with_q, st, p);
+        }
+        return start_mixed(relax_mixed_kernel<RW, PL, DI, RPT, false, 0, false, SX>, grid, block, base, st, p);
     } else {
         return static_cast<int>(hipErrorInvalidValue);  // unreachable: every k-step is held
     }
@@ -797,7 +966,7 @@ int launch_mixed_shape(const MixedParams& p, dim3 grid, dim3 block, int waves, h
 
 int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void* stream) {
     if (p.cols < 1 || p.cols > 16 || rw < 1 || rw > kMixedMaxRowTilesPerWave || p.rt < 2 ||
-        p.rt > 2 * kMixedSmallRowTiles || (p.rt > kMixedSmallRowTiles && rw != 4))
+        p.rt > 2 * kMixedSmallRowTiles || (p.rt > kMixedSmallRowTiles && rw != 4) || (p.simplex && !dicg))
         return static_cast<int>(hipErrorInvalidValue);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(mixed_reset_kernel, dim3(1), dim3(1), 0, st, p.counter);
@@ -805,15 +974,19 @@ int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void
     const int rpt = mixed_rows_per_lane(p.cols);
     const dim3 grid(blocks), block(waves * kWave);
     int e = 0;
-#define GB_MIXED_RPT(RW, PL, DI)                                                        \
-    do {                                                                                \
-        if (rpt == 1) e = launch_mixed_shape<RW, PL, DI, 1>(p, grid, block, waves, st);  \
-        else if (rpt == 2) e = launch_mixed_shape<RW, PL, DI, 2>(p, grid, block, waves, st); \
-        else e = launch_mixed_shape<RW, PL, DI, 4>(p, grid, block, waves, st);          \
+#define GB_MIXED_RPT_SX(RW, PL, DI, SX)                                                      \
+    do {                                                                                     \
+        if (rpt == 1) e = launch_mixed_shape<RW, PL, DI, 1, SX>(p, grid, block, waves, st);  \
+        else if (rpt == 2) e = launch_mixed_shape<RW, PL, DI, 2, SX>(p, grid, block, waves, st); \
+        else e = launch_mixed_shape<RW, PL, DI, 4, SX>(p, grid, block, waves, st);           \
     } while (0)
+#define GB_MIXED_RPT(RW, PL, DI) GB_MIXED_RPT_SX(RW, PL, DI, false)
 #define GB_MIXED(RW)                                                     \
     case RW:                                                             \
-        if (p.planes == 1) {                                             \
+        if (p.simplex) {                                                 \
+            if (p.planes == 1) GB_MIXED_RPT_SX(RW, 1, true, true);       \
+            else GB_MIXED_RPT_SX(RW, 3, true, true);                     \
+        } else if (p.planes == 1) {                                      \
             if (dicg) GB_MIXED_RPT(RW, 1, true);                         \
             else GB_MIXED_RPT(RW, 1, false);                             \
         } else {                                                         \
@@ -831,6 +1004,7 @@ int launch_relax_mixed(const MixedParams& p, bool dicg, int rw, int blocks, void
     }
 #undef GB_MIXED
 #undef GB_MIXED_RPT
+#undef GB_MIXED_RPT_SX
     if (e != 0) return e;
     return static_cast<int>(hipGetLastError());
 }

@@ -1217,6 +1217,43 @@ extern "C" void orc_relax_fast_batch_lmo(const orc_objective* op, const orc_lmo*
             sx_vertex(h, x);
         } else {
             for (int j = 0; j < n; ++j) x[j] = (initial == 0 && !supplied) ? ub[j] : xn[j];
+            if (sx) {  // clip, put near-bound coordinates on their bound, scale onto the mass limit
+                const double atol = 1e-6, rtol = 1e-4;
+                std::vector<char> fr(n, 0);
+                for (int j = 0; j < n; ++j) {
+                    double v = std::fmin(std::fmax(x[j], lb[j]), ub[j]);
+                    if (std::fabs(v - lb[j]) <= atol + rtol * std::fmax(std::fabs(v), std::fabs(lb[j]))) v = lb[j];
+                    else if (std::fabs(v - ub[j]) <= atol + rtol * std::fmax(std::fabs(v), std::fabs(ub[j]))) v = ub[j];
+                    else fr[j] = 1;
+                    x[j] = v;
+                }
+                const double tx = block_sum(n, 64, [&](double acc, int j) { return acc + x[j]; });
+                const double tdn = block_sum(n, 64, [&](double acc, int j) { return fr[j] ? acc + (x[j] - lb[j]) : acc; });
+                const double tup = block_sum(n, 64, [&](double acc, int j) { return fr[j] ? acc + (ub[j] - x[j]) : acc; });
+                const double tl = block_sum(n, 64, [&](double acc, int j) { return acc + lb[j]; });
+                const double tu = block_sum(n, 64, [&](double acc, int j) { return acc + ub[j]; });
+                if (tx > rhs) {
+                    const double e = tx - rhs;
+                    if (tdn > e) {
+                        const double f = e / tdn;
+                        for (int j = 0; j < n; ++j)
+                            if (fr[j]) x[j] = std::fma(lb[j] - x[j], f, x[j]);
+                    } else {
+                        const double f = (rhs - tl) / (tx - tl);
+                        for (int j = 0; j < n; ++j) x[j] = std::fma(x[j] - lb[j], f, lb[j]);
+                    }
+                } else if (eq && tx < rhs) {
+                    const double sh = rhs - tx;
+                    if (tup > sh) {
+                        const double f = sh / tup;
+                        for (int j = 0; j < n; ++j)
+                            if (fr[j]) x[j] = std::fma(ub[j] - x[j], f, x[j]);
+                    } else {
+                        const double f = sh / (tu - tx);
+                        for (int j = 0; j < n; ++j) x[j] = std::fma(ub[j] - x[j], f, x[j]);
+                    }
+                }
+            }
         }
         F.gemv(x, qx);
         auto grad = [&]() { for (int j = 0; j < n; ++j) g[j] = o.grad_scale * qx[j] + F.lsc[j]; };

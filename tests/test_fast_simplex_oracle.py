@@ -201,3 +201,35 @@ def test_batched_simplex_bnb_reaches_enumerated_optimum(orc):
                         eps=1e-7, eps_min=1e-9, abs_gap_tol=1e-9, rel_gap_tol=1e-9, form=1)
     assert abs(res["objective"] - best) <= 1e-9 * (1 + abs(best))
     assert res["x"].sum() <= card
+
+
+@pytest.mark.parametrize("equality,free_value", [(False, 0.6), (True, 0.6), (True, 0.4), (False, 0.4)])
+def test_supplied_point_is_put_on_the_faces_and_the_mass_limit(orc, equality, free_value):
+    """A supplied point (an FP32 iterate of the mixed pass, say) near bounds
+    and off the mass limit: coordinates within the face tolerance of a bound
+    land on it, and the free ones absorb the excess (or, for an equality, the
+    shortfall) in proportion to their room; a point within the limit of an
+    inequality keeps its free coordinates."""
+    n, card = 40, 10
+    _, _, obj, _ = simplex_problem(orc, n, seed=8, card=card)
+    lmo = orc.simplex_lmo(float(card), np.ones(n, dtype=np.uint8), equality)
+    x0 = np.zeros(n)
+    x0[:6] = 1.0 - 3e-7                # within the face tolerance of 1
+    x0[6:12] = 2e-8                    # within the face tolerance of 0
+    x0[12:20] = free_value             # free: 6 + 8 * 0.6 = 10.8 > 10, or 6 + 3.2 = 9.2 < 10
+    x0[20] = -1e-3                     # below its bound: clipped to 0
+    X = x0[None, :].copy()
+    r = orc.relax_fast_batch(obj, True, X, f64(np.zeros((1, n))), f64(np.ones((1, n))), np.full(1, 1e-9),
+                             np.zeros(1, np.int32), np.array([0x80], np.uint8), flags=np.ones(n, np.uint8),
+                             max_iters=0, lmo=lmo)
+    x = r["x"][0]
+    assert np.all(x[:6] == 1.0) and np.all(x[6:12] == 0.0) and x[20] == 0.0 and np.all(x[21:] == 0.0)
+    free = x[12:20]
+    if free_value > 0.5:       # excess: each free coordinate gives up 0.8 / 4.8 of its value
+        assert np.allclose(free, 0.6 - 0.6 * (0.8 / 4.8), rtol=0, atol=1e-15)
+        assert abs(x.sum() - card) <= 1e-12
+    elif equality:             # shortfall: each takes 0.8 / 4.8 of its room up to 1
+        assert np.allclose(free, 0.4 + 0.6 * (0.8 / 4.8), rtol=0, atol=1e-15)
+        assert abs(x.sum() - card) <= 1e-12
+    else:                      # within the limit: unchanged
+        assert np.all(free == 0.4)

@@ -1,9 +1,9 @@
 """The heterogeneous batch at its configured shape: 256 binary QPs with n
 log-uniform in [32, 512], each with a box or a bounded-simplex LMO
 (sum x <= ceil(n / 4)) chosen by seed and 1024 open nodes, relaxed by one
-gb_relax_instances call (box instances in the mixed form with 3 bf16 planes
-of Q and the FP64 certification pass, simplex instances in the batched FP64
-form).  Every node's outputs are checked for what must hold whatever the
+gb_relax_instances call (every instance in the mixed form: 3 bf16 planes of
+Q on MFMA, then the FP64 certification pass; over a simplex pass 1 picks the
+greedy vertices with one wave per node column).  Every node's outputs are checked for what must hold whatever the
 arithmetic (valid status, feasible iterate, FP64 bound f - gap no greater
 than the best vertex found), and four sampled instances are compared bit
 for bit with the oracle on a slice of their nodes."""
@@ -47,7 +47,7 @@ def build(count=COUNT, nodes=NODES):
             lmo = gb.LmoDesc(gb.LMO_SIMPLEX, float(K), 0, flags.data_ptr(), 0)
         e = {"k": k, "n": n, "kind": kind, "K": K, "Q": Q, "c": c, "LB": LB, "UB": UB, "EPS": EPS,
              "obj": obj, "lmo": lmo, "keep": (dQt, dc),
-             "mixed": gb.MixedQ(obj, 3) if kind == "box" else None,
+             "mixed": gb.MixedQ(obj, 3),
              "x": torch.from_numpy(LB).cuda(), "lb": torch.from_numpy(LB).cuda(), "ub": torch.from_numpy(UB).cuda(),
              "eps": torch.from_numpy(EPS).cuda(), "iterations": torch.zeros(nodes, dtype=torch.int32, device="cuda"),
              "status": torch.zeros(nodes, dtype=torch.uint8, device="cuda"), "gap": torch.zeros(nodes, **f64),
@@ -98,10 +98,9 @@ def check_properties(e):
 
 
 def compare_with_oracle(e, alg, rows):
-    """A slice of one instance's nodes against the oracle: the batched FP64
-    form over the simplex directly, the mixed form through its pass-1 output
-    (a solo call of pass 1 only, which equals the in-call pass 1) and the
-    oracle's FP64 pass."""
+    """A slice of one instance's nodes against the oracle: the mixed form
+    through its pass-1 output (a solo call of pass 1 only, which equals the
+    in-call pass 1) and the oracle's FP64 pass from it (box or simplex)."""
     import torch
     from oracle import pyoracle as orc
     n = e["n"]
@@ -109,12 +108,6 @@ def compare_with_oracle(e, alg, rows):
     LB, UB, EPS = e["LB"][rows], e["UB"][rows], e["EPS"][rows]
     got = results(e, rows)
     ones = np.ones(n, np.uint8)
-    if e["kind"] == "simplex":
-        o = orc.relax_fast_batch(orc.dense_objective(e["Q"], e["c"], 2.0, 0.5, 0.0), True, LB.copy(), LB.copy(),
-                                 UB.copy(), EPS.copy(), np.zeros(N, np.int32), np.zeros(N, np.uint8), flags=ones,
-                                 max_iters=BUDGET, refresh=64, threads=8, lmo=orc.simplex_lmo(e["K"], ones))
-        assert_bit_equal(got, o, f"simplex instance {e['k']} n={n}")
-        return
     f64 = dict(dtype=torch.float64, device="cuda")
     x = torch.from_numpy(LB.copy()).cuda()
     it = torch.zeros(N, dtype=torch.int32, device="cuda")
@@ -128,8 +121,9 @@ def compare_with_oracle(e, alg, rows):
     o = orc.relax_fast_batch(orc.dense_objective(e["Q"], e["c"], 2.0, 0.5, 0.0), True, x.cpu().numpy(), LB.copy(),
                              UB.copy(), EPS.copy(), it.cpu().numpy(), st.cpu().numpy(), flags=ones,
                              max_iters=max(1, BUDGET // 16), refresh=64, threads=8,
-                             best_vertex=bv.cpu().numpy(), best_vertex_val=bvv.cpu().numpy())
-    assert_bit_equal(got, o, f"box instance {e['k']} n={n}")
+                             best_vertex=bv.cpu().numpy(), best_vertex_val=bvv.cpu().numpy(),
+                             lmo=orc.simplex_lmo(e["K"], ones) if e["kind"] == "simplex" else None)
+    assert_bit_equal(got, o, f"{e['kind']} instance {e['k']} n={n}")
 
 
 def test_c3_full_shape_one_call():
