@@ -1,8 +1,9 @@
-// Mixed-precision batched relaxation of dense quadratics over 0/1 boxes,
-// gfx950 (the precision form of the batched B&B: BF16 MFMA products, FP32
-// iterates, partial sums and step; the bound that decides pruning and the
-// incumbent value are certified afterwards in FP64 by the batched FP64
-// kernel, which continues from the iterate written here).
+// Mixed-precision batched relaxation of dense quadratics over 0/1 boxes and
+// 0/1 bounded simplices, gfx950 (the precision form of the batched B&B:
+// BF16 MFMA products, FP32 iterates, partial sums and step; the bound that
+// decides pruning and the incumbent value are certified afterwards in FP64
+// by the batched FP64 kernel, which continues from the iterate written
+// here).
 //
 // Layout: a workgroup of W waves carries up to 16 node columns in lockstep;
 // wave w owns the row tiles t = w + W j (j < RW, t < RT) of every column.
@@ -42,6 +43,14 @@
 // form's init vertex); nodes with bounds other than 0/1 and continuations
 // are left to the FP64 kernel.  A node's trajectory depends only on the node
 // and the launch shape (RW, RPT), not on the column or tile that serves it.
+//
+// Over a bounded simplex (SX, DICG) the vertices are not elementwise: each
+// pass stages the columns' gradients and row flags in LDS, one wave per
+// column picks v and the in-face a with the greedy's selection
+// (mixed_simplex_column), and the rows read their bits back (two barriers);
+// the maximum step adds the mass limit from the column sums of x and v - a.
+// A column starts at the greedy vertex of the artificial direction, and a
+// node whose lower bounds exceed the right side is left to the FP64 kernel.
 #include <type_traits>
 
 #include "gb_device.h"

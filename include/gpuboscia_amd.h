@@ -321,7 +321,11 @@ typedef struct {
  * variable, node bounds 0 or 1), where the greedy vertex is the first
  * rhs - sum lb eligible coordinates in (gradient, index) order, as the
  * reference's; a node outside that scope is not relaxed and gets status
- * 0xFF (value and gap NaN).
+ * 0xFF (value and gap NaN).  Over a simplex a supplied point is first
+ * clipped to the node's bounds, coordinates within the face tolerance
+ * (1e-6 + 1e-4 |bound|) are put on their bound, and the free ones are
+ * scaled onto the mass limit if the point exceeds it (or, for an equality,
+ * falls short of it).
  */
 gb_status gb_relax_batch_fast(const gb_objective_desc* objective,
                               const gb_algorithm_desc* algorithm,
@@ -377,7 +381,10 @@ typedef struct {
 } gb_mixed_options;
 
 /*
- * Mixed-precision relaxation of N nodes of a dense quadratic over a 0/1 box.
+ * Mixed-precision relaxation of N nodes of a dense quadratic over a 0/1 box,
+ * or (DICG only) over a bounded / equality simplex of 0/1 variables with an
+ * integral right side, whose vertices pass 1 takes from the greedy on the
+ * FP32 gradient (one wave per node column).
  * Pass 1 (BF16 MFMA): 16 nodes per wave64 in lockstep, products Q v / Q a
  * from the prepared planes with FP32 accumulation, FP32 iterates, FP64
  * step and reductions, DICG (or FW) with the exact step, per-node refill
