@@ -71,12 +71,13 @@ int launch_mixed_prepare(const double* Qt, const double* lin, double gs, double 
 //  red         per-column partial sums of each wave, mixed_red(sx) floats;
 //  fetch, ok   fetched node ids, per-wave scope flags;
 //  sel         (simplex) per column: the gradient of every row (float), the
-//              row flags (byte) and the column's feasibility (int);
+//              row flags (byte), the rows' v and a (a byte each) and the
+//              column's feasibility (int);
 //  qf          (optional) the Q fragments, planes x rt x rt/2 KB.
 // At most 8 + 8 + 32 + 8 KB for n <= 256 and 16 + 16 + 64 + 4 KB for n <= 512,
 // plus the fragments when copied.
 struct MixedLds {
-    size_t vops, aops, xch, red, fetch, ok, sg, sf, sok, qf, total;
+    size_t vops, aops, xch, red, fetch, ok, sg, sf, sr, sok, qf, total;
 };
 // reduction slots per (wave, column): 8, or 12 with the simplex's mass sums
 __host__ __device__ constexpr int mixed_red(bool sx) { return sx ? 12 : 8; }
@@ -104,6 +105,8 @@ __host__ __device__ inline MixedLds mixed_lds_layout(int rt, int waves, int rw, 
     o += sx ? sizeof(float) * 16 * np : 0;
     L.sf = o;
     o += sx ? 16 * np : 0;
+    L.sr = o;  // selection results: v bytes of every column, then a bytes
+    o += sx ? 2 * 16 * np : 0;
     L.sok = o;
     o += sx ? sizeof(int) * 16 : 0;
     o = (o + 15) & ~static_cast<size_t>(15);

@@ -46,9 +46,10 @@
 //
 // Over a bounded simplex (SX, DICG) the vertices are not elementwise: each
 // pass stages the columns' gradients and row flags in LDS, one wave per
-// column picks v and the in-face a with the greedy's selection
-// (mixed_simplex_column), and the rows read their bits back (two barriers);
-// the maximum step adds the mass limit from the column sums of x and v - a.
+// column picks v and another the in-face a with the greedy's selection
+// (mixed_simplex_v, mixed_simplex_a), and the rows read them back (two
+// barriers); the maximum step adds the mass limit from the column sums of x
+// and v - a.
 // A column starts at the greedy vertex of the artificial direction, and a
 // node whose lower bounds exceed the right side is left to the FP64 kernel.
 #include <type_traits>
@@ -180,22 +181,22 @@ __global__ void mixed_prepare_kernel(const double* Qt, const double* lin, double
 
 constexpr int kPhaseDone = 0, kPhaseInit = 1, kPhaseActive = 2, kPhaseFetch = 3;
 
-// Simplex vertices of one node column, computed by one wave from the
+// Simplex vertices of one node column, computed by a wave from the
 // column's rows staged in LDS: g (float) and flags (bit 0: lb = 1, bit 1:
-// ub = 1, bit 2: x at a bound, bit 3: x at its upper bound).  The forward
-// vertex v is the greedy's for g (for the artificial direction
-// g_j = -1 - j / (n + 1) while the column initialises); the in-face vertex a
-// the greedy's over the free rows of the face for -g, the face keeping the
-// full mass when sum x (txm) equals rhs within tolerance (lmo.cuh:517-618,
-// :814-863; 0/1 capacities, so wave_select gives their vertices exactly on
-// the FP32 gradient).  Results go back into the flags as bits 4 (v) and 5
-// (a); *ok = 0 for a node the set cannot hold (sum lb > rhs, or for an
-// equality sum ub < rhs).
+// ub = 1, bit 2: x at a bound, bit 3: x at its upper bound).  Two tasks per
+// column, on different waves: the forward vertex v, the greedy's for g (for
+// the artificial direction g_j = -1 - j / (n + 1) while the column
+// initialises), and the in-face vertex a, the greedy's over the free rows of
+// the face for -g, the face keeping the full mass when sum x (txm) equals rhs
+// within tolerance (lmo.cuh:517-618, :814-863; 0/1 capacities, so
+// wave_select gives their vertices exactly on the FP32 gradient).  Each
+// task writes one byte per row (0 or 1); the v task also writes *ok = 0 for
+// a node the set cannot hold (sum lb > rhs, or for an equality sum ub < rhs).
 template <int NTS>
-__device__ __forceinline__ void mixed_simplex_column(const float* sgc, uint8_t* sfc, int n, double rhs, bool eq,
-                                                     bool ini, float txm, int lane, int* ok) {
+__device__ __forceinline__ void mixed_simplex_v(const float* sgc, const uint8_t* sfc, uint8_t* out, int n, double rhs,
+                                                bool eq, bool ini, int lane, int* ok) {
     unsigned long long key[NTS];
-    bool c[NTS], sel[NTS], a[NTS];
+    bool c[NTS], sel[NTS];
     uint32_t f[NTS];
     int lbc = 0, ubc = 0;
 #pragma unroll
@@ -209,38 +210,43 @@ __device__ __forceinline__ void mixed_simplex_column(const float* sgc, uint8_t* 
         lbc += __popcll(__ballot(in && (f[t] & 1u)));
         ubc += __popcll(__ballot(in && (f[t] & 2u)));
     }
-    const bool feasible = lbc <= rhs + 1e-9 && (!eq || ubc >= rhs - 1e-9);
     wave_select<NTS>(key, c, rhs - lbc, sel, lane);
-    uint32_t vb[NTS];
-#pragma unroll
-    for (int t = 0; t < NTS; ++t) vb[t] = ((f[t] & 1u) || sel[t]) ? 16u : 0u;
-    int mass = 0;
-#pragma unroll
-    for (int t = 0; t < NTS; ++t) {
-        const bool in = lane + kWave * t < n;
-        a[t] = (f[t] & 1u) || (f[t] & 12u) == 12u;  // lb, or ub where x sits at its upper bound
-        mass += __popcll(__ballot(in && a[t]));
-    }
-    if (!ini) {
-        const bool force = eq || fabs(static_cast<double>(txm) - rhs) <=
-                                     1e-6 + 1e-4 * fmax(fabs(static_cast<double>(txm)), fabs(rhs));
-#pragma unroll
-        for (int t = 0; t < NTS; ++t) {
-            const int j = lane + kWave * t;
-            const bool in = j < n;
-            const double h = in ? -static_cast<double>(sgc[j]) : 0.0;
-            key[t] = order_key(h);
-            c[t] = in && !(f[t] & 4u) && (f[t] & 2u) && !a[t] && (force || h <= 0.0);
-        }
-        wave_select<NTS>(key, c, rhs - mass, sel, lane);
-    }
 #pragma unroll
     for (int t = 0; t < NTS; ++t) {
         const int j = lane + kWave * t;
-        const bool ab = !ini && (a[t] || sel[t]);
-        if (j < n) sfc[j] = static_cast<uint8_t>(f[t] | vb[t] | (ab ? 32u : 0u));
+        if (j < n) out[j] = ((f[t] & 1u) || sel[t]) ? 1 : 0;
     }
-    if (lane == 0) *ok = feasible ? 1 : 0;
+    if (lane == 0) *ok = (lbc <= rhs + 1e-9 && (!eq || ubc >= rhs - 1e-9)) ? 1 : 0;
+}
+
+template <int NTS>
+__device__ __forceinline__ void mixed_simplex_a(const float* sgc, const uint8_t* sfc, uint8_t* out, int n, double rhs,
+                                                bool eq, float txm, int lane) {
+    unsigned long long key[NTS];
+    bool c[NTS], sel[NTS], a[NTS];
+    int mass = 0;
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) {
+        const int j = lane + kWave * t;
+        const bool in = j < n;
+        const uint32_t f = in ? sfc[j] : 0u;
+        a[t] = (f & 1u) || (f & 12u) == 12u;  // lb, or ub where x sits at its upper bound
+        mass += __popcll(__ballot(in && a[t]));
+        const double h = in ? -static_cast<double>(sgc[j]) : 0.0;
+        key[t] = order_key(h);
+        c[t] = in && !(f & 4u) && (f & 2u) && !a[t];
+        sel[t] = h <= 0.0;  // eligibility without the full-mass face
+    }
+    const bool force = eq || fabs(static_cast<double>(txm) - rhs) <=
+                                 1e-6 + 1e-4 * fmax(fabs(static_cast<double>(txm)), fabs(rhs));
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) c[t] = c[t] && (force || sel[t]);
+    wave_select<NTS>(key, c, rhs - mass, sel, lane);
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) {
+        const int j = lane + kWave * t;
+        if (j < n) out[j] = (a[t] || sel[t]) ? 1 : 0;
+    }
 }
 
 // The wave's Q fragments are the same in every pass: where they fit beside
@@ -283,6 +289,7 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
     const MixedLds L = mixed_lds_layout(RT, W, RW, QL ? mixed_q_bytes(RT, PL) : 0, SX);
     float* sg = reinterpret_cast<float*>(mixed_smem + L.sg);
     uint8_t* sf = reinterpret_cast<uint8_t*>(mixed_smem + L.sf);
+    uint8_t* sr = reinterpret_cast<uint8_t*>(mixed_smem + L.sr);
     int* sok = reinterpret_cast<int*>(mixed_smem + L.sok);
     uint4* vops = reinterpret_cast<uint4*>(mixed_smem + L.vops);
     uint4* aops = reinterpret_cast<uint4*>(mixed_smem + L.aops);
@@ -435,8 +442,8 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
         float gp = 0.0f, fp = 0.0f;
         float g[E];
         if constexpr (SX) {
-            // The column's rows go to LDS, one wave per column picks both
-            // vertices, and the rows read their bits back.
+            // The column's rows go to LDS, a wave per column and vertex picks
+            // v or a, and the rows read them back.
             const int NP = 16 * RT;
             float* sgc = sg + col * NP;
             uint8_t* sfc = sf + col * NP;
@@ -458,19 +465,36 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
             }
             __syncthreads();
             const int nts = (n + kWave - 1) / kWave;
-            for (int k = w; k < kWave / G; k += W) {  // the tile's 4 RPT columns
+            uint8_t* sv = sr;                 // v of every column
+            uint8_t* sa = sr + 16 * NP;       // a of every column
+            // tasks 2k (v of column k) and 2k + 1 (a of column k) over the waves
+            for (int task = w; task < 2 * (kWave / G); task += W) {
+                const int k = task >> 1;
                 const int kp = __builtin_amdgcn_readlane(phase, k * G);
                 if (kp != kPhaseActive && kp != kPhaseInit) continue;
-                const float kt = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, txm), k * G));
                 const bool kini = kp == kPhaseInit;
                 const float* gk = sg + k * NP;
-                uint8_t* fk = sf + k * NP;
-                if constexpr (BIG) {
-                    mixed_simplex_column<8>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
-                } else {
-                    if (nts <= 1) mixed_simplex_column<1>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
-                    else if (nts <= 2) mixed_simplex_column<2>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
-                    else mixed_simplex_column<4>(gk, fk, n, P.rhs, P.equality != 0, kini, kt, lane, sok + k);
+                const uint8_t* fk = sf + k * NP;
+                const bool eq = P.equality != 0;
+                if ((task & 1) == 0) {
+                    uint8_t* o = sv + k * NP;
+                    if constexpr (BIG) {
+                        mixed_simplex_v<8>(gk, fk, o, n, P.rhs, eq, kini, lane, sok + k);
+                    } else {
+                        if (nts <= 1) mixed_simplex_v<1>(gk, fk, o, n, P.rhs, eq, kini, lane, sok + k);
+                        else if (nts <= 2) mixed_simplex_v<2>(gk, fk, o, n, P.rhs, eq, kini, lane, sok + k);
+                        else mixed_simplex_v<4>(gk, fk, o, n, P.rhs, eq, kini, lane, sok + k);
+                    }
+                } else if (!kini) {
+                    uint8_t* o = sa + k * NP;
+                    const float kt = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, txm), k * G));
+                    if constexpr (BIG) {
+                        mixed_simplex_a<8>(gk, fk, o, n, P.rhs, eq, kt, lane);
+                    } else {
+                        if (nts <= 1) mixed_simplex_a<1>(gk, fk, o, n, P.rhs, eq, kt, lane);
+                        else if (nts <= 2) mixed_simplex_a<2>(gk, fk, o, n, P.rhs, eq, kt, lane);
+                        else mixed_simplex_a<4>(gk, fk, o, n, P.rhs, eq, kt, lane);
+                    }
                 }
             }
             __syncthreads();
@@ -479,9 +503,8 @@ __global__ void __launch_bounds__(BIG ? 512 : (RW == 1 ? 512 : 64 * ((16 + RW - 
                 const int t = w + W * (e / RPT);
                 const int row = 16 * t + RPT * m + e % RPT;
                 if (t < RT && row < n) {
-                    const uint32_t fl = sfc[row];
-                    vm |= ((fl >> 4) & 1u) << e;
-                    am |= ((fl >> 5) & 1u) << e;
+                    vm |= static_cast<uint32_t>(sr[col * NP + row] & 1u) << e;
+                    am |= static_cast<uint32_t>(sr[(16 + col) * NP + row] & 1u) << e;
                 }
             }
             if ((act || ini) && sok[col] == 0) {
