@@ -473,11 +473,15 @@ def qap_root_proof(args):
             "rows": rows}
 
 
-def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, rel_gap=1e-6, device=False):
+def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, rel_gap=1e-6, device=False, card=0):
+    """card > 0: over the bounded simplex sum x <= card (0/1 variables)."""
     import torch
     dQt = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
     dc = torch.from_numpy(c).cuda()
     obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
+    ii = torch.ones(n, dtype=torch.uint8, device="cuda")
+    lmo = (gb.LmoDesc(gb.LMO_SIMPLEX, float(card), 0, ii.data_ptr(), 0) if card > 0
+           else gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0))
     s = gb.default_settings()
     s.abs_gap_tol = 1e-6
     s.rel_gap_tol = rel_gap
@@ -494,7 +498,7 @@ def solve_row(gb, Q, c, n, form, learned=None, max_nodes=0, time_limit=120.0, re
     # the device-resident search: frontier, pruning and branching on the GPU
     # (the host driver's one-stream schedule)
     s.device_frontier = 1 if device else 0
-    r = gb.solve(n, [0] * n, [1] * n, range(n), obj, gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0), s)
+    r = gb.solve(n, [0] * n, [1] * n, range(n), obj, lmo, s)
     return {"seconds": round(r["elapsed_seconds"], 5), "relax_seconds": round(r["relax_seconds"], 5),
             "nodes": r["nodes_processed"], "fw_iterations": r["fw_iterations"], "objective": r["objective"],
             "dual_bound": r["dual_bound"], "proven": not r["reached_time_limit"]}
@@ -548,6 +552,37 @@ def time_to_optimum(args):
             out.append(row)
             say(f"time-to-optimum n={n} seed={seed}: " +
                 " ".join(f"{k} {row[k]['seconds']:.4f}s/{row[k]['nodes']}n" for k, _, _, _ in forms))
+    return out
+
+
+def simplex_time_to_optimum(args):
+    """gb_solve to the proven optimum (abs = rel gap 1e-6) over the bounded
+    simplex sum x <= ceil(n / 10) (binary QPs built as the heterogeneous
+    batch's, A of 1.5 n x n; at ceil(n / 4) the box optimum is inside the
+    simplex and the searches are the box's): plain form (reference defaults),
+    batched FP64 form and mixed form (three bf16 planes + FP64
+    certification), each after an untimed node-capped run on another
+    instance of the size."""
+    import gpuboscia_amd as gb
+    from gpuboscia_amd import instances
+    forms = [("plain", 0), ("batched", 1), ("mixed", 2)]
+    out = []
+    for n in [int(v) for v in args.sx_sizes.split(",") if v]:
+        card = (n + 9) // 10
+        for seed in (1, 2):
+            Q, c, sha = instances.binary_qp(n, seed=seed)
+            Qw, cw, _ = instances.binary_qp(n, seed=100 + seed)
+            row = {"n": n, "seed": seed, "card": card, "sha256": sha[:16]}
+            for key, form in forms:
+                solve_row(gb, Qw, cw, n, form, max_nodes=4096, card=card)
+                row[key] = solve_row(gb, Q, c, n, form, card=card, time_limit=args.sx_seconds)
+            fp = row["plain"]["objective"]
+            row["same_optimum"] = all(abs(row[k]["objective"] - fp) <= 1e-6 * max(1.0, abs(fp)) for k, _ in forms)
+            for key, _ in forms[1:]:
+                row[f"speedup_{key}"] = round(row["plain"]["seconds"] / max(row[key]["seconds"], 1e-9), 2)
+            out.append(row)
+            say(f"simplex time-to-optimum n={n} seed={seed} card={card}: " +
+                " ".join(f"{k} {row[k]['seconds']:.4f}s/{row[k]['nodes']}n" for k, _ in forms))
     return out
 
 
@@ -631,6 +666,10 @@ def main():
     ap.add_argument("--mb-max-n", type=int, default=65536, help="largest n of the MultiBlock leg")
     ap.add_argument("--no-qap", action="store_true", help="skip the Birkhoff QAP root-proof leg")
     ap.add_argument("--only-qap", action="store_true", help="run only the Birkhoff QAP root-proof leg")
+    ap.add_argument("--no-sx", action="store_true", help="skip the bounded-simplex time-to-optimum leg")
+    ap.add_argument("--only-sx", action="store_true", help="run only the bounded-simplex time-to-optimum leg")
+    ap.add_argument("--sx-sizes", default="40,60,80", help="binary-QP sizes of the bounded-simplex leg")
+    ap.add_argument("--sx-seconds", type=float, default=60.0, help="time limit of each bounded-simplex solve")
     ap.add_argument("--qap-sizes", default="64,128,200", help="QAP orders of the FP32 form")
     ap.add_argument("--qap-plain-sizes", default="64", help="QAP orders also solved by the plain FP64 form")
     ap.add_argument("--qap-seconds", type=float, default=120.0, help="time limit of each QAP solve")
@@ -653,6 +692,9 @@ def main():
         return
     if args.only_qap:
         print(json.dumps({"qap_root_proof": qap_root_proof(args)}), flush=True)
+        return
+    if args.only_sx:
+        print(json.dumps({"simplex_time_to_optimum": simplex_time_to_optimum(args)}), flush=True)
         return
     n = args.n
     Q, c, sha = instances.binary_qp(n, seed=1)
@@ -733,6 +775,8 @@ def main():
         result["multiblock"] = multiblock_large_n(args)
     if not args.no_qap:
         result["qap_root_proof"] = qap_root_proof(args)
+    if not args.no_sx:
+        result["simplex_time_to_optimum"] = simplex_time_to_optimum(args)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, Q, c, LB, UB, EPS)
     if not args.no_tto:
