@@ -29,8 +29,9 @@ def say(*a):
     sys.stdout.flush()
 
 
-def nodes01(n, N, card, seed):
-    """0/1 node bounds (node 0 the root); node N-1 infeasible (sum lb > card)."""
+def nodes01(n, N, card, seed, equality=False):
+    """0/1 node bounds (node 0 the root); node N-1 infeasible (sum lb > card)
+    and, for an equality, node N-2 too (sum ub < card)."""
     rng = np.random.default_rng(seed)
     LB, UB = np.zeros((N, n)), np.ones((N, n))
     for i in range(1, N):
@@ -42,6 +43,10 @@ def nodes01(n, N, card, seed):
             LB[i, np.flatnonzero(LB[i])[0]] = 0.0
     LB[N - 1, : card + 1] = 1.0
     UB[N - 1, : card + 1] = 1.0
+    if equality:
+        LB[N - 2, :] = 0.0
+        UB[N - 2, :] = 0.0
+        UB[N - 2, : card - 1] = 1.0
     return LB, UB
 
 
@@ -97,7 +102,8 @@ def test_mixed_simplex_pass1_and_fp64_pass(n, planes, equality, columns):
     card = max(2, (n + 3) // 4)
     C = Case(n, card, equality, planes, seed=n + 3)
     N = 48
-    LB, UB = nodes01(n, N, card, seed=n)
+    LB, UB = nodes01(n, N, card, seed=n, equality=equality)
+    infeasible = [N - 1] + ([N - 2] if equality else [])
     eps = np.full(N, 1e-6)
     say(f"[mixed-simplex] n={n} planes={planes} equality={equality} columns={columns} card={card}")
     b = C.buffers(LB, UB, eps)
@@ -106,7 +112,7 @@ def test_mixed_simplex_pass1_and_fp64_pass(n, planes, equality, columns):
     # pass 1: feasible iterates and vertices; infeasible nodes untouched
     for i in range(N):
         x = p1["x"][i]
-        if i == N - 1:
+        if i in infeasible:  # left to the FP64 pass
             assert p1["status"][i] == 0 and p1["iterations"][i] == 0
             continue
         assert p1["iterations"][i] > 0 and p1["status"][i] == gb.NODE_INITIAL_POINT_SUPPLIED
@@ -134,7 +140,7 @@ def test_mixed_simplex_pass1_and_fp64_pass(n, planes, equality, columns):
                              lmo=orc.simplex_lmo(float(card), np.ones(n, np.uint8), equality))
     assert_bit_equal(got, o, f"FP64 pass n={n}")
     ok = np.isin(got["status"], (0, 1, 3, 4))
-    assert got["status"][N - 1] == gb.NODE_INFEASIBLE
+    assert all(got["status"][i] == gb.NODE_INFEASIBLE for i in infeasible)
     assert np.all(got["x"][ok].sum(axis=1) <= card + 1e-9)
 
 
@@ -143,7 +149,7 @@ def test_mixed_simplex_full_relaxation_reaches_fp64_values(n, equality):
     card = (n + 3) // 4
     C = Case(n, card, equality, 3, seed=n + 11)
     N = 64
-    LB, UB = nodes01(n, N, card, seed=2 * n)
+    LB, UB = nodes01(n, N, card, seed=2 * n, equality=equality)
     eps = np.full(N, 1e-7)
     say(f"[mixed-simplex] full relaxation n={n} equality={equality}")
     bm = C.buffers(LB, UB, eps)
