@@ -232,3 +232,41 @@ def test_instances_with_mixed_simplex_entries_equal_solo_calls():
     for k, (a, b) in enumerate(zip(together, solo)):
         assert_bit_equal({q: a[q].cpu().numpy() for q in keys}, {q: b[q].cpu().numpy() for q in keys},
                          f"entry {k}")
+
+
+@pytest.mark.parametrize("n,rw,columns,planes", [
+    (100, 1, 2, 3), (100, 2, 8, 3), (100, 3, 16, 1), (100, 4, 4, 3), (150, 3, 4, 3), (150, 2, 16, 1),
+    (300, 4, 2, 3), (300, 4, 16, 1)])
+def test_mixed_simplex_tile_shapes(n, rw, columns, planes):
+    """Every tile shape (row tiles per wave, node columns per tile) computes
+    the same algorithm: pass 1 stays feasible and the whole call equals the
+    oracle's FP64 pass from pass 1's output."""
+    import torch
+    from oracle import pyoracle as orc
+    card = (n + 3) // 4
+    C = Case(n, card, False, planes, seed=n + 29)
+    N = 40
+    LB, UB = nodes01(n, N, card, seed=3 * n + rw)
+    eps = np.full(N, 1e-6)
+    say(f"[mixed-simplex] shape n={n} rw={rw} columns={columns} planes={planes}")
+
+    def run(fp64):
+        b = C.buffers(LB, UB, eps)
+        gb.relax_batch_mixed(C.mq, C.obj, C.alg, C.lmo, N, 150, b["x"], b["lb"], b["ub"], b["eps"], b["iterations"],
+                             b["status"], b["gap"], b["value"], b["best_vertex"], b["best_vertex_val"], C.flags, 1e-6,
+                             None, fp64_max_iters=fp64, row_tiles_per_wave=rw, columns=columns)
+        torch.cuda.synchronize()
+        return host(b)
+
+    p1 = run(-1)
+    for i in range(N - 1):
+        x = p1["x"][i]
+        assert p1["iterations"][i] > 0, i
+        assert np.all(x >= LB[i] - 1e-6) and np.all(x <= UB[i] + 1e-6) and x.sum() <= card + 1e-3 * (1 + card), i
+    got = run(0)
+    o = orc.relax_fast_batch(orc.dense_objective(C.Q, C.c, 2.0, 0.5, 0.0), True, p1["x"].copy(), LB.copy(),
+                             UB.copy(), eps.copy(), p1["iterations"].copy(), p1["status"].copy(),
+                             flags=np.ones(n, np.uint8), max_iters=150 // 16, refresh=64, threads=8,
+                             best_vertex=p1["best_vertex"].copy(), best_vertex_val=p1["best_vertex_val"].copy(),
+                             lmo=orc.simplex_lmo(float(card), np.ones(n, np.uint8)))
+    assert_bit_equal(got, o, f"shape n={n} rw={rw} columns={columns}")
