@@ -37,6 +37,12 @@
 //   portfolio LMO ..................... LP optima of tiny instances (scipy, committed
 //                                       fixtures tests/golden/portfolio_lmo.json)
 //   QAP objective ..................... finite differences, optimum 0 at P=Pi^T
+//   batched form's 0/1 simplex ........ vertices equal the plain greedy restatement
+//   (orc_fast_simplex_vertex / _inface) (compute_v / compute_inface) on random 0/1
+//                                       cases with ties and signed zeros; forward
+//                                       vertex = enumerated minimum; supplied-point
+//                                       handoff by closed forms; relaxation values =
+//                                       the plain form's; search = enumeration
 #include "gb_oracle.h"
 
 #include <algorithm>
@@ -1034,11 +1040,13 @@ extern "C" void orc_relax_batch(const orc_objective* o, const orc_alg* a, const 
 }
 
 // ---------------------------------------------------------------------------
-// Batched-form relaxation (box LMO, dense objective), mirroring the wave64
-// kernel gpuboscia_amd/csrc/relax_fast.hip operation for operation: one wave
-// (64 lanes) per node, incremental FP64 products, exact quadratic step,
-// cutoff stop, final direct certificate.  No reference counterpart: pinned
-// by the same KATs as the plain form (optima, enumeration) and by the kernel.
+// Batched-form relaxation (dense objective over a box, or over a bounded
+// simplex of 0/1 variables), mirroring the wave64 kernel
+// gpuboscia_amd/csrc/relax_fast.hip operation for operation: one wave (64
+// lanes) per node, incremental FP64 products, exact quadratic step, cutoff
+// stop, final direct certificate.  No reference counterpart: pinned by the
+// same KATs as the plain form (optima, enumeration), by the plain form's
+// greedy for the simplex vertices, and by the kernel.
 // ---------------------------------------------------------------------------
 namespace {
 
