@@ -929,13 +929,13 @@ gb_status validate_settings(const gb_problem* pr, const gb_objective_desc* o, co
     if (s->form != 0 && l->kind == GB_LMO_SIMPLEX) {
         // the batched and mixed forms' simplex relaxes 0/1 nodes: every
         // variable integer with root bounds 0 or 1 (branching keeps them so)
-        bool ok = pr->n_int_vars == pr->n_vars && s->branch_rule != GB_BRANCH_CUSTOM && !s->device_frontier;
+        bool ok = pr->n_int_vars == pr->n_vars && s->branch_rule != GB_BRANCH_CUSTOM;
         for (int j = 0; j < pr->n_vars && ok; ++j)
             ok = (pr->lb[j] == 0.0 || pr->lb[j] == 1.0) && (pr->ub[j] == 0.0 || pr->ub[j] == 1.0);
         if (!ok)
             return fail(GB_ERR_INVALID_ARGUMENT,
-                        "the batched form's simplex needs a 0/1 problem (every variable integer, bounds 0 or 1), "
-                        "no custom branching and the host search");
+                        "the batched form's simplex needs a 0/1 problem (every variable integer, bounds 0 or 1) "
+                        "and no custom branching");
     }
     const bool qap32 = s->form == 2 && o->kind == GB_OBJ_QAP;
     if (qap32 && (l->kind != GB_LMO_BIRKHOFF || l->order != o->order || o->order > kQapMaxOrder

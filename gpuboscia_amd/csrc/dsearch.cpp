@@ -117,8 +117,10 @@ gb_status device_search(const gb_problem* pr, const gb_objective_desc* o, const 
     const int B = s->gpu_batch_size;
     if (s->form != 1 && s->form != 2)
         return fail(GB_ERR_INVALID_ARGUMENT, "device_frontier runs the batched forms (form 1 or 2)");
-    if (o->kind != GB_OBJ_DENSE_QUADRATIC || l->kind != GB_LMO_BOX || n > kMaxFastN)
-        return fail(GB_ERR_INVALID_ARGUMENT, "device_frontier needs a dense quadratic over a box, n <= 512");
+    if (o->kind != GB_OBJ_DENSE_QUADRATIC || (l->kind != GB_LMO_BOX && !fast_simplex_ok(l)) || n > kMaxFastN)
+        return fail(GB_ERR_INVALID_ARGUMENT,
+                    "device_frontier needs a dense quadratic over a box or a simplex with an integral right side, "
+                    "n <= 512");
     if (pr->n_int_vars > n) return fail(GB_ERR_INVALID_ARGUMENT, "device_frontier needs distinct integer variables");
     if (B < 1 || B > kDsMaxBatch)
         return fail(GB_ERR_INVALID_ARGUMENT, "device_frontier needs 1 <= gpu_batch_size <= 1024");
@@ -157,6 +159,7 @@ gb_status device_search(const gb_problem* pr, const gb_objective_desc* o, const 
     L.rel_tol = s->rel_gap_tol;
     L.rule = s->branch_rule;
     L.branch_k = s->branch_rule == GB_BRANCH_POW2 ? s->branch_k : 1;
+    L.child_warm = l->kind == GB_LMO_SIMPLEX ? 0 : 1;
     L.store.capacity = C;
     Arena arena;
     arena.plan(&L.S, 1);

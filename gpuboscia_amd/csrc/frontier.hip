@@ -517,7 +517,7 @@ __global__ void ds_post_emit_kernel(DsLaunch L) {
             L.store.parent[s] = A.id[r];
             L.store.depth[s] = A.depth[r] + 1;
             L.store.riters[s] = 0;
-            L.store.warm[s] = 1;
+            L.store.warm[s] = L.child_warm ? 1 : 0;
             L.post.fresh[off + c] = DsEntry{nb, id0 + c, s};
         }
     }

@@ -104,6 +104,8 @@ struct DsLaunch {
     double abs_tol, rel_tol;
     int rule;                  // GB_BRANCH_POW2 or GB_BRANCH_MOST_FRACTIONAL
     int branch_k;
+    int child_warm;            // children start at the parent's clipped iterate (a box);
+                               // 0: at the LMO vertex (a simplex, where clipping may break sum x <= rhs)
 };
 
 // Kernels of one round, in order; relax launches of A and B go between.

@@ -113,3 +113,27 @@ def test_unsupported_settings_are_rejected():
     s = settings(1, gb.BRANCH_LEARNED, 1024, True)
     with pytest.raises(RuntimeError, match="device_frontier"):
         gb.solve(20, [0] * 20, [1] * 20, range(20), obj, BOX, s)
+
+
+@pytest.mark.parametrize("n,seed,card", [(30, 1, 3), (60, 2, 6)])
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("batch", [1024, 16])
+def test_device_search_over_simplex_equals_host_search(n, seed, card, form, batch):
+    """Over the bounded simplex sum x <= card (0/1 variables) children start
+    at the LMO vertex in both drivers; the trees are the same."""
+    Q, c, _ = instances.binary_qp(n, seed=seed)
+    obj, keep = dense(Q, c)
+    ii = dev(np.ones(n, np.uint8))
+    lmo = gb.LmoDesc(gb.LMO_SIMPLEX, float(card), 0, ii.data_ptr(), 0)
+    out = []
+    for device in (False, True):
+        say(f"[dsearch] simplex n={n} seed={seed} card={card} form={form} batch={batch} "
+            f"{'device' if device else 'host'}")
+        s = settings(form, gb.BRANCH_POW2, batch, device)
+        if form == 2:
+            s.algorithm.algorithm = gb.ALG_DICG
+        r = gb.solve(n, [0] * n, [1] * n, range(n), obj, lmo, s)
+        say(f"[dsearch]   objective {r['objective']!r} nodes {r['nodes_processed']}")
+        out.append(r)
+    assert_same(out[0], out[1], f"simplex n={n} form={form} batch={batch}")
+    assert sum(out[1]["x"]) <= card
