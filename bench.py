@@ -668,7 +668,7 @@ def main():
     ap.add_argument("--only-qap", action="store_true", help="run only the Birkhoff QAP root-proof leg")
     ap.add_argument("--no-sx", action="store_true", help="skip the bounded-simplex time-to-optimum leg")
     ap.add_argument("--only-sx", action="store_true", help="run only the bounded-simplex time-to-optimum leg")
-    ap.add_argument("--sx-sizes", default="40,60,80", help="binary-QP sizes of the bounded-simplex leg")
+    ap.add_argument("--sx-sizes", default="40,60,80,100", help="binary-QP sizes of the bounded-simplex leg")
     ap.add_argument("--sx-seconds", type=float, default=60.0, help="time limit of each bounded-simplex solve")
     ap.add_argument("--qap-sizes", default="64,128,200", help="QAP orders of the FP32 form")
     ap.add_argument("--qap-plain-sizes", default="64", help="QAP orders also solved by the plain FP64 form")

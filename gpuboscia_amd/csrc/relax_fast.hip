@@ -720,7 +720,7 @@ constexpr int kRowSums = 3;  // fma-chain sums per exchange
 
 // Every node of a batch is meant to be in flight at once (the launcher uses
 // this layout only then), so the kernel asks for four waves per SIMD.
-template <int NT>
+template <int NT, bool SX = false>
 __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams P) {
     extern __shared__ __align__(16) double lds_fast[];
     const int lane = lane_id();
@@ -919,6 +919,41 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
         columns += count;
     };
 
+    // Simplex (SX): the greedy's selection over the node's rows.  Every wave
+    // stages its rows' keys and flags in LDS (the spaces of the changed-column
+    // lists, free between updates), then runs the same wave_select over all
+    // rows, so every wave holds the same result; each lane keeps its row's.
+    // want: rhs - sum lb (forward), or rhs - the mass of the rows flagged
+    // `base` (in-face) when use_mass.
+    unsigned long long* skey = reinterpret_cast<unsigned long long*>(dlist);
+    uint8_t* sflag = reinterpret_cast<uint8_t*>(klist);
+    auto select_rows = [&](double h, bool cand, bool base, double want, bool use_mass) -> bool {
+        if (in) {
+            skey[j] = order_key(h);
+            sflag[j] = static_cast<uint8_t>((cand ? 1u : 0u) | (base ? 2u : 0u));
+        }
+        __syncthreads();
+        unsigned long long key[NT];
+        bool c[NT], sel[NT];
+        int mass = 0;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int jj = lane + kWave * t;
+            const bool ii = jj < n;
+            key[t] = ii ? skey[jj] : 0ull;
+            const uint32_t f = ii ? sflag[jj] : 0u;
+            c[t] = (f & 1u) != 0u;
+            mass += __popcll(__ballot((f & 2u) != 0u));
+        }
+        wave_select<NT>(key, c, use_mass ? want - mass : want, sel, lane);
+        bool mine = false;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            if (t == w) mine = sel[t];
+        __syncthreads();
+        return mine;
+    };
+
     while (true) {
         if (threadIdx.x == 0) misc[NT] = atomicAdd(P.counter, 1);
         __syncthreads();
@@ -943,7 +978,40 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
         Ex e{};
         e.ok = !(in && (isnan(lb) || isnan(ub) || lb > ub || (fl && ceil(lb) > floor(ub))));
         e.ok2 = true;
-        exchange(e, 0, false, true);
+        double lbsum = 0.0;
+        if constexpr (SX) {
+            // the 0/1 scope, and Lmo<GB_LMO_SIMPLEX>::feasible with the
+            // one-wave kernel's sums (exact: the values are 0 and 1)
+            e.ok2 = !(in && (!P.is_int[j] || (lb != 0.0 && lb != 1.0) || (ub != 0.0 && ub != 1.0)));
+            e.a[0] = in ? fmax(0.0, lb) : 0.0;
+            e.b[0] = 1.0;
+            e.a[1] = in ? ub : 0.0;
+            e.b[1] = 1.0;
+            exchange(e, 2, false, true);
+        } else {
+            exchange(e, 0, false, true);
+        }
+        if (SX && e.all_ok && !e.all_ok2) {
+            if (threadIdx.x == 0) {
+                P.status[node] = 0xFF;
+                P.value[node] = __builtin_nan("");
+                P.gap[node] = __builtin_nan("");
+                P.best_val[node] = kNone;
+                P.iters[node] = initial;
+            }
+            continue;
+        }
+        if (SX && e.all_ok) {
+            Ex h{};
+            h.ok = !(in && P.flags && !fl);  // all integer (no continuous variable)
+            h.ok2 = true;
+            exchange(h, 0, false, true);
+            const double ms = e.sum[0];
+            bool feas = ms <= P.rhs + 1e-9;
+            if (P.equality) feas = feas && e.sum[1] >= P.rhs - 1e-9 && (!h.all_ok || fabs(P.rhs - round(P.rhs)) <= 1e-9);
+            e.all_ok = feas;
+            lbsum = ms;
+        }
         if (!e.all_ok) {
             if (threadIdx.x == 0) {
                 P.status[node] = GB_NODE_INFEASIBLE;
@@ -974,6 +1042,55 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
         };
 
         double x = in ? ((initial == 0 && !supplied) ? ub : xn[j]) : 0.0;
+        if constexpr (SX) {
+            if (initial == 0 && !supplied) {
+                // the greedy vertex of the artificial direction g_j = -1 - j / (n + 1)
+                const double h = -1.0 - (double)j / (n + 1.0);
+                const bool sel = select_rows(h, in && ub > lb && (P.equality || h <= 0.0), false, P.rhs - lbsum, false);
+                x = in ? (sel ? ub : lb) : 0.0;
+            } else {
+                // a supplied point: clipped, put on its faces and on the mass
+                // limit (the one-wave kernel's sums and formulas)
+                bool fr = false;
+                if (in) {
+                    double vv = fmin(fmax(x, lb), ub);
+                    if (fabs(vv - lb) <= atol + rtol * fmax(fabs(vv), fabs(lb))) vv = lb;
+                    else if (fabs(vv - ub) <= atol + rtol * fmax(fabs(vv), fabs(ub))) vv = ub;
+                    else fr = true;
+                    x = vv;
+                }
+                Ex m1{}, m2{};
+                m1.a[0] = in ? x : 0.0;
+                m1.b[0] = 1.0;
+                m1.a[1] = fr ? x - lb : 0.0;
+                m1.b[1] = 1.0;
+                m1.a[2] = fr ? ub - x : 0.0;
+                m1.b[2] = 1.0;
+                exchange(m1, 3, false, false);
+                m2.a[0] = in ? lb : 0.0;
+                m2.b[0] = 1.0;
+                m2.a[1] = in ? ub : 0.0;
+                m2.b[1] = 1.0;
+                exchange(m2, 2, false, false);
+                const double tx = m1.sum[0], tdn = m1.sum[1], tup = m1.sum[2], tl = m2.sum[0], tu = m2.sum[1];
+                if (tx > P.rhs) {
+                    const double e0 = tx - P.rhs;
+                    if (tdn > e0) {
+                        if (fr) x = fma(lb - x, e0 / tdn, x);
+                    } else {
+                        x = fma(x - lb, (P.rhs - tl) / (tx - tl), lb);
+                    }
+                } else if (P.equality && tx < P.rhs) {
+                    const double sh = P.rhs - tx;
+                    if (tup > sh) {
+                        if (fr) x = fma(ub - x, sh / tup, x);
+                    } else {
+                        x = fma(ub - x, sh / (tu - tx), x);
+                    }
+                }
+                if (!in) x = 0.0;
+            }
+        }
         double qx = 0.0, v, qv = 0.0, a = 0.0, qa = 0.0, g;
         if (P.rescore && P.best_val[node] < kNone) {
             const double z = in ? bvn[j] : 0.0;
@@ -988,13 +1105,45 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
             gemv(x, qx);
         }
         auto grad = [&]() { g = P.gs * qx + lsc; };
-        auto fwd = [&]() { return (g > 0.0) ? lb : ub; };
+        double txm = 0.0;  // simplex: sum x at the last in-face vertex (the mass limit reuses it)
+        auto fwd = [&]() {
+            if constexpr (SX) {
+                const bool sel = select_rows(g, in && ub > lb && (P.equality || g <= 0.0), false, P.rhs - lbsum, false);
+                return sel ? ub : lb;
+            } else {
+                return (g > 0.0) ? lb : ub;
+            }
+        };
         auto away = [&]() {
-            const double ut = atol + rtol * fabs(ub);
-            const double lt = atol + rtol * fabs(lb);
-            if (fabs(x - ub) <= ut) return ub;
-            if (fabs(x - lb) <= lt) return lb;
-            return (-g > 0.0) ? lb : ub;
+            if constexpr (SX) {
+                // in-face vertex for -g (lmo.cuh:814-863): rows at a bound stay,
+                // the face keeps the full mass when sum x == rhs within tolerance
+                bool fixed = false;
+                double base = lb;
+                if (in) {
+                    if (fabs(x - lb) <= atol + rtol * fmax(fabs(x), fabs(lb))) {
+                        fixed = true;
+                    } else if (fabs(x - ub) <= atol + rtol * fmax(fabs(x), fabs(ub))) {
+                        fixed = true;
+                        base = ub;
+                    }
+                }
+                Ex mx{};
+                mx.a[0] = in ? x : 0.0;
+                mx.b[0] = 1.0;
+                exchange(mx, 1, false, false);
+                txm = mx.sum[0];
+                const bool force = P.equality || fabs(txm - P.rhs) <= atol + rtol * fmax(fabs(txm), fabs(P.rhs));
+                const bool cand = in && !fixed && ub > base && (force || -g <= 0.0);
+                const bool sel = select_rows(-g, cand, in && base != 0.0, P.rhs, true);
+                return in ? (sel ? ub : base) : 0.0;
+            } else {
+                const double ut = atol + rtol * fabs(ub);
+                const double lt = atol + rtol * fabs(lb);
+                if (fabs(x - ub) <= ut) return ub;
+                if (fabs(x - lb) <= lt) return lb;
+                return (-g > 0.0) ? lb : ub;
+            }
         };
         grad();
         v = fwd();
@@ -1053,10 +1202,17 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
                 score_in(r, 0, a, qa);
                 r.ok2 = true;
                 const double dv = a - v;  // direction a - v as the reference
-                r.mn = !in ? __builtin_inf()
-                       : dv < 0.0 ? (ub - x) / -dv
-                       : dv > 0.0 ? (x - lb) / dv
-                                  : __builtin_inf();
+                if constexpr (SX) {
+                    r.mn = !in ? __builtin_inf()
+                           : dv > 1e-6 ? (x - lb) / dv
+                           : dv < -1e-6 ? (ub - x) / -dv
+                                        : __builtin_inf();
+                } else {
+                    r.mn = !in ? __builtin_inf()
+                           : dv < 0.0 ? (ub - x) / -dv
+                           : dv > 0.0 ? (x - lb) / dv
+                                      : __builtin_inf();
+                }
                 d = v - a;
                 qd = qv - qa;
                 r.a[1] = -g;
@@ -1065,7 +1221,30 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
                 r.b[2] = qd;
                 exchange(r, 3, true, true);
                 score_out(r, 0, a);
-                gmax = fmax(0.0, fmin(r.min, 1.0));
+                if constexpr (SX) {
+                    // the mass limit (lmo.cuh:865-906): sum x is the in-face
+                    // vertex's (x has not moved since), sum (a - v) an exact
+                    // integer from the vertices' counts
+                    const int cnt_a = __popcll(__ballot(in && a != 0.0));
+                    const int cnt_v = __popcll(__ballot(in && v != 0.0));
+                    misc[NT + 1 + 2 * w] = cnt_a;
+                    misc[NT + 2 + 2 * w] = cnt_v;
+                    __syncthreads();
+                    int ta = 0, tv = 0;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        ta += misc[NT + 1 + 2 * t];
+                        tv += misc[NT + 2 + 2 * t];
+                    }
+                    __syncthreads();
+                    const double td = (double)(ta - tv);
+                    double agg;
+                    if (P.equality && fabs(td) > 1e-9) agg = 0.0;
+                    else agg = td < 0.0 ? fmin(1.0, (txm - P.rhs) / td) : 1.0;
+                    gmax = fmax(0.0, fmin(r.min, agg));
+                } else {
+                    gmax = fmax(0.0, fmin(r.min, 1.0));
+                }
             } else {
                 d = v - x;
                 qd = qv - qx;
@@ -1113,12 +1292,23 @@ __global__ void __launch_bounds__(64 * NT, 4) relax_fast_rows_kernel(FastParams 
         const double fgap = s.sum[1];
         const double fval = s.sum[2] + P.kappa;
         uint8_t st = (fgap >= 0.0 && fgap < eps) ? 1 : 0;
+        bool take = true;
+        if (SX && P.flags) {  // Lmo<GB_LMO_SIMPLEX>::rounded_feasible
+            Ex rf{};
+            rf.a[0] = in ? cand : 0.0;
+            rf.b[0] = 1.0;
+            rf.ok = !(in && !(cand >= -P.ctol));
+            rf.ok2 = true;
+            exchange(rf, 1, false, true);
+            const double tot = rf.sum[0];
+            take = rf.all_ok && (P.equality ? fabs(tot - P.rhs) <= P.ctol : tot <= P.rhs + P.ctol);
+        }
         Ex c{};
         if (P.flags) score_in(c, 0, cand, qc);
         else c.ok = true;
         c.ok2 = !(in && (!isfinite(x) || (fl && fabs(x - round(x)) > P.ctol)));
         exchange(c, P.flags ? 1 : 0, false, true);
-        if (P.flags) score_out(c, 0, cand);
+        if (P.flags && take) score_out(c, 0, cand);
         if (in) xn[j] = x;
         if (c.all_ok2) st = st ? GB_NODE_INTEGRAL_CONVERGED : GB_NODE_INTEGRAL_UNCONVERGED;
         if (threadIdx.x == 0) {
@@ -1372,14 +1562,14 @@ int launch_scatter_iterates(int n, int N, const unsigned long long* dst, const d
 // list, the exchange buffers, counts and the node id.
 static size_t rows_lds_bytes(int n_pad, int nt) {
     const size_t doubles = (size_t)n_pad * 7 / 2 + (size_t)2 * nt * kWave * (2 * kRowSums + 1);
-    const size_t ints = (size_t)2 * nt * kWave * 2 + nt + 1;
+    const size_t ints = (size_t)2 * nt * kWave * 2 + nt + 1 + 2 * nt;  // + the simplex's per-wave counts
     return doubles * 8 + ints * 4;
 }
 
 // Nodes relax_fast_rows_kernel<NT> keeps in flight on this device with
 // `lds` bytes of LDS per workgroup (occupancy x CUs), cached per device and
 // LDS size.
-template <int NT>
+template <int NT, bool SX>
 static int rows_capacity(size_t lds) {
     static std::mutex mu;
     static std::map<std::pair<int, size_t>, int> cached;
@@ -1389,7 +1579,7 @@ static int rows_capacity(size_t lds) {
     const auto it = cached.find({dev, lds});
     if (it != cached.end()) return it->second;
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_fast_rows_kernel<NT>, NT * kWave, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_fast_rows_kernel<NT, SX>, NT * kWave, lds) !=
             hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
@@ -1400,18 +1590,28 @@ int launch_relax_fast(const FastParams& p, int waves_per_block, int blocks, void
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, p.counter);
     const bool sx = p.simplex != 0;
-    if (waves_per_block == 0 && p.n > 64 && !sx) {
+    // (over a simplex the rows layout pays a selection barrier per vertex,
+    // which only more than two waves per node repay: measured on 1024 nodes,
+    // 3.4 against 2.7 ms at n = 80, 5.4 against 8.0 ms at n = 200)
+    if (waves_per_block == 0 && p.n > (sx ? 128 : 64)) {
         // One node per workgroup when the whole batch is in flight at once;
-        // otherwise the one-wave-per-node layout keeps more nodes resident
-        // (the simplex LMO runs in the one-wave-per-node layout only).
+        // otherwise the one-wave-per-node layout keeps more nodes resident.
         const int nt = p.n <= 128 ? 2 : (p.n <= 256 ? 4 : 8);
         const size_t lds = rows_lds_bytes(p.n_pad, nt);
-        const int cap = nt == 2 ? rows_capacity<2>(lds) : nt == 4 ? rows_capacity<4>(lds) : rows_capacity<8>(lds);
+        int cap = 0;
+        if (sx) cap = nt == 2 ? rows_capacity<2, true>(lds) : nt == 4 ? rows_capacity<4, true>(lds) : rows_capacity<8, true>(lds);
+        else cap = nt == 2 ? rows_capacity<2, false>(lds) : nt == 4 ? rows_capacity<4, false>(lds) : rows_capacity<8, false>(lds);
         if (p.N <= cap) {
             const dim3 grid(p.N < blocks || blocks <= 0 ? p.N : blocks), block(nt * kWave);
-            if (nt == 2) hipLaunchKernelGGL(relax_fast_rows_kernel<2>, grid, block, lds, st, p);
-            else if (nt == 4) hipLaunchKernelGGL(relax_fast_rows_kernel<4>, grid, block, lds, st, p);
-            else hipLaunchKernelGGL(relax_fast_rows_kernel<8>, grid, block, lds, st, p);
+            if (sx) {
+                if (nt == 2) hipLaunchKernelGGL((relax_fast_rows_kernel<2, true>), grid, block, lds, st, p);
+                else if (nt == 4) hipLaunchKernelGGL((relax_fast_rows_kernel<4, true>), grid, block, lds, st, p);
+                else hipLaunchKernelGGL((relax_fast_rows_kernel<8, true>), grid, block, lds, st, p);
+            } else {
+                if (nt == 2) hipLaunchKernelGGL((relax_fast_rows_kernel<2, false>), grid, block, lds, st, p);
+                else if (nt == 4) hipLaunchKernelGGL((relax_fast_rows_kernel<4, false>), grid, block, lds, st, p);
+                else hipLaunchKernelGGL((relax_fast_rows_kernel<8, false>), grid, block, lds, st, p);
+            }
             return (int)hipGetLastError();
         }
     }

@@ -297,7 +297,8 @@ typedef struct {
     int refresh_interval;     /* recompute Q x directly every k iterations (0 = never) */
     int waves_per_block;      /* 1..4: nodes resident per workgroup, one wave each;
                                  0: one node per workgroup with its rows split over
-                                 ceil(n/64) waves (n > 64; n <= 64 uses 4) */
+                                 ceil(n/64) waves when the batch fits in flight
+                                 (n > 64, over a simplex n > 128; else 4) */
     int blocks;               /* persistent workgroups (0 = automatic) */
     long long* stats;         /* optional device [N][2]: Q columns read by the incremental
                                  updates (a column serving Q v and Q a in one gather counts

@@ -83,7 +83,10 @@ def run_pair(Q, c, dicg, LB, UB, X, ST, card, equality, eps, max_iters, is_int=N
     (20, True, False, False, 4), (20, False, False, False, 4), (48, True, True, False, 1),
     (64, True, False, True, 4), (100, True, False, False, 0), (100, False, True, False, 4),
     (128, False, False, True, 4),
-    (200, True, False, True, 4), (300, True, False, False, 2), (512, True, True, False, 4)])
+    (200, True, False, True, 4), (300, True, False, False, 2), (512, True, True, False, 4),
+    # wpb 0: one node per workgroup with its rows over the waves (the batch fits in flight)
+    (80, True, True, True, 0), (128, False, False, False, 0), (200, True, False, True, 0),
+    (256, True, True, False, 0), (300, True, False, True, 0), (512, True, False, False, 0)])
 def test_fast_simplex_kernel_parity(n, dicg, equality, warm, wpb):
     Q, c, _ = instances.binary_qp(n, seed=17)
     N = 40
