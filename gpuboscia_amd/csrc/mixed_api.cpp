@@ -478,14 +478,16 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
             gb_status st;
             if (b.mixed) {
                 // k entries share the CUs: widen the tiles so that together
-                // they cover the chip once, within the default shape's
-                // rows-per-lane class (same arithmetic, so each entry still
-                // equals its solo call).
+                // they cover the chip once, up to the MFMA's 16 node columns.
+                // A tile reads its Q fragments once per iteration whatever
+                // its width, so wider tiles cut the fragment traffic per node
+                // (at 1024 nodes and k = 8: 16 columns instead of the solo
+                // default's 4).  Each entry equals its solo call with
+                // gb_mixed_options.columns = clamp(ceil(N k / CUs), default, 16).
                 gb_mixed_options mo{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
                 const int def = mixed_default_shape(b.mixed->rt, b.N).cols;
-                const int cap = 4 * mixed_rows_per_lane(def);
                 const int want = cus > 0 ? (int)(((long long)b.N * k + cus - 1) / cus) : def;
-                mo.columns = want < def ? def : (want > cap ? cap : want);
+                mo.columns = want < def ? def : (want > 16 ? 16 : want);
                 st = relax_mixed_impl(b.mixed, b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,
                                       b.integer_flags, candidate_tol, b.cutoff, &mo, s, counter);

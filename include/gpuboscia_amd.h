@@ -424,14 +424,20 @@ gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
  * compile time, device_resident_solver.cuh:362, :1217-1218): independent
  * node batches of different problems and sizes relaxed in one call.  Each
  * entry is relaxed like gb_relax_batch_mixed (when `mixed` is set: dense
- * objective, box LMO, n <= 512), like gb_relax_batch_fast (other entries of
+ * objective, n <= 512, a box LMO or, under DICG, a simplex with an integral
+ * right side over 0/1 nodes), like gb_relax_batch_fast (other entries of
  * a dense objective with n <= 512 over a box, or over a simplex with an
  * integral right side, whose nodes must then be 0/1), or like
  * gb_relax_batch (every other entry: FP64, the algorithm's own line
  * search, cutoff unused), with the entry's own buffers, on one of
  * `concurrency` internal streams (1..8) forked from and joined back into
  * `stream`, so small batches of different instances share the GPU.  Every
- * entry's results equal those of its solo call.  After one call on the same
+ * entry's results equal those of its solo call; a mixed entry's solo call
+ * being the one with gb_mixed_options.columns = min(16, max(c0, ceil(N k /
+ * CUs))), where c0 is the automatic choice ceil(N / 256) clamped to 1..16
+ * and k = min(concurrency, count): the entries share the chip, so their
+ * tiles widen towards the MFMA's 16 node columns, which cuts the Q fragment
+ * reads per node (the same arithmetic as that solo call).  After one call on the same
  * buffers (which makes the library's allocations) a call can be captured in
  * a HIP graph on `stream` and replayed.
  */

@@ -97,10 +97,19 @@ def check_properties(e):
     return int(r["iterations"].sum()), int(np.isin(st, (1, 4)).sum())
 
 
-def compare_with_oracle(e, alg, rows):
+def in_call_columns(N, k):
+    """Node columns per tile of a mixed entry in a k-entry gb_relax_instances
+    call: min(16, max(ceil(N / 256), ceil(N k / CUs)))."""
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    return min(16, max(min(16, max(1, -(-N // 256))), -(-N * k // cus)))
+
+
+def compare_with_oracle(e, alg, rows, columns):
     """A slice of one instance's nodes against the oracle: the mixed form
-    through its pass-1 output (a solo call of pass 1 only, which equals the
-    in-call pass 1) and the oracle's FP64 pass from it (box or simplex)."""
+    through its pass-1 output (a solo call of pass 1 only with the in-call
+    tile width, which equals the in-call pass 1) and the oracle's FP64 pass
+    from it (box or simplex)."""
     import torch
     from oracle import pyoracle as orc
     n = e["n"]
@@ -116,7 +125,7 @@ def compare_with_oracle(e, alg, rows):
     bv, bvv = torch.zeros((N, n), **f64), torch.full((N,), 1e300, **f64)
     gb.relax_batch_mixed(e["mixed"], e["obj"], alg, e["lmo"], N, BUDGET, x, torch.from_numpy(LB).cuda(),
                          torch.from_numpy(UB).cuda(), torch.from_numpy(EPS).cuda(), it, st, gap, val, bv, bvv,
-                         e["flags"], 1e-6, None, fp64_max_iters=-1)
+                         e["flags"], 1e-6, None, fp64_max_iters=-1, columns=columns)
     torch.cuda.synchronize()
     o = orc.relax_fast_batch(orc.dense_objective(e["Q"], e["c"], 2.0, 0.5, 0.0), True, x.cpu().numpy(), LB.copy(),
                              UB.copy(), EPS.copy(), it.cpu().numpy(), st.cpu().numpy(), flags=ones,
@@ -148,4 +157,4 @@ def test_c3_full_shape_one_call():
         ks = sorted((e for e in entries if e["kind"] == kind), key=lambda e: e["n"])
         for e in (ks[0], ks[-1]):
             say(f"[c3] oracle comparison: {kind} instance {e['k']} n={e['n']}")
-            compare_with_oracle(e, alg, slice(0, 48))
+            compare_with_oracle(e, alg, slice(0, 48), in_call_columns(NODES, min(8, COUNT)))

@@ -87,6 +87,50 @@ def test_instances_equal_solo_runs(concurrency):
         assert int(a["iterations"].sum().item()) > 0
 
 
+def test_instances_widen_mixed_tiles():
+    """Large mixed entries sharing the chip run wider tiles (up to the MFMA's
+    16 node columns): each equals its solo call with
+    columns = min(16, max(ceil(N / 256), ceil(N k / CUs))), k = min(concurrency, count)."""
+    import torch
+    specs = [(100, 1024, 0), (300, 700, 75), (60, 1024, 15), (450, 512, 0)]
+    alg = gb.algorithm(dicg=True, line_search=gb.LS_QUADRATIC_EXACT)
+    budget, conc = 60, 4
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+
+    def entry(k, n, N, card):
+        Q, c, _ = instances.binary_qp(n, seed=900 + k)
+        dQt, dc = dev(np.ascontiguousarray(Q.T)), dev(c)
+        obj = gb.ObjectiveDesc(gb.OBJ_DENSE, n, dQt.data_ptr(), dc.data_ptr(), 2.0, 0.5, 0.0, 0, 0, 0)
+        LB, UB, eps = instances.open_nodes(n, N, 950 + k, card=card)
+        flags = dev(np.ones(n, np.uint8))
+        lmo = (gb.LmoDesc(gb.LMO_SIMPLEX, float(card), 0, flags.data_ptr(), 0) if card
+               else gb.LmoDesc(gb.LMO_BOX, 1.0, 0, 0, 0))
+        f64 = dict(dtype=torch.float64, device="cuda")
+        return {"obj": obj, "lmo": lmo, "keep": (dQt, dc), "mixed": gb.MixedQ(obj, 3),
+                "x": dev(LB.copy()), "lb": dev(LB), "ub": dev(UB), "eps": dev(eps),
+                "iterations": torch.zeros(N, dtype=torch.int32, device="cuda"),
+                "status": torch.zeros(N, dtype=torch.uint8, device="cuda"),
+                "gap": torch.zeros(N, **f64), "value": torch.zeros(N, **f64),
+                "best_vertex": torch.zeros((N, n), **f64), "best_vertex_val": torch.full((N,), 1e300, **f64),
+                "flags": flags, "cutoff": None}
+
+    together = [entry(k, *sp) for k, sp in enumerate(specs)]
+    solo = [entry(k, *sp) for k, sp in enumerate(specs)]
+    k = min(conc, len(specs))
+    cols = [min(16, max(min(16, max(1, -(-N // 256))), -(-N * k // cus))) for _, N, _ in specs]
+    say(f"[instances] wide mixed tiles: columns {cols} on {cus} CUs")
+    assert max(cols) == 16 and min(cols) < 16
+    gb.relax_instances(together, alg, budget, concurrency=conc)
+    for e, cc in zip(solo, cols):
+        gb.relax_batch_mixed(e["mixed"], e["obj"], alg, e["lmo"], e["x"].shape[0], budget, e["x"], e["lb"],
+                             e["ub"], e["eps"], e["iterations"], e["status"], e["gap"], e["value"],
+                             e["best_vertex"], e["best_vertex_val"], e["flags"], 1e-6, e["cutoff"], columns=cc)
+    torch.cuda.synchronize()
+    for i, (a, b) in enumerate(zip(together, solo)):
+        assert_bit_equal(results(a), results(b), f"entry {i} n={specs[i][0]} columns={cols[i]}")
+        assert int(a["iterations"].sum().item()) > 0
+
+
 def test_instances_reject_bad_entries():
     e = make_entry(40, 8, 1, 0)
     e["obj"] = gb.ObjectiveDesc(gb.OBJ_DENSE, 40, 0, 0, 2.0, 0.5, 0.0, 0, 0, 0)  # no Qt
