@@ -453,6 +453,14 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
         }
         int cus = 0;
         GB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        // Entries sharing the chip widen their mixed tiles (below).  A graph
+        // replay overlaps the fork streams' kernels less than eager streams
+        // do (measured: 666 ms per replay of the 256-instance batch with
+        // widened tiles, 511 ms at the one-stream width, 382 ms eager and
+        // widened), so a call under capture keeps the one-stream width.
+        hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
+        GB_HIP(hipStreamIsCapturing(caller, &capturing));
+        const int sharing = capturing == hipStreamCaptureStatusNone ? k : 1;
         // Entries of the plain FP64 form use their fork stream's work
         // vectors: size them for the largest such entry first, so that no
         // entry grows them (and waits on its stream) mid-call.
@@ -477,16 +485,17 @@ extern "C" gb_status gb_relax_instances(int count, const gb_instance_batch* batc
             int* counter = lease[i % k].ptr;
             gb_status st;
             if (b.mixed) {
-                // k entries share the CUs: widen the tiles so that together
-                // they cover the chip once, up to the MFMA's 16 node columns.
-                // A tile reads its Q fragments once per iteration whatever
-                // its width, so wider tiles cut the fragment traffic per node
-                // (at 1024 nodes and k = 8: 16 columns instead of the solo
-                // default's 4).  Each entry equals its solo call with
-                // gb_mixed_options.columns = clamp(ceil(N k / CUs), default, 16).
+                // `sharing` entries share the CUs: widen the tiles so that
+                // together they cover the chip once, up to the MFMA's 16 node
+                // columns.  A tile reads its Q fragments once per iteration
+                // whatever its width, so wider tiles cut the fragment traffic
+                // per node (at 1024 nodes and 8 streams: 16 columns instead
+                // of the solo default's 4).  Each entry equals its solo call
+                // with gb_mixed_options.columns =
+                // clamp(ceil(N sharing / CUs), default, 16).
                 gb_mixed_options mo{0, 0, 0, 1e-6f, 0, nullptr, nullptr, nullptr};
                 const int def = mixed_default_shape(b.mixed->rt, b.N).cols;
-                const int want = cus > 0 ? (int)(((long long)b.N * k + cus - 1) / cus) : def;
+                const int want = cus > 0 ? (int)(((long long)b.N * sharing + cus - 1) / cus) : def;
                 mo.columns = want < def ? def : (want > 16 ? 16 : want);
                 st = relax_mixed_impl(b.mixed, b.objective, algorithm, b.lmo, b.N, max_iters, b.x, b.lb, b.ub, b.eps,
                                       b.iterations, b.status, b.gap, b.value, b.best_vertex, b.best_vertex_val,

@@ -435,9 +435,12 @@ gb_status gb_relax_batch_mixed(const gb_mixed_q* q,
  * entry's results equal those of its solo call; a mixed entry's solo call
  * being the one with gb_mixed_options.columns = min(16, max(c0, ceil(N k /
  * CUs))), where c0 is the automatic choice ceil(N / 256) clamped to 1..16
- * and k = min(concurrency, count): the entries share the chip, so their
- * tiles widen towards the MFMA's 16 node columns, which cuts the Q fragment
- * reads per node (the same arithmetic as that solo call).  After one call on the same
+ * and k = min(concurrency, count), or k = 1 when `stream` is capturing (a
+ * graph replay overlaps the streams less than eager calls do): the entries
+ * share the chip, so their tiles widen towards the MFMA's 16 node columns,
+ * which cuts the Q fragment reads per node (the same arithmetic as that
+ * solo call).  A captured call thus equals an eager call with concurrency 1.
+ * After one call on the same
  * buffers (which makes the library's allocations) a call can be captured in
  * a HIP graph on `stream` and replayed.
  */

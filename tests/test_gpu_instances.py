@@ -130,6 +130,41 @@ def test_instances_widen_mixed_tiles():
         assert_bit_equal(results(a), results(b), f"entry {i} n={specs[i][0]} columns={cols[i]}")
         assert int(a["iterations"].sum().item()) > 0
 
+    # under capture the call keeps the one-stream width: a replay equals an
+    # eager call with concurrency 1
+    one = [entry(k, *sp) for k, sp in enumerate(specs)]
+    gb.relax_instances(one, alg, budget, concurrency=1)
+    torch.cuda.synchronize()
+    cap = [entry(k, *sp) for k, sp in enumerate(specs)]
+    init = [(e["x"].clone(), e["iterations"].clone(), e["status"].clone()) for e in cap]
+
+    def refill():
+        for e, (x0, it0, st0) in zip(cap, init):
+            e["x"].copy_(x0)
+            e["iterations"].copy_(it0)
+            e["status"].copy_(st0)
+            e["best_vertex_val"].fill_(1e300)
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        gb.relax_instances(cap, alg, budget, concurrency=conc, stream=s.cuda_stream)  # allocations
+        refill()
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        gb.relax_instances(cap, alg, budget, concurrency=conc, stream=s.cuda_stream)
+    s.synchronize()
+    for rep in range(2):
+        with torch.cuda.stream(s):
+            refill()
+        s.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        say(f"[instances] wide tiles: captured replay {rep}")
+        for i, (a, b) in enumerate(zip(cap, one)):
+            assert_bit_equal(results(a), results(b), f"replay {rep} entry {i} n={specs[i][0]}")
+    del g
+
 
 def test_instances_reject_bad_entries():
     e = make_entry(40, 8, 1, 0)
